@@ -1,0 +1,112 @@
+/*
+ * hr_oracle.h -- FP64 CPU restatement of PyRITE's transport and scoring hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
+ * never as the thing measured or shipped.  The product path (pyrite_amd) must
+ * not link, import or execute anything under oracle/.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the PyRITE source tree, src/pyrite/...).  Arithmetic follows the reference's
+ * expression order and is compiled with -ffp-contract=off so no multiply-add is
+ * fused that the reference (NumPy / Numba without fastmath) does not fuse.
+ *
+ * Pinning status per function is listed in oracle/README.md.
+ */
+#ifndef HR_ORACLE_H
+#define HR_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- counter-addressed SplitMix64 streams (transport/kinematics.py:198-252) */
+uint64_t or_splitmix64(uint64_t x);
+uint64_t or_stream_key(uint64_t seed, uint64_t electron);
+double or_stream_uniform(uint64_t key, uint64_t counter);
+void or_stream_keys(uint64_t seed, int64_t start, int64_t n, uint64_t *out);
+uint64_t or_urban_stream_key(uint64_t stream_key);
+uint64_t or_urban_flight_key(uint64_t urban_key, int64_t flight, int64_t substep);
+
+/* ---- stopping power (transport/stopping.py:384-411, straggling.py:305-329) */
+double or_dEds_spliced_element(double J, double k, double coeff, double E_cross, double E);
+double or_dEds_spliced_packed(const double *J, const double *k, const double *coeff,
+                              const double *E_cross, int n_el, double E);
+
+/* ---- Urban straggling (transport/straggling.py:131-363) */
+/* Returns the positive sampled loss [keV]; *counter is advanced. */
+double or_urban_sample_compound(const double *Z, const double *J, const double *k,
+                                const double *coeff, const double *E_cross, int n_el, double E,
+                                double s, double stopping_scale, uint64_t key, uint64_t *counter);
+/* Analytic (mean, variance) of one element's Urban loss (straggling.py:204-226). */
+void or_urban_moments_element(double Z, double J, double C, double E, double s, double *mean,
+                              double *var);
+int64_t or_urban_poisson(double lam, uint64_t key, uint64_t *counter);
+
+/* ---- LUT interpolation (transport/lut.py:94-124) */
+void or_lut_index_frac(double E, double log_E_min, double inv_dlogE, int n_energy, int *i,
+                       double *f);
+
+/* ---- direction rotation (transport/core_geometry.py:16-36), in place */
+void or_rotate_direction(double *d, double cos_t, double phi);
+
+/* ---- per-electron LUT transport core (transport/cores.py:365-1250,
+ *      transport/_jit_lut_kernel.py:64-574) */
+typedef struct or_transport_args {
+    /* run */
+    int64_t e_start, e_count, cap;
+    const uint64_t *stream_key; /* [Ne] */
+    /* control */
+    int32_t max_steps, energy_model; /* 1 = midpoint, 0 = frozen */
+    double max_dE_frac;
+    int32_t straggle_on;
+    /* geometry */
+    int32_t n_layers;
+    const double *internal_bounds; /* [n_layers-1] */
+    double z_total;
+    int32_t finite_footprint;
+    double width_ang, height_ang;
+    const int32_t *L_nel;   /* [n_layers] */
+    const double *L_top;    /* [n_layers] */
+    const double *L_bot;    /* [n_layers] */
+    /* materials (padded per layer, stride max_el) */
+    int32_t max_el;
+    const double *L_Z, *L_J, *L_k, *L_coeff, *L_E_cross;
+    /* lut */
+    double lut_log_E_min, lut_inv_dlogE;
+    int32_t lut_n_energy;
+    const double *lut_total_rate; /* [n_layers, n_E] */
+    const double *lut_dEds;       /* [n_layers, n_E] */
+    const double *lut_inv_beta;   /* [n_E] */
+    const double *lut_cdf;        /* [n_layers, max_el, n_E] */
+    const double *lut_alpha;      /* [n_layers, max_el, n_E] */
+    /* state (global electron index) */
+    const uint8_t *alive;
+    double *clock, *pos, *dirs, *E_keV;
+    const double *E_cut;
+    double *stragg_dE;
+    /* rows, slot = i*cap + s (batching.py:401-425 midpoint schema) */
+    double *seg_dir, *seg_mid, *seg_len, *seg_E, *seg_t0;
+    int64_t *seg_id;
+    int16_t *seg_lay;
+    double *seg_E_end, *seg_t_end;
+    int64_t *seg_flight, *seg_substep;
+    int8_t *seg_event;
+    /* per local electron */
+    int64_t *seg_count;
+    int8_t *exit_code;
+} or_transport_args;
+
+/* Runs electrons [e_start, e_start+e_count) with `threads` OpenMP threads
+ * (<=0: all).  Each electron is independent, so the result does not depend on
+ * the thread count. */
+void or_transport_lut(const or_transport_args *a, int threads);
+
+/* ---- scoring (declared in the scoring section) */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,233 @@
+"""ctypes front end of the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg.  The product package never imports this module.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+_dp = C.c_void_p
+
+
+def build() -> Path:
+    """Compile the oracle with its Makefile (gcc, OpenMP)."""
+    res = subprocess.run(["make", "-C", str(HERE), "-s"], capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{res.stdout}\n{res.stderr}")
+    return LIB_PATH
+
+
+class TransportArgs(C.Structure):
+    _fields_ = [
+        ("e_start", C.c_int64), ("e_count", C.c_int64), ("cap", C.c_int64),
+        ("stream_key", _dp),
+        ("max_steps", C.c_int32), ("energy_model", C.c_int32),
+        ("max_dE_frac", C.c_double), ("straggle_on", C.c_int32),
+        ("n_layers", C.c_int32), ("internal_bounds", _dp), ("z_total", C.c_double),
+        ("finite_footprint", C.c_int32), ("width_ang", C.c_double), ("height_ang", C.c_double),
+        ("L_nel", _dp), ("L_top", _dp), ("L_bot", _dp),
+        ("max_el", C.c_int32),
+        ("L_Z", _dp), ("L_J", _dp), ("L_k", _dp), ("L_coeff", _dp), ("L_E_cross", _dp),
+        ("lut_log_E_min", C.c_double), ("lut_inv_dlogE", C.c_double), ("lut_n_energy", C.c_int32),
+        ("lut_total_rate", _dp), ("lut_dEds", _dp), ("lut_inv_beta", _dp), ("lut_cdf", _dp),
+        ("lut_alpha", _dp),
+        ("alive", _dp), ("clock", _dp), ("pos", _dp), ("dirs", _dp), ("E_keV", _dp),
+        ("E_cut", _dp), ("stragg_dE", _dp),
+        ("seg_dir", _dp), ("seg_mid", _dp), ("seg_len", _dp), ("seg_E", _dp), ("seg_t0", _dp),
+        ("seg_id", _dp), ("seg_lay", _dp), ("seg_E_end", _dp), ("seg_t_end", _dp),
+        ("seg_flight", _dp), ("seg_substep", _dp), ("seg_event", _dp),
+        ("seg_count", _dp), ("exit_code", _dp),
+    ]
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        L.or_splitmix64.restype = C.c_uint64
+        L.or_splitmix64.argtypes = [C.c_uint64]
+        L.or_stream_key.restype = C.c_uint64
+        L.or_stream_key.argtypes = [C.c_uint64, C.c_uint64]
+        L.or_stream_uniform.restype = C.c_double
+        L.or_stream_uniform.argtypes = [C.c_uint64, C.c_uint64]
+        L.or_stream_keys.restype = None
+        L.or_stream_keys.argtypes = [C.c_uint64, C.c_int64, C.c_int64, _dp]
+        L.or_urban_stream_key.restype = C.c_uint64
+        L.or_urban_stream_key.argtypes = [C.c_uint64]
+        L.or_urban_flight_key.restype = C.c_uint64
+        L.or_urban_flight_key.argtypes = [C.c_uint64, C.c_int64, C.c_int64]
+        L.or_dEds_spliced_element.restype = C.c_double
+        L.or_dEds_spliced_element.argtypes = [C.c_double] * 5
+        L.or_dEds_spliced_packed.restype = C.c_double
+        L.or_dEds_spliced_packed.argtypes = [_dp, _dp, _dp, _dp, C.c_int, C.c_double]
+        L.or_urban_sample_compound.restype = C.c_double
+        L.or_urban_sample_compound.argtypes = [_dp, _dp, _dp, _dp, _dp, C.c_int, C.c_double,
+                                              C.c_double, C.c_double, C.c_uint64,
+                                              C.POINTER(C.c_uint64)]
+        L.or_urban_moments_element.restype = None
+        L.or_urban_moments_element.argtypes = [C.c_double] * 5 + [C.POINTER(C.c_double)] * 2
+        L.or_urban_poisson.restype = C.c_int64
+        L.or_urban_poisson.argtypes = [C.c_double, C.c_uint64, C.POINTER(C.c_uint64)]
+        L.or_lut_index_frac.restype = None
+        L.or_lut_index_frac.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int,
+                                        C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.or_rotate_direction.restype = None
+        L.or_rotate_direction.argtypes = [_dp, C.c_double, C.c_double]
+        L.or_transport_lut.restype = None
+        L.or_transport_lut.argtypes = [C.POINTER(TransportArgs), C.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def splitmix64(x: int) -> int:
+    return int(lib().or_splitmix64(C.c_uint64(x)))
+
+
+def stream_key(seed: int, e: int) -> int:
+    return int(lib().or_stream_key(seed, e))
+
+
+def stream_uniform(key: int, counter: int) -> float:
+    return float(lib().or_stream_uniform(key, counter))
+
+
+def stream_keys(seed: int, n: int, start: int = 0) -> np.ndarray:
+    out = np.empty(n, dtype=np.uint64)
+    lib().or_stream_keys(seed, start, n, _p(out))
+    return out
+
+
+def dEds_element(J, k, coeff, E_cross, E):
+    return float(lib().or_dEds_spliced_element(J, k, coeff, E_cross, E))
+
+
+def dEds_packed(J, k, coeff, E_cross, E):
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (J, k, coeff, E_cross)]
+    return float(lib().or_dEds_spliced_packed(*[_p(x) for x in arrs], len(arrs[0]), E))
+
+
+def urban_sample_compound(Z, J, k, coeff, E_cross, E, s, key, counter=0, stopping_scale=1.0):
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (Z, J, k, coeff, E_cross)]
+    c = C.c_uint64(counter)
+    loss = lib().or_urban_sample_compound(*[_p(x) for x in arrs], len(arrs[0]), E, s,
+                                          stopping_scale, key, C.byref(c))
+    return float(loss), int(c.value)
+
+
+def urban_moments_element(Z, J, C_, E, s):
+    m, v = C.c_double(), C.c_double()
+    lib().or_urban_moments_element(Z, J, C_, E, s, C.byref(m), C.byref(v))
+    return m.value, v.value
+
+
+def urban_poisson(lam, key, counter=0):
+    c = C.c_uint64(counter)
+    n = lib().or_urban_poisson(lam, key, C.byref(c))
+    return int(n), int(c.value)
+
+
+def rotate_direction(d, cos_t, phi):
+    v = np.ascontiguousarray(d, dtype=np.float64).copy()
+    lib().or_rotate_direction(_p(v), cos_t, phi)
+    return v
+
+
+def lut_index_frac(E, log_E_min, inv_dlogE, n_energy):
+    i, f = C.c_int(), C.c_double()
+    lib().or_lut_index_frac(E, log_E_min, inv_dlogE, n_energy, C.byref(i), C.byref(f))
+    return i.value, f.value
+
+
+def run_transport(layers, lut, keys, pos, dirs, E_keV, E_cut, *, cap, e_start=0, e_count=None,
+                  alive=None, clock=None, max_steps=400_000, energy_model=1, max_dE_frac=0.0,
+                  straggle_on=False, finite=None, threads=0):
+    """Run the oracle core on host arrays; returns (rows dict, seg_count, exit_code, state).
+
+    ``layers``/``lut`` are the product's host tables (the kernel inputs);
+    ``pos``/``dirs``/``E_keV``/``clock`` are copied, never mutated.
+    """
+    Ne = len(E_keV)
+    if e_count is None:
+        e_count = Ne - e_start
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64).copy()  # noqa: E731
+    pos, dirs, E_keV = f64(pos), f64(dirs), f64(E_keV)
+    clock = np.zeros(Ne) if clock is None else f64(clock)
+    E_cut = f64(E_cut)
+    alive = np.ones(Ne, dtype=np.uint8) if alive is None else np.ascontiguousarray(alive, dtype=np.uint8)
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    stragg = np.zeros(Ne)
+    n = e_count * cap
+    rows = {
+        "seg_dir": np.zeros((n, 3)), "seg_mid": np.zeros((n, 3)), "seg_len": np.zeros(n),
+        "seg_E": np.zeros(n), "seg_t0": np.zeros(n), "seg_id": np.zeros(n, dtype=np.int64),
+        "seg_lay": np.zeros(n, dtype=np.int16), "seg_E_end": np.zeros(n), "seg_t_end": np.zeros(n),
+        "seg_flight": np.zeros(n, dtype=np.int64), "seg_substep": np.zeros(n, dtype=np.int64),
+        "seg_event": np.zeros(n, dtype=np.int8),
+    }
+    seg_count = np.zeros(e_count, dtype=np.int64)
+    exit_code = np.zeros(e_count, dtype=np.int8)
+    hold = dict(
+        internal_bounds=np.ascontiguousarray(layers.internal_bounds, dtype=np.float64),
+        L_nel=np.ascontiguousarray(layers.L_nel, dtype=np.int32),
+        L_top=np.ascontiguousarray(layers.L_top, dtype=np.float64),
+        L_bot=np.ascontiguousarray(layers.L_bot, dtype=np.float64),
+        L_Z=np.ascontiguousarray(layers.L_Z), L_J=np.ascontiguousarray(layers.L_J),
+        L_k=np.ascontiguousarray(layers.L_k), L_coeff=np.ascontiguousarray(layers.L_coeff),
+        L_E_cross=np.ascontiguousarray(layers.L_E_cross),
+        tr=np.ascontiguousarray(lut.total_rate), de=np.ascontiguousarray(lut.dEds),
+        ib=np.ascontiguousarray(lut.inv_beta), cdf=np.ascontiguousarray(lut.cdf),
+        al=np.ascontiguousarray(lut.alpha),
+    )
+    a = TransportArgs()
+    a.e_start, a.e_count, a.cap = e_start, e_count, cap
+    a.stream_key = _p(keys)
+    a.max_steps, a.energy_model, a.max_dE_frac = max_steps, energy_model, max_dE_frac
+    a.straggle_on = int(bool(straggle_on))
+    a.n_layers = layers.n_layers
+    a.internal_bounds = _p(hold["internal_bounds"])
+    a.z_total = layers.z_total
+    if finite is not None:
+        a.finite_footprint, a.width_ang, a.height_ang = 1, finite[0], finite[1]
+    a.L_nel, a.L_top, a.L_bot = _p(hold["L_nel"]), _p(hold["L_top"]), _p(hold["L_bot"])
+    a.max_el = layers.max_el
+    a.L_Z, a.L_J, a.L_k = _p(hold["L_Z"]), _p(hold["L_J"]), _p(hold["L_k"])
+    a.L_coeff, a.L_E_cross = _p(hold["L_coeff"]), _p(hold["L_E_cross"])
+    a.lut_log_E_min, a.lut_inv_dlogE, a.lut_n_energy = lut.log_E_min, lut.inv_dlogE, lut.n_energy
+    a.lut_total_rate, a.lut_dEds, a.lut_inv_beta = _p(hold["tr"]), _p(hold["de"]), _p(hold["ib"])
+    a.lut_cdf, a.lut_alpha = _p(hold["cdf"]), _p(hold["al"])
+    a.alive, a.clock, a.pos, a.dirs = _p(alive), _p(clock), _p(pos), _p(dirs)
+    a.E_keV, a.E_cut, a.stragg_dE = _p(E_keV), _p(E_cut), _p(stragg)
+    for k, v in rows.items():
+        setattr(a, k, _p(v))
+    a.seg_count, a.exit_code = _p(seg_count), _p(exit_code)
+    lib().or_transport_lut(C.byref(a), int(threads))
+    state = {"pos": pos, "dirs": dirs, "E_keV": E_keV, "clock": clock, "stragg_dE": stragg}
+    return rows, seg_count, exit_code, state
+
+
+def compact(rows, seg_count, cap):
+    """Electron-major / step-minor compaction (batching.py:573-600)."""
+    keep = (np.arange(cap)[None, :] < seg_count[:, None]).reshape(-1)
+    return {k: v[keep] for k, v in rows.items()}
+
+
+os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")

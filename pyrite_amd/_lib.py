@@ -1,0 +1,147 @@
+"""ctypes binding of libhiprite.so (the C ABI declared in include/hiprite.h).
+
+The library is loaded from ``pyrite_amd/_lib/libhiprite.so`` (built in-tree by
+``pyrite_amd.build``).  There is no fallback: when the library is missing or
+fails to load, every entry point raises :class:`HipriteUnavailable`.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libhiprite.so"
+
+HR_OK = 0
+HR_ERR_INVALID = 1
+HR_ERR_HIP = 2
+HR_ERR_OOM = 3
+HR_ERR_UNSUPPORTED = 4
+
+_dp = C.c_void_p  # device pointer
+
+
+class HipriteUnavailable(RuntimeError):
+    """libhiprite.so is not built or cannot be loaded."""
+
+
+class HipriteError(RuntimeError):
+    """A libhiprite entry point returned a non-zero hr_status."""
+
+    def __init__(self, status: int, where: str, text: str):
+        super().__init__(f"{where} failed with hr_status={status}: {text}")
+        self.status = status
+
+
+class HipriteOOM(HipriteError, MemoryError):
+    """Device allocation failed inside libhiprite."""
+
+
+class TransportArgs(C.Structure):
+    """Mirror of ``hr_transport_args`` (include/hiprite.h)."""
+
+    _fields_ = [
+        ("e_start", C.c_int64),
+        ("e_count", C.c_int64),
+        ("cap", C.c_int64),
+        ("stream_key", _dp),
+        ("max_steps", C.c_int32),
+        ("energy_model", C.c_int32),
+        ("max_dE_frac", C.c_double),
+        ("straggle_on", C.c_int32),
+        ("n_layers", C.c_int32),
+        ("internal_bounds", _dp),
+        ("z_total", C.c_double),
+        ("finite_footprint", C.c_int32),
+        ("width_ang", C.c_double),
+        ("height_ang", C.c_double),
+        ("L_nel", _dp),
+        ("L_top", _dp),
+        ("L_bot", _dp),
+        ("max_el", C.c_int32),
+        ("L_Z", _dp),
+        ("L_J", _dp),
+        ("L_k", _dp),
+        ("L_coeff", _dp),
+        ("L_E_cross", _dp),
+        ("lut_log_E_min", C.c_double),
+        ("lut_inv_dlogE", C.c_double),
+        ("lut_n_energy", C.c_int32),
+        ("lut_total_rate", _dp),
+        ("lut_dEds", _dp),
+        ("lut_inv_beta", _dp),
+        ("lut_cdf", _dp),
+        ("lut_alpha", _dp),
+        ("alive", _dp),
+        ("clock", _dp),
+        ("pos", _dp),
+        ("dirs", _dp),
+        ("E_keV", _dp),
+        ("E_cut", _dp),
+        ("stragg_dE", _dp),
+        ("seg_dir", _dp),
+        ("seg_mid", _dp),
+        ("seg_len", _dp),
+        ("seg_E", _dp),
+        ("seg_t0", _dp),
+        ("seg_id", _dp),
+        ("seg_lay", _dp),
+        ("seg_E_end", _dp),
+        ("seg_t_end", _dp),
+        ("seg_flight", _dp),
+        ("seg_substep", _dp),
+        ("seg_event", _dp),
+        ("seg_count", _dp),
+        ("exit_code", _dp),
+        ("block_size", C.c_int32),
+    ]
+
+
+# name -> (restype, argtypes); every symbol include/hiprite.h declares.
+SIGNATURES = {
+    "hr_last_error": (C.c_char_p, []),
+    "hr_version": (C.c_char_p, []),
+    "hr_device_count": (C.c_int32, []),
+    "hr_stream_keys": (C.c_int32, [C.c_uint64, C.c_int64, C.c_int64, _dp, _dp]),
+    "hr_transport_run": (C.c_int32, [C.POINTER(TransportArgs), _dp]),
+}
+
+_LIB = None
+
+
+def load(path: Path | None = None):
+    """Load libhiprite.so once; raise :class:`HipriteUnavailable` when absent."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise HipriteUnavailable(
+            f"{p} is not built; run `python -m pyrite_amd.build` (hipcc, gfx950)"
+        )
+    try:
+        lib = C.CDLL(str(p))
+    except OSError as exc:  # pragma: no cover - depends on the ROCm install
+        raise HipriteUnavailable(f"cannot load {p}: {exc}") from exc
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def check(status: int, where: str) -> None:
+    """Raise the typed error for a non-zero hr_status."""
+    if status == HR_OK:
+        return
+    text = (load().hr_last_error() or b"").decode(errors="replace")
+    if status == HR_ERR_OOM:
+        raise HipriteOOM(status, where, text)
+    raise HipriteError(status, where, text)
+
+
+def call(name: str, *args) -> None:
+    """Call a status-returning entry point and raise on failure."""
+    check(getattr(load(), name)(*args), name)

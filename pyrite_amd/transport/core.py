@@ -1,0 +1,273 @@
+"""The ``hip`` transport core and its capacity-replay batch driver.
+
+Mirrors the reference's per-electron core contract and driver:
+
+* ``core((e_start, e_count, cap, keys), control, geometry, materials, lut,
+  state, scratch, (seg_count, exit_code), (straggle_on, stragg_dE))`` --
+  ``src/pyrite/montecarlo/transport/batching.py:547-555`` and the CUDA launcher
+  signature ``_jit_launch.py:234-252``.  The core mutates ``pos``, ``dirs``,
+  ``E_keV`` and ``clock`` in place, writes row ``s`` of local electron ``i`` to
+  slot ``i*cap + s`` only while ``s < cap`` and reports the true count.
+* ``drive_batches`` -- ``batching.py:477-628``: snapshot, launch, one
+  device-to-host read of ``seg_count.max()``, replay at the measured capacity,
+  compaction into electron-major / step-minor rows.
+
+Device memory is torch.cuda tensors; the kernel is ``hr_transport_run`` of
+libhiprite.so.  There is no CPU fallback: without the library or a GPU the
+core raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from .. import _lib
+from ..backend import require_device
+from .tables import LayerTables, TransportLUT
+
+EXIT_CUTOFF_STOPPED = 0
+EXIT_BACKSCATTERED = 1
+EXIT_TRANSMITTED = 2
+EXIT_SIDE = 3
+EXIT_STEP_LIMITED = 4
+EXIT_NOT_ENTERED = 5
+
+ROW_FIELDS = (
+    "seg_dir", "seg_mid", "seg_len", "seg_E", "seg_t0", "seg_id", "seg_lay",
+    "seg_E_end", "seg_t_end", "seg_flight", "seg_substep", "seg_event",
+)
+
+
+class TransportStepLimitError(RuntimeError):
+    """A history reached ``max_steps`` (``transport/events.py:19-31``)."""
+
+
+@dataclass(frozen=True)
+class PerElectronTransportConfig:
+    """Batching policy (``batching.py:228-290``), sized for 288 GB of HBM.
+
+    No field changes results.  ``seg_capacity`` is the first guess of slots per
+    electron; ``scratch_budget_bytes`` bounds the ``(batch, cap)`` row grid;
+    ``capacity_headroom`` scales the measured maximum on replay.
+    """
+
+    seg_capacity: int = 256
+    scratch_budget_bytes: int = 16 * 1024**3
+    capacity_headroom: float = 1.25
+    max_batch: int | None = 1 << 21
+    probe_electrons: int = 0
+    kernel_threads: int = 256
+
+
+DEFAULT_CONFIG = PerElectronTransportConfig()
+
+# float64 dir(3)+mid(3)+len+E+t0, int64 id, int16 layer  (batching.py:398-400)
+_ROW_BYTES_FROZEN = 9 * 8 + 8 + 2
+# + float64 E_end, t_end; int64 flight, substep; int8 event
+_ROW_BYTES_MIDPOINT = _ROW_BYTES_FROZEN + 8 + 8 + 8 + 8 + 1
+_STATUS_BYTES = 8 + 1
+
+
+def _ptr(t) -> int:
+    return 0 if t is None else int(t.data_ptr())
+
+
+class DeviceTables:
+    """Layer, element and LUT tables resident on the device (uploaded once)."""
+
+    def __init__(self, layers: LayerTables, lut: TransportLUT, device):
+        import torch
+
+        def f64(a):
+            return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=device)
+
+        self.layers = layers
+        self.lut = lut
+        self.internal_bounds = f64(layers.internal_bounds)
+        self.L_nel = torch.as_tensor(np.ascontiguousarray(layers.L_nel, dtype=np.int32), device=device)
+        self.L_top = f64(layers.L_top)
+        self.L_bot = f64(layers.L_bot)
+        self.L_Z = f64(layers.L_Z)
+        self.L_J = f64(layers.L_J)
+        self.L_k = f64(layers.L_k)
+        self.L_coeff = f64(layers.L_coeff)
+        self.L_E_cross = f64(layers.L_E_cross)
+        self.total_rate = f64(lut.total_rate)
+        self.dEds = f64(lut.dEds)
+        self.inv_beta = f64(lut.inv_beta)
+        self.cdf = f64(lut.cdf)
+        self.alpha = f64(lut.alpha)
+
+
+@dataclass
+class TransportControl:
+    """``control`` tuple of the core (``cores.py:184``)."""
+
+    max_steps: int = 400_000
+    energy_model: int = 1  # 1 midpoint, 0 frozen
+    max_dE_frac: float = 0.0
+    straggle_on: bool = False
+    finite_footprint: bool = False
+    width_ang: float = 0.0
+    height_ang: float = 0.0
+
+
+def make_hip_transport_core(tables: DeviceTables, control: TransportControl):
+    """Return ``core(run, state, scratch, out, straggling, stream=None)``.
+
+    ``run = (e_start, e_count, cap, keys)``; ``state = (alive, clock, pos,
+    dirs, E_keV, E_cut)`` device tensors indexed by global electron id;
+    ``scratch`` a dict of row tensors sized ``e_count*cap`` (``ROW_FIELDS``);
+    ``out = (seg_count, exit_code)``; ``straggling = (straggle_on, stragg_dE)``.
+    """
+    lib = _lib.load()
+    L, lut = tables.layers, tables.lut
+
+    def core(run, state, scratch, out, straggling, stream=None, block_size=0):
+        e_start, e_count, cap, keys = run
+        alive, clock, pos, dirs, E_keV, E_cut = state
+        seg_count, exit_code = out
+        straggle_on, stragg_dE = straggling
+        a = _lib.TransportArgs()
+        a.e_start, a.e_count, a.cap = int(e_start), int(e_count), int(cap)
+        a.stream_key = _ptr(keys)
+        a.max_steps = int(control.max_steps)
+        a.energy_model = int(control.energy_model)
+        a.max_dE_frac = float(control.max_dE_frac)
+        a.straggle_on = int(bool(straggle_on))
+        a.n_layers = int(L.n_layers)
+        a.internal_bounds = _ptr(tables.internal_bounds)
+        a.z_total = float(L.z_total)
+        a.finite_footprint = int(bool(control.finite_footprint))
+        a.width_ang = float(control.width_ang)
+        a.height_ang = float(control.height_ang)
+        a.L_nel, a.L_top, a.L_bot = _ptr(tables.L_nel), _ptr(tables.L_top), _ptr(tables.L_bot)
+        a.max_el = int(L.max_el)
+        a.L_Z, a.L_J, a.L_k = _ptr(tables.L_Z), _ptr(tables.L_J), _ptr(tables.L_k)
+        a.L_coeff, a.L_E_cross = _ptr(tables.L_coeff), _ptr(tables.L_E_cross)
+        a.lut_log_E_min = float(lut.log_E_min)
+        a.lut_inv_dlogE = float(lut.inv_dlogE)
+        a.lut_n_energy = int(lut.n_energy)
+        a.lut_total_rate, a.lut_dEds = _ptr(tables.total_rate), _ptr(tables.dEds)
+        a.lut_inv_beta, a.lut_cdf, a.lut_alpha = _ptr(tables.inv_beta), _ptr(tables.cdf), _ptr(tables.alpha)
+        a.alive, a.clock, a.pos, a.dirs = _ptr(alive), _ptr(clock), _ptr(pos), _ptr(dirs)
+        a.E_keV, a.E_cut, a.stragg_dE = _ptr(E_keV), _ptr(E_cut), _ptr(stragg_dE)
+        for name in ROW_FIELDS:
+            setattr(a, name, _ptr(scratch.get(name)))
+        a.seg_count, a.exit_code = _ptr(seg_count), _ptr(exit_code)
+        a.block_size = int(block_size)
+        _lib.check(lib.hr_transport_run(C.byref(a), C.c_void_p(stream)), "hr_transport_run")
+
+    return core
+
+
+def alloc_rows(m: int, cap: int, midpoint: bool, device):
+    """Scratch row grid ``(m*cap)`` in the reference's column dtypes."""
+    import torch
+
+    n = int(m) * int(cap)
+    rows = {
+        "seg_dir": torch.empty((n, 3), dtype=torch.float64, device=device),
+        "seg_mid": torch.empty((n, 3), dtype=torch.float64, device=device),
+        "seg_len": torch.empty(n, dtype=torch.float64, device=device),
+        "seg_E": torch.empty(n, dtype=torch.float64, device=device),
+        "seg_t0": torch.empty(n, dtype=torch.float64, device=device),
+        "seg_id": torch.empty(n, dtype=torch.int64, device=device),
+        "seg_lay": torch.empty(n, dtype=torch.int16, device=device),
+    }
+    if midpoint:
+        rows.update(
+            seg_E_end=torch.empty(n, dtype=torch.float64, device=device),
+            seg_t_end=torch.empty(n, dtype=torch.float64, device=device),
+            seg_flight=torch.empty(n, dtype=torch.int64, device=device),
+            seg_substep=torch.empty(n, dtype=torch.int64, device=device),
+            seg_event=torch.empty(n, dtype=torch.int8, device=device),
+        )
+    return rows
+
+
+def _capacity_for(seen_max, config):
+    """``batching.py:462-469``: never below ``seen_max``."""
+    return max(1, seen_max, int(np.ceil(seen_max * config.capacity_headroom)))
+
+
+def _batch_electrons(e, cap, Ne, config, row_bytes):
+    """``batching.py:431-480``: electrons whose ``(batch, cap)`` grid fits the budget."""
+    per = cap * int(row_bytes) + _STATUS_BYTES
+    n = int(config.scratch_budget_bytes // per)
+    limit = (1 << 62) if config.max_batch is None else int(config.max_batch)
+    n = min(max(1, min(n, limit)), Ne - e)
+    if e == 0 and config.probe_electrons > 0:
+        n = min(n, int(config.probe_electrons))
+    return n
+
+
+@dataclass
+class DriveResult:
+    rows: dict  # compacted, electron-major / step-minor device tensors
+    n_segments: int
+    counts: dict  # exit-code tallies
+    seg_count: object  # per-electron true segment counts (device int64)
+    exit_code: object  # per-electron exit codes (device int8)
+    replays: int
+
+
+def drive_batches(core, keys, state, stragg_dE, control: TransportControl, Ne: int, device,
+                  config: PerElectronTransportConfig = DEFAULT_CONFIG, stream=None) -> DriveResult:
+    """Capacity-replayed batches (``batching.py:477-628``), rows kept on device."""
+    import torch
+
+    alive, clock, pos, dirs, E_keV, E_cut = state
+    midpoint = control.energy_model == 1
+    row_bytes = _ROW_BYTES_MIDPOINT if midpoint else _ROW_BYTES_FROZEN
+    cap = max(1, int(config.seg_capacity))
+    seen_max = 0
+    e = 0
+    batches = []
+    counts_all = []
+    exit_all = []
+    replays = 0
+    while e < Ne:
+        m = _batch_electrons(e, cap, Ne, config, row_bytes)
+        sl = slice(e, e + m)
+        snap = (pos[sl].clone(), dirs[sl].clone(), E_keV[sl].clone(), clock[sl].clone())
+        while True:
+            rows = alloc_rows(m, cap, midpoint, device)
+            seg_count = torch.zeros(m, dtype=torch.int64, device=device)
+            exit_code = torch.zeros(m, dtype=torch.int8, device=device)
+            core((e, m, cap, keys), state, rows, (seg_count, exit_code),
+                 (control.straggle_on, stragg_dE), stream=stream, block_size=config.kernel_threads)
+            needed = int(seg_count.max().item())  # the one device sync per batch
+            seen_max = max(seen_max, needed)
+            if needed <= cap:
+                break
+            replays += 1
+            cap = _capacity_for(seen_max, config)
+            pos[sl], dirs[sl], E_keV[sl], clock[sl] = snap
+            m = _batch_electrons(e, cap, Ne, config, row_bytes)
+            sl = slice(e, e + m)
+            snap = tuple(x[:m] for x in snap)
+        keep = (torch.arange(cap, device=device)[None, :] < seg_count[:, None]).reshape(-1)
+        batches.append({k: v[keep] for k, v in rows.items()})
+        counts_all.append(seg_count)
+        exit_all.append(exit_code)
+        e += m
+        if seen_max > 0:
+            cap = _capacity_for(seen_max, config)
+    joined = {k: torch.cat([b[k] for b in batches]) for k in batches[0]} if batches else alloc_rows(0, 1, midpoint, device)
+    seg_count = torch.cat(counts_all) if counts_all else torch.zeros(0, dtype=torch.int64, device=device)
+    exit_code = torch.cat(exit_all) if exit_all else torch.zeros(0, dtype=torch.int8, device=device)
+    codes = torch.bincount(exit_code.to(torch.int64), minlength=6).cpu().numpy()
+    counts = {
+        "n_cutoff_stopped": int(codes[EXIT_CUTOFF_STOPPED]),
+        "n_backscattered": int(codes[EXIT_BACKSCATTERED]),
+        "n_transmitted": int(codes[EXIT_TRANSMITTED]),
+        "n_side_exited": int(codes[EXIT_SIDE]),
+        "n_step_limited": int(codes[EXIT_STEP_LIMITED]),
+        "n_not_entered": int(codes[EXIT_NOT_ENTERED]),
+    }
+    n_seg = int(joined["seg_len"].shape[0])
+    return DriveResult(joined, n_seg, counts, seg_count, exit_code, replays)

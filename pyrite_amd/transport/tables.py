@@ -1,0 +1,258 @@
+"""Host-side transport tables: per-layer element rows and the ln-E LUT.
+
+Restates, in NumPy, the reference's host table build:
+
+* per-layer element coefficients -- ``src/pyrite/montecarlo/transport/layer_tables.py:24-154``
+  (``coeff = n/0.602214076 * Z``, ``k = 0.731 + 0.0688 log10 Z``,
+  SR rate numerator ``5.21e-21 Z^2 4 pi n_cm3``, Joy screening ``3.4e-3 Z^0.67``);
+* the Joy-Luo / Berger-Seltzer crossover bisection --
+  ``src/pyrite/montecarlo/transport/stopping.py:312-344``;
+* the uniform-in-ln(E) transport LUT -- ``src/pyrite/montecarlo/transport/lut.py:28-437``
+  (1024 intervals/decade, 256..16384 nodes, pinned endpoints).
+
+The arrays produced here are the kernel inputs.  They are float64, C-contiguous
+and laid out exactly as the device kernels index them.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from .elements import TRANSPORT_ELEMENTS
+
+MC2_KEV = 510.99895
+BS_PREFACTOR = 1.535e-6
+LN2 = 0.6931471805599453
+_LN10 = np.log(10.0)
+_MIN_DLOG_E = 8.0 * float(np.finfo(np.float32).eps)
+
+
+# ---- stopping (stopping.py:180-260, 384-411) --------------------------------
+def _dEds_joy_luo(Z, A, J, rho, E):
+    k = 0.731 + 0.0688 * np.log10(Z)
+    return -7.85e-4 * rho * Z / (A * E) * np.log(1.166 * (E + k * J) / J)
+
+
+def _dEds_berger_seltzer(Z, A, J, rho, E):
+    tau = E / MC2_KEV
+    gamma = 1.0 + tau
+    beta_sq = 1.0 - 1.0 / (gamma * gamma)
+    f_minus = 1.0 - beta_sq + (tau * tau / 8.0 - (2.0 * tau + 1.0) * LN2) / (gamma * gamma)
+    I_rel = J / MC2_KEV
+    log_term = np.log(tau * tau * (tau + 2.0) / (2.0 * I_rel * I_rel))
+    return -BS_PREFACTOR * rho * Z / (A * beta_sq) * (log_term + f_minus)
+
+
+_CROSSOVER_CACHE: dict[str, float] = {}
+
+
+def crossover_keV(element: str) -> float:
+    """Energy where Joy-Luo and Berger-Seltzer stopping agree (stopping.py:312-344).
+
+    200-step bisection on ``[2 J, 300]`` keV; density cancels in the ratio.
+    """
+    cached = _CROSSOVER_CACHE.get(element)
+    if cached is not None:
+        return cached
+    p = TRANSPORT_ELEMENTS[element]
+    Z, A, J = float(p["Z"]), float(p["A"]), float(p["J_keV"])
+
+    def diff(E):
+        return _dEds_joy_luo(Z, A, J, 1.0, E) - _dEds_berger_seltzer(Z, A, J, 1.0, E)
+
+    lo, hi = 2.0 * J, 300.0
+    if diff(lo) * diff(hi) > 0.0:
+        raise ValueError(f"no Joy-Luo/Berger-Seltzer crossover in [{lo}, {hi}] keV for {element}")
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if diff(lo) * diff(mid) <= 0.0:
+            hi = mid
+        else:
+            lo = mid
+    out = 0.5 * (lo + hi)
+    _CROSSOVER_CACHE[element] = out
+    return out
+
+
+def spliced_dEds(J, k, coeff, E_cross, E):
+    """Vectorised spliced stopping [keV/A], negative (stopping.py:347-381)."""
+    E = np.asarray(E, dtype=np.float64)
+    tau = E / MC2_KEV
+    gamma = 1.0 + tau
+    beta_sq = 1.0 - 1.0 / (gamma * gamma)
+    f_minus = 1.0 - beta_sq + (tau * tau / 8.0 - (2.0 * tau + 1.0) * LN2) / (gamma * gamma)
+    jl = np.zeros_like(E)
+    bs = np.zeros_like(E)
+    for Ji, ki, ci, Ec in zip(J, k, coeff, E_cross):
+        use_bs = E >= Ec
+        I_rel = Ji / MC2_KEV
+        jl += np.where(use_bs, 0.0, ci * np.log(1.166 * (E + ki * Ji) / Ji))
+        bs += np.where(use_bs, ci * (np.log(tau * tau * (tau + 2.0) / (2.0 * I_rel * I_rel)) + f_minus), 0.0)
+    return -7.85e-4 / E * jl - BS_PREFACTOR / beta_sq * bs
+
+
+# ---- layers (layer_tables.py:24-154) ----------------------------------------
+@dataclass
+class LayerTables:
+    """Padded ``(n_layers, max_el)`` element rows plus geometry of a layer stack."""
+
+    n_layers: int
+    max_el: int
+    L_nel: np.ndarray  # int32 [n_layers]
+    L_top: np.ndarray  # [n_layers]
+    L_bot: np.ndarray  # [n_layers]
+    internal_bounds: np.ndarray  # [max(n_layers-1, 1)]
+    z_total: float
+    L_Z: np.ndarray  # [n_layers, max_el]
+    L_J: np.ndarray
+    L_k: np.ndarray
+    L_coeff: np.ndarray
+    L_E_cross: np.ndarray
+    L_ncm3: np.ndarray
+    L_sr_rate_numer: np.ndarray
+    L_sr_joy_numer: np.ndarray
+    compositions: list
+
+
+def build_layer_tables(layers) -> LayerTables:
+    """``layers = [(z_top, z_bot, [(element, n_per_A3), ...]), ...]`` top first."""
+    n_layers = len(layers)
+    max_el = max(len(c) for _, _, c in layers)
+    shape = (n_layers, max_el)
+    out = {name: np.zeros(shape) for name in ("Z", "J", "k", "coeff", "E_cross", "ncm3", "srn", "sjn")}
+    nel = np.zeros(n_layers, dtype=np.int32)
+    for L, (_, _, comp) in enumerate(layers):
+        nel[L] = len(comp)
+        for i, (el, n_i) in enumerate(comp):
+            p = TRANSPORT_ELEMENTS[el]
+            Z = float(p["Z"])
+            out["Z"][L, i] = Z
+            out["J"][L, i] = float(p["J_keV"])
+            out["k"][L, i] = 0.731 + 0.0688 * np.log10(Z)
+            out["coeff"][L, i] = (n_i / 0.602214076) * Z
+            out["E_cross"][L, i] = crossover_keV(el)
+            ncm3 = n_i * 1e24
+            out["ncm3"][L, i] = ncm3
+            out["srn"][L, i] = 5.21e-21 * Z * Z * np.float64(4.0) * np.float64(np.pi) * ncm3
+            out["sjn"][L, i] = np.float64(3.4e-3) * Z ** np.float64(0.67)
+    tops = np.array([float(l[0]) for l in layers])
+    bots = np.array([float(l[1]) for l in layers])
+    if n_layers > 1:
+        internal = bots[:-1].copy()
+    else:
+        internal = np.zeros(1)
+    return LayerTables(
+        n_layers=n_layers,
+        max_el=max_el,
+        L_nel=nel,
+        L_top=tops,
+        L_bot=bots,
+        internal_bounds=internal,
+        z_total=float(bots[-1]),
+        L_Z=out["Z"],
+        L_J=out["J"],
+        L_k=out["k"],
+        L_coeff=out["coeff"],
+        L_E_cross=out["E_cross"],
+        L_ncm3=out["ncm3"],
+        L_sr_rate_numer=out["srn"],
+        L_sr_joy_numer=out["sjn"],
+        compositions=[list(c) for _, _, c in layers],
+    )
+
+
+# ---- LUT (lut.py:28-437) ----------------------------------------------------
+@dataclass
+class TransportLUT:
+    log_E_min: float
+    inv_dlogE: float
+    n_energy: int
+    E_grid: np.ndarray
+    total_rate: np.ndarray  # [n_layers, n_E]
+    dEds: np.ndarray  # [n_layers, n_E]
+    inv_beta: np.ndarray  # [n_E]
+    cdf: np.ndarray  # [n_layers, max_el, n_E]
+    alpha: np.ndarray  # [n_layers, max_el, n_E]
+    max_rel_interp_error: float = 0.0
+
+
+def _sr_rates(srn, sjn, E):
+    """Screened-Rutherford rate [1/cm] (scattering.py:208-218)."""
+    a = sjn / E
+    rel = (E + 511.0) / (E + 1024.0)
+    return srn / (E * E) / (a * (1.0 + a)) * (rel * rel)
+
+
+def _evaluate_nodes(tables: LayerTables, E_grid):
+    n_layers, max_el = tables.n_layers, tables.max_el
+    n = E_grid.size
+    total = np.empty((n_layers, n))
+    dEds = np.empty((n_layers, n))
+    cdf = np.ones((n_layers, max_el, n))
+    alpha = np.zeros((n_layers, max_el, n))
+    g = 1.0 + E_grid / 510.99895
+    beta = np.sqrt(1.0 - 1.0 / (g * g))
+    inv_beta = 1.0 / beta
+    for L in range(n_layers):
+        m = int(tables.L_nel[L])
+        dEds[L] = spliced_dEds(tables.L_J[L, :m], tables.L_k[L, :m], tables.L_coeff[L, :m], tables.L_E_cross[L, :m], E_grid)
+        rates = np.empty((m, n))
+        for i in range(m):
+            rates[i] = _sr_rates(tables.L_sr_rate_numer[L, i], tables.L_sr_joy_numer[L, i], E_grid)
+            alpha[L, i] = tables.L_sr_joy_numer[L, i] / E_grid
+        layer_total = rates.sum(axis=0)
+        if np.any(~np.isfinite(layer_total)) or np.any(layer_total <= 0.0):
+            raise ValueError("transport LUT produced a non-positive elastic rate")
+        total[L] = layer_total
+        c = np.cumsum(rates, axis=0) / layer_total[None, :]
+        c[-1, :] = 1.0
+        cdf[L, :m, :] = c
+    return total, dEds, inv_beta, cdf, alpha
+
+
+def build_transport_lut(E_min_keV, E_max_keV, tables: LayerTables, intervals_per_decade=1024,
+                        min_points=256, max_points=16384, audit=False) -> TransportLUT:
+    """Uniform-ln(E) LUT over ``[E_min, E_max]`` (lut.py:275-437)."""
+    E_min_keV = float(E_min_keV)
+    E_max_keV = float(E_max_keV)
+    if not (np.isfinite(E_min_keV) and np.isfinite(E_max_keV)) or E_min_keV <= 0.0:
+        raise ValueError("transport LUT energy bounds must be finite and positive")
+    if E_max_keV <= E_min_keV:
+        E_max_keV = float(np.nextafter(E_min_keV, np.inf))
+    log_E_min = float(np.log(E_min_keV))
+    log_E_max = float(np.log(E_max_keV))
+    log_span = log_E_max - log_E_min
+    decades = log_span / _LN10
+    requested = int(np.ceil(decades * float(intervals_per_decade))) + 1
+    n_energy = min(max_points, max(min_points, requested))
+    if log_span / (n_energy - 1) < _MIN_DLOG_E:
+        n_energy = max(2, int(log_span / _MIN_DLOG_E) + 1)
+    inv_dlogE = np.float64((n_energy - 1) / log_span)
+    logE = np.linspace(log_E_min, log_E_max, n_energy, dtype=np.float64)
+    E_grid = np.exp(logE)
+    E_grid[0] = E_min_keV
+    E_grid[-1] = E_max_keV
+    total, dEds, inv_beta, cdf, alpha = _evaluate_nodes(tables, E_grid)
+    err = 0.0
+    if audit:
+        for w in (0.25, 0.5, 0.75):
+            Es = np.exp((1.0 - w) * logE[:-1] + w * logE[1:])
+            st, sd, sb, _, sa = _evaluate_nodes(tables, Es)
+            for node, samp in ((total, st), (dEds, sd), (inv_beta[None, :], sb[None, :])):
+                pred = (1.0 - w) * node[..., :-1] + w * node[..., 1:]
+                e = np.abs(pred - samp) / np.abs(samp)
+                err = max(err, float(np.nanmax(e)))
+    return TransportLUT(
+        log_E_min=log_E_min,
+        inv_dlogE=float(inv_dlogE),
+        n_energy=int(n_energy),
+        E_grid=E_grid,
+        total_rate=np.ascontiguousarray(total),
+        dEds=np.ascontiguousarray(dEds),
+        inv_beta=np.ascontiguousarray(inv_beta),
+        cdf=np.ascontiguousarray(cdf),
+        alpha=np.ascontiguousarray(alpha),
+        max_rel_interp_error=err,
+    )

@@ -1,0 +1,54 @@
+"""Shared transport cases for the oracle and GPU parity tests."""
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from pyrite_amd.transport.tables import build_layer_tables, build_transport_lut
+
+
+@dataclass
+class Case:
+    name: str
+    E0: float
+    Ne: int
+    layers: list
+    seed: int
+    E_cut: float = 5.0
+    straggle: bool = False
+    energy_model: int = 1
+    max_dE_frac: float = 0.0
+    beam_dir: tuple = (0.0, 0.0, 1.0)
+    finite: tuple | None = None
+
+    def tables(self):
+        lt = build_layer_tables(self.layers)
+        lut = build_transport_lut(self.E_cut, self.E0, lt)
+        return lt, lut
+
+    def state(self):
+        bd = np.asarray(self.beam_dir, dtype=float)
+        bd = bd / np.linalg.norm(bd)
+        pos = np.zeros((self.Ne, 3))
+        dirs = np.tile(bd, (self.Ne, 1))
+        E = np.full(self.Ne, self.E0)
+        E_cut = np.full(self.Ne, self.E_cut)
+        return pos, dirs, E, E_cut
+
+
+SI_30KEV = Case("si30", 30.0, 400, [(0.0, 5.0e4, [("Si", 0.04996)])], seed=11)
+SI_30KEV_STRAGGLE = Case("si30s", 30.0, 400, [(0.0, 5.0e4, [("Si", 0.04996)])], seed=11, straggle=True)
+HOPG_5MEV = Case(
+    "hopg5", 5000.0, 300, [(0.0, 1.0e6, [("C", 0.1136)])], seed=1, straggle=True,
+    beam_dir=(-np.sin(np.deg2rad(0.5)), 0.0, np.cos(np.deg2rad(0.5))),
+)
+MULTILAYER = Case(
+    "stack", 40.0, 300,
+    [(0.0, 2.0e3, [("W", 0.0632)]), (2.0e3, 3.0e4, [("Si", 0.025), ("C", 0.05)])],
+    seed=5, straggle=True, max_dE_frac=0.05,
+)
+FRAMED = Case(
+    "prism", 60.0, 300, [(0.0, 4.0e4, [("Ge", 0.0442)])], seed=3,
+    beam_dir=(0.3, 0.1, 0.9), finite=(2.0e4, 1.5e4), energy_model=0,
+)
+CASES = [SI_30KEV, SI_30KEV_STRAGGLE, HOPG_5MEV, MULTILAYER, FRAMED]
