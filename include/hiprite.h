@@ -115,6 +115,103 @@ typedef struct hr_transport_args {
 
 hr_status hr_transport_run(const hr_transport_args *args, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Incoherent PXR/CBS line prologue over dumped segments.
+ * Replaces the batched line route's per-(segment, reflection) steps 1-7,
+ * montecarlo/spectrum/cxr/_batched.py:461-728 (_batched_block,
+ * _line_amplitudes, _escape_weights) with the fused device bodies of
+ * montecarlo/spectrum/cxr/_fused.py:22-389 and the exact line elements of
+ * montecarlo/spectrum/cxr/_recoil.py:35-76, for a laterally infinite slab.
+ * One call covers n_dir detector directions; line (d, j, g) is written to
+ * index (d*n_seg + j)*n_g + g as (E_r [eV], a [1/eV], w).  The direction's
+ * solid-angle weight and `scale` (1/Ne) are folded into w, so summing the
+ * lines' sinc^2 profiles gives the solid-angle-integrated per-electron
+ * spectrum (mc_cxr_spectrum_solid_angle, cxr/_spectrum.py:401).  Rejected
+ * pairs are written as (0, 0, 0).  Segment records and tables are FP32 (the
+ * reference's device REAL); the arithmetic is FP64.
+ * ---------------------------------------------------------------------- */
+typedef struct hr_line_args {
+    int64_t n_seg;
+    const float *r_mid; /* [n_seg,3] A */
+    const float *v_hat; /* [n_seg,3] */
+    const float *L;     /* [n_seg] A */
+    const float *E;     /* [n_seg] representative energy keV */
+    double thickness;   /* slab [A] */
+    int32_t n_dir;
+    const double *n_hat;      /* [n_dir,3] sample frame */
+    const double *dir_weight; /* [n_dir] solid angle [sr] */
+    int32_t n_g;
+    const double *g;    /* [n_g,3] 1/A */
+    const double *wm;   /* [n_g] orientation weight */
+    const int32_t *row; /* [n_g] coupling-table row */
+    const double *e_s;  /* [n_dir,n_g,3] polarization basis */
+    const double *e_p;  /* [n_dir,n_g,3] */
+    int64_t n_tab;
+    const float *E_tab;     /* [n_tab] eV ascending */
+    const float *delta_tab; /* [n_tab] 1 - Re n */
+    const float *chi_re, *chi_im, *u_re, *u_im; /* [rows, n_tab] */
+    int32_t n_mu;
+    const float *log_mu; /* [n_mu, n_tab] log(1/A) per absorber element */
+    double lo_keep, hi_keep; /* resonance keep window [eV] */
+    double scale;            /* multiplies every weight (1/Ne) */
+    double *E_r;             /* [n_dir*n_seg*n_g] */
+    double *a;
+    float *w;
+} hr_line_args;
+
+hr_status hr_cxr_lines(const hr_line_args *args, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Dense sinc^2 line reduction: out[k] += sum_j w_j [sin x/x]^2,
+ * x = a_j (E_k - E_r,j), x = 0 -> 1.
+ * Replaces montecarlo/spectrum/cxr_jit_kernel.py:561 run_reduction_kernel
+ * (and the CPU twin cxr/_cpu_reduce.py:26 _sincsq_reduce).  FP64 phase,
+ * FP32 profile, FP64 accumulation; no atomics (fixed-order partial sums).
+ * `uniform` selects the grid E_k = E0 + k*dE (E_grid may then be NULL).
+ * Uses a library-held per-device workspace: concurrent calls on different
+ * streams of one device must not overlap.
+ * ---------------------------------------------------------------------- */
+typedef struct hr_sincsq_args {
+    int64_t n_lines;
+    const double *E_r;
+    const double *a;
+    const float *w;
+    int64_t n_E;
+    const double *E_grid; /* [n_E] eV, or NULL when uniform */
+    int32_t uniform;
+    double E0, dE;
+    double *out; /* [n_E] FP64, accumulated */
+} hr_sincsq_args;
+
+hr_status hr_sincsq_reduce(const hr_sincsq_args *args, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Bethe-Heitler + Elwert bremsstrahlung track-length reduction:
+ * out[k] += sum_seg sum_el n_el L dsigma/dk(T, k) * <exp(-mu_k path)>.
+ * Replaces montecarlo/spectrum/brem_jit_kernel.py:779 run_brem_reduction_kernel
+ * (cell :76-153; incident prefactor brem.py:396-423); the caller divides by
+ * 4 pi Ne (brem.py:1169).  FP32 cells, FP64 sums, single absorber layer,
+ * 1..4 elements.  Same workspace rule as hr_sincsq_reduce.
+ * ---------------------------------------------------------------------- */
+typedef struct hr_brem_args {
+    int64_t n_seg;
+    const float *r_mid; /* [n_seg,3] A (the escape path reads z) */
+    const float *v_hat; /* [n_seg,3] */
+    const float *L;     /* [n_seg] A */
+    const float *E_keV; /* [n_seg] representative energy */
+    double thickness;   /* slab [A]; escape paths from both segment ends */
+    double n_hat[3];    /* observation direction (by value) */
+    int32_t n_el;     /* 1..4 */
+    double Z[4];      /* atomic numbers (by value) */
+    double n_cm3[4];  /* number densities [1/cm^3] (by value) */
+    int64_t n_E;
+    const double *E_grid_eV; /* [n_E] */
+    const float *mu;         /* [n_E] 1/A total attenuation */
+    double *out;             /* [n_E] FP64, accumulated */
+} hr_brem_args;
+
+hr_status hr_brem_reduce(const hr_brem_args *args, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

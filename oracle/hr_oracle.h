@@ -104,7 +104,47 @@ typedef struct or_transport_args {
  * the thread count. */
 void or_transport_lut(const or_transport_args *a, int threads);
 
-/* ---- scoring (declared in the scoring section) */
+/* ---- scoring: per (segment, reflection) line prologue for one detector
+ *      direction (spectrum/cxr/_batched.py:461-728), dense sinc^2 reduction
+ *      (cxr/_cpu_reduce.py:26-45) and BH+Elwert brem (brem.py:829-1239).
+ *      Segment records and tables are the FP32 inputs the device reads;
+ *      the oracle widens them and computes in FP64. */
+typedef struct or_line_args {
+    int64_t n_seg;
+    const float *r_mid; /* [n,3] A */
+    const float *v_hat; /* [n,3] */
+    const float *L;     /* [n] A (parent length; one escape piece) */
+    const float *E;     /* [n] representative energy, keV */
+    double thickness;   /* slab [A] */
+    double n_hat[3];    /* detector direction, sample frame */
+    int32_t n_g;
+    const double *g;    /* [n_g,3] 1/A */
+    const double *wm;   /* [n_g] orientation weight */
+    const int32_t *row; /* [n_g] coupling-table row */
+    const double *e_s;  /* [n_g,3] polarization basis */
+    const double *e_p;  /* [n_g,3] */
+    int64_t n_tab;
+    const float *E_tab;     /* [n_tab] eV ascending */
+    const float *delta_tab; /* [n_tab] 1 - Re n */
+    const float *chi_re, *chi_im, *u_re, *u_im; /* [n_rows, n_tab] */
+    int32_t n_mu;
+    const float *log_mu; /* [n_mu, n_tab] log(1/A) */
+    double lo_keep, hi_keep;
+} or_line_args;
+
+/* Line (E_r [eV], a [1/eV], w) per (segment j, reflection g) at j*n_g+g;
+ * rejected pairs get w = 0 and good = 0. */
+void or_cxr_lines(const or_line_args *p, double *E_r, double *a, double *w, uint8_t *good,
+                  int threads);
+/* out[k] += sum_j w_j [sin x/x]^2, x = a_j (E_k - E_r,j) (x = 0 -> 1e-20) */
+void or_sincsq_reduce(const double *E_r, const double *a, const double *w, int64_t n,
+                      const double *E_grid, int64_t n_E, double *out, int threads);
+/* out[k] += sum_seg sum_el n L dsigma/dk (BH+Elwert) * <exp(-mu_k path)>;
+ * caller divides by 4 pi Ne. */
+void or_brem_reduce(const float *r_mid, const float *v_hat, const float *L_ang,
+                    const float *E_keV, int64_t n_seg, double thickness, const double *n_hat,
+                    const double *Z, const double *n_cm3, int n_el, const double *E_grid_eV,
+                    const float *mu_inv_ang, int64_t n_E, double *out, int threads);
 
 #ifdef __cplusplus
 }

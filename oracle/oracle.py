@@ -50,6 +50,17 @@ class TransportArgs(C.Structure):
     ]
 
 
+class LineArgs(C.Structure):
+    _fields_ = [
+        ("n_seg", C.c_int64), ("r_mid", _dp), ("v_hat", _dp), ("L", _dp), ("E", _dp),
+        ("thickness", C.c_double), ("n_hat", C.c_double * 3),
+        ("n_g", C.c_int32), ("g", _dp), ("wm", _dp), ("row", _dp), ("e_s", _dp), ("e_p", _dp),
+        ("n_tab", C.c_int64), ("E_tab", _dp), ("delta_tab", _dp),
+        ("chi_re", _dp), ("chi_im", _dp), ("u_re", _dp), ("u_im", _dp),
+        ("n_mu", C.c_int32), ("log_mu", _dp), ("lo_keep", C.c_double), ("hi_keep", C.c_double),
+    ]
+
+
 _LIB = None
 
 
@@ -90,6 +101,13 @@ def lib():
         L.or_rotate_direction.argtypes = [_dp, C.c_double, C.c_double]
         L.or_transport_lut.restype = None
         L.or_transport_lut.argtypes = [C.POINTER(TransportArgs), C.c_int]
+        L.or_cxr_lines.restype = None
+        L.or_cxr_lines.argtypes = [C.POINTER(LineArgs), _dp, _dp, _dp, _dp, C.c_int]
+        L.or_sincsq_reduce.restype = None
+        L.or_sincsq_reduce.argtypes = [_dp, _dp, _dp, C.c_int64, _dp, C.c_int64, _dp, C.c_int]
+        L.or_brem_reduce.restype = None
+        L.or_brem_reduce.argtypes = [_dp, _dp, _dp, _dp, C.c_int64, C.c_double, _dp, _dp, _dp,
+                                     C.c_int, _dp, _dp, C.c_int64, _dp, C.c_int]
         _LIB = L
     return _LIB
 
@@ -231,3 +249,63 @@ def compact(rows, seg_count, cap):
 
 
 os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")
+
+
+# ---- scoring ---------------------------------------------------------------
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def cxr_lines(seg, thickness, n_hat, tables, e_s, e_p, threads=0):
+    """Lines (E_r, a, w, good) for one direction; ``seg`` holds FP32 r_mid, v_hat, L, E.
+
+    ``tables`` carries g [n_g,3], wm, row, E_tab, delta_tab, chi_re, chi_im,
+    u_re, u_im, log_mu (FP32 tables) and lo_keep, hi_keep.
+    """
+    hold = {
+        "r_mid": _f32(seg["r_mid"]), "v_hat": _f32(seg["v_hat"]), "L": _f32(seg["L"]),
+        "E": _f32(seg["E"]), "g": _f64(tables["g"]), "wm": _f64(tables["wm"]),
+        "row": np.ascontiguousarray(tables["row"], dtype=np.int32),
+        "e_s": _f64(e_s), "e_p": _f64(e_p),
+        "E_tab": _f32(tables["E_tab"]), "delta_tab": _f32(tables["delta_tab"]),
+        "chi_re": _f32(tables["chi_re"]), "chi_im": _f32(tables["chi_im"]),
+        "u_re": _f32(tables["u_re"]), "u_im": _f32(tables["u_im"]), "log_mu": _f32(tables["log_mu"]),
+    }
+    a = LineArgs()
+    n = len(hold["L"])
+    a.n_seg = n
+    for k in ("r_mid", "v_hat", "L", "E", "g", "wm", "row", "e_s", "e_p", "E_tab", "delta_tab",
+              "chi_re", "chi_im", "u_re", "u_im", "log_mu"):
+        setattr(a, k, _p(hold[k]))
+    a.thickness = float(thickness)
+    a.n_hat = (C.c_double * 3)(*[float(x) for x in n_hat])
+    a.n_g = len(hold["wm"])
+    a.n_tab = len(hold["E_tab"])
+    a.n_mu = hold["log_mu"].reshape(-1, a.n_tab).shape[0]
+    a.lo_keep, a.hi_keep = float(tables["lo_keep"]), float(tables["hi_keep"])
+    m = n * a.n_g
+    E_r, aw, w = np.zeros(m), np.zeros(m), np.zeros(m)
+    good = np.zeros(m, dtype=np.uint8)
+    lib().or_cxr_lines(C.byref(a), _p(E_r), _p(aw), _p(w), _p(good), int(threads))
+    return E_r, aw, w, good.astype(bool)
+
+
+def sincsq_reduce(E_r, a, w, E_grid, out=None, threads=0):
+    E_r, a, w, E_grid = _f64(E_r), _f64(a), _f64(w), _f64(E_grid)
+    out = np.zeros(len(E_grid)) if out is None else out
+    lib().or_sincsq_reduce(_p(E_r), _p(a), _p(w), len(E_r), _p(E_grid), len(E_grid), _p(out),
+                           int(threads))
+    return out
+
+
+def brem_reduce(seg, thickness, n_hat, Z, n_cm3, E_grid_eV, mu, threads=0):
+    hold = [_f32(seg["r_mid"]), _f32(seg["v_hat"]), _f32(seg["L"]), _f32(seg["E"])]
+    nh, Zs, ns, Eg, mu32 = _f64(n_hat), _f64(Z), _f64(n_cm3), _f64(E_grid_eV), _f32(mu)
+    out = np.zeros(len(Eg))
+    lib().or_brem_reduce(*[_p(x) for x in hold], len(hold[2]), float(thickness), _p(nh), _p(Zs),
+                         _p(ns), len(Zs), _p(Eg), _p(mu32), len(Eg), _p(out), int(threads))
+    return out

@@ -97,6 +97,57 @@ class TransportArgs(C.Structure):
     ]
 
 
+class LineArgs(C.Structure):
+    """Mirror of ``hr_line_args``."""
+
+    _fields_ = [
+        ("n_seg", C.c_int64),
+        ("r_mid", _dp), ("v_hat", _dp), ("L", _dp), ("E", _dp),
+        ("thickness", C.c_double),
+        ("n_dir", C.c_int32),
+        ("n_hat", _dp), ("dir_weight", _dp),
+        ("n_g", C.c_int32),
+        ("g", _dp), ("wm", _dp), ("row", _dp), ("e_s", _dp), ("e_p", _dp),
+        ("n_tab", C.c_int64),
+        ("E_tab", _dp), ("delta_tab", _dp),
+        ("chi_re", _dp), ("chi_im", _dp), ("u_re", _dp), ("u_im", _dp),
+        ("n_mu", C.c_int32),
+        ("log_mu", _dp),
+        ("lo_keep", C.c_double), ("hi_keep", C.c_double),
+        ("scale", C.c_double),
+        ("E_r", _dp), ("a", _dp), ("w", _dp),
+    ]
+
+
+class SincsqArgs(C.Structure):
+    """Mirror of ``hr_sincsq_args``."""
+
+    _fields_ = [
+        ("n_lines", C.c_int64),
+        ("E_r", _dp), ("a", _dp), ("w", _dp),
+        ("n_E", C.c_int64),
+        ("E_grid", _dp),
+        ("uniform", C.c_int32),
+        ("E0", C.c_double), ("dE", C.c_double),
+        ("out", _dp),
+    ]
+
+
+class BremArgs(C.Structure):
+    """Mirror of ``hr_brem_args``."""
+
+    _fields_ = [
+        ("n_seg", C.c_int64),
+        ("r_mid", _dp), ("v_hat", _dp), ("L", _dp), ("E_keV", _dp),
+        ("thickness", C.c_double), ("n_hat", C.c_double * 3),
+        ("n_el", C.c_int32),
+        ("Z", C.c_double * 4), ("n_cm3", C.c_double * 4),
+        ("n_E", C.c_int64),
+        ("E_grid_eV", _dp), ("mu", _dp),
+        ("out", _dp),
+    ]
+
+
 # name -> (restype, argtypes); every symbol include/hiprite.h declares.
 SIGNATURES = {
     "hr_last_error": (C.c_char_p, []),
@@ -104,6 +155,9 @@ SIGNATURES = {
     "hr_device_count": (C.c_int32, []),
     "hr_stream_keys": (C.c_int32, [C.c_uint64, C.c_int64, C.c_int64, _dp, _dp]),
     "hr_transport_run": (C.c_int32, [C.POINTER(TransportArgs), _dp]),
+    "hr_cxr_lines": (C.c_int32, [C.POINTER(LineArgs), _dp]),
+    "hr_sincsq_reduce": (C.c_int32, [C.POINTER(SincsqArgs), _dp]),
+    "hr_brem_reduce": (C.c_int32, [C.POINTER(BremArgs), _dp]),
 }
 
 _LIB = None

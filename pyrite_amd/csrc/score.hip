@@ -1,0 +1,573 @@
+// score.hip -- per-segment line prologue, sinc^2 reduction and BH+Elwert
+// bremsstrahlung on gfx950.
+//
+// Algorithms follow PyRITE's batched incoherent line route and brem reducer
+// (src/pyrite/montecarlo/spectrum/cxr/_batched.py:461-728, cxr/_fused.py,
+// cxr/_recoil.py:35-76, cxr_jit_kernel.py:32-613, brem_jit_kernel.py:76-153,
+// segment_escape.py:286-300).  Precision policy (MI355X-first):
+//  * the prologue (resonance root, couplings, amplitudes, weights) runs in FP64:
+//    it is once per line, and keeping the resonance energy E_r and the sinc slope
+//    a in FP64 keeps the phase a(E - E_r) exact to ~1e-11 turns;
+//  * the reduction cell evaluates the phase u = a(E - E_r)/pi in FP64 (one FMA
+//    on a uniform grid), reduces it to r = u - rint(u) in FP64, and takes
+//    sin(pi r)^2 / (pi u)^2 in FP32; every cell is accumulated in FP64;
+//  * the brem cell is FP32 (as the reference's), its sums FP64.
+// Reductions never use atomics: each workgroup writes a partial spectrum and a
+// second pass adds the partials in a fixed order, so results are bitwise
+// reproducible for a fixed launch shape.
+#include <math.h>
+
+#include "hr_common.hpp"
+
+namespace hr {
+namespace {
+
+constexpr double kHbarc = 1973.269804;     // materials/crystal.py:48
+constexpr double kAlpha = 1.0 / 137.035999; // crystal.py:49
+constexpr double kMe = 510998.95;          // crystal.py:50
+constexpr double kPi = 3.141592653589793;
+constexpr double kRootRtol = 1e-3;         // cxr/_kernels.py:675
+constexpr double kReCm2 = 7.9407877e-26;   // brem.py:63
+constexpr double kMc2 = 510.99895;
+
+__device__ __forceinline__ int64_t bracket(const float *__restrict__ grid, int64_t n, double x) {
+    // numpy searchsorted(side="left") + clip(., 1, n-1)   (_kernels.py:328-349)
+    int64_t y;
+    if (isnan(x)) {
+        y = n;
+    } else if (x > (double)grid[n - 1]) {
+        y = n;
+    } else {
+        int64_t lo = 0, hi = n - 1;
+        while (lo < hi) {
+            const int64_t m = lo + ((hi - lo) >> 1);
+            if ((double)grid[m] < x) lo = m + 1;
+            else hi = m;
+        }
+        y = hi;
+    }
+    return y < 1 ? 1 : (y > n - 1 ? n - 1 : y);
+}
+
+__device__ __forceinline__ double slab_path(double z, double thickness, double nz) {
+    const double inv_nz = 1.0 / fmax(fabs(nz), 1.0e-12);
+    const double depth = z < 0.0 ? 0.0 : (z > thickness ? thickness : z);
+    return nz < 0.0 ? depth * inv_nz : (thickness - depth) * inv_nz;
+}
+
+__device__ __forceinline__ double mean_transmission(double t0, double t1) {
+    const double lo = fmin(t0, t1);
+    const double d = fabs(t1 - t0);
+    const double ratio = d > 1.0e-4 ? -expm1(-d) / d : 1.0 - 0.5 * d + d * d / 6.0;
+    return exp(-lo) * ratio;
+}
+
+// One (segment, direction) pair: loops over reflections, writes n_g lines.
+__global__ void __launch_bounds__(256) cxr_lines_kernel(hr_line_args p) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = p.n_seg * (int64_t)p.n_dir;
+    if (tid >= total) return;
+    const int64_t j = tid % p.n_seg;
+    const int d = (int)(tid / p.n_seg);
+    const double nx = p.n_hat[3 * d], ny = p.n_hat[3 * d + 1], nz = p.n_hat[3 * d + 2];
+    const double dir_w = p.dir_weight[d] * p.scale;
+    const double rz = p.r_mid[3 * j + 2];
+    const double vhx = p.v_hat[3 * j], vhy = p.v_hat[3 * j + 1], vhz = p.v_hat[3 * j + 2];
+    const double L = p.L[j];
+    const double E = p.E[j];
+    // segment kinematics (_setup.py:981-985, _kernels.py:678-699)
+    const double gg0 = 1.0 + E / 510.99895;
+    const double beta = sqrt(1.0 - 1.0 / (gg0 * gg0));
+    const double vx = beta * vhx, vy = beta * vhy, vz = beta * vhz;
+    const double gamma = 1.0 + E / (kMe / 1e3);
+    const double one_minus_beta = 1.0 / (gamma * gamma * (1.0 + beta));
+    const double ddx = vhx - nx, ddy = vhy - ny, ddz = vhz - nz;
+    const double denom_row = one_minus_beta + beta * (0.5 * (ddx * ddx + ddy * ddy + ddz * ddz));
+    const double t_L = L / beta;
+    double gz = -1.0 / nz;  // slab escape gradient -e/(e.n), e = +z (segment_escape.py:78-117)
+    if (rz < 0.0 || rz > p.thickness) gz = 0.0;
+    const double v_dot_grad = vz * gz;
+    const double grad2 = gz * gz;
+    const double half = 0.5 * L;
+    const double path0 = slab_path(rz - half * vhz, p.thickness, nz);
+    const double path1 = slab_path(rz + half * vhz, p.thickness, nz);
+    const double inv_E = kHbarc / (gamma * kMe);
+    const double h = kHbarc / kMe;
+    const int64_t n_tab = p.n_tab;
+    const int64_t out0 = ((int64_t)d * p.n_seg + j) * p.n_g;
+    for (int g = 0; g < p.n_g; ++g) {
+        const double Gx = p.g[3 * g], Gy = p.g[3 * g + 1], Gz = p.g[3 * g + 2];
+        const double g2 = Gx * Gx + Gy * Gy + Gz * Gz;
+        const double n_dot_g = Gx * nx + Gy * ny + Gz * nz;
+        const double r_num = 0.5 * g2 * inv_E;  // _fused.py:335-345
+        const double r_den = n_dot_g * inv_E;
+        const double denom_vac = denom_row + r_den;
+        const double vdg = vx * Gx + vy * Gy + vz * Gz;
+        const double v_dot_g = vdg - r_num;
+        // in-medium first-order Snell root, three fixed-point passes (_kernels.py:702-766)
+        double denom = denom_vac, previous = denom, delta = 0.0;
+        for (int pass = 0; pass < 3; ++pass) {
+            const double Ex = kHbarc * (v_dot_g / denom);
+            const int64_t i = bracket(p.E_tab, n_tab, Ex);
+            const double x0 = p.E_tab[i - 1];
+            const double fr = (Ex - x0) / ((double)p.E_tab[i] - x0);
+            const double f0 = p.delta_tab[i - 1];
+            double y = f0 + fr * ((double)p.delta_tab[i] - f0);
+            if (Ex <= (double)p.E_tab[0]) y = p.delta_tab[0];
+            if (Ex >= (double)p.E_tab[n_tab - 1]) y = p.delta_tab[n_tab - 1];
+            delta = y;
+            previous = denom;
+            denom = denom_vac - delta * v_dot_grad;
+        }
+        bool settled = fabs(denom - previous) <= kRootRtol * fabs(denom);
+        settled = settled && (2.0 * fabs(delta) * grad2 < 1.0);
+        if (!settled) denom = __builtin_nan("");
+        const double n_dot_g_blk = n_dot_g + delta * (gz * Gz);
+        const double omega = (vdg - r_num) / denom;  // _kernels.py:624-652
+        const double k_dot_v = omega * ((1.0 - denom) + r_den);
+        const double k_dot_g = omega * n_dot_g_blk;
+        const double detuning = g2 + 2.0 * k_dot_g;
+        const double radicand = grad2 == 0.0 ? 1.0 : 1.0 - 2.0 * delta + delta * delta * grad2;
+        const double k_mag = omega * sqrt(radicand);
+        const double E_res = kHbarc * omega;
+        bool keep = (E_res > p.lo_keep) && (E_res > 10.0) && (E_res < p.hi_keep);
+        const double oe = omega * inv_E;
+        const double slope = denom * (1.0 / (1.0 - oe));
+        keep = keep && (denom > 0.0) && (oe <= 1.0 - 1.0 / gamma);
+        double w = 0.0, a_width = 0.0, Er = 0.0;
+        if (keep) {
+            const int64_t i = bracket(p.E_tab, n_tab, E_res);
+            const double x0 = p.E_tab[i - 1], x1 = p.E_tab[i];
+            const double fr = (E_res - x0) / (x1 - x0);
+            const bool below = E_res <= (double)p.E_tab[0];
+            const bool above = E_res >= (double)p.E_tab[n_tab - 1];
+            const double bounded = fmin(fmax(E_res, x0), x1);
+            const double log_frac = log1p((bounded - x0) / x0) / log1p((x1 - x0) / x0);
+            const int64_t base = (int64_t)p.row[g] * n_tab;
+            double tab[4];
+            const float *tables[4] = {p.chi_re, p.chi_im, p.u_re, p.u_im};
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const float *T = tables[t] + base;
+                const double t0 = T[i - 1];
+                double y = t0 + fr * ((double)T[i] - t0);
+                if (below) y = T[0];
+                if (above) y = T[n_tab - 1];
+                tab[t] = y;
+            }
+            double mu = 0.0;
+            for (int el = 0; el < p.n_mu; ++el) {
+                const float *T = p.log_mu + (int64_t)el * n_tab;
+                const double l0 = T[i - 1];
+                double lm = l0 + log_frac * ((double)T[i] - l0);
+                if (below) lm = T[0];
+                if (above) lm = T[n_tab - 1];
+                mu += exp(lm);
+            }
+            double A2 = 0.0;
+            const double gg = gamma * gamma;
+            const double k2 = k_mag * k_mag;
+            // _recoil.py:35-76, polarization-independent parts hoisted
+            const double n_1 = h * g2 - 2.0 * gamma * vdg;
+            const double n_2 = n_1 + 2.0 * h * k_dot_g;
+            const double n_f = h * (g2 + 2.0 * k_dot_g + k2) - 2.0 * gamma * (vdg + k_dot_v);
+            const double e_1 = sqrt(gg + h * n_1);
+            const double e_2 = sqrt(gg + h * (h * k2 - 2.0 * gamma * k_dot_v));
+            const double e_f = sqrt(gg + h * n_f);
+            const double s_if = gamma + e_f;
+            const double s_1 = e_1 + gamma, s_3 = e_1 + e_f, s_4 = gamma + e_2;
+            const double sigma = s_if + e_1 + e_2;
+            const int64_t pol_base = ((int64_t)d * p.n_g + g) * 3;
+#pragma unroll
+            for (int pol = 0; pol < 2; ++pol) {
+                const double *e = (pol == 0 ? p.e_s : p.e_p) + pol_base;
+                const double v_dot_e = vx * e[0] + vy * e[1] + vz * e[2];
+                const double k_dot_e = omega * delta * (gz * e[2]);
+                const double g_dot_e = Gx * e[0] + Gy * e[1] + Gz * e[2];
+                const double ge_ke = g_dot_e + k_dot_e;
+                const double pxr_num = (-n_f * ge_ke - k2 * (2.0 * gamma * v_dot_e - h * ge_ke)) / s_if;
+                const double x = 2.0 * gamma * v_dot_e - h * k_dot_e;
+                const double f_cbs = 2.0 * g_dot_e * s_1 / (s_3 * n_1) +
+                                     x * (n_1 * (n_f / s_if) * sigma - 2.0 * s_1 * s_4 * k_dot_g) /
+                                         (s_3 * s_4 * n_1 * n_2);
+                const double f_pxr = pxr_num / detuning;
+                const double re = tab[0] * f_pxr + tab[2] * f_cbs;
+                const double im = tab[1] * f_pxr + tab[3] * f_cbs;
+                A2 += re * re + im * im;
+            }
+            const double T_abs = mean_transmission(path0 * mu, path1 * mu);
+            const double pref = kAlpha * omega / (4.0 * kPi * kPi * kHbarc) * (t_L * t_L) * T_abs;
+            const double wt = pref * A2 * p.wm[g];
+            if (isfinite(wt) && wt > 0.0) {
+                w = wt * dir_w;
+                a_width = slope * t_L / (2.0 * kHbarc);
+                Er = E_res;
+            }
+        }
+        p.E_r[out0 + g] = Er;
+        p.a[out0 + g] = a_width;
+        p.w[out0 + g] = (float)w;
+    }
+}
+
+// sin(pi r) for |r| <= 1/2: odd Taylor series of sin(pi r) through r^13
+// (coefficients (-1)^k pi^(2k+1)/(2k+1)!), truncation error below 7e-10.
+__device__ __forceinline__ float sinpi_half(float r) {
+    constexpr double P1 = kPi;
+    constexpr double P3 = -kPi * kPi * kPi / 6.0;
+    constexpr double P5 = kPi * kPi * kPi * kPi * kPi / 120.0;
+    constexpr double P7 = -P5 * kPi * kPi / 42.0;
+    constexpr double P9 = -P7 * kPi * kPi / 72.0;
+    constexpr double P11 = -P9 * kPi * kPi / 110.0;
+    constexpr double P13 = -P11 * kPi * kPi / 156.0;
+    const float r2 = r * r;
+    float s = (float)P13;
+    s = fmaf(s, r2, (float)P11);
+    s = fmaf(s, r2, (float)P9);
+    s = fmaf(s, r2, (float)P7);
+    s = fmaf(s, r2, (float)P5);
+    s = fmaf(s, r2, (float)P3);
+    s = fmaf(s, r2, (float)P1);
+    return s * r;
+}
+
+constexpr int kRedThreads = 256;
+constexpr int kLineTile = 256;  // lines staged in LDS per step
+
+// Dense sum over a uniform grid E_k = E0 + k dE.  Block b takes line tiles
+// b, b + gridDim.x, ...; thread t owns bins t + kRedThreads*m (m < BINS_PER_THREAD)
+// of the bin window starting at blockIdx.y * kRedThreads * BINS.
+template <int BINS>
+__global__ void __launch_bounds__(kRedThreads) sincsq_uniform_kernel(
+    const double *__restrict__ E_r, const double *__restrict__ a, const float *__restrict__ w,
+    int64_t n_lines, double E0, double dE, int64_t n_E, double *__restrict__ partial) {
+    __shared__ double s_u0[kLineTile];
+    __shared__ double s_du[kLineTile];
+    __shared__ float s_w[kLineTile];
+    const int t = threadIdx.x;
+    const int64_t bin0 = (int64_t)blockIdx.y * kRedThreads * BINS;
+    double acc[BINS], kd[BINS];
+#pragma unroll
+    for (int m = 0; m < BINS; ++m) {
+        acc[m] = 0.0;
+        kd[m] = (double)(bin0 + t + (int64_t)m * kRedThreads);
+    }
+    const double inv_pi = 1.0 / kPi;
+    const int64_t n_tiles = (n_lines + kLineTile - 1) / kLineTile;
+    for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        const int64_t l = tile * kLineTile + t;
+        // u(E_k) = a (E_k - E_r)/pi = u0 + k du
+        if (l < n_lines) {
+            const double al = a[l] * inv_pi;
+            s_u0[t] = al * (E0 - E_r[l]);
+            s_du[t] = al * dE;
+            s_w[t] = w[l];
+        } else {
+            s_u0[t] = 0.0;
+            s_du[t] = 0.0;
+            s_w[t] = 0.0f;
+        }
+        __syncthreads();
+        for (int q = 0; q < kLineTile; ++q) {
+            const float wq = s_w[q];
+            if (wq == 0.0f) continue;  // uniform across the block
+            const double u0 = s_u0[q], du = s_du[q];
+#pragma unroll
+            for (int m = 0; m < BINS; ++m) {
+                const double u = fma(kd[m], du, u0);
+                const double rn = rint(u);
+                const float r = (float)(u - rn);
+                const float uf = (float)u;
+                const float s = sinpi_half(r);
+                const float denom = (float)(kPi * kPi) * uf * uf;
+                // u == 0 is x == 0: the reference substitutes x = 1e-20, sinc -> 1
+                const float cell = uf == 0.0f ? wq : wq * (s * s) / denom;
+                acc[m] += (double)cell;
+            }
+        }
+        __syncthreads();
+    }
+    double *out = partial + (int64_t)blockIdx.x * n_E;
+#pragma unroll
+    for (int m = 0; m < BINS; ++m) {
+        const int64_t k = bin0 + t + (int64_t)m * kRedThreads;
+        if (k < n_E) out[k] = acc[m];
+    }
+}
+
+// General grid: x = a (E_k - E_r) formed in FP64 per cell.
+template <int BINS>
+__global__ void __launch_bounds__(kRedThreads) sincsq_general_kernel(
+    const double *__restrict__ E_r, const double *__restrict__ a, const float *__restrict__ w,
+    int64_t n_lines, const double *__restrict__ E_grid, int64_t n_E, double *__restrict__ partial) {
+    __shared__ double s_er[kLineTile];
+    __shared__ double s_a[kLineTile];
+    __shared__ float s_w[kLineTile];
+    const int t = threadIdx.x;
+    const int64_t bin0 = (int64_t)blockIdx.y * kRedThreads * BINS;
+    double acc[BINS], Ek[BINS];
+#pragma unroll
+    for (int m = 0; m < BINS; ++m) {
+        acc[m] = 0.0;
+        const int64_t k = bin0 + t + (int64_t)m * kRedThreads;
+        Ek[m] = k < n_E ? E_grid[k] : 0.0;
+    }
+    const double inv_pi = 1.0 / kPi;
+    const int64_t n_tiles = (n_lines + kLineTile - 1) / kLineTile;
+    for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        const int64_t l = tile * kLineTile + t;
+        s_er[t] = l < n_lines ? E_r[l] : 0.0;
+        s_a[t] = l < n_lines ? a[l] * inv_pi : 0.0;
+        s_w[t] = l < n_lines ? w[l] : 0.0f;
+        __syncthreads();
+        for (int q = 0; q < kLineTile; ++q) {
+            const float wq = s_w[q];
+            if (wq == 0.0f) continue;
+            const double er = s_er[q], aq = s_a[q];
+#pragma unroll
+            for (int m = 0; m < BINS; ++m) {
+                const double u = aq * (Ek[m] - er);
+                const double rn = rint(u);
+                const float r = (float)(u - rn);
+                const float uf = (float)u;
+                const float s = sinpi_half(r);
+                const float denom = (float)(kPi * kPi) * uf * uf;
+                const float cell = uf == 0.0f ? wq : wq * (s * s) / denom;
+                acc[m] += (double)cell;
+            }
+        }
+        __syncthreads();
+    }
+    double *out = partial + (int64_t)blockIdx.x * n_E;
+#pragma unroll
+    for (int m = 0; m < BINS; ++m) {
+        const int64_t k = bin0 + t + (int64_t)m * kRedThreads;
+        if (k < n_E) out[k] = acc[m];
+    }
+}
+
+// out[k] += sum_b partial[b][k], b ascending (fixed order).
+__global__ void __launch_bounds__(256) sum_partials_kernel(const double *__restrict__ partial,
+                                                           int64_t n_part, int64_t n_E,
+                                                           double *__restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_E) return;
+    double s = 0.0;
+    for (int64_t b = 0; b < n_part; ++b) s += partial[b * n_E + k];
+    out[k] += s;
+}
+
+// BH+Elwert brem over segment tiles (brem.py:396-423, brem_jit_kernel.py:76-153).
+constexpr int kBremTile = 128;
+template <int BINS>
+__global__ void __launch_bounds__(kRedThreads) brem_kernel(hr_brem_args p, double *__restrict__ partial) {
+    __shared__ float s_T[kBremTile], s_pi[kBremTile], s_p0[kBremTile], s_p1[kBremTile];
+    __shared__ float s_pref[kBremTile * 4];
+    const int t = threadIdx.x;
+    const int64_t bin0 = (int64_t)blockIdx.y * kRedThreads * BINS;
+    const int n_el = p.n_el;
+    double acc[BINS];
+    float kev[BINS], keV[BINS], mu[BINS];
+#pragma unroll
+    for (int m = 0; m < BINS; ++m) {
+        acc[m] = 0.0;
+        const int64_t k = bin0 + t + (int64_t)m * kRedThreads;
+        kev[m] = k < p.n_E ? (float)p.E_grid_eV[k] : 0.0f;
+        keV[m] = kev[m] * 1.0e-3f;
+        mu[m] = k < p.n_E ? p.mu[k] : 0.0f;
+    }
+    float zi[4];
+    for (int el = 0; el < 4; ++el) zi[el] = el < n_el ? (float)(2.0 * kPi * p.Z[el] * kAlpha) : 0.0f;
+    const int64_t n_tiles = (p.n_seg + kBremTile - 1) / kBremTile;
+    for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        if (t < kBremTile) {
+            const int64_t j = tile * kBremTile + t;
+            if (j < p.n_seg) {
+                const float T = p.E_keV[j];
+                const float pi_ = sqrtf(T * (T + 2.0f * (float)kMc2)) / (float)kMc2;
+                const float bi = pi_ / (1.0f + T / (float)kMc2);
+                s_T[t] = T;
+                s_pi[t] = pi_;
+                // escape paths from both ends along n_hat (segment_escape.py:190-211)
+                const double half = 0.5 * (double)p.L[j];
+                const double rz = p.r_mid[3 * j + 2], vz = p.v_hat[3 * j + 2];
+                s_p0[t] = (float)slab_path(rz - half * vz, p.thickness, p.n_hat[2]);
+                s_p1[t] = (float)slab_path(rz + half * vz, p.thickness, p.n_hat[2]);
+                for (int el = 0; el < 4; ++el) {
+                    float pref = 0.0f;
+                    if (el < n_el) {
+                        // n L 1e-8 (16/3) alpha r_e^2 Z^2 beta_i (1 - exp(-2 pi Z alpha/beta_i)) / p_i^2
+                        const double den_i = 1.0 - exp(-(double)zi[el] / (double)bi);
+                        pref = (float)(p.n_cm3[el] * (double)p.L[j] * 1.0e-8 * (16.0 / 3.0) * kAlpha * kReCm2 *
+                                       p.Z[el] * p.Z[el] * (double)bi * den_i / ((double)pi_ * (double)pi_));
+                    }
+                    s_pref[t * 4 + el] = pref;
+                }
+            } else {
+                s_T[t] = 0.0f;
+                s_pi[t] = 0.0f;
+                s_p0[t] = 0.0f;
+                s_p1[t] = 0.0f;
+                for (int el = 0; el < 4; ++el) s_pref[t * 4 + el] = 0.0f;
+            }
+        }
+        __syncthreads();
+        const int64_t left = p.n_seg - tile * kBremTile;
+        const int cnt = left < kBremTile ? (int)left : kBremTile;
+        for (int q = 0; q < cnt; ++q) {
+            const float T = s_T[q], pi_ = s_pi[q], p0 = s_p0[q], p1 = s_p1[q];
+#pragma unroll
+            for (int m = 0; m < BINS; ++m) {
+                const float Tf = T - keV[m];
+                if (!(Tf > 1.0e-6f) || !(keV[m] > 0.0f)) continue;
+                const float pf = sqrtf(Tf * (Tf + 2.0f * (float)kMc2)) / (float)kMc2;
+                const float bf = pf / (1.0f + Tf / (float)kMc2);
+                const float dp = fmaxf(pi_ - pf, 1.0e-30f);
+                const float born = __logf((pi_ + pf) / dp);
+                float cell = 0.0f;
+                for (int el = 0; el < n_el; ++el) {
+                    const float den_f = 1.0f - __expf(-zi[el] / bf);
+                    cell += s_pref[q * 4 + el] * born / (fmaxf(kev[m], 1.0e-30f) * bf * den_f);
+                }
+                const float t0 = p0 * mu[m], t1 = p1 * mu[m];
+                const float lo = fminf(t0, t1);
+                const float dd = fabsf(t1 - t0);
+                const float ratio = dd > 1.0e-3f ? -expm1f(-dd) / dd : 1.0f - 0.5f * dd + dd * dd * (1.0f / 6.0f);
+                acc[m] += (double)(cell * (__expf(-lo) * ratio));
+            }
+        }
+        __syncthreads();
+    }
+    double *out = partial + (int64_t)blockIdx.x * p.n_E;
+#pragma unroll
+    for (int m = 0; m < BINS; ++m) {
+        const int64_t k = bin0 + t + (int64_t)m * kRedThreads;
+        if (k < p.n_E) out[k] = acc[m];
+    }
+}
+
+struct Workspace {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int cu = 0;
+};
+Workspace g_ws[64];
+
+hr_status workspace(size_t need, void *&ptr, int &cu) {
+    int dev = 0;
+    HR_CHECK_HIP(hipGetDevice(&dev));
+    HR_REQUIRE(dev >= 0 && dev < 64, "device index %d out of range", dev);
+    Workspace &w = g_ws[dev];
+    if (w.cu == 0) {
+        hipDeviceProp_t prop;
+        HR_CHECK_HIP(hipGetDeviceProperties(&prop, dev));
+        w.cu = prop.multiProcessorCount;
+    }
+    if (w.bytes < need) {
+        if (w.ptr) HR_CHECK_HIP(hipFree(w.ptr));
+        w.ptr = nullptr;
+        w.bytes = 0;
+        HR_CHECK_HIP(hipMalloc(&w.ptr, need));
+        w.bytes = need;
+    }
+    ptr = w.ptr;
+    cu = w.cu;
+    return HR_OK;
+}
+
+// Blocks along the line axis: enough to fill the chip several times over, but
+// never more than there are line tiles.
+int64_t line_blocks(int cu, int64_t n_tiles, int64_t bin_tiles) {
+    int64_t want = (int64_t)cu * 8 / (bin_tiles > 0 ? bin_tiles : 1);
+    if (want < 1) want = 1;
+    return n_tiles < want ? (n_tiles > 0 ? n_tiles : 1) : want;
+}
+
+}  // namespace
+}  // namespace hr
+
+extern "C" hr_status hr_cxr_lines(const hr_line_args *args, void *stream) {
+    using namespace hr;
+    HR_REQUIRE(args != nullptr, "hr_cxr_lines: null args");
+    const hr_line_args &p = *args;
+    HR_REQUIRE(p.n_seg >= 0 && p.n_dir >= 1 && p.n_g >= 1 && p.n_tab >= 2 && p.n_mu >= 1,
+               "hr_cxr_lines: bad sizes (n_seg=%lld n_dir=%d n_g=%d n_tab=%lld n_mu=%d)",
+               (long long)p.n_seg, p.n_dir, p.n_g, (long long)p.n_tab, p.n_mu);
+    if (p.n_seg == 0) return HR_OK;
+    HR_REQUIRE(p.r_mid && p.v_hat && p.L && p.E && p.n_hat && p.dir_weight && p.g && p.wm && p.row &&
+                   p.e_s && p.e_p && p.E_tab && p.delta_tab && p.chi_re && p.chi_im && p.u_re && p.u_im &&
+                   p.log_mu && p.E_r && p.a && p.w,
+               "hr_cxr_lines: null pointer");
+    const int64_t total = p.n_seg * (int64_t)p.n_dir;
+    const int block = 256;
+    const int64_t grid = (total + block - 1) / block;
+    HR_REQUIRE(grid < (1LL << 31), "hr_cxr_lines: %lld segment-direction pairs exceed one launch",
+               (long long)total);
+    hipLaunchKernelGGL(cxr_lines_kernel, dim3((unsigned)grid), dim3(block), 0, (hipStream_t)stream, p);
+    HR_CHECK_HIP(hipGetLastError());
+    return HR_OK;
+}
+
+extern "C" hr_status hr_sincsq_reduce(const hr_sincsq_args *args, void *stream) {
+    using namespace hr;
+    HR_REQUIRE(args != nullptr, "hr_sincsq_reduce: null args");
+    const hr_sincsq_args &p = *args;
+    HR_REQUIRE(p.n_lines >= 0 && p.n_E >= 0, "hr_sincsq_reduce: negative size");
+    if (p.n_lines == 0 || p.n_E == 0) return HR_OK;
+    HR_REQUIRE(p.E_r && p.a && p.w && p.out, "hr_sincsq_reduce: null pointer");
+    HR_REQUIRE(p.uniform || p.E_grid, "hr_sincsq_reduce: a non-uniform grid needs E_grid");
+    constexpr int BINS = 4;
+    const int64_t bin_tiles = (p.n_E + kRedThreads * BINS - 1) / (kRedThreads * BINS);
+    const int64_t n_tiles = (p.n_lines + kLineTile - 1) / kLineTile;
+    void *ws = nullptr;
+    int cu = 0;
+    hr_status st = workspace(0, ws, cu);
+    if (st != HR_OK) return st;
+    const int64_t gx = line_blocks(cu, n_tiles, bin_tiles);
+    const size_t need = (size_t)gx * (size_t)p.n_E * sizeof(double);
+    st = workspace(need, ws, cu);
+    if (st != HR_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    dim3 grid((unsigned)gx, (unsigned)bin_tiles);
+    if (p.uniform) {
+        hipLaunchKernelGGL(sincsq_uniform_kernel<BINS>, grid, dim3(kRedThreads), 0, s, p.E_r, p.a, p.w,
+                           p.n_lines, p.E0, p.dE, p.n_E, (double *)ws);
+    } else {
+        hipLaunchKernelGGL(sincsq_general_kernel<BINS>, grid, dim3(kRedThreads), 0, s, p.E_r, p.a, p.w,
+                           p.n_lines, p.E_grid, p.n_E, (double *)ws);
+    }
+    HR_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(sum_partials_kernel, dim3((unsigned)((p.n_E + 255) / 256)), dim3(256), 0, s,
+                       (const double *)ws, gx, p.n_E, p.out);
+    HR_CHECK_HIP(hipGetLastError());
+    return HR_OK;
+}
+
+extern "C" hr_status hr_brem_reduce(const hr_brem_args *args, void *stream) {
+    using namespace hr;
+    HR_REQUIRE(args != nullptr, "hr_brem_reduce: null args");
+    const hr_brem_args &p = *args;
+    HR_REQUIRE(p.n_seg >= 0 && p.n_E >= 0, "hr_brem_reduce: negative size");
+    HR_REQUIRE(p.n_el >= 1 && p.n_el <= 4, "hr_brem_reduce: 1..4 elements supported, got %d", p.n_el);
+    if (p.n_seg == 0 || p.n_E == 0) return HR_OK;
+    HR_REQUIRE(p.r_mid && p.v_hat && p.L && p.E_keV && p.E_grid_eV && p.mu && p.out,
+               "hr_brem_reduce: null pointer");
+    constexpr int BINS = 4;
+    const int64_t bin_tiles = (p.n_E + kRedThreads * BINS - 1) / (kRedThreads * BINS);
+    const int64_t n_tiles = (p.n_seg + kBremTile - 1) / kBremTile;
+    void *ws = nullptr;
+    int cu = 0;
+    hr_status st = workspace(0, ws, cu);
+    if (st != HR_OK) return st;
+    const int64_t gx = line_blocks(cu, n_tiles, bin_tiles);
+    st = workspace((size_t)gx * (size_t)p.n_E * sizeof(double), ws, cu);
+    if (st != HR_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(brem_kernel<BINS>, dim3((unsigned)gx, (unsigned)bin_tiles), dim3(kRedThreads), 0, s, p,
+                       (double *)ws);
+    HR_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(sum_partials_kernel, dim3((unsigned)((p.n_E + 255) / 256)), dim3(256), 0, s,
+                       (const double *)ws, gx, p.n_E, p.out);
+    HR_CHECK_HIP(hipGetLastError());
+    return HR_OK;
+}

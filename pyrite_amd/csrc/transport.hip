@@ -78,7 +78,10 @@ __device__ double element_dEds(double J, double k, double coeff, double E_cross,
 
 // straggling.py:229-263: inverse-CDF Poisson in bounded chunks, one draw each.
 __device__ int64_t urban_poisson(double lam, uint64_t key, uint64_t &counter) {
-    if (lam <= 0.0) return 0;
+    // Same as lam <= 0 for every finite mean; also turns a NaN or absurd mean
+    // (only reachable from corrupted state) into no draw instead of an
+    // unbounded chunk loop.
+    if (!(lam > 0.0) || !(lam <= 1.0e12)) return 0;
     const int64_t chunks = (int64_t)ceil(lam / kChunk);
     const double chunk_lam = lam / (double)chunks;
     const double p0 = exp(-chunk_lam);
@@ -542,9 +545,16 @@ extern "C" hr_status hr_transport_run(const hr_transport_args *args, void *strea
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, w->counter);
     HR_CHECK_HIP(hipGetLastError());
-    const int block = 256;
+    // Whole wave64s, at most the kernel's launch bound of 256 lanes.
+    const int block = a.block_size <= 0 ? 256 : (a.block_size >= 256 ? 256 : pick_block(a.block_size, 256));
+    int occupancy = w->occupancy;
+    if (block != 256) {
+        int blocks = 0;
+        HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, transport_kernel, block, 0));
+        occupancy = blocks > 0 ? blocks : 1;
+    }
     const int64_t need = (a.e_count + block - 1) / block;
-    const int64_t resident = (int64_t)w->cu * w->occupancy;
+    const int64_t resident = (int64_t)w->cu * occupancy;
     const int64_t grid = need < resident ? need : resident;
     hipLaunchKernelGGL(transport_kernel, dim3((unsigned)grid), dim3(block), 0, s, a, w->counter);
     HR_CHECK_HIP(hipGetLastError());
