@@ -1,0 +1,73 @@
+"""One plain-method case on the FP64 CPU oracle: transport, then line + brem scoring.
+
+TEST INFRASTRUCTURE ONLY: the checker for the end-to-end parity tests and the
+CPU baseline leg of bench.py.  It consumes the same host tables the device
+path uploads (layer/LUT tables, line tables, detector rays) and runs every
+step in the oracle's FP64 C code with OpenMP over histories / bins.
+"""
+
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import oracle
+
+
+def run_case(pipe_like, start, n, threads=0, timings=None):
+    """Histories ``[start, start+n)`` of the case described by ``pipe_like``.
+
+    ``pipe_like`` carries ``cfg``, ``layers``, ``lut``, ``beam_dir``,
+    ``n_hats``, ``weights``, ``n_hat``, ``E_grid``, ``line_tables`` (a
+    ``CasePipeline`` or ``CaseTables``).  Returns ``(lines, brem_sum, n_rows)``:
+    the solid-angle line spectrum summed over these histories (not divided by
+    Ne) and the raw brem sum ``sum n L dsigma/dk T`` at the central direction
+    (divide by ``4 pi Ne`` and multiply by Omega for the cone).
+    """
+    cfg = pipe_like.cfg
+    t0 = time.perf_counter()
+    keys = oracle.stream_keys(cfg.seed, n, start=start)
+    pos = np.zeros((n, 3))
+    dirs = np.tile(pipe_like.beam_dir, (n, 1))
+    E = np.full(n, cfg.E0_keV)
+    E_cut = np.full(n, cfg.E_cut_keV)
+    cap = 512
+    while True:
+        rows, cnt, code, st = oracle.run_transport(
+            pipe_like.layers, pipe_like.lut, keys, pos, dirs, E, E_cut, cap=cap,
+            max_steps=cfg.max_steps, energy_model=1, straggle_on=cfg.straggling, threads=threads)
+        if cnt.max() <= cap:
+            break
+        cap = int(cnt.max() * 1.25) + 1
+    r = oracle.compact(rows, cnt, cap)
+    t1 = time.perf_counter()
+    seg = {
+        "r_mid": r["seg_mid"].astype(np.float32), "v_hat": r["seg_dir"].astype(np.float32),
+        "L": r["seg_len"].astype(np.float32),
+        "E": (0.5 * (r["seg_E"] + r["seg_E_end"])).astype(np.float32),
+    }
+    from pyrite_amd.spectrum.crystal import polarization_pairs
+
+    tab = pipe_like.line_tables.as_dict()
+    lines = np.zeros(pipe_like.E_grid.size)
+    for d, n_hat in enumerate(pipe_like.n_hats):
+        es, ep = polarization_pairs(n_hat, tab["g"])
+        E_r, a, w, good = oracle.cxr_lines(seg, cfg.thickness_ang, n_hat, tab, es, ep, threads=threads)
+        w = np.where(good, w * pipe_like.weights[d], 0.0)
+        oracle.sincsq_reduce(E_r, a, w, pipe_like.E_grid, out=lines, threads=threads)
+    t2 = time.perf_counter()
+    from pyrite_amd.spectrum import attenuation
+    from pyrite_amd.transport.elements import TRANSPORT_ELEMENTS
+
+    comp = list(cfg.composition)
+    mu = attenuation.mu_total_inv_ang(comp, pipe_like.E_grid).astype(np.float32)
+    brem = oracle.brem_reduce(seg, cfg.thickness_ang, pipe_like.n_hat,
+                              [float(TRANSPORT_ELEMENTS[e]["Z"]) for e, _ in comp],
+                              [float(x) * 1e24 for _, x in comp], pipe_like.E_grid, mu, threads=threads)
+    t3 = time.perf_counter()
+    if timings is not None:
+        timings["transport"] = timings.get("transport", 0.0) + (t1 - t0)
+        timings["lines"] = timings.get("lines", 0.0) + (t2 - t1)
+        timings["brem"] = timings.get("brem", 0.0) + (t3 - t2)
+    return lines, brem, len(seg["L"])

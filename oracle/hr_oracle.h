@@ -132,6 +132,10 @@ typedef struct or_line_args {
     double lo_keep, hi_keep;
 } or_line_args;
 
+/* Exact second-order (pxr_num, f_cbs) of one polarization (cxr/_recoil.py:35-76). */
+void or_quantum_line_factors(double h, double gamma, double vdg, double k_dot_v, double k_dot_g,
+                             double g2, double k2, double v_dot_e, double g_dot_e, double k_dot_e,
+                             double *out2);
 /* Line (E_r [eV], a [1/eV], w) per (segment j, reflection g) at j*n_g+g;
  * rejected pairs get w = 0 and good = 0. */
 void or_cxr_lines(const or_line_args *p, double *E_r, double *a, double *w, uint8_t *good,

@@ -112,6 +112,13 @@ static void quantum_line_factors(double h, double gamma, double vdg, double k_do
              x * (n_1 * (n_f / s_if) * sigma - 2.0 * s_1 * s_4 * k_dot_g) / (s_3 * s_4 * n_1 * n_2);
 }
 
+void or_quantum_line_factors(double h, double gamma, double vdg, double k_dot_v, double k_dot_g,
+                             double g2, double k2, double v_dot_e, double g_dot_e, double k_dot_e,
+                             double *out2) {
+    quantum_line_factors(h, gamma, vdg, k_dot_v, k_dot_g, g2, k2, v_dot_e, g_dot_e, k_dot_e, &out2[0],
+                         &out2[1]);
+}
+
 void or_cxr_lines(const or_line_args *p, double *E_r_out, double *a_out, double *w_out,
                   uint8_t *good_out, int threads) {
     const double nx = p->n_hat[0], ny = p->n_hat[1], nz = p->n_hat[2];

@@ -101,6 +101,8 @@ def lib():
         L.or_rotate_direction.argtypes = [_dp, C.c_double, C.c_double]
         L.or_transport_lut.restype = None
         L.or_transport_lut.argtypes = [C.POINTER(TransportArgs), C.c_int]
+        L.or_quantum_line_factors.restype = None
+        L.or_quantum_line_factors.argtypes = [C.c_double] * 10 + [_dp]
         L.or_cxr_lines.restype = None
         L.or_cxr_lines.argtypes = [C.POINTER(LineArgs), _dp, _dp, _dp, _dp, C.c_int]
         L.or_sincsq_reduce.restype = None
@@ -309,3 +311,10 @@ def brem_reduce(seg, thickness, n_hat, Z, n_cm3, E_grid_eV, mu, threads=0):
     lib().or_brem_reduce(*[_p(x) for x in hold], len(hold[2]), float(thickness), _p(nh), _p(Zs),
                          _p(ns), len(Zs), _p(Eg), _p(mu32), len(Eg), _p(out), int(threads))
     return out
+
+
+def quantum_line_factors(h, gamma, vdg, k_dot_v, k_dot_g, g2, k2, v_dot_e, g_dot_e, k_dot_e):
+    out = np.zeros(2)
+    lib().or_quantum_line_factors(h, gamma, vdg, k_dot_v, k_dot_g, g2, k2, v_dot_e, g_dot_e, k_dot_e,
+                                  _p(out))
+    return float(out[0]), float(out[1])

@@ -1,0 +1,145 @@
+"""Transport -> scoring pipeline for one beam/target case, device resident.
+
+The reference runs one case as ``_transport_case`` then ``_spectrum_case``
+(``montecarlo/runner/case_driver.py:142-152``) with the segment arrays kept on
+the device between them (``keep_segments_on_device``).  ``CasePipeline`` is
+that contract on the hip backend: tables are built and uploaded once, every
+batch of histories is transported by ``hr_transport_run`` and scored by
+``hr_cxr_lines`` + ``hr_sincsq_reduce`` and ``hr_brem_reduce`` into FP64 device
+tallies, and histories are addressed by global electron id, so batches
+``[s*Ne, (s+1)*Ne)`` are slices of one large run.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .backend import require_device
+from .spectrum.geometry import cone_detector, tilted_geometry
+from .spectrum.lines import BremScorer, LineScorer, line_tables, segments_fp32
+from .transport.core import (
+    DeviceTables,
+    PerElectronTransportConfig,
+    TransportControl,
+    drive_batches,
+    make_hip_transport_core,
+)
+from .transport.tables import build_layer_tables, build_transport_lut
+
+
+def bin_centers(lo_eV, hi_eV, n):
+    edges = np.linspace(lo_eV, hi_eV, n + 1)
+    return 0.5 * (edges[:-1] + edges[1:])
+
+
+@dataclass
+class CaseConfig:
+    """One plain-method case (defaults: 5 MeV pencil beam on 100 um HOPG, (002))."""
+
+    E0_keV: float = 5000.0
+    Ne: int = 100_000
+    thickness_ang: float = 1.0e6
+    composition: list = field(default_factory=lambda: [("C", 0.1136)])
+    crystal: str = "hopg"
+    hkl_list: list = field(default_factory=lambda: [(0, 0, 2)])
+    tilt_deg: float = 0.5
+    theta_obs_deg: float = 90.0
+    cone_half_angle_rad: float = 0.01
+    n_side: int = 3
+    n_bins: int = 1024
+    E_lo_eV: float = 1000.0
+    E_hi_eV: float = 50000.0
+    E_cut_keV: float = 5.0
+    seed: int = 1
+    straggling: bool = True
+    max_steps: int = 400_000
+
+    def grid(self):
+        return bin_centers(self.E_lo_eV, self.E_hi_eV, self.n_bins)
+
+
+class CaseTables:
+    """Host-side inputs of a case: layer/LUT tables, geometry, detector rays, line tables."""
+
+    def __init__(self, cfg: CaseConfig):
+        self.cfg = cfg
+        self.layers = build_layer_tables([(0.0, cfg.thickness_ang, list(cfg.composition))])
+        self.lut = build_transport_lut(cfg.E_cut_keV, cfg.E0_keV, self.layers)
+        th, tilt = np.deg2rad(cfg.theta_obs_deg), np.deg2rad(cfg.tilt_deg)
+        self.beam_dir, self.n_hat = tilted_geometry(th, tilt)
+        self.n_hats, self.weights = cone_detector(th, tilt, cfg.cone_half_angle_rad, cfg.n_side)
+        self.omega = float(self.weights.sum())
+        self.E_grid = cfg.grid()
+        self.line_tables = line_tables(cfg.crystal, cfg.hkl_list, self.E_grid, cfg.composition)
+
+
+class CasePipeline(CaseTables):
+    def __init__(self, cfg: CaseConfig, transport_config: PerElectronTransportConfig | None = None):
+        import torch
+
+        super().__init__(cfg)
+        self.dev = require_device().torch_device
+        dev = self.dev
+        self.tcfg = transport_config or PerElectronTransportConfig()
+        self.tables = DeviceTables(self.layers, self.lut, dev)
+        self.control = TransportControl(max_steps=cfg.max_steps, energy_model=1, straggle_on=cfg.straggling)
+        self.core = make_hip_transport_core(self.tables, self.control)
+        self.lines = LineScorer(self.line_tables, self.E_grid, cfg.thickness_ang, self.n_hats, self.weights, dev)
+        self.brem = BremScorer(cfg.composition, self.E_grid, cfg.thickness_ang, self.n_hat, dev)
+        n = cfg.Ne
+        self.keys = torch.empty(n, dtype=torch.uint64, device=dev)
+        self.pos = torch.empty((n, 3), dtype=torch.float64, device=dev)
+        self.dirs = torch.empty((n, 3), dtype=torch.float64, device=dev)
+        self.E = torch.empty(n, dtype=torch.float64, device=dev)
+        self.clock = torch.empty(n, dtype=torch.float64, device=dev)
+        self.E_cut = torch.full((n,), cfg.E_cut_keV, dtype=torch.float64, device=dev)
+        self.alive = torch.ones(n, dtype=torch.uint8, device=dev)
+        self.stragg = torch.zeros(n, dtype=torch.float64, device=dev)
+        self.beam_d = torch.as_tensor(self.beam_dir, device=dev)
+        self.spec_lines = torch.zeros(self.E_grid.size, dtype=torch.float64, device=dev)
+        self.spec_brem = torch.zeros(self.E_grid.size, dtype=torch.float64, device=dev)
+        self.histories = 0
+        self.segments = 0
+        self.timers = {}
+
+    def transport(self, batch_index: int):
+        """Transport histories ``[b*Ne, (b+1)*Ne)``; returns the device segment rows."""
+        cfg = self.cfg
+        n = cfg.Ne
+        _lib.call("hr_stream_keys", int(cfg.seed), int(batch_index) * n, n, self.keys.data_ptr(), None)
+        self.pos.zero_()
+        self.dirs.copy_(self.beam_d.expand(n, 3))
+        self.E.fill_(cfg.E0_keV)
+        self.clock.zero_()
+        res = drive_batches(self.core, self.keys,
+                            (self.alive, self.clock, self.pos, self.dirs, self.E, self.E_cut),
+                            self.stragg, self.control, n, self.dev, self.tcfg)
+        return res
+
+    def score(self, rows, Ne):
+        seg = segments_fp32({
+            "r_mid": rows["seg_mid"], "v_hat": rows["seg_dir"], "L_ang": rows["seg_len"],
+            "E_keV": rows["seg_E"], "E_repr_keV": 0.5 * (rows["seg_E"] + rows["seg_E_end"]),
+        }, self.dev)
+        self.lines.spectrum(seg, Ne, self.spec_lines)
+        self.brem.spectrum(seg, Ne, self.spec_brem)
+        return seg
+
+    def step(self, batch_index: int, n_batches_total: int = 1):
+        """One batch of ``Ne`` histories through transport and scoring.
+
+        Tallies are per incident electron of the whole run when
+        ``n_batches_total`` batches are summed.
+        """
+        res = self.transport(batch_index)
+        self.score(res.rows, self.cfg.Ne * n_batches_total)
+        self.histories += self.cfg.Ne
+        self.segments += res.n_segments
+        return res
+
+    def spectra(self):
+        """``(lines [photons/eV/electron], brem [photons/eV/electron] over the cone)``."""
+        return self.spec_lines.cpu().numpy(), self.spec_brem.cpu().numpy() * self.omega

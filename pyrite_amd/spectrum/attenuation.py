@@ -23,7 +23,9 @@ from ..transport.elements import TRANSPORT_ELEMENTS
 
 DATA = Path(__file__).resolve().parent.parent / "data" / "xcom"
 N_A = 6.02214076e23
-COL_PHOTO, COL_TOTAL = 3, 6  # columns after E_MeV: coh, incoh, photo, pn, pe, tot_coh, tot_no
+# numeric columns of a row: E_MeV, coherent, incoherent, photoelectric, pair_nuclear,
+# pair_electron, total_with_coh, total_without_coh
+COL_PHOTO, COL_TOTAL = 3, 6
 
 
 @lru_cache(maxsize=None)
@@ -45,11 +47,12 @@ def _table(element: str):
         rows.append(vals)
     a = np.asarray(rows)
     E = a[:, 0] * 1.0e6
-    return E, a[:, 1 + COL_TOTAL] * scale, a[:, 1 + COL_PHOTO] * scale
+    return E, a[:, COL_TOTAL] * scale, a[:, COL_PHOTO] * scale
 
 
 def _loglog(E_nodes, v_nodes, E):
-    E = np.asarray(E, dtype=float)
+    # clamped to the tabulated range (1 keV - 100 GeV): no power-law extrapolation
+    E = np.clip(np.asarray(E, dtype=float), E_nodes[0], E_nodes[-1])
     i = np.searchsorted(E_nodes, E, side="right") - 1
     i = np.clip(i, 0, E_nodes.size - 2)
     x0, x1 = E_nodes[i], E_nodes[i + 1]

@@ -1,0 +1,140 @@
+"""HIP scoring kernels against the FP64 oracle on identical segments (GPU).
+
+Bars: line parameters to ~1e-12 (both FP64; FP32 weights to 1e-6), the dense
+sinc^2 spectrum and the brem spectrum to <= 1e-6 relative per bin (FP32 cells,
+FP64 phases and sums), and the worked three-energy block exactly.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _segments(rng, n, thickness=1.0e6, E_range=(1000.0, 20000.0), sigma=1e-3, tilt=np.deg2rad(0.5)):
+    """Synthetic FP32 segments: positions uniform in the slab, Gaussian directions, log-uniform E."""
+    r = np.stack([rng.uniform(-1e5, 1e5, n), rng.uniform(-1e5, 1e5, n), rng.uniform(0, thickness, n)], 1)
+    d = np.stack([rng.normal(0, sigma, n) - np.sin(tilt), rng.normal(0, sigma, n), np.ones(n)], 1)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    E = np.exp(rng.uniform(np.log(E_range[0]), np.log(E_range[1]), n))
+    L = rng.exponential(1.0e4, n)
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)  # noqa: E731
+    return dict(r_mid=f32(r), v_hat=f32(d), L=f32(L), E=f32(E))
+
+
+def _dev(seg, device):
+    import torch
+
+    return {k: torch.as_tensor(v, device=device) for k, v in seg.items()}
+
+
+def _hopg_setup(E_grid, n_side=3):
+    from pyrite_amd.spectrum.geometry import cone_detector
+    from pyrite_amd.spectrum.lines import line_tables
+
+    tables = line_tables("hopg", [(0, 0, 2)], E_grid)
+    n_hats, weights = cone_detector(np.pi / 2, np.deg2rad(0.5), 0.01, n_side)
+    return tables, n_hats, weights
+
+
+@pytest.mark.parametrize("kind", ["hopg", "synthetic"])
+def test_lines_match_oracle(oracle, device, kind):
+    from pyrite_amd.spectrum.crystal import polarization_pairs
+    from pyrite_amd.spectrum.lines import LineScorer, synthetic_line_tables
+
+    rng = np.random.default_rng(7)
+    E_grid = np.linspace(1000.0, 50000.0, 1024)
+    if kind == "hopg":
+        tables, n_hats, weights = _hopg_setup(E_grid)
+        seg = _segments(rng, 3000, E_range=(4900.0, 5000.0))
+    else:
+        tables = synthetic_line_tables(8, E_grid, rng)
+        n_hats = np.array([[np.sin(1.2), 0.0, np.cos(1.2)], [0.0, np.sin(0.9), -np.cos(0.9)]])
+        weights = np.array([1e-4, 2e-4])
+        seg = _segments(rng, 3000, thickness=5.0e6)
+    thick = 1.0e6 if kind == "hopg" else 5.0e6
+    sc = LineScorer(tables, E_grid, thick, n_hats, weights, device)
+    E_r, a, w = (x.cpu().numpy() for x in sc.lines(_dev(seg, device), 1.0 / 1000.0))
+    n_g, n = len(tables.wm), len(seg["L"])
+    for d in range(len(weights)):
+        es, ep = polarization_pairs(n_hats[d], tables.g)
+        oE, oa, ow, og = oracle.cxr_lines(seg, thick, n_hats[d], tables.as_dict(), es, ep)
+        sl = slice(d * n * n_g, (d + 1) * n * n_g)
+        gE, ga, gw = E_r[sl], a[sl], w[sl]
+        want_w = ow * weights[d] / 1000.0
+        kept = og & (ow > 0)
+        # the FP32 weight may underflow for lines attenuated by exp(-100) and more
+        flushed = kept & (gw == 0)
+        assert np.all(want_w[flushed] < 1e-37)
+        kept &= gw > 0
+        assert np.all(~(gw > 0) | (og & (ow > 0)))
+        np.testing.assert_allclose(gE[kept], oE[kept], rtol=1e-12)
+        np.testing.assert_allclose(ga[kept], oa[kept], rtol=1e-12)
+        np.testing.assert_allclose(gw[kept], want_w[kept], rtol=1e-6, atol=1e-37)
+
+
+@pytest.mark.parametrize("uniform", [True, False])
+def test_sincsq_matches_oracle(oracle, device, uniform):
+    import torch
+
+    from pyrite_amd.spectrum.lines import LineScorer
+
+    rng = np.random.default_rng(11)
+    E_grid = np.linspace(1000.0, 50000.0, 1024)
+    if not uniform:
+        E_grid = np.sort(rng.uniform(1000.0, 50000.0, 777))
+    tables, n_hats, weights = _hopg_setup(np.linspace(1000.0, 50000.0, 1024))
+    seg = _segments(rng, 20000, E_range=(4900.0, 5000.0))
+    sc = LineScorer(tables, E_grid, 1.0e6, n_hats, weights, device)
+    E_r, a, w = sc.lines(_dev(seg, device), 1.0 / 20000.0)
+    out = torch.zeros(E_grid.size, dtype=torch.float64, device=device)
+    sc.reduce(E_r, a, w, out)
+    got = out.cpu().numpy()
+    want = oracle.sincsq_reduce(E_r.cpu().numpy(), a.cpu().numpy(), w.cpu().numpy().astype(np.float64), E_grid)
+    assert np.all(np.isfinite(got)) and want.max() > 0
+    rel = np.abs(got - want) / np.abs(want)
+    assert rel.max() <= 1e-6, (rel.max(), int(rel.argmax()))
+
+
+def test_sincsq_worked_block_on_device(device):
+    import ctypes as C
+
+    import torch
+
+    from pyrite_amd import _lib
+
+    n = 33
+    E_r = torch.zeros(n, dtype=torch.float64, device=device)
+    a = torch.ones(n, dtype=torch.float64, device=device)
+    w = torch.zeros(n, dtype=torch.float32, device=device)
+    for j, (er, aw, ww) in {0: (0, math.pi / 2, 1), 1: (1, math.pi / 2, 2), 2: (2, math.pi / 2, 3), 32: (0, math.pi, 4)}.items():
+        E_r[j], a[j], w[j] = er, aw, ww
+    out = torch.zeros(3, dtype=torch.float64, device=device)
+    args = _lib.SincsqArgs()
+    args.n_lines, args.E_r, args.a, args.w = n, E_r.data_ptr(), a.data_ptr(), w.data_ptr()
+    args.n_E, args.E_grid, args.uniform, args.E0, args.dE = 3, None, 1, 0.0, 1.0
+    args.out = out.data_ptr()
+    _lib.check(_lib.load().hr_sincsq_reduce(C.byref(args), None), "hr_sincsq_reduce")
+    pi2 = math.pi**2
+    np.testing.assert_allclose(out.cpu().numpy(), [5 + 8 / pi2, 2 + 16 / pi2, 3 + 8 / pi2], rtol=1e-7)
+
+
+def test_brem_matches_oracle(oracle, device):
+    from pyrite_amd.spectrum import attenuation
+    from pyrite_amd.spectrum.lines import BremScorer
+
+    rng = np.random.default_rng(5)
+    E_grid = np.linspace(1000.0, 50000.0, 1024)
+    seg = _segments(rng, 20000, E_range=(4900.0, 5000.0))
+    comp = [("C", 0.1136)]
+    n_hat = np.array([np.cos(np.deg2rad(0.5)), 0.0, np.sin(np.deg2rad(0.5))])
+    sc = BremScorer(comp, E_grid, 1.0e6, n_hat, device)
+    got = sc.spectrum(_dev(seg, device), 1.0).cpu().numpy() * 4.0 * np.pi
+    mu = attenuation.mu_total_inv_ang(comp, E_grid).astype(np.float32)
+    want = oracle.brem_reduce(seg, 1.0e6, n_hat, [6.0], [0.1136e24], E_grid, mu)
+    # FP32 cells: exp(-tau) underflows near tau ~ 88 where FP64 keeps 1e-38 of a
+    # contribution, so bins are compared relative to themselves plus 1e-12 of the peak
+    err = np.abs(got - want)
+    assert np.all(err <= 1e-5 * np.abs(want) + 1e-12 * want.max()), (err / np.maximum(want, 1e-300)).max()
