@@ -154,6 +154,7 @@ typedef struct hr_line_args {
     const float *log_mu; /* [n_mu, n_tab] log(1/A) per absorber element */
     double lo_keep, hi_keep; /* resonance keep window [eV] */
     double scale;            /* multiplies every weight (1/Ne) */
+    const float *row_weight; /* [n_seg] statistical weight of each row, or NULL (= 1) */
     double *E_r;             /* [n_dir*n_seg*n_g] */
     double *a;
     float *w;
@@ -207,10 +208,116 @@ typedef struct hr_brem_args {
     int64_t n_E;
     const double *E_grid_eV; /* [n_E] */
     const float *mu;         /* [n_E] 1/A total attenuation */
+    const float *row_weight; /* [n_seg] statistical weight of each row, or NULL (= 1) */
     double *out;             /* [n_E] FP64, accumulated */
 } hr_brem_args;
 
 hr_status hr_brem_reduce(const hr_brem_args *args, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Importance network (learned variance reduction).
+ * No counterpart in the reference: PyRITE has no learned component (its
+ * closest variance reduction is detector-cone splitting, src/pyrite/
+ * _cone_splitting.py:57, transport/cone_split.py:223).  The network predicts
+ * the (log) detected-photon yield to go from an electron state
+ * x = (E/E0, ux, uy, uz, depth/thickness, tilt): 6 -> 64 -> 64 -> 1 with SiLU.
+ * Parameters: FP32 master vector of hr_mlp_param_count() = 4673 floats in the
+ * order W1[64][6], b1[64], W2[64][64], b2[64], w3[64], b3, plus BF16 shadows
+ * of W1 (K padded to 8) and W2 read by the MFMA operands
+ * (v_mfma_f32_32x32x8bf16_1k, FP32 accumulation).  x, h1 and dz2 are rounded
+ * to BF16 before each GEMM.  All pointers are device pointers.
+ * ---------------------------------------------------------------------- */
+typedef struct hr_mlp {
+    float *params; /* [4673] FP32 master */
+    uint16_t *w1b; /* [64*8]  BF16 shadow of W1 */
+    uint16_t *w2b; /* [64*64] BF16 shadow of W2 */
+} hr_mlp;
+
+int32_t hr_mlp_param_count(void);
+/* Bytes of the gradient workspace hr_mlp_grad needs for a batch of n. */
+size_t hr_mlp_workspace_bytes(int64_t n);
+/* Refresh the BF16 shadows from the master vector. */
+hr_status hr_mlp_shadow(const float *params_d, uint16_t *w1b_d, uint16_t *w2b_d, void *stream);
+/* y[s] = net(x[s]) for s < n; x is [n,6] FP32. */
+hr_status hr_mlp_forward(const hr_mlp *net, const float *x_d, int64_t n, float *y_d, void *stream);
+/* Gradient of L = mean_s (net(x_s) - t_s)^2 w.r.t. the master vector (BF16
+ * roundings straight-through); fixed-order reduction over per-wave partials. */
+hr_status hr_mlp_grad(const hr_mlp *net, const float *x_d, const float *t_d, int64_t n,
+                      float *workspace_d, float *grad_d, float *loss_d, void *stream);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) on the master vector; refreshes shadows. */
+hr_status hr_mlp_adam(hr_mlp *net, const float *grad_d, float *m1_d, float *m2_d, float lr,
+                      int32_t step, void *stream);
+
+/* ---- learned variance reduction: round-based transport and weight window.
+ * No counterpart in the reference (its variance reduction is likelihood-ratio
+ * cone splitting, src/pyrite/_cone_splitting.py:57 and
+ * transport/cone_split.py:223, plus line roulette, src/pyrite/_line_roulette.py:29);
+ * these entry points let an importance network split and roulette particles
+ * between transport rounds with weight-preserving decisions. */
+
+/* Advances particles [e_start, e_start+e_count) of a bank by at most `rows`
+ * analog steps (1 <= rows <= cap), each from a fresh state: optical depth,
+ * flight and substep restart and draws come from stream_key[e] (counter 0).
+ * Rows go to slot (e-e_start)*cap + s as in hr_transport_run; exit_code is
+ * 6 (suspended) for particles still alive after `rows` steps.  Bank state is
+ * updated in place.  alive[] is not read. */
+hr_status hr_vr_round(const hr_transport_args *args, int32_t rows, void *stream);
+
+/* x[i] = (E/E0, u_x, u_y, u_z, z/thickness, tilt_deg/3) as FP32 [n,6].  With
+ * seg_len non-NULL, pos holds row midpoints and z is taken at the row start. */
+hr_status hr_vr_features(int64_t n, const double *pos, const double *dirs, const double *E,
+                         const double *seg_len, double E0_keV, double thickness_ang,
+                         double tilt_deg, float *x, void *stream);
+
+typedef struct hr_vr_window_args {
+    int64_t n;
+    const int8_t *exit_code;      /* [n] only 6 (suspended) particles continue */
+    const float *log_importance;  /* [n] network output y */
+    const double *weight;         /* [n] */
+    const uint64_t *key;          /* [n] decision draw: counter 2^40 of the key */
+    double log_I_ref;             /* target weight w_t = clamp(exp(log_I_ref - y), w_min, w_max) */
+    double w_min, w_max;
+    int32_t n_max;                /* copies per particle, 1..1024 */
+    int32_t *count;               /* [n] out: copies (0 = ended or rouletted) */
+    double *w_child;              /* [n] out: weight of each copy */
+} hr_vr_window_args;
+/* r = w/w_t; r >= 1: n = floor(r) + [u < frac(r)], capped at n_max, copies of
+ * weight w/n; r < 1: one copy of weight w_t with probability r. */
+hr_status hr_vr_window(const hr_vr_window_args *args, void *stream);
+
+typedef struct hr_vr_spawn_args {
+    int64_t n;
+    const int32_t *count;    /* [n] */
+    const int64_t *offset;   /* [n] exclusive prefix sum of count */
+    const double *w_child;   /* [n] */
+    const double *pos, *dirs, *E, *clock;
+    const uint64_t *key;
+    const int64_t *prim;     /* [n] primary (incident electron) id */
+    double *out_pos, *out_dirs, *out_E, *out_clock, *out_weight;
+    uint64_t *out_key;       /* copy j of key k: splitmix64(splitmix64(k ^ S) + PHI*(j+1)) */
+    int64_t *out_prim;
+} hr_vr_spawn_args;
+hr_status hr_vr_spawn(const hr_vr_spawn_args *args, void *stream);
+
+typedef struct hr_vr_gather_args {
+    int64_t n, rows;               /* particles; slots per particle (the round's cap) */
+    const int64_t *seg_count;      /* [n] */
+    const int64_t *row_offset;     /* [n] exclusive prefix sum of min(seg_count, rows) */
+    const double *weight;          /* [n] */
+    const int64_t *prim;           /* [n] or NULL */
+    const double *seg_mid, *seg_dir, *seg_len, *seg_E, *seg_E_end; /* slot-major rows */
+    float *r_mid, *v_hat, *L, *E;  /* out: FP32 scoring records, E = (E + E_end)/2 */
+    float *row_weight;             /* out */
+    int64_t *row_prim;             /* out or NULL */
+} hr_vr_gather_args;
+hr_status hr_vr_gather(const hr_vr_gather_args *args, void *stream);
+
+/* mass[j] = sum over the segment's n_dir*n_g lines with lo <= E_r < hi of
+ * w*pi/(a*dE): the offset-averaged content of a narrow sinc^2 line on a
+ * uniform grid of spacing dE.  Lines at (d*n_seg + j)*n_g + g. */
+hr_status hr_line_mass(const double *E_r, const double *a, const float *w, int64_t n_seg,
+                       int32_t n_dir, int32_t n_g, double lo_eV, double hi_eV, double dE_eV,
+                       double *mass, void *stream);
 
 #ifdef __cplusplus
 }

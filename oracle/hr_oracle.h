@@ -143,6 +143,19 @@ void or_cxr_lines(const or_line_args *p, double *E_r, double *a, double *w, uint
 /* out[k] += sum_j w_j [sin x/x]^2, x = a_j (E_k - E_r,j) (x = 0 -> 1e-20) */
 void or_sincsq_reduce(const double *E_r, const double *a, const double *w, int64_t n,
                       const double *E_grid, int64_t n_E, double *out, int threads);
+/* ---- importance MLP (no reference counterpart; definition in or_mlp.c) */
+double or_bf16(double v);
+void or_mlp_set_rounding(int on); /* 1: device bfloat16 roundings (default), 0: exact */
+void or_mlp_forward(const float *params, const float *x, int64_t n, double *y);
+double or_mlp_loss_grad(const float *params, const float *x, const float *t, int64_t n,
+                        double *grad);
+/* ---- learned variance reduction (no reference counterpart; or_vr.c) */
+uint64_t or_vr_child_key(uint64_t key, int64_t j);
+void or_vr_window(int64_t n, const int8_t *exit_code, const float *log_importance, const double *weight,
+                  const uint64_t *key, double log_I_ref, double w_min, double w_max, int32_t n_max,
+                  int32_t *count, double *w_child);
+void or_line_mass(const double *E_r, const double *a, const float *w, int64_t n_seg, int32_t n_dir,
+                  int32_t n_g, double lo, double hi, double dE, double *mass);
 /* out[k] += sum_seg sum_el n L dsigma/dk (BH+Elwert) * <exp(-mu_k path)>;
  * caller divides by 4 pi Ne. */
 void or_brem_reduce(const float *r_mid, const float *v_hat, const float *L_ang,

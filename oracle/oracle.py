@@ -107,6 +107,20 @@ def lib():
         L.or_cxr_lines.argtypes = [C.POINTER(LineArgs), _dp, _dp, _dp, _dp, C.c_int]
         L.or_sincsq_reduce.restype = None
         L.or_sincsq_reduce.argtypes = [_dp, _dp, _dp, C.c_int64, _dp, C.c_int64, _dp, C.c_int]
+        L.or_bf16.restype = C.c_double
+        L.or_bf16.argtypes = [C.c_double]
+        L.or_mlp_forward.restype = None
+        L.or_mlp_forward.argtypes = [_dp, _dp, C.c_int64, _dp]
+        L.or_mlp_loss_grad.restype = C.c_double
+        L.or_mlp_loss_grad.argtypes = [_dp, _dp, _dp, C.c_int64, _dp]
+        L.or_vr_child_key.restype = C.c_uint64
+        L.or_vr_child_key.argtypes = [C.c_uint64, C.c_int64]
+        L.or_vr_window.restype = None
+        L.or_vr_window.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double,
+                                   C.c_int32, _dp, _dp]
+        L.or_line_mass.restype = None
+        L.or_line_mass.argtypes = [_dp, _dp, _dp, C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                   C.c_double, _dp]
         L.or_brem_reduce.restype = None
         L.or_brem_reduce.argtypes = [_dp, _dp, _dp, _dp, C.c_int64, C.c_double, _dp, _dp, _dp,
                                      C.c_int, _dp, _dp, C.c_int64, _dp, C.c_int]
@@ -313,8 +327,51 @@ def brem_reduce(seg, thickness, n_hat, Z, n_cm3, E_grid_eV, mu, threads=0):
     return out
 
 
+def bf16(v):
+    return float(lib().or_bf16(float(v)))
+
+
+def mlp_forward(params, x):
+    p, xx = _f32(params), _f32(x).reshape(-1, 6)
+    y = np.zeros(xx.shape[0])
+    lib().or_mlp_forward(_p(p), _p(xx), xx.shape[0], _p(y))
+    return y
+
+
+def mlp_loss_grad(params, x, t):
+    p, xx, tt = _f32(params), _f32(x).reshape(-1, 6), _f32(t)
+    g = np.zeros(p.size)
+    loss = lib().or_mlp_loss_grad(_p(p), _p(xx), _p(tt), xx.shape[0], _p(g))
+    return float(loss), g
+
+
 def quantum_line_factors(h, gamma, vdg, k_dot_v, k_dot_g, g2, k2, v_dot_e, g_dot_e, k_dot_e):
     out = np.zeros(2)
     lib().or_quantum_line_factors(h, gamma, vdg, k_dot_v, k_dot_g, g2, k2, v_dot_e, g_dot_e, k_dot_e,
                                   _p(out))
     return float(out[0]), float(out[1])
+
+
+def vr_child_key(key: int, j: int) -> int:
+    return int(lib().or_vr_child_key(C.c_uint64(key), int(j)))
+
+
+def vr_window(exit_code, log_importance, weight, key, log_I_ref, w_min, w_max, n_max):
+    ec = np.ascontiguousarray(exit_code, dtype=np.int8)
+    y = _f32(log_importance)
+    w = _f64(weight)
+    k = np.ascontiguousarray(key, dtype=np.uint64)
+    n = ec.size
+    count = np.zeros(n, dtype=np.int32)
+    wc = np.zeros(n)
+    lib().or_vr_window(n, _p(ec), _p(y), _p(w), _p(k), float(log_I_ref), float(w_min), float(w_max),
+                       int(n_max), _p(count), _p(wc))
+    return count, wc
+
+
+def line_mass(E_r, a, w, n_seg, n_dir, n_g, lo, hi, dE):
+    Er, aa, ww = _f64(E_r), _f64(a), _f32(w)
+    out = np.zeros(int(n_seg))
+    lib().or_line_mass(_p(Er), _p(aa), _p(ww), int(n_seg), int(n_dir), int(n_g), float(lo), float(hi),
+                       float(dE), _p(out))
+    return out

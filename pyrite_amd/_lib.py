@@ -115,6 +115,7 @@ class LineArgs(C.Structure):
         ("log_mu", _dp),
         ("lo_keep", C.c_double), ("hi_keep", C.c_double),
         ("scale", C.c_double),
+        ("row_weight", _dp),
         ("E_r", _dp), ("a", _dp), ("w", _dp),
     ]
 
@@ -144,8 +145,52 @@ class BremArgs(C.Structure):
         ("Z", C.c_double * 4), ("n_cm3", C.c_double * 4),
         ("n_E", C.c_int64),
         ("E_grid_eV", _dp), ("mu", _dp),
+        ("row_weight", _dp),
         ("out", _dp),
     ]
+
+
+class VrWindowArgs(C.Structure):
+    """Mirror of ``hr_vr_window_args``."""
+
+    _fields_ = [
+        ("n", C.c_int64),
+        ("exit_code", _dp), ("log_importance", _dp), ("weight", _dp), ("key", _dp),
+        ("log_I_ref", C.c_double), ("w_min", C.c_double), ("w_max", C.c_double),
+        ("n_max", C.c_int32),
+        ("count", _dp), ("w_child", _dp),
+    ]
+
+
+class VrSpawnArgs(C.Structure):
+    """Mirror of ``hr_vr_spawn_args``."""
+
+    _fields_ = [
+        ("n", C.c_int64),
+        ("count", _dp), ("offset", _dp), ("w_child", _dp),
+        ("pos", _dp), ("dirs", _dp), ("E", _dp), ("clock", _dp),
+        ("key", _dp), ("prim", _dp),
+        ("out_pos", _dp), ("out_dirs", _dp), ("out_E", _dp), ("out_clock", _dp), ("out_weight", _dp),
+        ("out_key", _dp), ("out_prim", _dp),
+    ]
+
+
+class VrGatherArgs(C.Structure):
+    """Mirror of ``hr_vr_gather_args``."""
+
+    _fields_ = [
+        ("n", C.c_int64), ("rows", C.c_int64),
+        ("seg_count", _dp), ("row_offset", _dp), ("weight", _dp), ("prim", _dp),
+        ("seg_mid", _dp), ("seg_dir", _dp), ("seg_len", _dp), ("seg_E", _dp), ("seg_E_end", _dp),
+        ("r_mid", _dp), ("v_hat", _dp), ("L", _dp), ("E", _dp),
+        ("row_weight", _dp), ("row_prim", _dp),
+    ]
+
+
+class MlpNet(C.Structure):
+    """Mirror of ``hr_mlp``."""
+
+    _fields_ = [("params", _dp), ("w1b", _dp), ("w2b", _dp)]
 
 
 # name -> (restype, argtypes); every symbol include/hiprite.h declares.
@@ -158,6 +203,19 @@ SIGNATURES = {
     "hr_cxr_lines": (C.c_int32, [C.POINTER(LineArgs), _dp]),
     "hr_sincsq_reduce": (C.c_int32, [C.POINTER(SincsqArgs), _dp]),
     "hr_brem_reduce": (C.c_int32, [C.POINTER(BremArgs), _dp]),
+    "hr_mlp_param_count": (C.c_int32, []),
+    "hr_mlp_workspace_bytes": (C.c_size_t, [C.c_int64]),
+    "hr_mlp_shadow": (C.c_int32, [_dp, _dp, _dp, _dp]),
+    "hr_mlp_forward": (C.c_int32, [C.POINTER(MlpNet), _dp, C.c_int64, _dp, _dp]),
+    "hr_mlp_grad": (C.c_int32, [C.POINTER(MlpNet), _dp, _dp, C.c_int64, _dp, _dp, _dp, _dp]),
+    "hr_mlp_adam": (C.c_int32, [C.POINTER(MlpNet), _dp, _dp, _dp, C.c_float, C.c_int32, _dp]),
+    "hr_vr_round": (C.c_int32, [C.POINTER(TransportArgs), C.c_int32, _dp]),
+    "hr_vr_features": (C.c_int32, [C.c_int64, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp, _dp]),
+    "hr_vr_window": (C.c_int32, [C.POINTER(VrWindowArgs), _dp]),
+    "hr_vr_spawn": (C.c_int32, [C.POINTER(VrSpawnArgs), _dp]),
+    "hr_vr_gather": (C.c_int32, [C.POINTER(VrGatherArgs), _dp]),
+    "hr_line_mass": (C.c_int32, [_dp, _dp, _dp, C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                 C.c_double, _dp, _dp]),
 }
 
 _LIB = None

@@ -70,7 +70,7 @@ __global__ void __launch_bounds__(256) cxr_lines_kernel(hr_line_args p) {
     const int64_t j = tid % p.n_seg;
     const int d = (int)(tid / p.n_seg);
     const double nx = p.n_hat[3 * d], ny = p.n_hat[3 * d + 1], nz = p.n_hat[3 * d + 2];
-    const double dir_w = p.dir_weight[d] * p.scale;
+    const double dir_w = p.dir_weight[d] * p.scale * (p.row_weight ? (double)p.row_weight[j] : 1.0);
     const double rz = p.r_mid[3 * j + 2];
     const double vhx = p.v_hat[3 * j], vhy = p.v_hat[3 * j + 1], vhz = p.v_hat[3 * j + 2];
     const double L = p.L[j];
@@ -398,7 +398,8 @@ __global__ void __launch_bounds__(kRedThreads) brem_kernel(hr_brem_args p, doubl
                     if (el < n_el) {
                         // n L 1e-8 (16/3) alpha r_e^2 Z^2 beta_i (1 - exp(-2 pi Z alpha/beta_i)) / p_i^2
                         const double den_i = 1.0 - exp(-(double)zi[el] / (double)bi);
-                        pref = (float)(p.n_cm3[el] * (double)p.L[j] * 1.0e-8 * (16.0 / 3.0) * kAlpha * kReCm2 *
+                        const double rw = p.row_weight ? (double)p.row_weight[j] : 1.0;
+                        pref = (float)(rw * p.n_cm3[el] * (double)p.L[j] * 1.0e-8 * (16.0 / 3.0) * kAlpha * kReCm2 *
                                        p.Z[el] * p.Z[el] * (double)bi * den_i / ((double)pi_ * (double)pi_));
                     }
                     s_pref[t * 4 + el] = pref;
@@ -422,7 +423,9 @@ __global__ void __launch_bounds__(kRedThreads) brem_kernel(hr_brem_args p, doubl
                 if (!(Tf > 1.0e-6f) || !(keV[m] > 0.0f)) continue;
                 const float pf = sqrtf(Tf * (Tf + 2.0f * (float)kMc2)) / (float)kMc2;
                 const float bf = pf / (1.0f + Tf / (float)kMc2);
-                const float dp = fmaxf(pi_ - pf, 1.0e-30f);
+                // p_i - p_f = k (T_i + T_f + 2 m) / (m^2 (p_i + p_f)): no cancellation when k << T_i
+                const float dp = fmaxf(keV[m] * (T + Tf + 2.0f * (float)kMc2) /
+                                           ((float)(kMc2 * kMc2) * (pi_ + pf)), 1.0e-30f);
                 const float born = __logf((pi_ + pf) / dp);
                 float cell = 0.0f;
                 for (int el = 0; el < n_el; ++el) {
@@ -433,7 +436,7 @@ __global__ void __launch_bounds__(kRedThreads) brem_kernel(hr_brem_args p, doubl
                 const float lo = fminf(t0, t1);
                 const float dd = fabsf(t1 - t0);
                 const float ratio = dd > 1.0e-3f ? -expm1f(-dd) / dd : 1.0f - 0.5f * dd + dd * dd * (1.0f / 6.0f);
-                acc[m] += (double)(cell * (__expf(-lo) * ratio));
+                acc[m] += (double)(cell * (expf(-lo) * ratio));
             }
         }
         __syncthreads();

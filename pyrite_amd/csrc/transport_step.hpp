@@ -1,0 +1,434 @@
+// transport_step.hpp -- one step of PyRITE's per-electron LUT core on gfx950.
+//
+// Physics and draw order follow src/pyrite/montecarlo/transport/cores.py:365-1250
+// and its CUDA twin _jit_lut_kernel.py:64-574: screened-Rutherford elastic
+// flights from a uniform-ln(E) LUT, spliced Joy-Luo/Berger-Seltzer stopping
+// with the midpoint rule, Urban straggling on a per-(flight, substep) key
+// domain, slab or prism geometry, and the cap/slot row contract.  Shared by the
+// analog history kernel (transport.hip) and the round-based kernel of the
+// learned variance reduction (vr.hip).
+#pragma once
+#include <math.h>
+
+#include "hr_common.hpp"
+
+namespace hr {
+namespace {
+
+constexpr double kMc2 = 510.99895;
+constexpr double kBs = 1.535e-6;
+constexpr double kLn2 = 0.6931471805599453;
+constexpr double kE0 = 1.0e-2;      // Urban E_0 [keV]          straggling.py:117
+constexpr double kE2PerZ2 = 1.0e-2; // Urban E_2 / Z^2 [keV]    straggling.py:118
+constexpr double kRate = 0.55;      // Urban r                  straggling.py:119
+constexpr double kChunk = 64.0;     // Poisson chunk mean       straggling.py:125
+constexpr double kPi = 3.141592653589793;
+
+enum : int8_t { EV_ELASTIC = 0, EV_SUBSTEP = 1, EV_LAYER = 2, EV_EXIT_TOP = 3, EV_EXIT_BOTTOM = 4,
+                EV_EXIT_SIDE = 5, EV_CUTOFF = 7 };
+enum : int8_t { EXIT_CUTOFF = 0, EXIT_BACK = 1, EXIT_TRANS = 2, EXIT_SIDE = 3, EXIT_STEP_LIMITED = 4,
+                EXIT_NOT_ENTERED = 5, EXIT_SUSPENDED = 6 };
+
+struct LutPos {
+    int i;
+    double f;
+};
+
+// lut.py:94-110: upper clamp tested first so a non-finite coordinate lands low.
+__device__ __forceinline__ LutPos lut_pos(double E, double log_E_min, double inv_dlogE, int n_energy) {
+    const double x = (log(E) - log_E_min) * inv_dlogE;
+    const int last = n_energy - 1;
+    LutPos p;
+    if (x >= (double)last) {
+        p.i = last - 1;
+        p.f = 1.0;
+    } else if (x > 0.0) {
+        p.i = (int)x;
+        p.f = x - (double)p.i;
+    } else {
+        p.i = 0;
+        p.f = 0.0;
+    }
+    return p;
+}
+
+__device__ __forceinline__ double lerp(const double *__restrict__ t, LutPos p) {
+    const double lo = t[p.i];
+    return lo + p.f * (t[p.i + 1] - lo);
+}
+
+// straggling.py:305-329, one element of the spliced stopping, delta = 0.
+__device__ double element_dEds(double J, double k, double coeff, double E_cross, double E) {
+    const double tau = E / kMc2;
+    const double gamma = 1.0 + tau;
+    const double beta_sq = 1.0 - 1.0 / (gamma * gamma);
+    if (E < E_cross) return -7.85e-4 / E * coeff * log(1.166 * (E + k * J) / J);
+    const double f_minus = 1.0 - beta_sq + (tau * tau / 8.0 - (2.0 * tau + 1.0) * kLn2) / (gamma * gamma);
+    const double I_rel = J / kMc2;
+    return -kBs / beta_sq * coeff * (log(tau * tau * (tau + 2.0) / (2.0 * I_rel * I_rel)) + f_minus - 0.0);
+}
+
+// straggling.py:229-263: inverse-CDF Poisson in bounded chunks, one draw each.
+__device__ int64_t urban_poisson(double lam, uint64_t key, uint64_t &counter) {
+    // Same as lam <= 0 for every finite mean; also turns a NaN or absurd mean
+    // (only reachable from corrupted state) into no draw instead of an
+    // unbounded chunk loop.
+    if (!(lam > 0.0) || !(lam <= 1.0e12)) return 0;
+    const int64_t chunks = (int64_t)ceil(lam / kChunk);
+    const double chunk_lam = lam / (double)chunks;
+    const double p0 = exp(-chunk_lam);
+    int64_t total = 0;
+    for (int64_t c = 0; c < chunks; ++c) {
+        const double u = stream_uniform(key, counter);
+        counter += 1ULL;
+        double p = p0, cdf = p0;
+        int64_t k = 0;
+        while (u >= cdf) {
+            k += 1;
+            p = p * chunk_lam / (double)k;
+            const double next = cdf + p;
+            if (next <= cdf) break;
+            cdf = next;
+        }
+        total += k;
+    }
+    return total;
+}
+
+// straggling.py:131-201 and 278-363: Bragg-additive Urban loss of one layer row.
+__device__ double urban_compound(const double *__restrict__ Zr, const double *__restrict__ Jr,
+                                 const double *__restrict__ kr, const double *__restrict__ cr,
+                                 const double *__restrict__ xr, int n_el, double E, double s,
+                                 uint64_t key) {
+    uint64_t counter = 0;
+    double loss = 0.0;
+    const double tau = E / kMc2;
+    const double gamma = 1.0 + tau;
+    const double beta_sq = 1.0 - 1.0 / (gamma * gamma);
+    const double two = 2.0 * kMc2 * tau * (tau + 2.0);
+    const double T_up = 0.5 * E;
+    for (int el = 0; el < n_el; ++el) {
+        const double Z = Zr[el], J = Jr[el];
+        const double C = -element_dEds(J, kr[el], cr[el], xr[el], E) * 1.0;
+        bool valid = T_up > kE0 && C > 0.0;
+        double L_I = 0.0;
+        if (valid) {
+            L_I = log(two / J) - beta_sq;
+            valid = L_I > 0.0;
+        }
+        double dE = 0.0;
+        if (!valid || !(s > 0.0)) {
+            dE = C * s;
+        } else {
+            const double e2 = kE2PerZ2 * Z * Z;
+            double f2 = Z > 2.0 ? 2.0 / Z : 1.0;
+            double f1 = 1.0, e1 = J;
+            bool resolved = false;
+            if (Z > 2.0 && e2 < T_up && log(two / e2) - beta_sq > 0.0) {
+                const double f1c = 1.0 - f2;
+                const double e1c = exp((log(J) - f2 * log(e2)) / f1c);
+                if (log(two / e1c) - beta_sq > 0.0) {
+                    f1 = f1c;
+                    e1 = e1c;
+                    resolved = true;
+                }
+            }
+            if (!resolved) {
+                f1 = 1.0;
+                e1 = J;
+                f2 = 0.0;
+            }
+            const double soft = C * (1.0 - kRate) / L_I;
+            const double s1 = soft * (f1 / e1) * (log(two / e1) - beta_sq);
+            const double s2 = f2 > 0.0 ? soft * (f2 / e2) * (log(two / e2) - beta_sq) : 0.0;
+            const double s3 = C * kRate * (T_up - kE0) / (kE0 * T_up * log(T_up / kE0));
+            const int64_t n1 = urban_poisson(s1 * s, key, counter);
+            const int64_t n2 = urban_poisson(s2 * s, key, counter);
+            const int64_t n3 = urban_poisson(s3 * s, key, counter);
+            dE = (double)n1 * e1 + (double)n2 * e2;
+            for (int64_t q = 0; q < n3; ++q) {
+                const double u = stream_uniform(key, counter);
+                counter += 1ULL;
+                dE += kE0 / (1.0 - u * (T_up - kE0) / T_up);
+            }
+        }
+        loss += dE;
+    }
+    return loss;
+}
+
+struct Lane {
+    double px, py, pz, dx, dy, dz, E, clock, tau, E_cut, stragg;
+    uint64_t key, draws;
+    int64_t e, nseg, flight, substep;
+    int32_t steps;
+};
+
+__device__ __forceinline__ void load_lane(const hr_transport_args &a, int64_t e, Lane &s) {
+    s.e = e;
+    s.px = a.pos[3 * e];
+    s.py = a.pos[3 * e + 1];
+    s.pz = a.pos[3 * e + 2];
+    s.dx = a.dirs[3 * e];
+    s.dy = a.dirs[3 * e + 1];
+    s.dz = a.dirs[3 * e + 2];
+    s.E = a.E_keV[e];
+    s.clock = a.clock[e];
+    s.E_cut = a.E_cut[e];
+    s.key = a.stream_key[e];
+    s.tau = -1.0;
+    s.stragg = 0.0;
+    s.draws = 0;
+    s.nseg = 0;
+    s.flight = 0;
+    s.substep = 0;
+    s.steps = 0;
+}
+
+__device__ __forceinline__ void store_lane(const hr_transport_args &a, const Lane &s, int8_t code) {
+    const int64_t e = s.e, i = e - a.e_start;
+    a.pos[3 * e] = s.px;
+    a.pos[3 * e + 1] = s.py;
+    a.pos[3 * e + 2] = s.pz;
+    a.dirs[3 * e] = s.dx;
+    a.dirs[3 * e + 1] = s.dy;
+    a.dirs[3 * e + 2] = s.dz;
+    a.E_keV[e] = s.E;
+    a.clock[e] = s.clock;
+    if (a.straggle_on) a.stragg_dE[e] = s.stragg;
+    a.seg_count[i] = s.nseg;
+    a.exit_code[i] = code;
+}
+
+// One loop iteration of cores.py:408-1239 (per_electron, lut).  Returns the
+// exit code when the history ends, or -1 when it continues.
+__device__ int step_lane(const hr_transport_args &a, Lane &s) {
+    const int nE = a.lut_n_energy;
+    const int max_el = a.max_el;
+    int L = 0;
+    if (a.n_layers != 1) {
+        int lo = 0, hi = a.n_layers - 1;  // searchsorted(side="right")
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (a.internal_bounds[mid] <= s.pz) lo = mid + 1;
+            else hi = mid;
+        }
+        L = lo;
+    }
+    const int n_el = a.L_nel[L];
+    const double z_top = a.L_top[L], z_bot = a.L_bot[L];
+    const double E = s.E;
+    const LutPos p = lut_pos(E, a.lut_log_E_min, a.lut_inv_dlogE, nE);
+    const double *rate_row = a.lut_total_rate + (size_t)L * nE;
+    const double *dEds_row = a.lut_dEds + (size_t)L * nE;
+    const double total_rate = lerp(rate_row, p);
+    const double lam = 1e8 / total_rate;
+    if (s.tau < 0.0) {
+        s.tau = -log(stream_uniform(s.key, s.draws));
+        s.draws += 1ULL;
+    }
+    double step = s.tau * lam;
+    const double dx = s.dx, dy = s.dy, dz = s.dz, px = s.px, py = s.py, pz = s.pz;
+    bool up = false, dn = false, side = false;
+    if (a.finite_footprint) {
+        double best = INFINITY;
+        int face = -1;
+        const double hw = 0.5 * a.width_ang, hh = 0.5 * a.height_ang;
+        double t;
+        if (dx < 0.0) { t = (-hw - px) / dx; if (t > 0.0 && t < best) { best = t; face = 0; } }
+        if (dx > 0.0) { t = (hw - px) / dx; if (t > 0.0 && t < best) { best = t; face = 1; } }
+        if (dy < 0.0) { t = (-hh - py) / dy; if (t > 0.0 && t < best) { best = t; face = 2; } }
+        if (dy > 0.0) { t = (hh - py) / dy; if (t > 0.0 && t < best) { best = t; face = 3; } }
+        if (dz < 0.0) { t = (z_top - pz) / dz; if (t > 0.0 && t < best) { best = t; face = 4; } }
+        if (dz > 0.0) { t = (z_bot - pz) / dz; if (t > 0.0 && t < best) { best = t; face = 5; } }
+        if (step > best) {
+            step = best;
+            up = face == 4;
+            dn = face == 5;
+            side = face >= 0 && face <= 3;
+        }
+    } else if (dz < 0.0) {
+        const double sb = (pz - z_top) / (-dz);
+        if (step > sb) { step = sb; up = true; }
+    } else if (dz > 0.0) {
+        const double sb = (z_bot - pz) / dz;
+        if (step > sb) { step = sb; dn = true; }
+    }
+    bool exit_top = up && z_top <= 0.0;
+    bool exit_bot = dn && z_bot >= a.z_total;
+    const double dEds = lerp(dEds_row, p);
+    const double inv_beta_j = lerp(a.lut_inv_beta, p);
+    bool cutoff = false, limited = false;
+    bool geometry_event = up || dn || side;
+    double E_end, t_end;
+    const bool midpoint = a.energy_model == 1;
+    if (a.straggle_on) {
+        if (a.max_dE_frac > 0.0) {
+            const double se = a.max_dE_frac * E / (-dEds);
+            if (se < step) {
+                step = se;
+                limited = true;
+                up = dn = exit_top = exit_bot = side = false;
+                geometry_event = false;
+            }
+        }
+        const uint64_t ukey = splitmix64(s.key ^ kUrbanSalt);
+        const uint64_t combined = ((uint64_t)s.flight << 32) + (uint64_t)s.substep;
+        const uint64_t fkey = splitmix64(ukey + kGolden * combined);
+        const size_t row = (size_t)L * max_el;
+        const double loss = urban_compound(a.L_Z + row, a.L_J + row, a.L_k + row, a.L_coeff + row,
+                                           a.L_E_cross + row, n_el, E, step, fkey);
+        s.stragg += loss;
+        const double delta_cut = E - s.E_cut;
+        if (loss > delta_cut || (loss == delta_cut && !geometry_event)) {
+            step = loss > 0.0 ? step * (delta_cut / loss) : 0.0;
+            cutoff = true;
+            limited = false;
+            up = dn = exit_top = exit_bot = side = false;
+            E_end = s.E_cut;
+        } else {
+            E_end = E - loss;
+        }
+        if (midpoint) {
+            const LutPos c = lut_pos(0.5 * (E + E_end), a.lut_log_E_min, a.lut_inv_dlogE, nE);
+            t_end = s.clock + step * lerp(a.lut_inv_beta, c);
+        } else {
+            t_end = s.clock + step * inv_beta_j;
+        }
+    } else {
+        double cut_dist;
+        if (midpoint) {
+            const LutPos c = lut_pos(0.5 * (E + s.E_cut), a.lut_log_E_min, a.lut_inv_dlogE, nE);
+            cut_dist = (s.E_cut - E) / lerp(dEds_row, c);
+        } else {
+            cut_dist = (s.E_cut - E) / dEds;
+        }
+        if (cut_dist < step || (cut_dist == step && !geometry_event)) {
+            step = cut_dist;
+            cutoff = true;
+            up = dn = exit_top = exit_bot = side = false;
+        }
+        if (a.max_dE_frac > 0.0 && !cutoff) {
+            const double se = a.max_dE_frac * E / (-dEds);
+            if (se < step) {
+                step = se;
+                limited = true;
+                up = dn = exit_top = exit_bot = side = false;
+            }
+        }
+        if (midpoint) {
+            if (cutoff) {
+                E_end = s.E_cut;
+            } else {
+                const double E_pred = E + dEds * step;
+                const LutPos m = lut_pos(0.5 * (E + E_pred), a.lut_log_E_min, a.lut_inv_dlogE, nE);
+                E_end = E + step * lerp(dEds_row, m);
+            }
+            const LutPos c = lut_pos(0.5 * (E + E_end), a.lut_log_E_min, a.lut_inv_dlogE, nE);
+            t_end = s.clock + step * lerp(a.lut_inv_beta, c);
+        } else {
+            E_end = cutoff ? s.E_cut : E + dEds * step;
+            t_end = s.clock + step * inv_beta_j;
+        }
+    }
+    int8_t event = EV_ELASTIC;
+    if (limited) event = EV_SUBSTEP;
+    else if (cutoff) event = EV_CUTOFF;
+    else if (exit_top) event = EV_EXIT_TOP;
+    else if (exit_bot) event = EV_EXIT_BOTTOM;
+    else if (side) event = EV_EXIT_SIDE;
+    else if (up || dn) event = EV_LAYER;
+
+    if (s.nseg < a.cap) {
+        const int64_t slot = (s.e - a.e_start) * a.cap + s.nseg;
+        a.seg_dir[3 * slot] = dx;
+        a.seg_dir[3 * slot + 1] = dy;
+        a.seg_dir[3 * slot + 2] = dz;
+        a.seg_mid[3 * slot] = px + 0.5 * step * dx;
+        a.seg_mid[3 * slot + 1] = py + 0.5 * step * dy;
+        a.seg_mid[3 * slot + 2] = pz + 0.5 * step * dz;
+        a.seg_len[slot] = step;
+        a.seg_E[slot] = E;
+        a.seg_t0[slot] = s.clock;
+        a.seg_id[slot] = s.e;
+        a.seg_lay[slot] = (int16_t)L;
+        if (midpoint) {
+            a.seg_E_end[slot] = E_end;
+            a.seg_t_end[slot] = t_end;
+            a.seg_flight[slot] = s.flight;
+            a.seg_substep[slot] = s.substep;
+            a.seg_event[slot] = event;
+        }
+    }
+    s.nseg += 1;
+    s.px = px + step * dx;
+    s.py = py + step * dy;
+    s.pz = pz + step * dz;
+    s.E = E_end;
+    s.clock = t_end;
+    s.tau -= step / lam;
+    if (s.tau < 0.0) s.tau = 0.0;
+    if (limited) {
+        s.substep += 1;
+        s.steps += 1;
+        return s.steps < a.max_steps ? -1 : EXIT_STEP_LIMITED;
+    }
+    if (exit_top || exit_bot || side || cutoff)
+        return exit_top ? EXIT_BACK : exit_bot ? EXIT_TRANS : side ? EXIT_SIDE : EXIT_CUTOFF;
+    s.flight += 1;
+    s.substep = 0;
+    s.tau = -1.0;
+    if (up || dn) {
+        s.pz += (s.dz > 0.0 ? 1.0 : -1.0) * 1e-6;
+        s.steps += 1;
+        return s.steps < a.max_steps ? -1 : EXIT_STEP_LIMITED;
+    }
+    // Elastic collision: element (cores.py:1094-1105), screened-Rutherford angle.
+    const size_t el_row = (size_t)L * max_el;
+    int i_el = 0;
+    if (n_el != 1) {
+        const double u = stream_uniform(s.key, s.draws);
+        s.draws += 1ULL;
+        i_el = n_el - 1;
+        for (int k = 0; k < n_el; ++k) {
+            if (lerp(a.lut_cdf + (el_row + k) * nE, p) > u) {
+                i_el = k;
+                break;
+            }
+        }
+    }
+    const LutPos q = lut_pos(s.E, a.lut_log_E_min, a.lut_inv_dlogE, nE);
+    const double alpha = lerp(a.lut_alpha + (el_row + i_el) * nE, q);
+    const double cos_u = stream_uniform(s.key, s.draws);
+    const double phi_u = stream_uniform(s.key, s.draws + 1ULL);
+    s.draws += 2ULL;
+    const double cos_t = 1.0 - 2.0 * alpha * cos_u / (1.0 + alpha - cos_u);
+    // core_geometry.py:16-36
+    const double s2 = 1.0 - cos_t * cos_t;
+    const double sin_t = sqrt(s2 > 0.0 ? s2 : 0.0);
+    double sin_phi, cos_phi;
+    const double phi = 2.0 * kPi * phi_u;
+    sin_phi = sin(phi);
+    cos_phi = cos(phi);
+    double refx, refy;
+    if (fabs(dx) < 0.9) { refx = 1.0; refy = 0.0; } else { refx = 0.0; refy = 1.0; }
+    double ux = -dz * refy, uy = dz * refx, uz = dx * refy - dy * refx;
+    const double u_mag = sqrt(ux * ux + uy * uy + uz * uz);
+    ux = ux / u_mag;
+    uy = uy / u_mag;
+    uz = uz / u_mag;
+    const double wx = dy * uz - dz * uy, wy = dz * ux - dx * uz, wz = dx * uy - dy * ux;
+    const double ra = sin_t * cos_phi, rb = sin_t * sin_phi;
+    const double ox = cos_t * dx + ra * ux + rb * wx;
+    const double oy = cos_t * dy + ra * uy + rb * wy;
+    const double oz = cos_t * dz + ra * uz + rb * wz;
+    const double mag = sqrt(ox * ox + oy * oy + oz * oz);
+    s.dx = ox / mag;
+    s.dy = oy / mag;
+    s.dz = oz / mag;
+    s.steps += 1;
+    return s.steps < a.max_steps ? -1 : EXIT_STEP_LIMITED;
+}
+
+}  // namespace
+}  // namespace hr

@@ -1,0 +1,258 @@
+// vr.hip -- round-based transport and weight-window kernels of the learned
+// variance reduction on gfx950.
+//
+// There is no counterpart in the reference: PyRITE's variance reduction is
+// likelihood-ratio detector-cone splitting on the CPU core
+// (src/pyrite/_cone_splitting.py:57, transport/cone_split.py:223) and line
+// roulette (src/pyrite/_line_roulette.py:29).  Here an importance network
+// (mlp.hip) scores particle states between transport rounds and a weight window
+// splits or roulettes them.  Every decision keeps the expected weight:
+//   split:    n copies of weight w/n (total weight exactly w),
+//   roulette: survive with probability w/w_t at weight w_t,
+// so every tally stays unbiased whatever the network predicts.
+//
+// Particles live in a structure-of-arrays bank (pos/dir/E/clock/weight/key/
+// primary id).  A round advances every particle by at most `rows` transport
+// rows with the analog step (transport_step.hpp), then the particle is
+// suspended.  Resuming a suspended particle from a fresh stream is exact:
+// the pending flight is exponential in optical depth (memoryless), so a new
+// optical depth is drawn, and the Urban key domain comes from the new key.
+#include "transport_step.hpp"
+
+namespace hr {
+namespace {
+
+constexpr uint64_t kDecisionCounter = 1ULL << 40;       // draw of a particle's split/roulette decision
+constexpr uint64_t kSpawnSalt = 0x6A09E667F3BCC909ULL;  // child-key domain (sqrt(2) fraction bits)
+
+__host__ __device__ __forceinline__ uint64_t child_key(uint64_t key, int64_t j) {
+    return splitmix64(splitmix64(key ^ kSpawnSalt) + kGolden * (uint64_t)(j + 1));
+}
+
+// One particle per thread: at most `rows` steps, then suspend.
+__global__ void __launch_bounds__(256) vr_round_kernel(hr_transport_args a, int32_t rows) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.e_count) return;
+    const int64_t e = a.e_start + i;
+    Lane s;
+    load_lane(a, e, s);
+    int code = -1;
+    while (code < 0 && s.nseg < rows) code = step_lane(a, s);
+    store_lane(a, s, code < 0 ? (int8_t)EXIT_SUSPENDED : (int8_t)code);
+}
+
+// Network inputs x = (E/E0, u_x, u_y, u_z, z/thickness, tilt/3 deg).  With a
+// row length array the position is the row midpoint and the row start is used.
+__global__ void __launch_bounds__(256) vr_features_kernel(int64_t n, const double *__restrict__ pos,
+                                                          const double *__restrict__ dirs,
+                                                          const double *__restrict__ E,
+                                                          const double *__restrict__ seg_len, double inv_E0,
+                                                          double inv_thickness, float tilt_feature,
+                                                          float *__restrict__ x) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double dz = dirs[3 * i + 2];
+    double z = pos[3 * i + 2];
+    if (seg_len) z -= 0.5 * seg_len[i] * dz;
+    x[6 * i + 0] = (float)(E[i] * inv_E0);
+    x[6 * i + 1] = (float)dirs[3 * i];
+    x[6 * i + 2] = (float)dirs[3 * i + 1];
+    x[6 * i + 3] = (float)dz;
+    x[6 * i + 4] = (float)(z * inv_thickness);
+    x[6 * i + 5] = tilt_feature;
+}
+
+// Weight window on the predicted log-importance y: target weight
+// w_t = clamp(exp(log_I_ref - y), w_min, w_max); ratio r = w / w_t.
+__global__ void __launch_bounds__(256) vr_window_kernel(hr_vr_window_args p) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    int32_t count = 0;
+    double wc = 0.0;
+    if (p.exit_code[i] == EXIT_SUSPENDED) {
+        const double w = p.weight[i];
+        double wt = exp(p.log_I_ref - (double)p.log_importance[i]);
+        wt = fmin(fmax(wt, p.w_min), p.w_max);
+        const double r = w / wt;
+        const double u = stream_uniform(p.key[i], kDecisionCounter);
+        if (r >= 1.0) {
+            double fl = floor(r);
+            if (u < r - fl) fl += 1.0;
+            const double cap = (double)p.n_max;
+            count = (int32_t)(fl < cap ? fl : cap);
+            wc = w / (double)count;
+        } else if (u < r) {
+            count = 1;
+            wc = wt;
+        }
+    }
+    p.count[i] = count;
+    p.w_child[i] = wc;
+}
+
+// Children of parent i occupy [offset[i], offset[i] + count[i]) of the output bank.
+__global__ void __launch_bounds__(256) vr_spawn_kernel(hr_vr_spawn_args p) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const int32_t c = p.count[i];
+    if (c <= 0) return;
+    const int64_t o = p.offset[i];
+    const double px = p.pos[3 * i], py = p.pos[3 * i + 1], pz = p.pos[3 * i + 2];
+    const double dx = p.dirs[3 * i], dy = p.dirs[3 * i + 1], dz = p.dirs[3 * i + 2];
+    const double E = p.E[i], clock = p.clock[i], wc = p.w_child[i];
+    const uint64_t key = p.key[i];
+    const int64_t prim = p.prim[i];
+    for (int32_t j = 0; j < c; ++j) {
+        const int64_t k = o + j;
+        p.out_pos[3 * k] = px;
+        p.out_pos[3 * k + 1] = py;
+        p.out_pos[3 * k + 2] = pz;
+        p.out_dirs[3 * k] = dx;
+        p.out_dirs[3 * k + 1] = dy;
+        p.out_dirs[3 * k + 2] = dz;
+        p.out_E[k] = E;
+        p.out_clock[k] = clock;
+        p.out_weight[k] = wc;
+        p.out_key[k] = child_key(key, j);
+        p.out_prim[k] = prim;
+    }
+}
+
+// Rows of a round (particle i, slot i*rows + s, s < seg_count[i]) gathered to
+// [row_offset[i], ...) as FP32 scoring records with the particle's weight.
+__global__ void __launch_bounds__(256) vr_gather_kernel(hr_vr_gather_args p) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.n) return;
+    const int64_t m = p.seg_count[i] < p.rows ? p.seg_count[i] : p.rows;
+    const int64_t o = p.row_offset[i];
+    const float w = (float)p.weight[i];
+    const int64_t prim = p.prim ? p.prim[i] : i;
+    for (int64_t s = 0; s < m; ++s) {
+        const int64_t src = i * p.rows + s, dst = o + s;
+        for (int c = 0; c < 3; ++c) {
+            p.r_mid[3 * dst + c] = (float)p.seg_mid[3 * src + c];
+            p.v_hat[3 * dst + c] = (float)p.seg_dir[3 * src + c];
+        }
+        p.L[dst] = (float)p.seg_len[src];
+        p.E[dst] = (float)(0.5 * (p.seg_E[src] + p.seg_E_end[src]));
+        p.row_weight[dst] = w;
+        if (p.row_prim) p.row_prim[dst] = prim;
+    }
+}
+
+// Expected window content of each segment's lines: a narrow sinc^2 line of
+// weight w and slope a deposits w * pi / (a * dE) on a uniform grid of
+// spacing dE, on average over its offset to the bins.  Lines are laid out
+// (d * n_seg + j) * n_g + g.
+__global__ void __launch_bounds__(256) line_mass_kernel(const double *__restrict__ E_r, const double *__restrict__ a,
+                                                        const float *__restrict__ w, int64_t n_seg, int32_t n_dir,
+                                                        int32_t n_g, double lo, double hi, double inv_dE,
+                                                        double *__restrict__ mass) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_seg) return;
+    double m = 0.0;
+    for (int d = 0; d < n_dir; ++d) {
+        for (int g = 0; g < n_g; ++g) {
+            const int64_t k = ((int64_t)d * n_seg + j) * n_g + g;
+            const double e = E_r[k], ak = a[k], wk = (double)w[k];
+            if (wk != 0.0 && ak > 0.0 && e >= lo && e < hi) m += wk * kPi * inv_dE / ak;
+        }
+    }
+    mass[j] = m;
+}
+
+unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+}  // namespace
+}  // namespace hr
+
+extern "C" {
+
+hr_status hr_vr_round(const hr_transport_args *a, int32_t rows, void *stream) {
+    using namespace hr;
+    HR_REQUIRE(a != nullptr, "hr_vr_round: null args");
+    HR_REQUIRE(rows >= 1 && rows <= a->cap, "hr_vr_round: rows %d outside [1, cap=%lld]", rows, (long long)a->cap);
+    HR_REQUIRE(a->e_count >= 0 && a->e_start >= 0, "hr_vr_round: bad electron range");
+    if (a->e_count == 0) return HR_OK;
+    HR_REQUIRE(a->lut_n_energy >= 2 && a->n_layers >= 1 && a->max_el >= 1, "hr_vr_round: bad tables");
+    HR_REQUIRE(a->pos && a->dirs && a->E_keV && a->clock && a->E_cut && a->stream_key, "hr_vr_round: null state");
+    HR_REQUIRE(a->seg_dir && a->seg_mid && a->seg_len && a->seg_E && a->seg_t0 && a->seg_id && a->seg_lay,
+               "hr_vr_round: null row buffer");
+    HR_REQUIRE(a->energy_model != 1 || (a->seg_E_end && a->seg_t_end && a->seg_flight && a->seg_substep &&
+                                        a->seg_event),
+               "hr_vr_round: midpoint rows need E_end/t_end/flight/substep/event");
+    HR_REQUIRE(!a->straggle_on || a->stragg_dE, "hr_vr_round: straggling needs stragg_dE");
+    HR_REQUIRE(a->seg_count && a->exit_code, "hr_vr_round: null seg_count/exit_code");
+    HR_REQUIRE(a->e_count < (1LL << 40), "hr_vr_round: too many particles");
+    hipLaunchKernelGGL(vr_round_kernel, dim3(blocks_for(a->e_count, 256)), dim3(256), 0, (hipStream_t)stream, *a,
+                       rows);
+    HR_CHECK_HIP(hipGetLastError());
+    return HR_OK;
+}
+
+hr_status hr_vr_features(int64_t n, const double *pos, const double *dirs, const double *E, const double *seg_len,
+                         double E0_keV, double thickness_ang, double tilt_deg, float *x, void *stream) {
+    using namespace hr;
+    HR_REQUIRE(n >= 0, "hr_vr_features: negative n");
+    if (n == 0) return HR_OK;
+    HR_REQUIRE(pos && dirs && E && x, "hr_vr_features: null pointer");
+    HR_REQUIRE(E0_keV > 0.0 && thickness_ang > 0.0, "hr_vr_features: E0 and thickness must be positive");
+    hipLaunchKernelGGL(vr_features_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, n, pos,
+                       dirs, E, seg_len, 1.0 / E0_keV, 1.0 / thickness_ang, (float)(tilt_deg / 3.0), x);
+    HR_CHECK_HIP(hipGetLastError());
+    return HR_OK;
+}
+
+hr_status hr_vr_window(const hr_vr_window_args *p, void *stream) {
+    using namespace hr;
+    HR_REQUIRE(p != nullptr && p->n >= 0, "hr_vr_window: bad args");
+    if (p->n == 0) return HR_OK;
+    HR_REQUIRE(p->exit_code && p->log_importance && p->weight && p->key && p->count && p->w_child,
+               "hr_vr_window: null pointer");
+    HR_REQUIRE(p->w_min > 0.0 && p->w_min <= p->w_max, "hr_vr_window: need 0 < w_min <= w_max");
+    HR_REQUIRE(p->n_max >= 1 && p->n_max <= 1024, "hr_vr_window: n_max %d outside [1, 1024]", p->n_max);
+    hipLaunchKernelGGL(vr_window_kernel, dim3(blocks_for(p->n, 256)), dim3(256), 0, (hipStream_t)stream, *p);
+    HR_CHECK_HIP(hipGetLastError());
+    return HR_OK;
+}
+
+hr_status hr_vr_spawn(const hr_vr_spawn_args *p, void *stream) {
+    using namespace hr;
+    HR_REQUIRE(p != nullptr && p->n >= 0, "hr_vr_spawn: bad args");
+    if (p->n == 0) return HR_OK;
+    HR_REQUIRE(p->count && p->offset && p->w_child && p->pos && p->dirs && p->E && p->clock && p->key && p->prim,
+               "hr_vr_spawn: null input");
+    HR_REQUIRE(p->out_pos && p->out_dirs && p->out_E && p->out_clock && p->out_weight && p->out_key &&
+                   p->out_prim,
+               "hr_vr_spawn: null output");
+    hipLaunchKernelGGL(vr_spawn_kernel, dim3(blocks_for(p->n, 256)), dim3(256), 0, (hipStream_t)stream, *p);
+    HR_CHECK_HIP(hipGetLastError());
+    return HR_OK;
+}
+
+hr_status hr_vr_gather(const hr_vr_gather_args *p, void *stream) {
+    using namespace hr;
+    HR_REQUIRE(p != nullptr && p->n >= 0 && p->rows >= 1, "hr_vr_gather: bad args");
+    if (p->n == 0) return HR_OK;
+    HR_REQUIRE(p->seg_count && p->row_offset && p->weight && p->seg_mid && p->seg_dir && p->seg_len && p->seg_E &&
+                   p->seg_E_end,
+               "hr_vr_gather: null input");
+    HR_REQUIRE(p->r_mid && p->v_hat && p->L && p->E && p->row_weight, "hr_vr_gather: null output");
+    hipLaunchKernelGGL(vr_gather_kernel, dim3(blocks_for(p->n, 256)), dim3(256), 0, (hipStream_t)stream, *p);
+    HR_CHECK_HIP(hipGetLastError());
+    return HR_OK;
+}
+
+hr_status hr_line_mass(const double *E_r, const double *a, const float *w, int64_t n_seg, int32_t n_dir,
+                       int32_t n_g, double lo_eV, double hi_eV, double dE_eV, double *mass, void *stream) {
+    using namespace hr;
+    HR_REQUIRE(n_seg >= 0 && n_dir >= 1 && n_g >= 1, "hr_line_mass: bad sizes");
+    if (n_seg == 0) return HR_OK;
+    HR_REQUIRE(E_r && a && w && mass, "hr_line_mass: null pointer");
+    HR_REQUIRE(dE_eV > 0.0, "hr_line_mass: dE must be positive");
+    hipLaunchKernelGGL(line_mass_kernel, dim3(blocks_for(n_seg, 256)), dim3(256), 0, (hipStream_t)stream, E_r, a,
+                       w, n_seg, n_dir, n_g, lo_eV, hi_eV, 1.0 / dE_eV, mass);
+    HR_CHECK_HIP(hipGetLastError());
+    return HR_OK;
+}
+}

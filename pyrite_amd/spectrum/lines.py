@@ -195,8 +195,12 @@ class LineScorer:
         self.record_events = False
         self.events = []  # ((start, end), cells) per reduction launch when recording
 
-    def lines(self, seg, scale, j0=0, n=None):
-        """(E_r, a, w) device tensors for segments ``[j0, j0+n)`` of ``seg``."""
+    def lines(self, seg, scale, j0=0, n=None, row_weight=None):
+        """(E_r, a, w) device tensors for segments ``[j0, j0+n)`` of ``seg``.
+
+        ``row_weight`` (device FP32 ``[n_seg]``) multiplies each segment's lines:
+        the statistical weight of a split or rouletted particle.
+        """
         import torch
 
         n_total = int(seg["L"].shape[0])
@@ -226,6 +230,7 @@ class LineScorer:
         args.log_mu = t.log_mu.data_ptr()
         args.lo_keep, args.hi_keep = self.tables.lo_keep, self.tables.hi_keep
         args.scale = float(scale)
+        args.row_weight = None if row_weight is None else row_weight.data_ptr() + 4 * j0
         args.E_r, args.a, args.w = E_r.data_ptr(), a.data_ptr(), w.data_ptr()
         _lib.check(self.lib.hr_cxr_lines(C.byref(args), _stream()), "hr_cxr_lines")
         return E_r, a, w
@@ -250,7 +255,7 @@ class LineScorer:
             ev[1].record()
             self.events.append((ev, args.n_lines * args.n_E))
 
-    def spectrum(self, seg, Ne, out=None):
+    def spectrum(self, seg, Ne, out=None, row_weight=None):
         """Solid-angle-integrated per-electron spectrum [photons/eV/electron] (device FP64)."""
         import torch
 
@@ -261,7 +266,7 @@ class LineScorer:
         step = max(1, LINE_BLOCK // per_seg)
         for j0 in range(0, n_total, step):
             n = min(step, n_total - j0)
-            E_r, a, w = self.lines(seg, 1.0 / float(Ne), j0, n)
+            E_r, a, w = self.lines(seg, 1.0 / float(Ne), j0, n, row_weight)
             self.reduce(E_r, a, w, out)
         return out
 
@@ -297,7 +302,7 @@ class BremScorer:
         self.device = device
         self.lib = _lib.load()
 
-    def reduce(self, seg, out):
+    def reduce(self, seg, out, row_weight=None):
         args = _lib.BremArgs()
         args.n_seg = int(seg["L"].shape[0])
         args.r_mid, args.v_hat = seg["r_mid"].data_ptr(), seg["v_hat"].data_ptr()
@@ -311,15 +316,16 @@ class BremScorer:
         args.n_E = int(self.E_grid.size)
         args.E_grid_eV = self.E_grid_d.data_ptr()
         args.mu = self.mu_d.data_ptr()
+        args.row_weight = None if row_weight is None else row_weight.data_ptr()
         args.out = out.data_ptr()
         _lib.check(self.lib.hr_brem_reduce(C.byref(args), _stream()), "hr_brem_reduce")
 
-    def spectrum(self, seg, Ne, out=None):
+    def spectrum(self, seg, Ne, out=None, row_weight=None):
         """Density [photons/eV/sr/electron] (device FP64): the sum over 4 pi Ne."""
         import torch
 
         acc = torch.zeros(self.E_grid.size, dtype=torch.float64, device=self.device)
-        self.reduce(seg, acc)
+        self.reduce(seg, acc, row_weight)
         acc /= 4.0 * np.pi * float(Ne)
         if out is not None:
             out += acc

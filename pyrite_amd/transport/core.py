@@ -115,6 +115,45 @@ class TransportControl:
     height_ang: float = 0.0
 
 
+def transport_args(tables: DeviceTables, control: TransportControl, run, state, scratch, out, straggling,
+                   block_size=0) -> "_lib.TransportArgs":
+    """``hr_transport_args`` for the core's positional contract (pointers only)."""
+    L, lut = tables.layers, tables.lut
+    e_start, e_count, cap, keys = run
+    alive, clock, pos, dirs, E_keV, E_cut = state
+    seg_count, exit_code = out
+    straggle_on, stragg_dE = straggling
+    a = _lib.TransportArgs()
+    a.e_start, a.e_count, a.cap = int(e_start), int(e_count), int(cap)
+    a.stream_key = _ptr(keys)
+    a.max_steps = int(control.max_steps)
+    a.energy_model = int(control.energy_model)
+    a.max_dE_frac = float(control.max_dE_frac)
+    a.straggle_on = int(bool(straggle_on))
+    a.n_layers = int(L.n_layers)
+    a.internal_bounds = _ptr(tables.internal_bounds)
+    a.z_total = float(L.z_total)
+    a.finite_footprint = int(bool(control.finite_footprint))
+    a.width_ang = float(control.width_ang)
+    a.height_ang = float(control.height_ang)
+    a.L_nel, a.L_top, a.L_bot = _ptr(tables.L_nel), _ptr(tables.L_top), _ptr(tables.L_bot)
+    a.max_el = int(L.max_el)
+    a.L_Z, a.L_J, a.L_k = _ptr(tables.L_Z), _ptr(tables.L_J), _ptr(tables.L_k)
+    a.L_coeff, a.L_E_cross = _ptr(tables.L_coeff), _ptr(tables.L_E_cross)
+    a.lut_log_E_min = float(lut.log_E_min)
+    a.lut_inv_dlogE = float(lut.inv_dlogE)
+    a.lut_n_energy = int(lut.n_energy)
+    a.lut_total_rate, a.lut_dEds = _ptr(tables.total_rate), _ptr(tables.dEds)
+    a.lut_inv_beta, a.lut_cdf, a.lut_alpha = _ptr(tables.inv_beta), _ptr(tables.cdf), _ptr(tables.alpha)
+    a.alive, a.clock, a.pos, a.dirs = _ptr(alive), _ptr(clock), _ptr(pos), _ptr(dirs)
+    a.E_keV, a.E_cut, a.stragg_dE = _ptr(E_keV), _ptr(E_cut), _ptr(stragg_dE)
+    for name in ROW_FIELDS:
+        setattr(a, name, _ptr(scratch.get(name)))
+    a.seg_count, a.exit_code = _ptr(seg_count), _ptr(exit_code)
+    a.block_size = int(block_size)
+    return a
+
+
 def make_hip_transport_core(tables: DeviceTables, control: TransportControl):
     """Return ``core(run, state, scratch, out, straggling, stream=None)``.
 
@@ -124,41 +163,9 @@ def make_hip_transport_core(tables: DeviceTables, control: TransportControl):
     ``out = (seg_count, exit_code)``; ``straggling = (straggle_on, stragg_dE)``.
     """
     lib = _lib.load()
-    L, lut = tables.layers, tables.lut
 
     def core(run, state, scratch, out, straggling, stream=None, block_size=0):
-        e_start, e_count, cap, keys = run
-        alive, clock, pos, dirs, E_keV, E_cut = state
-        seg_count, exit_code = out
-        straggle_on, stragg_dE = straggling
-        a = _lib.TransportArgs()
-        a.e_start, a.e_count, a.cap = int(e_start), int(e_count), int(cap)
-        a.stream_key = _ptr(keys)
-        a.max_steps = int(control.max_steps)
-        a.energy_model = int(control.energy_model)
-        a.max_dE_frac = float(control.max_dE_frac)
-        a.straggle_on = int(bool(straggle_on))
-        a.n_layers = int(L.n_layers)
-        a.internal_bounds = _ptr(tables.internal_bounds)
-        a.z_total = float(L.z_total)
-        a.finite_footprint = int(bool(control.finite_footprint))
-        a.width_ang = float(control.width_ang)
-        a.height_ang = float(control.height_ang)
-        a.L_nel, a.L_top, a.L_bot = _ptr(tables.L_nel), _ptr(tables.L_top), _ptr(tables.L_bot)
-        a.max_el = int(L.max_el)
-        a.L_Z, a.L_J, a.L_k = _ptr(tables.L_Z), _ptr(tables.L_J), _ptr(tables.L_k)
-        a.L_coeff, a.L_E_cross = _ptr(tables.L_coeff), _ptr(tables.L_E_cross)
-        a.lut_log_E_min = float(lut.log_E_min)
-        a.lut_inv_dlogE = float(lut.inv_dlogE)
-        a.lut_n_energy = int(lut.n_energy)
-        a.lut_total_rate, a.lut_dEds = _ptr(tables.total_rate), _ptr(tables.dEds)
-        a.lut_inv_beta, a.lut_cdf, a.lut_alpha = _ptr(tables.inv_beta), _ptr(tables.cdf), _ptr(tables.alpha)
-        a.alive, a.clock, a.pos, a.dirs = _ptr(alive), _ptr(clock), _ptr(pos), _ptr(dirs)
-        a.E_keV, a.E_cut, a.stragg_dE = _ptr(E_keV), _ptr(E_cut), _ptr(stragg_dE)
-        for name in ROW_FIELDS:
-            setattr(a, name, _ptr(scratch.get(name)))
-        a.seg_count, a.exit_code = _ptr(seg_count), _ptr(exit_code)
-        a.block_size = int(block_size)
+        a = transport_args(tables, control, run, state, scratch, out, straggling, block_size)
         _lib.check(lib.hr_transport_run(C.byref(a), C.c_void_p(stream)), "hr_transport_run")
 
     return core
