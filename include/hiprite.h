@@ -248,6 +248,90 @@ hr_status hr_mlp_grad(const hr_mlp *net, const float *x_d, const float *t_d, int
 hr_status hr_mlp_adam(hr_mlp *net, const float *grad_d, float *m1_d, float *m2_d, float lr,
                       int32_t step, void *stream);
 
+/* ---- many configurations in one persistent work-queue transport kernel
+ * Replaces a host loop of per-case transports (montecarlo/runner/case_driver.py:
+ * 142-152 over transport/batching.py:477-628 per case) for tally-only
+ * campaigns: every history of every configuration is a queue item; item i of
+ * configuration c (offsets[c] <= i < offsets[c+1]) is its electron
+ * e = i - offsets[c] with stream key splitmix64(seed + PHI*(e+1))
+ * (transport/kinematics.py:220), a pencil beam from the origin along beam_dir.
+ * No rows are stored.  Tally columns per configuration (FP64):
+ *   0..6  histories ending with exit code 0..6
+ *   7     rows, 8 rows^2
+ *   9     sum of exit energies of transmitted histories [keV]
+ *   10    sum of exit energies of backscattered histories [keV]
+ *   11    energy deposited (E0 minus the energy carried out through a face) [keV] */
+#define HR_QUEUE_TALLY 12
+typedef struct hr_queue_config {
+    hr_transport_args t; /* control, geometry, materials, LUT; run, state and row fields unused */
+    uint64_t seed;
+    double E0_keV, E_cut_keV;
+    double beam_dir[3];
+} hr_queue_config;
+typedef struct hr_queue_launch {
+    const hr_queue_config *configs; /* device [n_cfg], n_cfg <= 1024 */
+    int32_t n_cfg;
+    const int64_t *offsets;         /* device [n_cfg + 1] */
+    int64_t total;                  /* offsets[n_cfg] */
+    double *tally;                  /* device [n_cfg, HR_QUEUE_TALLY] */
+    int32_t accumulate;             /* 0: the tally is zeroed first (by a kernel, graph-safe) */
+    int32_t block_size;             /* 0 (256) or 64..256 in wave64 steps */
+    int32_t per_claim;              /* consecutive items a lane claims at once, 1..1024 */
+} hr_queue_launch;
+hr_status hr_queue_run(const hr_queue_launch *launch, void *stream);
+/* The same launch captured once into a hipGraph (reset + queue kernel) and
+ * replayed by hr_queue_graph_launch; the graph owns its own claim counter. */
+hr_status hr_queue_graph_create(const hr_queue_launch *launch, void **graph);
+hr_status hr_queue_graph_launch(void *graph, void *stream);
+hr_status hr_queue_graph_destroy(void *graph);
+
+/* ---- support-windowed line reduction with the cycle-averaged far tail
+ * Replaces the per-row node-window route: row_window_plan and
+ * _sincsq_window_reduce (src/pyrite/montecarlo/spectrum/cxr/_node_window.py:225-250,
+ * :469-522) with the NodeTail deposit/apply (:86-180).  Exact sinc^2 on the
+ * bins within |a (E - E_r)| <= x_c of each line (the reference default is
+ * x_c = 100, DEFAULT_NODE_SUPPORT_CUTOFF), the tail m2/x^2 + m3/x^3 + m4/x^4
+ * elsewhere; broad windows (a quarter of the grid) are exact everywhere.
+ * Uniform grids E0 + k dE with n_E <= 4096. */
+hr_status hr_window_reduce(const double *E_r, const double *a, const float *w, int64_t n_lines,
+                           int64_t n_E, double E0, double dE, double x_c, double *out, void *stream);
+
+typedef struct hr_window_args {
+    const hr_line_args *lines; /* prologue inputs; its E_r/a/w outputs are not used */
+    int64_t n_E;
+    double E0, dE;             /* uniform grid [eV] */
+    double x_c;                /* exact support |a (E - E_r)| <= x_c */
+    double *out;               /* [n_dir, n_E] FP64 per-direction (pixel) spectra, accumulated */
+} hr_window_args;
+/* Per-direction spectra: the line prologue of hr_cxr_lines fused with the
+ * windowed reduction, one spectrum per detector direction (pixel), without
+ * materialising the lines (n_seg * n_dir * n_g of them). */
+hr_status hr_window_spectra(const hr_window_args *args, void *stream);
+
+/* ---- incident states on the entrance face
+ * Replaces initial_beam_positions + primary_directions for an ungrooved
+ * analytic beam (src/pyrite/montecarlo/transport/beam_entry.py:23-126,
+ * :347-359; transverse.py:197-224; geometry.py:330-394).  Draw c of electron e
+ * is u = ((splitmix64(k_e + PHI*(c+1)) >> 12) + 1/2) 2^-52 with
+ * k_e = splitmix64(root + PHI*(e+1)) (kinematics.py:267-312) and the normal is
+ * Phi^-1(u).  root is the SeedSequence child root the reference uses:
+ * spawn(2)[1] for the spot, spawn(5)[4] for the Twiss beam. */
+typedef struct hr_beam_args {
+    int64_t start, n;        /* global electron ids [start, start+n) */
+    int32_t mode;            /* 0 pencil, 1 Gaussian spot, 2 Courant-Snyder (Twiss) */
+    uint64_t root;
+    double sigma_x, sigma_y; /* spot sigma [A] (mode 1) */
+    double twiss_x[3];       /* (sqrt(eps beta) [mm], sqrt(eps/beta) [rad], alpha) (mode 2) */
+    double twiss_y[3];
+    int32_t projected;       /* nonzero tilt: project lab offsets onto the z = 0 face */
+    double e_x[3], e_y[3];   /* sample-frame lab spot axes (columns of R^T) */
+    double beam_dir[3];      /* nominal unit beam axis, sample frame */
+    double basis[9];         /* beam-frame basis, row-major; columns 0, 1 carry (x', y') */
+    double E0_keV;
+    double *pos, *dirs, *E_keV, *clock; /* out, local index */
+} hr_beam_args;
+hr_status hr_beam_entry(const hr_beam_args *args, void *stream);
+
 /* ---- learned variance reduction: round-based transport and weight window.
  * No counterpart in the reference (its variance reduction is likelihood-ratio
  * cone splitting, src/pyrite/_cone_splitting.py:57 and
@@ -277,12 +361,21 @@ typedef struct hr_vr_window_args {
     const uint64_t *key;          /* [n] decision draw: counter 2^40 of the key */
     double log_I_ref;             /* target weight w_t = clamp(exp(log_I_ref - y), w_min, w_max) */
     double w_min, w_max;
+    double ratio_lo, ratio_hi;    /* window: split when w/w_t > ratio_hi, roulette when < ratio_lo */
     int32_t n_max;                /* copies per particle, 1..1024 */
     int32_t *count;               /* [n] out: copies (0 = ended or rouletted) */
     double *w_child;              /* [n] out: weight of each copy */
+    const double *weight2;        /* [n] second tally's weight, or NULL */
+    double *w2_child;             /* [n] out: second weight of copy 0 (other copies carry 0), or NULL */
 } hr_vr_window_args;
-/* r = w/w_t; r >= 1: n = floor(r) + [u < frac(r)], capped at n_max, copies of
- * weight w/n; r < 1: one copy of weight w_t with probability r. */
+/* r = w/w_t; r > ratio_hi: n = floor(r) + [u < frac(r)], capped at n_max,
+ * copies of weight w/n; r < ratio_lo: one copy of weight w_t with probability
+ * r; otherwise the particle continues unchanged (one copy of weight w).  The
+ * band [ratio_lo, ratio_hi] (0 < ratio_lo <= 1 <= ratio_hi) keeps noisy
+ * importance estimates from splitting and rouletting the same particle back
+ * and forth.  The second
+ * weight (a tally that is not split, e.g. brem) stays with copy 0 (w2) and is
+ * divided by the survival probability r on roulette. */
 hr_status hr_vr_window(const hr_vr_window_args *args, void *stream);
 
 typedef struct hr_vr_spawn_args {
@@ -296,6 +389,8 @@ typedef struct hr_vr_spawn_args {
     double *out_pos, *out_dirs, *out_E, *out_clock, *out_weight;
     uint64_t *out_key;       /* copy j of key k: splitmix64(splitmix64(k ^ S) + PHI*(j+1)) */
     int64_t *out_prim;
+    const double *w2_child;  /* [n] or NULL */
+    double *out_weight2;     /* copy 0: w2_child, other copies 0 (NULL with w2_child) */
 } hr_vr_spawn_args;
 hr_status hr_vr_spawn(const hr_vr_spawn_args *args, void *stream);
 
@@ -309,6 +404,8 @@ typedef struct hr_vr_gather_args {
     float *r_mid, *v_hat, *L, *E;  /* out: FP32 scoring records, E = (E + E_end)/2 */
     float *row_weight;             /* out */
     int64_t *row_prim;             /* out or NULL */
+    const double *weight2;         /* [n] or NULL */
+    float *row_weight2;            /* out, or NULL */
 } hr_vr_gather_args;
 hr_status hr_vr_gather(const hr_vr_gather_args *args, void *stream);
 
@@ -318,6 +415,15 @@ hr_status hr_vr_gather(const hr_vr_gather_args *args, void *stream);
 hr_status hr_line_mass(const double *E_r, const double *a, const float *w, int64_t n_seg,
                        int32_t n_dir, int32_t n_g, double lo_eV, double hi_eV, double dE_eV,
                        double *mass, void *stream);
+
+/* Line roulette by exact mass: segment j's lines are kept surely when
+ * mass[j] >= m_thr, else with probability p = mass[j]/m_thr and weight 1/p
+ * (u = counter draw row0 + j of `key`); dropped lines get w = 0.  The
+ * per-primary two-stratum LineRoulette of the reference
+ * (src/pyrite/_line_roulette.py:29-80) applied per segment with a
+ * proportional keep probability. */
+hr_status hr_line_roulette(float *w, int64_t n_seg, int32_t n_dir, int32_t n_g, const double *mass,
+                           double m_thr, uint64_t key, int64_t row0, void *stream);
 
 #ifdef __cplusplus
 }

@@ -28,8 +28,11 @@ def run_case(pipe_like, start, n, threads=0, timings=None):
     cfg = pipe_like.cfg
     t0 = time.perf_counter()
     keys = oracle.stream_keys(cfg.seed, n, start=start)
-    pos = np.zeros((n, 3))
-    dirs = np.tile(pipe_like.beam_dir, (n, 1))
+    from pyrite_amd.transport.beam import beam_args
+
+    from . import beam
+
+    pos, dirs = beam.entry(beam_args(cfg.beam, cfg.seed, start, n, cfg.E0_keV, np.deg2rad(cfg.tilt_deg)), n, start)
     E = np.full(n, cfg.E0_keV)
     E_cut = np.full(n, cfg.E_cut_keV)
     cap = 512

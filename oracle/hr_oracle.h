@@ -149,11 +149,20 @@ void or_mlp_set_rounding(int on); /* 1: device bfloat16 roundings (default), 0: 
 void or_mlp_forward(const float *params, const float *x, int64_t n, double *y);
 double or_mlp_loss_grad(const float *params, const float *x, const float *t, int64_t n,
                         double *grad);
+/* ---- support-windowed sinc^2 reduction with the far tail (or_window.c,
+ *      spectrum/cxr/_node_window.py:86-250,454-522); uniform grid E0 + k dE */
+int or_window_plan(double E_r, double a, double E0, double dE, int64_t n, double x_c, int64_t *node,
+                   int64_t *lo, int64_t *hi, double *mu);
+void or_window_reduce(const double *E_r, const double *a, const float *w, int64_t n_lines, double E0,
+                      double dE, int64_t n, double x_c, double *out);
 /* ---- learned variance reduction (no reference counterpart; or_vr.c) */
 uint64_t or_vr_child_key(uint64_t key, int64_t j);
 void or_vr_window(int64_t n, const int8_t *exit_code, const float *log_importance, const double *weight,
-                  const uint64_t *key, double log_I_ref, double w_min, double w_max, int32_t n_max,
-                  int32_t *count, double *w_child);
+                  const uint64_t *key, double log_I_ref, double w_min, double w_max, double ratio_lo,
+                  double ratio_hi, int32_t n_max, int32_t *count, double *w_child, const double *weight2,
+                  double *w2_child);
+void or_line_roulette(float *w, int64_t n_seg, int32_t n_dir, int32_t n_g, const double *mass, double m_thr,
+                      uint64_t key, int64_t row0);
 void or_line_mass(const double *E_r, const double *a, const float *w, int64_t n_seg, int32_t n_dir,
                   int32_t n_g, double lo, double hi, double dE, double *mass);
 /* out[k] += sum_seg sum_el n L dsigma/dk (BH+Elwert) * <exp(-mu_k path)>;

@@ -113,11 +113,17 @@ def lib():
         L.or_mlp_forward.argtypes = [_dp, _dp, C.c_int64, _dp]
         L.or_mlp_loss_grad.restype = C.c_double
         L.or_mlp_loss_grad.argtypes = [_dp, _dp, _dp, C.c_int64, _dp]
+        L.or_window_reduce.restype = None
+        L.or_window_reduce.argtypes = [_dp, _dp, _dp, C.c_int64, C.c_double, C.c_double, C.c_int64,
+                                       C.c_double, _dp]
         L.or_vr_child_key.restype = C.c_uint64
         L.or_vr_child_key.argtypes = [C.c_uint64, C.c_int64]
         L.or_vr_window.restype = None
         L.or_vr_window.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double,
-                                   C.c_int32, _dp, _dp]
+                                   C.c_double, C.c_double, C.c_int32, _dp, _dp, _dp, _dp]
+        L.or_line_roulette.restype = None
+        L.or_line_roulette.argtypes = [_dp, C.c_int64, C.c_int32, C.c_int32, _dp, C.c_double, C.c_uint64,
+                                       C.c_int64]
         L.or_line_mass.restype = None
         L.or_line_mass.argtypes = [_dp, _dp, _dp, C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                    C.c_double, _dp]
@@ -356,7 +362,8 @@ def vr_child_key(key: int, j: int) -> int:
     return int(lib().or_vr_child_key(C.c_uint64(key), int(j)))
 
 
-def vr_window(exit_code, log_importance, weight, key, log_I_ref, w_min, w_max, n_max):
+def vr_window(exit_code, log_importance, weight, key, log_I_ref, w_min, w_max, n_max, weight2=None,
+              ratio_lo=1.0, ratio_hi=1.0):
     ec = np.ascontiguousarray(exit_code, dtype=np.int8)
     y = _f32(log_importance)
     w = _f64(weight)
@@ -364,9 +371,18 @@ def vr_window(exit_code, log_importance, weight, key, log_I_ref, w_min, w_max, n
     n = ec.size
     count = np.zeros(n, dtype=np.int32)
     wc = np.zeros(n)
+    w2 = None if weight2 is None else _f64(weight2)
+    w2c = None if weight2 is None else np.zeros(n)
     lib().or_vr_window(n, _p(ec), _p(y), _p(w), _p(k), float(log_I_ref), float(w_min), float(w_max),
-                       int(n_max), _p(count), _p(wc))
-    return count, wc
+                       float(ratio_lo), float(ratio_hi), int(n_max), _p(count), _p(wc), _p(w2), _p(w2c))
+    return (count, wc) if weight2 is None else (count, wc, w2c)
+
+
+def line_roulette(w, n_seg, n_dir, n_g, mass, m_thr, key, row0=0):
+    out = _f32(w).copy()
+    lib().or_line_roulette(_p(out), int(n_seg), int(n_dir), int(n_g), _p(_f64(mass)), float(m_thr),
+                           C.c_uint64(int(key)), int(row0))
+    return out
 
 
 def line_mass(E_r, a, w, n_seg, n_dir, n_g, lo, hi, dE):
@@ -374,4 +390,13 @@ def line_mass(E_r, a, w, n_seg, n_dir, n_g, lo, hi, dE):
     out = np.zeros(int(n_seg))
     lib().or_line_mass(_p(Er), _p(aa), _p(ww), int(n_seg), int(n_dir), int(n_g), float(lo), float(hi),
                        float(dE), _p(out))
+    return out
+
+
+def window_reduce(E_r, a, w, E0, dE, n_E, x_c, out=None):
+    """Support-windowed sinc^2 sum with the far tail on the grid E0 + k dE."""
+    Er, aa, ww = _f64(E_r), _f64(a), _f32(w)
+    if out is None:
+        out = np.zeros(int(n_E))
+    lib().or_window_reduce(_p(Er), _p(aa), _p(ww), Er.size, float(E0), float(dE), int(n_E), float(x_c), _p(out))
     return out

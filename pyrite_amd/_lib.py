@@ -150,6 +150,60 @@ class BremArgs(C.Structure):
     ]
 
 
+QUEUE_TALLY = 12
+
+
+class QueueConfig(C.Structure):
+    """Mirror of ``hr_queue_config``."""
+
+    _fields_ = [
+        ("t", TransportArgs),
+        ("seed", C.c_uint64),
+        ("E0_keV", C.c_double), ("E_cut_keV", C.c_double),
+        ("beam_dir", C.c_double * 3),
+    ]
+
+
+class QueueLaunch(C.Structure):
+    """Mirror of ``hr_queue_launch``."""
+
+    _fields_ = [
+        ("configs", _dp), ("n_cfg", C.c_int32),
+        ("offsets", _dp), ("total", C.c_int64),
+        ("tally", _dp),
+        ("accumulate", C.c_int32), ("block_size", C.c_int32), ("per_claim", C.c_int32),
+    ]
+
+
+class WindowArgs(C.Structure):
+    """Mirror of ``hr_window_args``."""
+
+    _fields_ = [
+        ("lines", C.POINTER(LineArgs)),
+        ("n_E", C.c_int64),
+        ("E0", C.c_double), ("dE", C.c_double),
+        ("x_c", C.c_double),
+        ("out", _dp),
+    ]
+
+
+class BeamArgs(C.Structure):
+    """Mirror of ``hr_beam_args``."""
+
+    _fields_ = [
+        ("start", C.c_int64), ("n", C.c_int64),
+        ("mode", C.c_int32),
+        ("root", C.c_uint64),
+        ("sigma_x", C.c_double), ("sigma_y", C.c_double),
+        ("twiss_x", C.c_double * 3), ("twiss_y", C.c_double * 3),
+        ("projected", C.c_int32),
+        ("e_x", C.c_double * 3), ("e_y", C.c_double * 3), ("beam_dir", C.c_double * 3),
+        ("basis", C.c_double * 9),
+        ("E0_keV", C.c_double),
+        ("pos", _dp), ("dirs", _dp), ("E_keV", _dp), ("clock", _dp),
+    ]
+
+
 class VrWindowArgs(C.Structure):
     """Mirror of ``hr_vr_window_args``."""
 
@@ -157,8 +211,10 @@ class VrWindowArgs(C.Structure):
         ("n", C.c_int64),
         ("exit_code", _dp), ("log_importance", _dp), ("weight", _dp), ("key", _dp),
         ("log_I_ref", C.c_double), ("w_min", C.c_double), ("w_max", C.c_double),
+        ("ratio_lo", C.c_double), ("ratio_hi", C.c_double),
         ("n_max", C.c_int32),
         ("count", _dp), ("w_child", _dp),
+        ("weight2", _dp), ("w2_child", _dp),
     ]
 
 
@@ -172,6 +228,7 @@ class VrSpawnArgs(C.Structure):
         ("key", _dp), ("prim", _dp),
         ("out_pos", _dp), ("out_dirs", _dp), ("out_E", _dp), ("out_clock", _dp), ("out_weight", _dp),
         ("out_key", _dp), ("out_prim", _dp),
+        ("w2_child", _dp), ("out_weight2", _dp),
     ]
 
 
@@ -184,6 +241,7 @@ class VrGatherArgs(C.Structure):
         ("seg_mid", _dp), ("seg_dir", _dp), ("seg_len", _dp), ("seg_E", _dp), ("seg_E_end", _dp),
         ("r_mid", _dp), ("v_hat", _dp), ("L", _dp), ("E", _dp),
         ("row_weight", _dp), ("row_prim", _dp),
+        ("weight2", _dp), ("row_weight2", _dp),
     ]
 
 
@@ -209,11 +267,21 @@ SIGNATURES = {
     "hr_mlp_forward": (C.c_int32, [C.POINTER(MlpNet), _dp, C.c_int64, _dp, _dp]),
     "hr_mlp_grad": (C.c_int32, [C.POINTER(MlpNet), _dp, _dp, C.c_int64, _dp, _dp, _dp, _dp]),
     "hr_mlp_adam": (C.c_int32, [C.POINTER(MlpNet), _dp, _dp, _dp, C.c_float, C.c_int32, _dp]),
+    "hr_beam_entry": (C.c_int32, [C.POINTER(BeamArgs), _dp]),
+    "hr_queue_run": (C.c_int32, [C.POINTER(QueueLaunch), _dp]),
+    "hr_queue_graph_create": (C.c_int32, [C.POINTER(QueueLaunch), C.POINTER(C.c_void_p)]),
+    "hr_queue_graph_launch": (C.c_int32, [_dp, _dp]),
+    "hr_queue_graph_destroy": (C.c_int32, [_dp]),
+    "hr_window_reduce": (C.c_int32, [_dp, _dp, _dp, C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                     _dp, _dp]),
+    "hr_window_spectra": (C.c_int32, [C.POINTER(WindowArgs), _dp]),
     "hr_vr_round": (C.c_int32, [C.POINTER(TransportArgs), C.c_int32, _dp]),
     "hr_vr_features": (C.c_int32, [C.c_int64, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp, _dp]),
     "hr_vr_window": (C.c_int32, [C.POINTER(VrWindowArgs), _dp]),
     "hr_vr_spawn": (C.c_int32, [C.POINTER(VrSpawnArgs), _dp]),
     "hr_vr_gather": (C.c_int32, [C.POINTER(VrGatherArgs), _dp]),
+    "hr_line_roulette": (C.c_int32, [_dp, C.c_int64, C.c_int32, C.c_int32, _dp, C.c_double, C.c_uint64,
+                                     C.c_int64, _dp]),
     "hr_line_mass": (C.c_int32, [_dp, _dp, _dp, C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                  C.c_double, _dp, _dp]),
 }

@@ -62,13 +62,10 @@ __device__ __forceinline__ double mean_transmission(double t0, double t1) {
     return exp(-lo) * ratio;
 }
 
-// One (segment, direction) pair: loops over reflections, writes n_g lines.
-__global__ void __launch_bounds__(256) cxr_lines_kernel(hr_line_args p) {
-    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t total = p.n_seg * (int64_t)p.n_dir;
-    if (tid >= total) return;
-    const int64_t j = tid % p.n_seg;
-    const int d = (int)(tid / p.n_seg);
+// Lines of one (segment j, direction d) pair: emit(g, E_r, a, w) for every
+// reflection g (w = 0 and E_r = a = 0 for a rejected pair).
+template <class Emit>
+__device__ __forceinline__ void segment_lines(const hr_line_args &p, int64_t j, int d, Emit &&emit) {
     const double nx = p.n_hat[3 * d], ny = p.n_hat[3 * d + 1], nz = p.n_hat[3 * d + 2];
     const double dir_w = p.dir_weight[d] * p.scale * (p.row_weight ? (double)p.row_weight[j] : 1.0);
     const double rz = p.r_mid[3 * j + 2];
@@ -94,7 +91,6 @@ __global__ void __launch_bounds__(256) cxr_lines_kernel(hr_line_args p) {
     const double inv_E = kHbarc / (gamma * kMe);
     const double h = kHbarc / kMe;
     const int64_t n_tab = p.n_tab;
-    const int64_t out0 = ((int64_t)d * p.n_seg + j) * p.n_g;
     for (int g = 0; g < p.n_g; ++g) {
         const double Gx = p.g[3 * g], Gy = p.g[3 * g + 1], Gz = p.g[3 * g + 2];
         const double g2 = Gx * Gx + Gy * Gy + Gz * Gz;
@@ -204,10 +200,23 @@ __global__ void __launch_bounds__(256) cxr_lines_kernel(hr_line_args p) {
                 Er = E_res;
             }
         }
+        emit(g, Er, a_width, w);
+    }
+}
+
+// One (segment, direction) pair per thread: writes its n_g lines.
+__global__ void __launch_bounds__(256) cxr_lines_kernel(hr_line_args p) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = p.n_seg * (int64_t)p.n_dir;
+    if (tid >= total) return;
+    const int64_t j = tid % p.n_seg;
+    const int d = (int)(tid / p.n_seg);
+    const int64_t out0 = ((int64_t)d * p.n_seg + j) * p.n_g;
+    segment_lines(p, j, d, [&](int g, double Er, double a_width, double w) {
         p.E_r[out0 + g] = Er;
         p.a[out0 + g] = a_width;
         p.w[out0 + g] = (float)w;
-    }
+    });
 }
 
 // sin(pi r) for |r| <= 1/2: odd Taylor series of sin(pi r) through r^13
@@ -357,6 +366,16 @@ __global__ void __launch_bounds__(256) sum_partials_kernel(const double *__restr
     out[k] += s;
 }
 
+constexpr float kMc2f = 510.99895f;
+constexpr float kInvMc2f = (float)(1.0 / 510.99895);
+constexpr double kLog2e = 1.4426950408889634;
+
+// Hardware single-instruction forms (v_rcp_f32, v_sqrt_f32, v_exp_f32): about
+// 1 ulp each, against the multi-instruction correctly rounded sequences.
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+
 // BH+Elwert brem over segment tiles (brem.py:396-423, brem_jit_kernel.py:76-153).
 constexpr int kBremTile = 128;
 template <int BINS>
@@ -376,8 +395,16 @@ __global__ void __launch_bounds__(kRedThreads) brem_kernel(hr_brem_args p, doubl
         keV[m] = kev[m] * 1.0e-3f;
         mu[m] = k < p.n_E ? p.mu[k] : 0.0f;
     }
-    float zi[4];
-    for (int el = 0; el < 4; ++el) zi[el] = el < n_el ? (float)(2.0 * kPi * p.Z[el] * kAlpha) : 0.0f;
+    float zi[4], zi2[4], inv_kev[BINS], mu2[BINS];
+    for (int el = 0; el < 4; ++el) {
+        zi[el] = el < n_el ? (float)(2.0 * kPi * p.Z[el] * kAlpha) : 0.0f;
+        zi2[el] = (float)((double)zi[el] * kLog2e);  // exp(-z/b) = exp2(-z log2(e) / b)
+    }
+#pragma unroll
+    for (int m = 0; m < BINS; ++m) {
+        inv_kev[m] = 1.0f / fmaxf(kev[m], 1.0e-30f);
+        mu2[m] = (float)((double)mu[m] * kLog2e);
+    }
     const int64_t n_tiles = (p.n_seg + kBremTile - 1) / kBremTile;
     for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
         if (t < kBremTile) {
@@ -416,27 +443,40 @@ __global__ void __launch_bounds__(kRedThreads) brem_kernel(hr_brem_args p, doubl
         const int64_t left = p.n_seg - tile * kBremTile;
         const int cnt = left < kBremTile ? (int)left : kBremTile;
         for (int q = 0; q < cnt; ++q) {
+            // a zero-weight row (a copy that does not carry this tally) adds nothing
+            if (s_pref[q * 4] == 0.0f && s_pref[q * 4 + 1] == 0.0f && s_pref[q * 4 + 2] == 0.0f &&
+                s_pref[q * 4 + 3] == 0.0f)
+                continue;
             const float T = s_T[q], pi_ = s_pi[q], p0 = s_p0[q], p1 = s_p1[q];
+            const float plo = fminf(p0, p1), dp01 = fabsf(p1 - p0);
 #pragma unroll
             for (int m = 0; m < BINS; ++m) {
                 const float Tf = T - keV[m];
                 if (!(Tf > 1.0e-6f) || !(keV[m] > 0.0f)) continue;
-                const float pf = sqrtf(Tf * (Tf + 2.0f * (float)kMc2)) / (float)kMc2;
-                const float bf = pf / (1.0f + Tf / (float)kMc2);
-                // p_i - p_f = k (T_i + T_f + 2 m) / (m^2 (p_i + p_f)): no cancellation when k << T_i
-                const float dp = fmaxf(keV[m] * (T + Tf + 2.0f * (float)kMc2) /
-                                           ((float)(kMc2 * kMc2) * (pi_ + pf)), 1.0e-30f);
-                const float born = __logf((pi_ + pf) / dp);
+                const float Tm = Tf + 2.0f * kMc2f;
+                const float pf = fast_sqrt(Tf * Tm) * kInvMc2f;
+                // beta_f = p_f / (1 + T_f/m);  (p_i + p_f)/(p_i - p_f) with
+                // p_i - p_f = k (T_i + T_f + 2m) / (m^2 (p_i + p_f)) (no cancellation for k << T_i)
+                const float rcp_bf = fast_rcp(pf * kMc2f) * (kMc2f + Tf);
+                const float spf = pi_ + pf;
+                const float born = __logf(spf * spf * (kMc2f * kMc2f) * fast_rcp(keV[m] * (T + Tm)));
+                const float base = born * inv_kev[m] * rcp_bf;
                 float cell = 0.0f;
                 for (int el = 0; el < n_el; ++el) {
-                    const float den_f = 1.0f - __expf(-zi[el] / bf);
-                    cell += s_pref[q * 4 + el] * born / (fmaxf(kev[m], 1.0e-30f) * bf * den_f);
+                    const float den_f = 1.0f - fast_exp2(-zi2[el] * rcp_bf);
+                    cell += s_pref[q * 4 + el] * base * fast_rcp(den_f);
                 }
-                const float t0 = p0 * mu[m], t1 = p1 * mu[m];
-                const float lo = fminf(t0, t1);
-                const float dd = fabsf(t1 - t0);
-                const float ratio = dd > 1.0e-3f ? -expm1f(-dd) / dd : 1.0f - 0.5f * dd + dd * dd * (1.0f / 6.0f);
-                acc[m] += (double)(cell * (expf(-lo) * ratio));
+                // segment-mean escape exp(-t_lo) (1 - exp(-d)) / d, d = |t1 - t0|
+                const float d = dp01 * mu[m];
+                float ratio;
+                if (d > 0.5f) {
+                    ratio = (1.0f - fast_exp2(-dp01 * mu2[m])) * fast_rcp(d);
+                } else {
+                    // Taylor series of (1 - e^-d)/d, truncation < 6e-10 on [0, 1/2]
+                    ratio = 1.0f + d * (-1.0f / 2 + d * (1.0f / 6 + d * (-1.0f / 24 + d * (1.0f / 120 +
+                            d * (-1.0f / 720 + d * (1.0f / 5040 + d * (-1.0f / 40320 + d * (1.0f / 362880))))))));
+                }
+                acc[m] += (double)(cell * (fast_exp2(-plo * mu2[m]) * ratio));
             }
         }
         __syncthreads();
@@ -447,6 +487,206 @@ __global__ void __launch_bounds__(kRedThreads) brem_kernel(hr_brem_args p, doubl
         const int64_t k = bin0 + t + (int64_t)m * kRedThreads;
         if (k < p.n_E) out[k] = acc[m];
     }
+}
+
+// ---- support-windowed reduction with the cycle-averaged far tail
+// (spectrum/cxr/_node_window.py:86-250 NodeTail / row_window_plan,
+// :454-522 _sincsq_window_reduce).  A line deposits its exact sinc^2 on the
+// bins within |a (E - E_r)| <= x_c (minus the tail there) and three tail
+// moments (m2, m3, m4) = w (1, 2 eps, 3 eps^2) / (2 a^2) at its node; a second
+// pass adds sum_node m2/x^2 + m3/x^3 + m4/x^4 everywhere.  Broad windows (a
+// quarter of the grid or more) and nodes outside [-n, 2n) are summed exactly
+// over the whole grid.  Per-direction spectra and in-grid moments live in LDS
+// (FP64 atomics); out-of-grid moments go to global memory.
+constexpr int kWinThreads = 512;
+constexpr int kWinMaxBins = 4096;
+
+struct WindowShared {
+    double spec[kWinMaxBins];
+    double dep[3][kWinMaxBins];
+};
+static_assert(sizeof(WindowShared) <= 160 * 1024, "window LDS");
+
+__device__ __forceinline__ double tail_value(double m2, double m3, double m4, double x) {
+    const double inv = 1.0 / x;
+    return (m2 + (m3 + m4 * inv) * inv) * inv * inv;
+}
+
+// Exact cell of the dense reducer at u = a (E - E_r) / pi (FP64 phase, FP32 profile).
+__device__ __forceinline__ double sincsq_cell(double u) {
+    const double rn = rint(u);
+    const float r = (float)(u - rn);
+    const float uf = (float)u;
+    const float sv = sinpi_half(r);
+    const float denom = (float)(kPi * kPi) * uf * uf;
+    return uf == 0.0f ? 1.0 : (double)((sv * sv) / denom);
+}
+
+struct WindowGrid {
+    int64_t n;
+    double E0, dE, x_c;
+};
+
+__device__ __forceinline__ void window_line(WindowShared &sh, double *__restrict__ dep_far, const WindowGrid &G,
+                                            double Er, double a, double w) {
+    const int64_t n = G.n;
+    const double nd = rint((Er - G.E0) / G.dE);
+    const double eps = Er - (G.E0 + nd * G.dE);
+    double R = ceil((fabs(eps) + G.x_c / a) / G.dE);
+    R = fmin(R, 4503599627370496.0);
+    const int64_t node = (int64_t)nd;
+    const bool conv_range = node + n >= 0 && node + n < 3 * n;
+    int64_t lo = 0, hi = n - 1;
+    bool conv = false;
+    if (conv_range) {
+        const int64_t RR = (int64_t)R;
+        const int64_t l = node - RR < 0 ? 0 : node - RR;
+        const int64_t h = node + RR > n - 1 ? n - 1 : node + RR;
+        if ((double)(h - l + 1) < 0.25 * (double)n) {
+            lo = l;
+            hi = h;
+            conv = true;
+        }
+    }
+    const double m2 = 0.5 / (a * a);
+    const double m3 = 2.0 * eps * m2, m4 = 3.0 * eps * eps * m2;
+    const double inv_pi = 1.0 / kPi;
+    const double al = a * inv_pi;
+    for (int64_t e = lo; e <= hi; ++e) {
+        double v = sincsq_cell(al * ((G.E0 + (double)e * G.dE) - Er));
+        if (conv && e != node) v -= tail_value(m2, m3, m4, (double)(e - node) * G.dE);
+        atomicAdd(&sh.spec[e], w * v);
+    }
+    if (conv) {
+        if (node >= 0 && node < n) {
+            atomicAdd(&sh.dep[0][node], w * m2);
+            atomicAdd(&sh.dep[1][node], w * m3);
+            atomicAdd(&sh.dep[2][node], w * m4);
+        } else {
+            const int64_t L = 3 * n;
+            atomicAdd(&dep_far[node + n], w * m2);
+            atomicAdd(&dep_far[L + node + n], w * m3);
+            atomicAdd(&dep_far[2 * L + node + n], w * m4);
+        }
+    }
+}
+
+__device__ __forceinline__ void window_zero(WindowShared &sh, int64_t n) {
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        sh.spec[i] = 0.0;
+        sh.dep[0][i] = 0.0;
+        sh.dep[1][i] = 0.0;
+        sh.dep[2][i] = 0.0;
+    }
+}
+
+// Block partial: [spec | m2 | m3 | m4] over the grid.
+__device__ __forceinline__ void window_flush(const WindowShared &sh, int64_t n, double *__restrict__ out) {
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        out[i] = sh.spec[i];
+        out[n + i] = sh.dep[0][i];
+        out[2 * n + i] = sh.dep[1][i];
+        out[3 * n + i] = sh.dep[2][i];
+    }
+}
+
+// Segments [chunk share) x direction blockIdx.y: prologue fused with the window deposit.
+__global__ void __launch_bounds__(kWinThreads) window_lines_kernel(hr_line_args p, WindowGrid G,
+                                                                   double *__restrict__ partial,
+                                                                   double *__restrict__ dep_far) {
+    __shared__ WindowShared sh;
+    const int d = blockIdx.y;
+    window_zero(sh, G.n);
+    __syncthreads();
+    const int64_t per = (p.n_seg + gridDim.x - 1) / gridDim.x;
+    const int64_t j0 = (int64_t)blockIdx.x * per;
+    const int64_t j1 = j0 + per < p.n_seg ? j0 + per : p.n_seg;
+    double *far = dep_far + (int64_t)d * 9 * G.n;
+    for (int64_t j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
+        segment_lines(p, j, d, [&](int, double Er, double a_width, double w64) {
+            const float wf = (float)w64;  // the line weight the dense route stores
+            if (wf == 0.0f || !(a_width > 0.0)) return;
+            window_line(sh, far, G, Er, a_width, (double)wf);
+        });
+    }
+    __syncthreads();
+    window_flush(sh, G.n, partial + ((int64_t)d * gridDim.x + blockIdx.x) * 4 * G.n);
+}
+
+// Given lines (one target): the windowed counterpart of hr_sincsq_reduce.
+__global__ void __launch_bounds__(kWinThreads) window_given_kernel(const double *__restrict__ E_r,
+                                                                   const double *__restrict__ a,
+                                                                   const float *__restrict__ w, int64_t n_lines,
+                                                                   WindowGrid G, double *__restrict__ partial,
+                                                                   double *__restrict__ dep_far) {
+    __shared__ WindowShared sh;
+    window_zero(sh, G.n);
+    __syncthreads();
+    for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < n_lines;
+         l += (int64_t)gridDim.x * blockDim.x) {
+        const float wf = w[l];
+        if (wf == 0.0f || !(a[l] > 0.0)) continue;
+        window_line(sh, dep_far, G, E_r[l], a[l], (double)wf);
+    }
+    __syncthreads();
+    window_flush(sh, G.n, partial + (int64_t)blockIdx.x * 4 * G.n);
+}
+
+// Sum the block partials of each direction in a fixed order; spectra to spec,
+// in-grid moments to the node slots [n, 2n) of the moment array.
+__global__ void __launch_bounds__(256) window_sum_kernel(const double *__restrict__ partial, int64_t n_blocks,
+                                                         int64_t n, int n_dir, double *__restrict__ spec,
+                                                         double *__restrict__ dep) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)n_dir * 4 * n) return;
+    const int64_t d = i / (4 * n), r = i % (4 * n);
+    double s = 0.0;
+    for (int64_t b = 0; b < n_blocks; ++b) s += partial[(d * n_blocks + b) * 4 * n + r];
+    const int64_t part = r / n, e = r % n;
+    if (part == 0) spec[d * n + e] = s;
+    else dep[d * 9 * n + (part - 1) * 3 * n + n + e] = s;
+}
+
+// out[d][e] += spec[d][e] + sum_q tail(dep[d][.][q], (e - (q - n)) dE) over node slots q in [q_lo, q_hi).
+constexpr int kConvTile = 256;
+__global__ void __launch_bounds__(kConvTile) window_tail_kernel(const double *__restrict__ spec,
+                                                                const double *__restrict__ dep, int64_t n,
+                                                                double dE, int64_t q_lo, int64_t q_hi,
+                                                                double *__restrict__ out) {
+    __shared__ double s_m[3][kConvTile];
+    __shared__ double s_inv[2 * kConvTile];
+    const int d = blockIdx.y;
+    const int64_t e0 = (int64_t)blockIdx.x * kConvTile;
+    const int t = threadIdx.x;
+    const int64_t e = e0 + t;
+    const double *D = dep + (int64_t)d * 9 * n;
+    const int64_t L = 3 * n;
+    double acc = 0.0;
+    for (int64_t q0 = q_lo; q0 < q_hi; q0 += kConvTile) {
+        const int64_t q = q0 + t;
+        const bool in = q < q_hi;
+        s_m[0][t] = in ? D[q] : 0.0;
+        s_m[1][t] = in ? D[L + q] : 0.0;
+        s_m[2][t] = in ? D[2 * L + q] : 0.0;
+        // offsets e - (q - n) over the tile: o = (e0 - q0 + n) + (t_e - t_q), t_e - t_q in [-255, 255]
+        for (int k = t; k < 2 * kConvTile; k += kConvTile) {
+            const int64_t o = (e0 - q0 + n) + (int64_t)(k - (kConvTile - 1));
+            s_inv[k] = o == 0 ? 0.0 : 1.0 / ((double)o * dE);
+        }
+        __syncthreads();
+        const int m = q_hi - q0 < kConvTile ? (int)(q_hi - q0) : kConvTile;
+        for (int j = 0; j < m; ++j) {
+            const double inv = s_inv[t - j + (kConvTile - 1)];
+            acc += (s_m[0][j] + (s_m[1][j] + s_m[2][j] * inv) * inv) * inv * inv;
+        }
+        __syncthreads();
+    }
+    if (e < n) out[(int64_t)d * n + e] += spec[(int64_t)d * n + e] + acc;
+}
+
+__global__ void fill_zero_kernel(double *__restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
 }
 
 struct Workspace {
@@ -573,4 +813,112 @@ extern "C" hr_status hr_brem_reduce(const hr_brem_args *args, void *stream) {
                        (const double *)ws, gx, p.n_E, p.out);
     HR_CHECK_HIP(hipGetLastError());
     return HR_OK;
+}
+
+namespace hr {
+namespace {
+
+hr_status window_setup(int64_t n_E, double E0, double dE, double x_c, WindowGrid &G) {
+    HR_REQUIRE(n_E >= 1 && n_E <= kWinMaxBins, "windowed reduction: n_E %lld outside [1, %d]", (long long)n_E,
+               kWinMaxBins);
+    HR_REQUIRE(dE > 0.0 && isfinite(E0), "windowed reduction: needs a uniform ascending grid");
+    HR_REQUIRE(x_c > 0.0, "windowed reduction: x_c must be positive");
+    G.n = n_E;
+    G.E0 = E0;
+    G.dE = dE;
+    G.x_c = x_c;
+    return HR_OK;
+}
+
+// Launch window_sum + window_tail over `blocks` partials per direction; the
+// workspace holds [partials | spec | dep] and dep was zeroed before the deposit.
+hr_status window_finish(double *partial, int64_t blocks, int n_dir, const WindowGrid &G, double *spec, double *dep,
+                        int64_t q_lo, int64_t q_hi, double *out, hipStream_t s) {
+    const int64_t n = G.n;
+    const int64_t tot = (int64_t)n_dir * 4 * n;
+    hipLaunchKernelGGL(window_sum_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, partial, blocks, n,
+                       n_dir, spec, dep);
+    HR_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(window_tail_kernel, dim3((unsigned)((n + kConvTile - 1) / kConvTile), (unsigned)n_dir),
+                       dim3(kConvTile), 0, s, spec, dep, n, G.dE, q_lo, q_hi, out);
+    HR_CHECK_HIP(hipGetLastError());
+    return HR_OK;
+}
+
+}  // namespace
+}  // namespace hr
+
+extern "C" hr_status hr_window_reduce(const double *E_r, const double *a, const float *w, int64_t n_lines,
+                                      int64_t n_E, double E0, double dE, double x_c, double *out, void *stream) {
+    using namespace hr;
+    HR_REQUIRE(n_lines >= 0, "hr_window_reduce: negative n_lines");
+    if (n_lines == 0 || n_E == 0) return HR_OK;
+    HR_REQUIRE(E_r && a && w && out, "hr_window_reduce: null pointer");
+    WindowGrid G;
+    hr_status st = window_setup(n_E, E0, dE, x_c, G);
+    if (st != HR_OK) return st;
+    void *ws = nullptr;
+    int cu = 0;
+    st = workspace(0, ws, cu);
+    if (st != HR_OK) return st;
+    int64_t blocks = (n_lines + kWinThreads - 1) / kWinThreads;
+    blocks = blocks < 2 * (int64_t)cu ? blocks : 2 * (int64_t)cu;
+    const int64_t n = n_E;
+    const size_t need = (size_t)(blocks * 4 * n + n + 9 * n) * sizeof(double);
+    st = workspace(need, ws, cu);
+    if (st != HR_OK) return st;
+    double *partial = (double *)ws, *spec = partial + blocks * 4 * n, *dep = spec + n;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((9 * n + 255) / 256)), dim3(256), 0, s, dep, 9 * n);
+    HR_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(window_given_kernel, dim3((unsigned)blocks), dim3(kWinThreads), 0, s, E_r, a, w, n_lines, G,
+                       partial, dep);
+    HR_CHECK_HIP(hipGetLastError());
+    return window_finish(partial, blocks, 1, G, spec, dep, 0, 3 * n, out, s);
+}
+
+extern "C" hr_status hr_window_spectra(const hr_window_args *args, void *stream) {
+    using namespace hr;
+    HR_REQUIRE(args != nullptr && args->lines != nullptr, "hr_window_spectra: null args");
+    const hr_line_args &p = *args->lines;
+    HR_REQUIRE(p.n_seg >= 0 && p.n_dir >= 1 && p.n_g >= 1, "hr_window_spectra: bad sizes");
+    if (p.n_seg == 0 || args->n_E == 0) return HR_OK;
+    HR_REQUIRE(p.r_mid && p.v_hat && p.L && p.E && p.n_hat && p.dir_weight && p.g && p.wm && p.row && p.e_s &&
+                   p.e_p && p.E_tab && p.delta_tab && p.chi_re && p.chi_im && p.u_re && p.u_im && p.log_mu &&
+                   args->out,
+               "hr_window_spectra: null pointer");
+    HR_REQUIRE(p.n_tab >= 2 && p.n_mu >= 1, "hr_window_spectra: tables too small");
+    HR_REQUIRE(p.n_dir <= 65535, "hr_window_spectra: at most 65535 directions");
+    WindowGrid G;
+    hr_status st = window_setup(args->n_E, args->E0, args->dE, args->x_c, G);
+    if (st != HR_OK) return st;
+    void *ws = nullptr;
+    int cu = 0;
+    st = workspace(0, ws, cu);
+    if (st != HR_OK) return st;
+    const int64_t n = G.n;
+    // about two resident blocks per CU in total (one block fills a CU's LDS)
+    int64_t chunks = (2 * (int64_t)cu + p.n_dir - 1) / p.n_dir;
+    const int64_t max_chunks = (p.n_seg + kWinThreads - 1) / kWinThreads;
+    chunks = chunks < max_chunks ? chunks : max_chunks;
+    chunks = chunks < 1 ? 1 : chunks;
+    const size_t need = (size_t)((int64_t)p.n_dir * (chunks * 4 * n + n + 9 * n)) * sizeof(double);
+    st = workspace(need, ws, cu);
+    if (st != HR_OK) return st;
+    double *partial = (double *)ws;
+    double *spec = partial + (int64_t)p.n_dir * chunks * 4 * n;
+    double *dep = spec + (int64_t)p.n_dir * n;
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t nd9 = (int64_t)p.n_dir * 9 * n;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((nd9 + 255) / 256)), dim3(256), 0, s, dep, nd9);
+    HR_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(window_lines_kernel, dim3((unsigned)chunks, (unsigned)p.n_dir), dim3(kWinThreads), 0, s, p,
+                       G, partial, dep);
+    HR_CHECK_HIP(hipGetLastError());
+    // node slots reachable from the keep window
+    int64_t q_lo = (int64_t)floor((p.lo_keep - G.E0) / G.dE) - 1 + n;
+    int64_t q_hi = (int64_t)ceil((p.hi_keep - G.E0) / G.dE) + 2 + n;
+    q_lo = q_lo < 0 ? 0 : (q_lo > 3 * n ? 3 * n : q_lo);
+    q_hi = q_hi < q_lo ? q_lo : (q_hi > 3 * n ? 3 * n : q_hi);
+    return window_finish(partial, chunks, p.n_dir, G, spec, dep, q_lo, q_hi, args->out, s);
 }

@@ -201,7 +201,9 @@ __device__ __forceinline__ void store_lane(const hr_transport_args &a, const Lan
 }
 
 // One loop iteration of cores.py:408-1239 (per_electron, lut).  Returns the
-// exit code when the history ends, or -1 when it continues.
+// exit code when the history ends, or -1 when it continues.  kRows = false
+// never writes rows (tally-only callers), whatever a.cap says.
+template <bool kRows = true>
 __device__ int step_lane(const hr_transport_args &a, Lane &s) {
     const int nE = a.lut_n_energy;
     const int max_el = a.max_el;
@@ -339,7 +341,7 @@ __device__ int step_lane(const hr_transport_args &a, Lane &s) {
     else if (side) event = EV_EXIT_SIDE;
     else if (up || dn) event = EV_LAYER;
 
-    if (s.nseg < a.cap) {
+    if (kRows && s.nseg < a.cap) {
         const int64_t slot = (s.e - a.e_start) * a.cap + s.nseg;
         a.seg_dir[3 * slot] = dx;
         a.seg_dir[3 * slot + 1] = dy;

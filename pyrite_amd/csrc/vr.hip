@@ -68,26 +68,34 @@ __global__ void __launch_bounds__(256) vr_window_kernel(hr_vr_window_args p) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.n) return;
     int32_t count = 0;
-    double wc = 0.0;
+    double wc = 0.0, w2c = 0.0;
+    const double w2 = p.weight2 ? p.weight2[i] : 0.0;
     if (p.exit_code[i] == EXIT_SUSPENDED) {
         const double w = p.weight[i];
         double wt = exp(p.log_I_ref - (double)p.log_importance[i]);
         wt = fmin(fmax(wt, p.w_min), p.w_max);
         const double r = w / wt;
         const double u = stream_uniform(p.key[i], kDecisionCounter);
-        if (r >= 1.0) {
+        if (r > p.ratio_hi) {
             double fl = floor(r);
             if (u < r - fl) fl += 1.0;
             const double cap = (double)p.n_max;
             count = (int32_t)(fl < cap ? fl : cap);
             wc = w / (double)count;
+            w2c = w2;
+        } else if (r >= p.ratio_lo) {
+            count = 1;
+            wc = w;
+            w2c = w2;
         } else if (u < r) {
             count = 1;
             wc = wt;
+            w2c = w2 / r;
         }
     }
     p.count[i] = count;
     p.w_child[i] = wc;
+    if (p.w2_child) p.w2_child[i] = w2c;
 }
 
 // Children of parent i occupy [offset[i], offset[i] + count[i]) of the output bank.
@@ -102,6 +110,7 @@ __global__ void __launch_bounds__(256) vr_spawn_kernel(hr_vr_spawn_args p) {
     const double E = p.E[i], clock = p.clock[i], wc = p.w_child[i];
     const uint64_t key = p.key[i];
     const int64_t prim = p.prim[i];
+    const double w2c = p.w2_child ? p.w2_child[i] : 0.0;
     for (int32_t j = 0; j < c; ++j) {
         const int64_t k = o + j;
         p.out_pos[3 * k] = px;
@@ -115,6 +124,7 @@ __global__ void __launch_bounds__(256) vr_spawn_kernel(hr_vr_spawn_args p) {
         p.out_weight[k] = wc;
         p.out_key[k] = child_key(key, j);
         p.out_prim[k] = prim;
+        if (p.out_weight2) p.out_weight2[k] = j == 0 ? w2c : 0.0;
     }
 }
 
@@ -126,6 +136,7 @@ __global__ void __launch_bounds__(256) vr_gather_kernel(hr_vr_gather_args p) {
     const int64_t m = p.seg_count[i] < p.rows ? p.seg_count[i] : p.rows;
     const int64_t o = p.row_offset[i];
     const float w = (float)p.weight[i];
+    const float w2 = p.weight2 ? (float)p.weight2[i] : 0.0f;
     const int64_t prim = p.prim ? p.prim[i] : i;
     for (int64_t s = 0; s < m; ++s) {
         const int64_t src = i * p.rows + s, dst = o + s;
@@ -136,6 +147,7 @@ __global__ void __launch_bounds__(256) vr_gather_kernel(hr_vr_gather_args p) {
         p.L[dst] = (float)p.seg_len[src];
         p.E[dst] = (float)(0.5 * (p.seg_E[src] + p.seg_E_end[src]));
         p.row_weight[dst] = w;
+        if (p.row_weight2) p.row_weight2[dst] = w2;
         if (p.row_prim) p.row_prim[dst] = prim;
     }
 }
@@ -159,6 +171,24 @@ __global__ void __launch_bounds__(256) line_mass_kernel(const double *__restrict
         }
     }
     mass[j] = m;
+}
+
+// Segment j's lines survive with probability min(1, mass/m_thr), weight 1/p.
+__global__ void __launch_bounds__(256) line_roulette_kernel(float *__restrict__ w, int64_t n_seg, int32_t n_dir,
+                                                            int32_t n_g, const double *__restrict__ mass,
+                                                            double m_thr, uint64_t key, int64_t row0) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_seg) return;
+    const double m = mass[j];
+    if (m >= m_thr) return;
+    const double pkeep = m > 0.0 ? m / m_thr : 0.0;
+    const double u = stream_uniform(key, (uint64_t)(row0 + j));
+    const float scale = u < pkeep ? (float)(1.0 / pkeep) : 0.0f;
+    for (int d = 0; d < n_dir; ++d)
+        for (int g = 0; g < n_g; ++g) {
+            const int64_t k = ((int64_t)d * n_seg + j) * n_g + g;
+            w[k] = w[k] * scale;
+        }
 }
 
 unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
@@ -210,6 +240,8 @@ hr_status hr_vr_window(const hr_vr_window_args *p, void *stream) {
     HR_REQUIRE(p->exit_code && p->log_importance && p->weight && p->key && p->count && p->w_child,
                "hr_vr_window: null pointer");
     HR_REQUIRE(p->w_min > 0.0 && p->w_min <= p->w_max, "hr_vr_window: need 0 < w_min <= w_max");
+    HR_REQUIRE(p->ratio_lo > 0.0 && p->ratio_lo <= 1.0 && p->ratio_hi >= 1.0,
+               "hr_vr_window: need 0 < ratio_lo <= 1 <= ratio_hi");
     HR_REQUIRE(p->n_max >= 1 && p->n_max <= 1024, "hr_vr_window: n_max %d outside [1, 1024]", p->n_max);
     hipLaunchKernelGGL(vr_window_kernel, dim3(blocks_for(p->n, 256)), dim3(256), 0, (hipStream_t)stream, *p);
     HR_CHECK_HIP(hipGetLastError());
@@ -252,6 +284,19 @@ hr_status hr_line_mass(const double *E_r, const double *a, const float *w, int64
     HR_REQUIRE(dE_eV > 0.0, "hr_line_mass: dE must be positive");
     hipLaunchKernelGGL(line_mass_kernel, dim3(blocks_for(n_seg, 256)), dim3(256), 0, (hipStream_t)stream, E_r, a,
                        w, n_seg, n_dir, n_g, lo_eV, hi_eV, 1.0 / dE_eV, mass);
+    HR_CHECK_HIP(hipGetLastError());
+    return HR_OK;
+}
+
+hr_status hr_line_roulette(float *w, int64_t n_seg, int32_t n_dir, int32_t n_g, const double *mass, double m_thr,
+                           uint64_t key, int64_t row0, void *stream) {
+    using namespace hr;
+    HR_REQUIRE(n_seg >= 0 && n_dir >= 1 && n_g >= 1 && row0 >= 0, "hr_line_roulette: bad sizes");
+    if (n_seg == 0) return HR_OK;
+    HR_REQUIRE(w && mass, "hr_line_roulette: null pointer");
+    HR_REQUIRE(m_thr > 0.0, "hr_line_roulette: m_thr must be positive");
+    hipLaunchKernelGGL(line_roulette_kernel, dim3(blocks_for(n_seg, 256)), dim3(256), 0, (hipStream_t)stream, w,
+                       n_seg, n_dir, n_g, mass, m_thr, key, row0);
     HR_CHECK_HIP(hipGetLastError());
     return HR_OK;
 }

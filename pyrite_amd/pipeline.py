@@ -20,6 +20,7 @@ from . import _lib
 from .backend import require_device
 from .spectrum.geometry import cone_detector, tilted_geometry
 from .spectrum.lines import BremScorer, LineScorer, line_tables, segments_fp32
+from .transport.beam import BeamSpec, fill_entry
 from .transport.core import (
     DeviceTables,
     PerElectronTransportConfig,
@@ -56,6 +57,7 @@ class CaseConfig:
     seed: int = 1
     straggling: bool = True
     max_steps: int = 400_000
+    beam: BeamSpec = field(default_factory=BeamSpec)
 
     def grid(self):
         return bin_centers(self.E_lo_eV, self.E_hi_eV, self.n_bins)
@@ -110,10 +112,8 @@ class CasePipeline(CaseTables):
         cfg = self.cfg
         n = cfg.Ne
         _lib.call("hr_stream_keys", int(cfg.seed), int(batch_index) * n, n, self.keys.data_ptr(), None)
-        self.pos.zero_()
-        self.dirs.copy_(self.beam_d.expand(n, 3))
-        self.E.fill_(cfg.E0_keV)
-        self.clock.zero_()
+        fill_entry(cfg.beam, cfg.seed, int(batch_index) * n, n, cfg.E0_keV, np.deg2rad(cfg.tilt_deg),
+                   self.pos, self.dirs, self.E, self.clock)
         res = drive_batches(self.core, self.keys,
                             (self.alive, self.clock, self.pos, self.dirs, self.E, self.E_cut),
                             self.stragg, self.control, n, self.dev, self.tcfg)

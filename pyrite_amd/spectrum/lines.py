@@ -195,20 +195,10 @@ class LineScorer:
         self.record_events = False
         self.events = []  # ((start, end), cells) per reduction launch when recording
 
-    def lines(self, seg, scale, j0=0, n=None, row_weight=None):
-        """(E_r, a, w) device tensors for segments ``[j0, j0+n)`` of ``seg``.
-
-        ``row_weight`` (device FP32 ``[n_seg]``) multiplies each segment's lines:
-        the statistical weight of a split or rouletted particle.
-        """
-        import torch
-
+    def line_args(self, seg, scale, j0=0, n=None, row_weight=None):
+        """``hr_line_args`` for segments ``[j0, j0+n)`` of ``seg`` (outputs unset)."""
         n_total = int(seg["L"].shape[0])
         n = n_total - j0 if n is None else n
-        m = n * self.n_dir * len(self.tables.wm)
-        E_r = torch.empty(m, dtype=torch.float64, device=self.device)
-        a = torch.empty(m, dtype=torch.float64, device=self.device)
-        w = torch.empty(m, dtype=torch.float32, device=self.device)
         t = self.dtab
         args = _lib.LineArgs()
         args.n_seg = n
@@ -231,6 +221,23 @@ class LineScorer:
         args.lo_keep, args.hi_keep = self.tables.lo_keep, self.tables.hi_keep
         args.scale = float(scale)
         args.row_weight = None if row_weight is None else row_weight.data_ptr() + 4 * j0
+        return args
+
+    def lines(self, seg, scale, j0=0, n=None, row_weight=None):
+        """(E_r, a, w) device tensors for segments ``[j0, j0+n)`` of ``seg``.
+
+        ``row_weight`` (device FP32 ``[n_seg]``) multiplies each segment's lines:
+        the statistical weight of a split or rouletted particle.
+        """
+        import torch
+
+        n_total = int(seg["L"].shape[0])
+        n = n_total - j0 if n is None else n
+        m = n * self.n_dir * len(self.tables.wm)
+        E_r = torch.empty(m, dtype=torch.float64, device=self.device)
+        a = torch.empty(m, dtype=torch.float64, device=self.device)
+        w = torch.empty(m, dtype=torch.float32, device=self.device)
+        args = self.line_args(seg, scale, j0, n, row_weight)
         args.E_r, args.a, args.w = E_r.data_ptr(), a.data_ptr(), w.data_ptr()
         _lib.check(self.lib.hr_cxr_lines(C.byref(args), _stream()), "hr_cxr_lines")
         return E_r, a, w
@@ -268,6 +275,25 @@ class LineScorer:
             n = min(step, n_total - j0)
             E_r, a, w = self.lines(seg, 1.0 / float(Ne), j0, n, row_weight)
             self.reduce(E_r, a, w, out)
+        return out
+
+    def window_spectra(self, seg, Ne, out=None, x_c=100.0, row_weight=None):
+        """Per-direction (pixel) spectra ``[n_dir, n_E]`` by the fused prologue +
+        support-windowed route (``node_support_cutoff = x_c``,
+        ``spectrum/cxr/_node_window.py``); the lines are never stored.  Row ``d``
+        carries direction ``d``'s weight (its solid angle)."""
+        import torch
+
+        if not self.uniform:
+            raise ValueError("the windowed route needs a uniform energy grid")
+        if out is None:
+            out = torch.zeros((self.n_dir, self.E_grid.size), dtype=torch.float64, device=self.device)
+        largs = self.line_args(seg, 1.0 / float(Ne), 0, None, row_weight)
+        wa = _lib.WindowArgs()
+        wa.lines = C.pointer(largs)
+        wa.n_E, wa.E0, wa.dE, wa.x_c = int(self.E_grid.size), self.E0, self.dE, float(x_c)
+        wa.out = out.data_ptr()
+        _lib.check(self.lib.hr_window_spectra(C.byref(wa), _stream()), "hr_window_spectra")
         return out
 
 

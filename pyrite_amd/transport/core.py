@@ -258,13 +258,19 @@ def drive_batches(core, keys, state, stragg_dE, control: TransportControl, Ne: i
             sl = slice(e, e + m)
             snap = tuple(x[:m] for x in snap)
         keep = (torch.arange(cap, device=device)[None, :] < seg_count[:, None]).reshape(-1)
-        batches.append({k: v[keep] for k, v in rows.items()})
+        idx = torch.nonzero(keep).squeeze(1)  # one stream compaction shared by every column
+        batches.append({k: v.index_select(0, idx) for k, v in rows.items()})
         counts_all.append(seg_count)
         exit_all.append(exit_code)
         e += m
         if seen_max > 0:
             cap = _capacity_for(seen_max, config)
-    joined = {k: torch.cat([b[k] for b in batches]) for k in batches[0]} if batches else alloc_rows(0, 1, midpoint, device)
+    if len(batches) == 1:
+        joined = batches[0]
+    elif batches:
+        joined = {k: torch.cat([b[k] for b in batches]) for k in batches[0]}
+    else:
+        joined = alloc_rows(0, 1, midpoint, device)
     seg_count = torch.cat(counts_all) if counts_all else torch.zeros(0, dtype=torch.int64, device=device)
     exit_code = torch.cat(exit_all) if exit_all else torch.zeros(0, dtype=torch.int8, device=device)
     codes = torch.bincount(exit_code.to(torch.int64), minlength=6).cpu().numpy()

@@ -53,9 +53,14 @@ class VRConfig:
     rows_per_round: int = 2
     horizon_rows: int = 2
     k_max: float = 16.0  # smallest target weight is 1/k_max
+    ratio_lo: float = 0.5  # weight window band around the target weight
+    ratio_hi: float = 2.0
+    tail_fraction: float = 0.01  # below this share of the batch alive, rounds take tail_rows rows
+    tail_rows: int = 32
     n_max: int = 32  # copies per split
     ref_quantile: float = 0.995  # states at this predicted quantile get w_t = 1/k_max
     peak_half_width_eV: float = 400.0
+    roulette_keep_mass: float = 0.99  # rows at or above the mass carrying this fraction are scored surely
     label_eps_frac: float = 1.0e-3  # floor of the regression target, relative to the mean label
     train_rows: int = 2_000_000
     epochs: int = 6
@@ -76,7 +81,8 @@ class ParticleBank:
         self.dirs = torch.zeros((m, 3), dtype=torch.float64, device=device)
         self.E = torch.zeros(m, dtype=torch.float64, device=device)
         self.clock = torch.zeros(m, dtype=torch.float64, device=device)
-        self.weight = torch.zeros(m, dtype=torch.float64, device=device)
+        self.weight = torch.zeros(m, dtype=torch.float64, device=device)  # line tally weight
+        self.weight2 = torch.zeros(m, dtype=torch.float64, device=device)  # brem tally weight
         self.key = torch.zeros(m, dtype=torch.uint64, device=device)
         self.prim = torch.zeros(m, dtype=torch.int64, device=device)
         self.E_cut = torch.full((m,), float(E_cut_keV), dtype=torch.float64, device=device)
@@ -101,8 +107,10 @@ class LearnedImportance:
         self.window_bins = (grid >= self.window[0]) & (grid < self.window[1])
         self.net = net or ImportanceNet(init_params(np.random.default_rng(self.vcfg.seed)), pipe.dev)
         self.log_I_ref = None
+        self.m_thr_unit = None  # line-roulette threshold for a line scale of 1 (per-electron masses x Ne)
         self.train_history = []
         self._banks = None
+        self._rows_R = 0
         self.rounds = 0
 
     # -- case geometry ------------------------------------------------------
@@ -116,10 +124,10 @@ class LearnedImportance:
         n = np.asarray(p.n_hat, dtype=float)
         return float(1973.269804 * abs(v @ g) / (1.0 - v @ n))
 
-    def peak_tally(self, spec_lines, spec_brem_density):
-        """Photons per incident electron in the peak window (lines + brem over the cone)."""
-        m = self.window_bins
-        return float((spec_lines[m].sum() + self.pipe.omega * spec_brem_density[m].sum()) * self.dE)
+    def peak_tally(self, spec_lines):
+        """PXR/CBS line photons per incident electron in the peak window (the
+        brem continuum under the peak is a separate tally)."""
+        return float(spec_lines[self.window_bins].sum() * self.dE)
 
     # -- labels -------------------------------------------------------------
     def _features(self, n, pos, dirs, E, seg_len, out):
@@ -128,8 +136,9 @@ class LearnedImportance:
                   None if seg_len is None else seg_len.data_ptr(), float(cfg.E0_keV), float(cfg.thickness_ang),
                   float(cfg.tilt_deg), out.data_ptr(), _stream())
 
-    def row_masses(self, seg, Ne):
-        """Expected peak-window content of every row's lines (device FP64 ``[n_rows]``)."""
+    def row_masses(self, seg, Ne, window=True):
+        """Expected peak-window content of every row's lines (device FP64 ``[n_rows]``);
+        ``window=False``: their whole-grid content."""
         import torch
 
         lines = self.pipe.lines
@@ -140,9 +149,36 @@ class LearnedImportance:
         for j0 in range(0, n_total, step):
             n = min(step, n_total - j0)
             E_r, a, w = lines.lines(seg, 1.0 / float(Ne), j0, n)
-            _lib.call("hr_line_mass", E_r.data_ptr(), a.data_ptr(), w.data_ptr(), n, n_dir, n_g,
-                      self.window[0], self.window[1], self.dE, mass[j0:].data_ptr(), _stream())
+            self._mass(E_r, a, w, n, mass[j0:], window=window)
         return mass
+
+    def _mass(self, E_r, a, w, n, out, window=True):
+        lines = self.pipe.lines
+        lo, hi = self.window if window else (-1.0e300, 1.0e300)
+        _lib.call("hr_line_mass", E_r.data_ptr(), a.data_ptr(), w.data_ptr(), n, lines.n_dir,
+                  len(lines.tables.wm), lo, hi, self.dE, out.data_ptr(), _stream())
+
+    def score_lines(self, seg, Ne, out, row_weight=None, roulette_key=None):
+        """Dense line spectrum of ``seg`` into ``out``; with ``roulette_key`` the
+        rows below the mass threshold play line roulette first."""
+        import torch
+
+        lines = self.pipe.lines
+        n_total = int(seg["L"].shape[0])
+        n_dir, n_g = lines.n_dir, len(lines.tables.wm)
+        step = max(1, LINE_BLOCK // (n_dir * n_g))
+        mass = None
+        for j0 in range(0, n_total, step):
+            n = min(step, n_total - j0)
+            E_r, a, w = lines.lines(seg, 1.0 / float(Ne), j0, n, row_weight)
+            if roulette_key is not None:
+                if mass is None:
+                    mass = torch.empty(min(step, n_total), dtype=torch.float64, device=self.pipe.dev)
+                self._mass(E_r, a, w, n, mass, window=False)
+                _lib.call("hr_line_roulette", w.data_ptr(), n, n_dir, n_g, mass.data_ptr(),
+                          float(self.m_thr_unit) / float(Ne), int(roulette_key), int(self._row0 + j0), _stream())
+            lines.reduce(E_r, a, w, out)
+        self._row0 += n_total
 
     def labels_from_rows(self, rows, Ne):
         """``(x [n,6] FP32, y [n] FP64 horizon yield)`` of compacted analog rows."""
@@ -155,6 +191,7 @@ class LearnedImportance:
         }, dev)
         n = int(seg["L"].shape[0])
         mass = self.row_masses(seg, Ne)
+        self._last_full_mass = self.row_masses(seg, Ne, window=False) * float(Ne)
         x = torch.empty((n, 6), dtype=torch.float32, device=dev)
         self._features(n, rows["seg_mid"], rows["seg_dir"], rows["seg_E"], rows["seg_len"], x)
         # horizon sums within each history (rows are electron-major / step-minor)
@@ -177,17 +214,32 @@ class LearnedImportance:
         p = self.pipe
         g = torch.Generator(device=p.dev)
         g.manual_seed(int(self.vcfg.seed))
-        xs, ys = [], []
+        xs, ys, ms = [], [], []
         per_batch = max(1, int(self.vcfg.train_rows) // max(1, int(n_batches)))
         for b in range(int(n_batches)):
             res = p.transport(first_batch + b)
             x, y = self.labels_from_rows(res.rows, p.cfg.Ne)
+            ms.append(self._last_full_mass)
             if x.shape[0] > per_batch:
                 idx = torch.randperm(x.shape[0], generator=g, device=p.dev)[:per_batch]
                 x, y = x[idx], y[idx]
             xs.append(x)
             ys.append(y)
+        self.calibrate_roulette(torch.cat(ms))
         return torch.cat(xs), torch.cat(ys)
+
+    def calibrate_roulette(self, full_mass):
+        """Threshold of the line roulette: rows at or above it carry
+        ``roulette_keep_mass`` of the total line mass (per-electron units x Ne)."""
+        import torch
+
+        m = torch.sort(full_mass[full_mass > 0], descending=True).values
+        if m.numel() == 0:
+            self.m_thr_unit = 1.0
+            return
+        c = torch.cumsum(m, 0)
+        k = int(torch.searchsorted(c, float(self.vcfg.roulette_keep_mass) * float(c[-1])))
+        self.m_thr_unit = float(m[min(k, m.numel() - 1)])
 
     def fit(self, x, y):
         """Train the network on ``log(eps + y)`` and set the window reference."""
@@ -204,21 +256,34 @@ class LearnedImportance:
         self.net.rescale_output(max(sd, 1e-6), mu)
         self.train_history = hist
         pred = self.net.forward(x.contiguous())
-        q = torch.quantile(pred[: min(pred.numel(), 1 << 24)].double(), float(self.vcfg.ref_quantile))
-        self.log_I_ref = float(q) - float(np.log(self.vcfg.k_max))
+        self._pred_sample = pred[: min(pred.numel(), 1 << 24)].double()
+        self.calibrate()
         return hist
+
+    def calibrate(self, ref_quantile=None, k_max=None):
+        """Place the window: states at the ``ref_quantile`` of the predicted
+        importance over the training rows get the smallest target weight 1/k_max."""
+        import torch
+
+        if ref_quantile is not None:
+            self.vcfg.ref_quantile = float(ref_quantile)
+        if k_max is not None:
+            self.vcfg.k_max = float(k_max)
+        q = torch.quantile(self._pred_sample, float(self.vcfg.ref_quantile))
+        self.log_I_ref = float(q) - float(np.log(self.vcfg.k_max))
 
     # -- production ---------------------------------------------------------
     def _alloc(self, Ne):
         v = self.vcfg
         cap = int(Ne * min(v.k_max, v.n_max) * 1.25) + 4096
-        if self._banks is not None and self._banks[0].capacity >= cap:
+        R = int(v.rows_per_round)
+        if self._banks is not None and self._banks[0].capacity >= cap and self._rows_R == R:
             return
+        self._rows_R = R
         import torch
 
         dev = self.pipe.dev
         self._banks = [ParticleBank(cap, dev, self.pipe.cfg.E_cut_keV) for _ in range(2)]
-        R = int(v.rows_per_round)
         self._rows = alloc_rows(cap, R, True, dev)
         self._seg_count = torch.zeros(cap, dtype=torch.int64, device=dev)
         self._exit = torch.zeros(cap, dtype=torch.int8, device=dev)
@@ -234,6 +299,8 @@ class LearnedImportance:
             "E": torch.empty(m, dtype=torch.float32, device=dev),
         }
         self._row_w = torch.empty(m, dtype=torch.float32, device=dev)
+        self._row_w2 = torch.empty(m, dtype=torch.float32, device=dev)
+        self._w2c = torch.empty(cap, dtype=torch.float64, device=dev)
 
     def _window(self, n, bank, use_vr, n_max):
         a = _lib.VrWindowArgs()
@@ -244,13 +311,17 @@ class LearnedImportance:
         if use_vr:
             a.log_I_ref = float(self.log_I_ref)
             a.w_min, a.w_max = 1.0 / float(self.vcfg.k_max), 1.0
+            a.ratio_lo, a.ratio_hi = float(self.vcfg.ratio_lo), float(self.vcfg.ratio_hi)
         else:
             a.log_I_ref, a.w_min, a.w_max = 0.0, 1.0, 1.0
+            a.ratio_lo, a.ratio_hi = 1.0, 1.0
         a.n_max = int(n_max)
         a.count, a.w_child = self._count.data_ptr(), self._wc.data_ptr()
+        a.weight2, a.w2_child = bank.weight2.data_ptr(), self._w2c.data_ptr()
         _lib.check(self.lib.hr_vr_window(C.byref(a), _stream()), "hr_vr_window")
 
-    def run_batch(self, batch_index, Ne, Ne_total, spec_lines, brem_acc, use_vr=True, stats=None):
+    def run_batch(self, batch_index, Ne, Ne_total, spec_lines, brem_acc, use_vr=True, stats=None,
+                  roulette=False):
         """Histories ``[b*Ne, (b+1)*Ne)`` in rounds; adds weighted tallies
         (lines per electron of ``Ne_total``; brem as the raw sum) to the outputs.
 
@@ -272,22 +343,28 @@ class LearnedImportance:
         src.E[:Ne].fill_(p.cfg.E0_keV)
         src.clock[:Ne].zero_()
         src.weight[:Ne].fill_(1.0)
+        src.weight2[:Ne].fill_(1.0)
+        self._row0 = 0
+        rkey = _roulette_key(p.cfg.seed, batch_index) if roulette else None
         src.prim[:Ne].copy_(torch.arange(int(batch_index) * Ne, (int(batch_index) + 1) * Ne, device=p.dev))
         n = Ne
         rows = self._rows
+        tail_n = max(64, int(v.tail_fraction * Ne))
         while n > 0:
             self.rounds += 1
-            a = transport_args(p.tables, p.control, (0, n, R, src.key),
+            # the last stragglers take long rounds: fewer launches, decisions rarer where they matter least
+            Rr = R if n > tail_n else max(R, min(int(v.tail_rows), (cap * R) // n))
+            a = transport_args(p.tables, p.control, (0, n, Rr, src.key),
                                (None, src.clock, src.pos, src.dirs, src.E, src.E_cut), rows,
                                (self._seg_count, self._exit), (p.control.straggle_on, src.stragg))
-            _lib.check(self.lib.hr_vr_round(C.byref(a), R, _stream()), "hr_vr_round")
+            _lib.check(self.lib.hr_vr_round(C.byref(a), Rr, _stream()), "hr_vr_round")
             # rows -> weighted FP32 scoring records
-            m = self._seg_count[:n].clamp(max=R)
+            m = self._seg_count[:n].clamp(max=Rr)
             incl = torch.cumsum(m, 0)
             n_rows = int(incl[-1])
             off = incl - m
             g = _lib.VrGatherArgs()
-            g.n, g.rows = n, R
+            g.n, g.rows = n, Rr
             g.seg_count, g.row_offset = self._seg_count.data_ptr(), off.data_ptr()
             g.weight, g.prim = src.weight.data_ptr(), None
             g.seg_mid, g.seg_dir = rows["seg_mid"].data_ptr(), rows["seg_dir"].data_ptr()
@@ -297,8 +374,10 @@ class LearnedImportance:
             g.r_mid, g.v_hat, g.L, g.E = s["r_mid"].data_ptr(), s["v_hat"].data_ptr(), s["L"].data_ptr(), \
                 s["E"].data_ptr()
             g.row_weight, g.row_prim = self._row_w.data_ptr(), None
+            g.weight2, g.row_weight2 = src.weight2.data_ptr(), self._row_w2.data_ptr()
             _lib.check(self.lib.hr_vr_gather(C.byref(g), _stream()), "hr_vr_gather")
             if stats is not None:
+                stats["rows2"] = stats.get("rows2", 0.0) + float((m.to(torch.float64) * src.weight2[:n]).sum())
                 w = src.weight[:n]
                 stats["rows"] = stats.get("rows", 0.0) + float((m.to(torch.float64) * w).sum())
                 codes = self._exit[:n].to(torch.int64)
@@ -310,9 +389,8 @@ class LearnedImportance:
                 stats["rounds"] = stats.get("rounds", 0) + 1
             if n_rows:
                 seg = {k: t[:n_rows] for k, t in s.items()}
-                rw = self._row_w[:n_rows]
-                p.lines.spectrum(seg, Ne_total, spec_lines, row_weight=rw)
-                p.brem.reduce(seg, brem_acc, row_weight=rw)
+                self.score_lines(seg, Ne_total, spec_lines, self._row_w[:n_rows], rkey)
+                p.brem.reduce(seg, brem_acc, row_weight=self._row_w2[:n_rows])
             # split / roulette the suspended particles
             if use_vr:
                 self._features(n, src.pos, src.dirs, src.E, None, self._x)
@@ -336,6 +414,7 @@ class LearnedImportance:
                 sp.out_pos, sp.out_dirs, sp.out_E = dst.pos.data_ptr(), dst.dirs.data_ptr(), dst.E.data_ptr()
                 sp.out_clock, sp.out_weight = dst.clock.data_ptr(), dst.weight.data_ptr()
                 sp.out_key, sp.out_prim = dst.key.data_ptr(), dst.prim.data_ptr()
+                sp.w2_child, sp.out_weight2 = self._w2c.data_ptr(), dst.weight2.data_ptr()
                 _lib.check(self.lib.hr_vr_spawn(C.byref(sp), _stream()), "hr_vr_spawn")
             src, dst = dst, src
             n = total
@@ -344,9 +423,11 @@ class LearnedImportance:
     def measure(self, n_batches, Ne, mode="vr", first_batch=0):
         """Per-batch peak tallies and wall time of ``n_batches`` batches of ``Ne``.
 
-        ``mode``: ``"vr"`` (learned importance), ``"rounds"`` (the round
-        machinery with every window at weight 1: analog physics, VR overheads)
-        or ``"analog"`` (the plain ``CasePipeline`` transport + scoring).
+        ``mode``: ``"analog"`` (the plain ``CasePipeline`` transport + scoring),
+        ``"roulette"`` (analog plus line roulette by exact row mass),
+        ``"rounds"`` (the round machinery with every window at weight 1),
+        ``"vr"`` (learned splitting/roulette plus line roulette) or
+        ``"vr_split"`` (learned splitting/roulette only).
         Each batch is an independent estimate (tallies per electron of the batch).
         """
         import torch
@@ -360,24 +441,37 @@ class LearnedImportance:
         for b in range(int(n_batches)):
             lines = torch.zeros(n_bins, dtype=torch.float64, device=dev)
             brem = torch.zeros(n_bins, dtype=torch.float64, device=dev)
-            if mode == "analog":
+            if mode in ("analog", "roulette"):
                 res = p.transport(first_batch + b)
                 seg = segments_fp32({
                     "r_mid": res.rows["seg_mid"], "v_hat": res.rows["seg_dir"], "L_ang": res.rows["seg_len"],
                     "E_keV": res.rows["seg_E"],
                     "E_repr_keV": 0.5 * (res.rows["seg_E"] + res.rows["seg_E_end"]),
                 }, dev)
-                p.lines.spectrum(seg, Ne, lines)
+                if mode == "analog":
+                    p.lines.spectrum(seg, p.cfg.Ne, lines)
+                else:
+                    self._row0 = 0
+                    self.score_lines(seg, p.cfg.Ne, lines, None, _roulette_key(p.cfg.seed, first_batch + b))
                 p.brem.reduce(seg, brem)
+            elif mode in ("rounds", "vr", "vr_split"):
+                self.run_batch(first_batch + b, Ne, Ne, lines, brem, use_vr=(mode != "rounds"),
+                               roulette=(mode == "vr"))
             else:
-                self.run_batch(first_batch + b, Ne, Ne, lines, brem, use_vr=(mode == "vr"))
-            brem /= 4.0 * np.pi * float(Ne)
+                raise ValueError(f"unknown mode {mode!r}")
             m = torch.as_tensor(self.window_bins, device=dev)
-            vals.append((lines[m].sum() + p.omega * brem[m].sum()) * self.dE)
+            vals.append(lines[m].sum() * self.dE)
         vals = torch.stack(vals).cpu().numpy()
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
         return vals, elapsed
+
+
+def _roulette_key(seed, batch_index):
+    """Stream key of a batch's line-roulette draws (disjoint from transport keys)."""
+    from ..transport.beam import child_stream_root
+
+    return (child_stream_root(int(seed), 7, 6) + 0x9E3779B97F4A7C15 * (int(batch_index) + 1)) % (1 << 64)
 
 
 def figure_of_merit(vals, elapsed):

@@ -46,3 +46,45 @@ def test_product_does_not_import_oracle():
     for path in root.rglob("*.py"):
         src = path.read_text()
         assert "oracle" not in re.findall(r"^\s*(?:from|import)\s+(\w+)", src, flags=re.M), path
+
+
+STRUCTS = {
+    "hr_transport_args": "TransportArgs",
+    "hr_line_args": "LineArgs",
+    "hr_sincsq_args": "SincsqArgs",
+    "hr_brem_args": "BremArgs",
+    "hr_mlp": "MlpNet",
+    "hr_vr_window_args": "VrWindowArgs",
+    "hr_vr_spawn_args": "VrSpawnArgs",
+    "hr_vr_gather_args": "VrGatherArgs",
+    "hr_beam_args": "BeamArgs",
+    "hr_window_args": "WindowArgs",
+    "hr_queue_config": "QueueConfig",
+    "hr_queue_launch": "QueueLaunch",
+}
+
+
+def test_ctypes_layouts_match_the_c_header(tmp_path):
+    """Every field of every ctypes mirror sits at the C compiler's offset."""
+    import shutil
+    import subprocess
+
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "hiprite.h"', "int main(void) {"]
+    for cname, pyname in STRUCTS.items():
+        cls = getattr(_lib, pyname)
+        lines.append(f'printf("{pyname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{pyname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", f"-I{HEADER.parent}", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    for row in filter(None, out):
+        pyname, field, value = row.split()
+        cls = getattr(_lib, pyname)
+        got = ctypes.sizeof(cls) if field == "size" else getattr(cls, field).offset
+        assert got == int(value), (pyname, field, got, value)

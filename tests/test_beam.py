@@ -1,0 +1,77 @@
+"""Incident beam entry: numpy oracle pinned by closed forms (CPU), device vs oracle (GPU)."""
+
+import numpy as np
+import pytest
+
+from pyrite_amd.transport.beam import BeamSpec, TwissPlane, beam_args, child_stream_root
+
+
+def test_counter_uniforms_follow_the_transport_stream(oracle):
+    from oracle import beam
+
+    root = 12345
+    u = beam.counter_uniforms(root, 3, 2, start=5)
+    for i in range(3):
+        key = oracle.splitmix64((root + 0x9E3779B97F4A7C15 * (5 + i + 1)) % 2**64)
+        for c in range(2):
+            z = oracle.splitmix64((key + 0x9E3779B97F4A7C15 * (c + 1)) % 2**64)
+            assert u[i, c] == ((z >> 12) + 0.5) * 2.0**-52
+    # a block equals the slice of one larger draw
+    np.testing.assert_array_equal(beam.counter_uniforms(root, 10, 2)[5:8], u)
+
+
+def test_twiss_moments_and_projection():
+    from oracle import beam
+
+    E0 = 10_000.0
+    spec = BeamSpec(kind="twiss", twiss_x=TwissPlane.from_normalized(20.0, 2.0, -0.7, E0),
+                    twiss_y=TwissPlane.from_normalized(10.0, 1.0, 0.3, E0))
+    n = 200_000
+    a = beam_args(spec, 7, 0, n, E0, 0.0)
+    pos, dirs = beam.entry(a, n)
+    for p, pl, col in ((0, spec.twiss_x, 0), (1, spec.twiss_y, 1)):
+        eps = pl.sigma_pos_mm * pl.slope_scale_rad
+        beta = pl.sigma_pos_mm / pl.slope_scale_rad
+        x = pos[:, p] * 1e-7
+        xp = dirs[:, col] / dirs[:, 2]
+        assert np.var(x) == pytest.approx(eps * beta, rel=0.02)
+        assert np.var(xp) == pytest.approx(eps * (1 + pl.alpha**2) / beta, rel=0.02)
+        assert np.mean(x * xp) == pytest.approx(-eps * pl.alpha, rel=0.05)
+    # tilted face: entry points lie on z = 0 and stretch by 1/cos(tilt) along the tilt azimuth
+    spot = BeamSpec(kind="spot", fwhm_mm=1.0)
+    tilt = np.deg2rad(30.0)
+    a0 = beam_args(spot, 7, 0, n, E0, 0.0)
+    a1 = beam_args(spot, 7, 0, n, E0, tilt)
+    p0, _ = beam.entry(a0, n)
+    p1, d1 = beam.entry(a1, n)
+    assert np.std(p1[:, 0]) / np.std(p0[:, 0]) == pytest.approx(1 / np.cos(tilt), rel=1e-9)
+    assert np.std(p1[:, 1]) == pytest.approx(np.std(p0[:, 1]), rel=1e-9)
+    assert np.std(p0[:, 0]) * 1e-7 == pytest.approx(1.0 / 2.3548200450309493, rel=0.01)
+
+
+def test_child_roots_are_distinct():
+    assert len({child_stream_root(1, 2, 1), child_stream_root(1, 5, 4), child_stream_root(2, 5, 4)}) == 3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,tilt", [("pencil", 0.5), ("spot", 0.0), ("spot", 1.0), ("twiss", 1.0)])
+def test_device_entry_matches_oracle(device, kind, tilt):
+    import torch
+
+    from oracle import beam
+    from pyrite_amd.transport.beam import fill_entry
+
+    E0 = 10_000.0
+    spec = {"pencil": BeamSpec(), "spot": BeamSpec(kind="spot", fwhm_mm=2.3548),
+            "twiss": BeamSpec.gaussian_waist(1.0, 5e-4, E0)}[kind]
+    n, start = 100_000, 1_000_000
+    pos = torch.empty((n, 3), dtype=torch.float64, device=device)
+    dirs = torch.empty((n, 3), dtype=torch.float64, device=device)
+    E = torch.empty(n, dtype=torch.float64, device=device)
+    clock = torch.empty(n, dtype=torch.float64, device=device)
+    fill_entry(spec, 3, start, n, E0, np.deg2rad(tilt), pos, dirs, E, clock)
+    want_p, want_d = beam.entry(beam_args(spec, 3, start, n, E0, np.deg2rad(tilt)), n, start)
+    # erfcinv and scipy's ndtri agree to a few ulp; positions then inherit them
+    np.testing.assert_allclose(pos.cpu().numpy(), want_p, rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(dirs.cpu().numpy(), want_d, rtol=1e-14, atol=1e-15)
+    assert torch.all(E == E0) and torch.all(clock == 0)

@@ -49,6 +49,7 @@ def test_window_and_spawn_match_oracle(oracle, device):
     a.n = n
     a.exit_code, a.log_importance, a.weight, a.key = (d[k].data_ptr() for k in ("ec", "y", "w", "k"))
     a.log_I_ref, a.w_min, a.w_max, a.n_max = args
+    a.ratio_lo, a.ratio_hi = 1.0, 1.0
     a.count, a.w_child = count.data_ptr(), wc.data_ptr()
     _lib.check(_lib.load().hr_vr_window(C.byref(a), _stream()), "hr_vr_window")
     got_c = count.cpu().numpy()
@@ -80,11 +81,51 @@ def test_window_and_spawn_match_oracle(oracle, device):
     j = np.arange(total) - np.repeat(off.cpu().numpy(), got_c)
     np.testing.assert_array_equal(out["prim"].cpu().numpy(), parent)
     np.testing.assert_array_equal(out["pos"].cpu().numpy(), pos.cpu().numpy()[parent])
-    np.testing.assert_array_equal(out["w"].cpu().numpy(), want_w[parent])
+    np.testing.assert_array_equal(out["w"].cpu().numpy(), wc.cpu().numpy()[parent])
     ck = out["key"].cpu().numpy()
     sample = rng.choice(total, 500, replace=False)
     for s in sample:
         assert int(ck[s]) == oracle.vr_child_key(int(keys[parent[s]]), int(j[s]))
+
+
+def test_second_weight_and_line_roulette_match_oracle(oracle, device):
+    import torch
+
+    from pyrite_amd import _lib
+
+    rng = np.random.default_rng(4)
+    n = 20_000
+    ec = np.where(rng.random(n) < 0.8, 6, rng.integers(0, 6, n)).astype(np.int8)
+    y = rng.normal(0.0, 2.0, n).astype(np.float32)
+    w = np.exp(rng.uniform(np.log(1 / 64), 0.0, n))
+    w2 = rng.uniform(0.5, 2.0, n)
+    keys = rng.integers(0, 2**63, n, dtype=np.uint64)
+    want_c, want_w, want_w2 = oracle.vr_window(ec, y, w, keys, 0.7, 1 / 16, 1.0, 32, weight2=w2,
+                                               ratio_lo=0.5, ratio_hi=2.0)
+    d = {k: torch.as_tensor(v, device=device) for k, v in (("ec", ec), ("y", y), ("w", w), ("k", keys), ("w2", w2))}
+    count = torch.empty(n, dtype=torch.int32, device=device)
+    wc = torch.empty(n, dtype=torch.float64, device=device)
+    w2c = torch.empty(n, dtype=torch.float64, device=device)
+    a = _lib.VrWindowArgs()
+    a.n = n
+    a.exit_code, a.log_importance, a.weight, a.key = (d[k].data_ptr() for k in ("ec", "y", "w", "k"))
+    a.log_I_ref, a.w_min, a.w_max, a.n_max = 0.7, 1 / 16, 1.0, 32
+    a.ratio_lo, a.ratio_hi = 0.5, 2.0
+    a.count, a.w_child = count.data_ptr(), wc.data_ptr()
+    a.weight2, a.w2_child = d["w2"].data_ptr(), w2c.data_ptr()
+    _lib.check(_lib.load().hr_vr_window(C.byref(a), _stream()), "hr_vr_window")
+    np.testing.assert_array_equal(count.cpu().numpy(), want_c)
+    np.testing.assert_allclose(wc.cpu().numpy(), want_w, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(w2c.cpu().numpy(), want_w2, rtol=1e-15, atol=0)
+    # line roulette: survivors bit for bit
+    n_seg, n_dir, n_g = 5000, 3, 2
+    lw = rng.uniform(0.1, 1.0, n_seg * n_dir * n_g).astype(np.float32)
+    mass = rng.uniform(0.0, 3.0, n_seg)
+    want = oracle.line_roulette(lw, n_seg, n_dir, n_g, mass, 2.0, key=99, row0=1000)
+    lw_d = torch.as_tensor(lw, device=device)
+    m_d = torch.as_tensor(mass, device=device)
+    _lib.call("hr_line_roulette", lw_d.data_ptr(), n_seg, n_dir, n_g, m_d.data_ptr(), 2.0, 99, 1000, _stream())
+    np.testing.assert_array_equal(lw_d.cpu().numpy(), want)
 
 
 def test_line_mass_matches_oracle(oracle, device):
@@ -181,8 +222,7 @@ def _weighted_observables(vr, n_batches, Ne, use_vr, first):
         lines = torch.zeros(p.E_grid.size, dtype=torch.float64, device=p.dev)
         brem = torch.zeros_like(lines)
         vr.run_batch(first + b, Ne, Ne, lines, brem, use_vr=use_vr, stats=st)
-        brem /= 4.0 * np.pi * Ne
-        peak = vr.peak_tally(lines.cpu().numpy(), brem.cpu().numpy())
+        peak = vr.peak_tally(lines.cpu().numpy())
         out.append((st["rows"] / Ne, st.get("exit_2", 0.0) / Ne, st.get("exit_1", 0.0) / Ne, peak))
     return np.array(out)
 
@@ -205,8 +245,7 @@ def _analog_observables(pipe, vr, n_batches, first):
         }, pipe.dev)
         pipe.lines.spectrum(seg, Ne, lines)
         pipe.brem.reduce(seg, brem)
-        brem /= 4.0 * np.pi * Ne
-        peak = vr.peak_tally(lines.cpu().numpy(), brem.cpu().numpy())
+        peak = vr.peak_tally(lines.cpu().numpy())
         out.append((res.n_segments / Ne, (code == 2).mean(), (code == 1).mean(), peak))
     return np.array(out)
 

@@ -88,3 +88,56 @@ def test_line_mass_matches_offset_averaged_reduction(oracle):
                 acc += out.sum()
             tot += acc / offs.size
         assert mass[j] == pytest.approx(tot, rel=2e-3), j
+
+
+def test_second_weight_stays_with_copy_zero(oracle):
+    ec, y, w, keys = _inputs(20000, seed=5)
+    w2 = np.random.default_rng(6).uniform(0.5, 2.0, w.size)
+    count, wc, w2c = oracle.vr_window(ec, y, w, keys, 0.5, 1 / 16, 1.0, 32, weight2=w2)
+    wt = np.clip(np.exp(0.5 - y.astype(float)), 1 / 16, 1.0)
+    r = w / wt
+    split = (ec == 6) & (r >= 1)
+    np.testing.assert_array_equal(w2c[split], w2[split])
+    rou = (ec == 6) & (r < 1)
+    surv = rou & (count == 1)
+    np.testing.assert_allclose(w2c[surv], w2[surv] / r[surv], rtol=1e-15)
+    # expectation over the roulette: sum of surviving w2 vs sum of w2
+    sd = np.sqrt((w2[rou] ** 2 * (1 - r[rou]) / r[rou]).sum())
+    assert abs(w2c[rou].sum() - w2[rou].sum()) < 4 * sd
+
+
+def test_line_roulette_keeps_the_expectation(oracle):
+    rng = np.random.default_rng(8)
+    n_seg, n_dir, n_g = 400, 3, 2
+    w = rng.uniform(0.1, 1.0, n_seg * n_dir * n_g).astype(np.float32)
+    mass = np.where(rng.random(n_seg) < 0.3, rng.uniform(2.0, 5.0, n_seg), rng.uniform(0.0, 1.0, n_seg))
+    m_thr = 2.0
+    acc = np.zeros_like(w, dtype=float)
+    trials = 400
+    for t in range(trials):
+        out = oracle.line_roulette(w, n_seg, n_dir, n_g, mass, m_thr, key=1234 + t)
+        acc += out
+        sure = np.repeat(np.repeat(mass >= m_thr, n_g)[None, :], n_dir, 0).reshape(-1)
+        np.testing.assert_array_equal(out[sure], w[sure])
+    mean = acc / trials
+    # a row of mass m survives with p = m/m_thr; its weight's estimate has sd w sqrt((1-p)/p / trials)
+    p = np.clip(mass / m_thr, 1e-12, 1.0)
+    pl = np.repeat(np.repeat(p, n_g)[None, :], n_dir, 0).reshape(-1)
+    sd = w * np.sqrt((1 - pl) / pl / trials)
+    ok = pl > 0.05
+    assert np.all(np.abs(mean[ok] - w[ok]) < 5 * sd[ok] + 1e-6)
+
+
+def test_window_band_leaves_moderate_ratios_alone(oracle):
+    ec, y, w, keys = _inputs(20000, seed=9)
+    count, wc = oracle.vr_window(ec, y, w, keys, 0.5, 1 / 16, 1.0, 32, ratio_lo=0.5, ratio_hi=2.0)
+    wt = np.clip(np.exp(0.5 - y.astype(float)), 1 / 16, 1.0)
+    r = w / wt
+    mid = (ec == 6) & (r >= 0.5) & (r <= 2.0)
+    assert mid.sum() > 1000
+    np.testing.assert_array_equal(count[mid], 1)
+    np.testing.assert_array_equal(wc[mid], w[mid])
+    hi = (ec == 6) & (r > 2.0)
+    np.testing.assert_allclose(count[hi] * wc[hi], w[hi], rtol=1e-15)
+    lo = (ec == 6) & (r < 0.5)
+    assert set(np.unique(count[lo])) <= {0, 1}
