@@ -10,9 +10,14 @@ in HBM before the timed region, and every step transports new histories
 (global electron ids [s*Ne, (s+1)*Ne)).
 
 Prints ONE JSON line.  `roofline` is for the dominant kernel (the dense sinc^2
-reduction), timed with HIP events around its launches inside the timed region;
-`cpu_baseline` times the FP64 C oracle (the CPU restatement of the reference
-path, OpenMP over histories and bins) on a bounded sample of the same workload.
+reduction: one hr_sincsq_reduce call = sincsq_uniform_kernel plus its fixed-order
+partial-sum kernel), timed with HIP events around its launches inside the timed
+region; `cpu_baseline` times the FP64 C oracle (the CPU restatement of the
+reference path, OpenMP over histories and bins) on a bounded sample of the same
+workload.  `fom` trains the importance network on analog tallies of the same
+case and compares the PXR-peak figure of merit 1/(rel_err^2 t) of the learned
+splitting/roulette run with the plain analog run (independent batches of
+65536 histories; after the timed region, not part of `value`).
 """
 
 from __future__ import annotations
@@ -38,7 +43,39 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=8000, help="histories in the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--breakdown", action="store_true", help="print per-phase times to stderr")
+    p.add_argument("--no-fom", action="store_true", help="skip the learned-importance figure of merit")
+    p.add_argument("--fom-batches", type=int, default=24)
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="also run one 1 GiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     return p.parse_args()
+
+
+def learned_fom(pipe_cfg, n_batches):
+    """PXR-peak figure of merit of the learned importance vs the plain analog run."""
+    import torch
+
+    from pyrite_amd.pipeline import CaseConfig, CasePipeline
+    from pyrite_amd.vr.importance import LearnedImportance, VRConfig, figure_of_merit
+
+    Ne = 65536
+    pipe = CasePipeline(CaseConfig(**{**pipe_cfg.__dict__, "Ne": Ne}))
+    vr = LearnedImportance(pipe, VRConfig())
+    t0 = time.perf_counter()
+    x, y = vr.training_set(8, first_batch=10_000)
+    vr.fit(x, y)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    vr.measure(1, Ne, "analog", first_batch=900)
+    vr.measure(1, Ne, "vr", first_batch=901)
+    out = {"histories_per_batch": Ne, "batches": n_batches, "training_s": t_train,
+           "training_histories": 8 * Ne, "window_eV": list(vr.window)}
+    for mode, first in (("analog", 0), ("roulette", 5000), ("vr", 1000)):
+        vals, t = vr.measure(n_batches, Ne, mode, first_batch=first)
+        mean, rel, fom = figure_of_merit(vals, t)
+        out[mode] = {"mean_photons_per_electron": mean, "rel_err": rel, "time_s": t, "fom": fom}
+    out["gain_learned_vs_analog"] = out["vr"]["fom"] / out["analog"]["fom"]
+    out["gain_line_roulette_only"] = out["roulette"]["fom"] / out["analog"]["fom"]
+    return out
 
 
 def cpu_baseline(case_tables, n_hist):
@@ -136,7 +173,7 @@ def main():
             "parallelism": "single GPU",
         },
         "roofline": {
-            "kernel": "sincsq_uniform_kernel (dense sinc^2 line reduction)",
+            "kernel": "hr_sincsq_reduce: sincsq_uniform_kernel + sum_partials_kernel (dense sinc^2 lines)",
             "bound": "mfma",
             "pipe": "vector ALU, FP32 peak (the listed fp32 dense figure equals the VALU FP32 peak)",
             "achieved": achieved,
@@ -153,6 +190,13 @@ def main():
     }
     if args.breakdown:
         print(json.dumps({"phase_s": phase, "segments": pipe.segments}), file=sys.stderr)
+    if args.copy_calibration:
+        src = torch.empty(1 << 30, dtype=torch.uint8, device=pipe.dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+    if not args.no_fom:
+        out["fom"] = learned_fom(cfg, args.fom_batches)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pipe, args.cpu_sample)
         out["speedup_vs_cpu_baseline"] = value / out["cpu_baseline"]["value"]

@@ -111,6 +111,11 @@ typedef struct hr_transport_args {
     int8_t *exit_code;
     /* launch shape: 0 picks the tuned default */
     int32_t block_size;
+    /* transport precision (transport/_jit_precision.py): 0 FP64; 1 mixed --
+     * FP32 copies of the LUT columns below and FP32 table queries, ln E,
+     * sqrt(1 - cos^2) and azimuth sin/cos, FP64 state, draws and straggling */
+    int32_t precision;
+    const float *lut_total_rate_f, *lut_dEds_f, *lut_inv_beta_f, *lut_cdf_f, *lut_alpha_f;
 } hr_transport_args;
 
 hr_status hr_transport_run(const hr_transport_args *args, void *stream);
@@ -277,6 +282,7 @@ typedef struct hr_queue_launch {
     int32_t accumulate;             /* 0: the tally is zeroed first (by a kernel, graph-safe) */
     int32_t block_size;             /* 0 (256) or 64..256 in wave64 steps */
     int32_t per_claim;              /* consecutive items a lane claims at once, 1..1024 */
+    int32_t precision;              /* 0 FP64, 1 mixed (every configuration carries FP32 table copies) */
 } hr_queue_launch;
 hr_status hr_queue_run(const hr_queue_launch *launch, void *stream);
 /* The same launch captured once into a hipGraph (reset + queue kernel) and

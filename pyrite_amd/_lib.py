@@ -94,6 +94,9 @@ class TransportArgs(C.Structure):
         ("seg_count", _dp),
         ("exit_code", _dp),
         ("block_size", C.c_int32),
+        ("precision", C.c_int32),
+        ("lut_total_rate_f", _dp), ("lut_dEds_f", _dp), ("lut_inv_beta_f", _dp), ("lut_cdf_f", _dp),
+        ("lut_alpha_f", _dp),
     ]
 
 
@@ -172,6 +175,7 @@ class QueueLaunch(C.Structure):
         ("offsets", _dp), ("total", C.c_int64),
         ("tally", _dp),
         ("accumulate", C.c_int32), ("block_size", C.c_int32), ("per_claim", C.c_int32),
+        ("precision", C.c_int32),
     ]
 
 

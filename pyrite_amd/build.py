@@ -36,8 +36,8 @@ def _hipcc() -> str:
     raise RuntimeError("hipcc not found; set HIPCC or install ROCm")
 
 
-def _compile(src: Path) -> Path:
-    obj = BUILD / (src.stem + ".o")
+def _compile(src: Path, build_dir: Path = BUILD, extra=()) -> Path:
+    obj = build_dir / (src.stem + ".o")
     flags = [
         _hipcc(),
         "-c",
@@ -53,6 +53,7 @@ def _compile(src: Path) -> Path:
     ]
     if src.name in REPRODUCIBLE:
         flags.append("-ffp-contract=off")
+    flags += list(extra)
     flags += [str(src), "-o", str(obj)]
     res = subprocess.run(flags, capture_output=True, text=True)
     if res.returncode != 0:
@@ -60,22 +61,28 @@ def _compile(src: Path) -> Path:
     return obj
 
 
-def build(verbose: bool = True) -> Path:
-    """Compile every unit and link the shared library; returns its path."""
-    BUILD.mkdir(exist_ok=True)
-    LIBDIR.mkdir(exist_ok=True)
+def build(verbose: bool = True, out: Path | None = None, extra_flags=()) -> Path:
+    """Compile every unit and link the shared library; returns its path.
+
+    ``out`` and ``extra_flags`` build a variant elsewhere (e.g. with
+    ``-DHR_TRANSPORT_WAVES_PER_EU=3``) without touching the in-tree library."""
+    lib = LIB if out is None else Path(out)
+    build_dir = BUILD if out is None else lib.parent / "_build"
+    build_dir.mkdir(parents=True, exist_ok=True)
+    lib.parent.mkdir(parents=True, exist_ok=True)
     sources = sorted(CSRC.glob("*.hip"))
     with ThreadPoolExecutor(max_workers=min(8, len(sources))) as pool:
-        objs = list(pool.map(_compile, sources))
-    tmp = LIB.with_suffix(".so.tmp")
+        objs = list(pool.map(lambda src: _compile(src, build_dir, extra_flags), sources))
+    LIB_ = lib
+    tmp = LIB_.with_suffix(".so.tmp")
     cmd = [_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, LIB_)
     if verbose:
-        print(f"built {LIB}")
-    return LIB
+        print(f"built {LIB_}")
+    return LIB_
 
 
 if __name__ == "__main__":

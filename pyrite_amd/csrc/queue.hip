@@ -41,6 +41,7 @@ __device__ __forceinline__ void tally_history(double *__restrict__ t, const Lane
     atomicAdd(&t[11], E0 - (escaped ? s.E : 0.0));
 }
 
+template <bool kMixed>
 __global__ void __launch_bounds__(256) queue_kernel(const hr_queue_config *__restrict__ cfg, int32_t n_cfg,
                                                     const int64_t *__restrict__ off_g, int64_t total,
                                                     double *__restrict__ tally, unsigned long long *next,
@@ -97,7 +98,7 @@ __global__ void __launch_bounds__(256) queue_kernel(const hr_queue_config *__res
         }
         if (__ballot(live || !drained) == 0) break;
         if (live) {
-            const int code = step_lane<false>(cfg[c].t, s);
+            const int code = step_lane<false, kMixed>(cfg[c].t, s);
             if (code >= 0) {
                 tally_history(tally + (int64_t)c * kTally, s, code, cfg[c].E0_keV);
                 live = false;
@@ -141,6 +142,7 @@ hr_status queue_check(const hr_queue_launch *L) {
     HR_REQUIRE(L->per_claim >= 1 && L->per_claim <= 1024, "queue: per_claim %d outside [1, 1024]", L->per_claim);
     HR_REQUIRE(L->block_size == 0 || (L->block_size % 64 == 0 && L->block_size >= 64 && L->block_size <= 256),
                "queue: block_size must be 64, 128, 192 or 256 (or 0)");
+    HR_REQUIRE(L->precision == 0 || L->precision == 1, "queue: precision must be 0 (fp64) or 1 (mixed)");
     return HR_OK;
 }
 
@@ -153,13 +155,17 @@ hr_status queue_enqueue(const hr_queue_launch *L, unsigned long long *counter, i
     HR_CHECK_HIP(hipGetLastError());
     if (L->total == 0) return HR_OK;
     int occ = 0;
-    HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, queue_kernel, block, 0));
+    HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, queue_kernel<false>, block, 0));
     occ = occ > 0 ? occ : 1;
     const int64_t need = (L->total / L->per_claim + block) / block + 1;
     const int64_t resident = (int64_t)cu * occ;
     const int64_t grid = need < resident ? need : resident;
-    hipLaunchKernelGGL(queue_kernel, dim3((unsigned)grid), dim3(block), 0, s, L->configs, L->n_cfg, L->offsets,
-                       L->total, L->tally, counter, L->per_claim);
+    if (L->precision == 1)
+        hipLaunchKernelGGL(queue_kernel<true>, dim3((unsigned)grid), dim3(block), 0, s, L->configs, L->n_cfg,
+                           L->offsets, L->total, L->tally, counter, L->per_claim);
+    else
+        hipLaunchKernelGGL(queue_kernel<false>, dim3((unsigned)grid), dim3(block), 0, s, L->configs, L->n_cfg,
+                           L->offsets, L->total, L->tally, counter, L->per_claim);
     HR_CHECK_HIP(hipGetLastError());
     return HR_OK;
 }

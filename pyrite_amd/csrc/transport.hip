@@ -37,7 +37,16 @@ __device__ __forceinline__ int64_t claim(unsigned long long *next, bool want, in
     return id < limit ? id : -1;
 }
 
-__global__ void __launch_bounds__(256) transport_kernel(hr_transport_args a,
+// Optional occupancy target (waves per SIMD) for tuning builds; the default
+// lets the compiler keep the whole step in registers.
+#ifdef HR_TRANSPORT_WAVES_PER_EU
+#define HR_TRANSPORT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(HR_TRANSPORT_WAVES_PER_EU, 8)))
+#else
+#define HR_TRANSPORT_OCCUPANCY
+#endif
+
+template <bool kMixed>
+__global__ void __launch_bounds__(256) HR_TRANSPORT_OCCUPANCY transport_kernel(hr_transport_args a,
                                                         unsigned long long *__restrict__ next) {
     Lane s;
     bool live = false;
@@ -64,7 +73,7 @@ __global__ void __launch_bounds__(256) transport_kernel(hr_transport_args a,
         }
         if (__ballot(live || !drained) == 0) break;
         if (live) {
-            const int code = step_lane(a, s);
+            const int code = step_lane<true, kMixed>(a, s);
             if (code >= 0) {
                 store_lane(a, s, (int8_t)code);
                 live = false;
@@ -93,7 +102,7 @@ hr_status device_work(DeviceWork *&w) {
         HR_CHECK_HIP(hipGetDeviceProperties(&prop, dev));
         w->cu = prop.multiProcessorCount;
         int blocks = 0;
-        HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, transport_kernel, 256, 0));
+        HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, transport_kernel<false>, 256, 0));
         w->occupancy = blocks > 0 ? blocks : 1;
     }
     return HR_OK;
@@ -122,6 +131,10 @@ extern "C" hr_status hr_transport_run(const hr_transport_args *args, void *strea
     HR_REQUIRE(a.cap == 0 || a.energy_model == 0 ||
                    (a.seg_E_end && a.seg_t_end && a.seg_flight && a.seg_substep && a.seg_event),
                "hr_transport_run: midpoint rows need E_end/t_end/flight/substep/event");
+    HR_REQUIRE(a.precision == 0 || a.precision == 1, "hr_transport_run: precision must be 0 (fp64) or 1 (mixed)");
+    HR_REQUIRE(a.precision == 0 || (a.lut_total_rate_f && a.lut_dEds_f && a.lut_inv_beta_f && a.lut_cdf_f &&
+                                    a.lut_alpha_f),
+               "hr_transport_run: mixed precision needs the FP32 table copies");
     DeviceWork *w = nullptr;
     hr_status st = device_work(w);
     if (st != HR_OK) return st;
@@ -133,13 +146,16 @@ extern "C" hr_status hr_transport_run(const hr_transport_args *args, void *strea
     int occupancy = w->occupancy;
     if (block != 256) {
         int blocks = 0;
-        HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, transport_kernel, block, 0));
+        HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, transport_kernel<false>, block, 0));
         occupancy = blocks > 0 ? blocks : 1;
     }
     const int64_t need = (a.e_count + block - 1) / block;
     const int64_t resident = (int64_t)w->cu * occupancy;
     const int64_t grid = need < resident ? need : resident;
-    hipLaunchKernelGGL(transport_kernel, dim3((unsigned)grid), dim3(block), 0, s, a, w->counter);
+    if (a.precision == 1)
+        hipLaunchKernelGGL(transport_kernel<true>, dim3((unsigned)grid), dim3(block), 0, s, a, w->counter);
+    else
+        hipLaunchKernelGGL(transport_kernel<false>, dim3((unsigned)grid), dim3(block), 0, s, a, w->counter);
     HR_CHECK_HIP(hipGetLastError());
     return HR_OK;
 }

@@ -57,6 +57,55 @@ __device__ __forceinline__ double lerp(const double *__restrict__ t, LutPos p) {
     return lo + p.f * (t[p.i + 1] - lo);
 }
 
+// Mixed precision (transport/_jit_precision.py:1-40): the table query ln E and
+// the interpolation run in FP32 on FP32 table copies; the result is widened.
+__device__ __forceinline__ LutPos lut_pos_f(double E, double log_E_min, double inv_dlogE, int n_energy) {
+    const float x = (logf((float)E) - (float)log_E_min) * (float)inv_dlogE;
+    const int last = n_energy - 1;
+    LutPos p;
+    if (x >= (float)last) {
+        p.i = last - 1;
+        p.f = 1.0;
+    } else if (x > 0.0f) {
+        p.i = (int)x;
+        p.f = (double)(x - (float)p.i);
+    } else {
+        p.i = 0;
+        p.f = 0.0;
+    }
+    return p;
+}
+
+__device__ __forceinline__ double lerp_f(const float *__restrict__ t, LutPos p) {
+    const float lo = t[p.i];
+    return (double)(lo + (float)p.f * (t[p.i + 1] - lo));
+}
+
+// Table access of one precision: kMixed selects the FP32 copies and FP32 queries.
+template <bool kMixed>
+struct Lut {
+    const hr_transport_args &a;
+    __device__ __forceinline__ LutPos pos(double E) const {
+        return kMixed ? lut_pos_f(E, a.lut_log_E_min, a.lut_inv_dlogE, a.lut_n_energy)
+                      : lut_pos(E, a.lut_log_E_min, a.lut_inv_dlogE, a.lut_n_energy);
+    }
+    __device__ __forceinline__ double rate(size_t off, LutPos p) const {
+        return kMixed ? lerp_f(a.lut_total_rate_f + off, p) : lerp(a.lut_total_rate + off, p);
+    }
+    __device__ __forceinline__ double dEds(size_t off, LutPos p) const {
+        return kMixed ? lerp_f(a.lut_dEds_f + off, p) : lerp(a.lut_dEds + off, p);
+    }
+    __device__ __forceinline__ double inv_beta(LutPos p) const {
+        return kMixed ? lerp_f(a.lut_inv_beta_f, p) : lerp(a.lut_inv_beta, p);
+    }
+    __device__ __forceinline__ double cdf(size_t off, LutPos p) const {
+        return kMixed ? lerp_f(a.lut_cdf_f + off, p) : lerp(a.lut_cdf + off, p);
+    }
+    __device__ __forceinline__ double alpha(size_t off, LutPos p) const {
+        return kMixed ? lerp_f(a.lut_alpha_f + off, p) : lerp(a.lut_alpha + off, p);
+    }
+};
+
 // straggling.py:305-329, one element of the spliced stopping, delta = 0.
 __device__ double element_dEds(double J, double k, double coeff, double E_cross, double E) {
     const double tau = E / kMc2;
@@ -202,9 +251,13 @@ __device__ __forceinline__ void store_lane(const hr_transport_args &a, const Lan
 
 // One loop iteration of cores.py:408-1239 (per_electron, lut).  Returns the
 // exit code when the history ends, or -1 when it continues.  kRows = false
-// never writes rows (tally-only callers), whatever a.cap says.
-template <bool kRows = true>
+// never writes rows (tally-only callers), whatever a.cap says.  kMixed is the
+// reference's "mixed" transport precision: FP32 table lookups, query
+// logarithms, sqrt(1 - cos^2) and azimuth sin/cos; every accumulated quantity,
+// free-path draw and the straggling stay FP64.
+template <bool kRows = true, bool kMixed = false>
 __device__ int step_lane(const hr_transport_args &a, Lane &s) {
+    const Lut<kMixed> lut{a};
     const int nE = a.lut_n_energy;
     const int max_el = a.max_el;
     int L = 0;
@@ -220,10 +273,9 @@ __device__ int step_lane(const hr_transport_args &a, Lane &s) {
     const int n_el = a.L_nel[L];
     const double z_top = a.L_top[L], z_bot = a.L_bot[L];
     const double E = s.E;
-    const LutPos p = lut_pos(E, a.lut_log_E_min, a.lut_inv_dlogE, nE);
-    const double *rate_row = a.lut_total_rate + (size_t)L * nE;
-    const double *dEds_row = a.lut_dEds + (size_t)L * nE;
-    const double total_rate = lerp(rate_row, p);
+    const LutPos p = lut.pos(E);
+    const size_t row_off = (size_t)L * nE;
+    const double total_rate = lut.rate(row_off, p);
     const double lam = 1e8 / total_rate;
     if (s.tau < 0.0) {
         s.tau = -log(stream_uniform(s.key, s.draws));
@@ -258,8 +310,8 @@ __device__ int step_lane(const hr_transport_args &a, Lane &s) {
     }
     bool exit_top = up && z_top <= 0.0;
     bool exit_bot = dn && z_bot >= a.z_total;
-    const double dEds = lerp(dEds_row, p);
-    const double inv_beta_j = lerp(a.lut_inv_beta, p);
+    const double dEds = lut.dEds(row_off, p);
+    const double inv_beta_j = lut.inv_beta(p);
     bool cutoff = false, limited = false;
     bool geometry_event = up || dn || side;
     double E_end, t_end;
@@ -292,16 +344,16 @@ __device__ int step_lane(const hr_transport_args &a, Lane &s) {
             E_end = E - loss;
         }
         if (midpoint) {
-            const LutPos c = lut_pos(0.5 * (E + E_end), a.lut_log_E_min, a.lut_inv_dlogE, nE);
-            t_end = s.clock + step * lerp(a.lut_inv_beta, c);
+            const LutPos c = lut.pos(0.5 * (E + E_end));
+            t_end = s.clock + step * lut.inv_beta(c);
         } else {
             t_end = s.clock + step * inv_beta_j;
         }
     } else {
         double cut_dist;
         if (midpoint) {
-            const LutPos c = lut_pos(0.5 * (E + s.E_cut), a.lut_log_E_min, a.lut_inv_dlogE, nE);
-            cut_dist = (s.E_cut - E) / lerp(dEds_row, c);
+            const LutPos c = lut.pos(0.5 * (E + s.E_cut));
+            cut_dist = (s.E_cut - E) / lut.dEds(row_off, c);
         } else {
             cut_dist = (s.E_cut - E) / dEds;
         }
@@ -323,11 +375,11 @@ __device__ int step_lane(const hr_transport_args &a, Lane &s) {
                 E_end = s.E_cut;
             } else {
                 const double E_pred = E + dEds * step;
-                const LutPos m = lut_pos(0.5 * (E + E_pred), a.lut_log_E_min, a.lut_inv_dlogE, nE);
-                E_end = E + step * lerp(dEds_row, m);
+                const LutPos m = lut.pos(0.5 * (E + E_pred));
+                E_end = E + step * lut.dEds(row_off, m);
             }
-            const LutPos c = lut_pos(0.5 * (E + E_end), a.lut_log_E_min, a.lut_inv_dlogE, nE);
-            t_end = s.clock + step * lerp(a.lut_inv_beta, c);
+            const LutPos c = lut.pos(0.5 * (E + E_end));
+            t_end = s.clock + step * lut.inv_beta(c);
         } else {
             E_end = cutoff ? s.E_cut : E + dEds * step;
             t_end = s.clock + step * inv_beta_j;
@@ -393,25 +445,34 @@ __device__ int step_lane(const hr_transport_args &a, Lane &s) {
         s.draws += 1ULL;
         i_el = n_el - 1;
         for (int k = 0; k < n_el; ++k) {
-            if (lerp(a.lut_cdf + (el_row + k) * nE, p) > u) {
+            if (lut.cdf((el_row + k) * nE, p) > u) {
                 i_el = k;
                 break;
             }
         }
     }
-    const LutPos q = lut_pos(s.E, a.lut_log_E_min, a.lut_inv_dlogE, nE);
-    const double alpha = lerp(a.lut_alpha + (el_row + i_el) * nE, q);
+    const LutPos q = lut.pos(s.E);
+    const double alpha = lut.alpha((el_row + i_el) * nE, q);
     const double cos_u = stream_uniform(s.key, s.draws);
     const double phi_u = stream_uniform(s.key, s.draws + 1ULL);
     s.draws += 2ULL;
     const double cos_t = 1.0 - 2.0 * alpha * cos_u / (1.0 + alpha - cos_u);
     // core_geometry.py:16-36
     const double s2 = 1.0 - cos_t * cos_t;
-    const double sin_t = sqrt(s2 > 0.0 ? s2 : 0.0);
-    double sin_phi, cos_phi;
+    double sin_t, sin_phi, cos_phi;
     const double phi = 2.0 * kPi * phi_u;
-    sin_phi = sin(phi);
-    cos_phi = cos(phi);
+    if (kMixed) {
+        const float s2f = (float)s2;
+        sin_t = (double)sqrtf(s2f > 0.0f ? s2f : 0.0f);
+        float sf, cf;
+        sincosf((float)phi, &sf, &cf);
+        sin_phi = (double)sf;
+        cos_phi = (double)cf;
+    } else {
+        sin_t = sqrt(s2 > 0.0 ? s2 : 0.0);
+        sin_phi = sin(phi);
+        cos_phi = cos(phi);
+    }
     double refx, refy;
     if (fabs(dx) < 0.9) { refx = 1.0; refy = 0.0; } else { refx = 0.0; refy = 1.0; }
     double ux = -dz * refy, uy = dz * refx, uz = dx * refy - dy * refx;

@@ -30,6 +30,7 @@ __host__ __device__ __forceinline__ uint64_t child_key(uint64_t key, int64_t j) 
 }
 
 // One particle per thread: at most `rows` steps, then suspend.
+template <bool kMixed>
 __global__ void __launch_bounds__(256) vr_round_kernel(hr_transport_args a, int32_t rows) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.e_count) return;
@@ -37,7 +38,7 @@ __global__ void __launch_bounds__(256) vr_round_kernel(hr_transport_args a, int3
     Lane s;
     load_lane(a, e, s);
     int code = -1;
-    while (code < 0 && s.nseg < rows) code = step_lane(a, s);
+    while (code < 0 && s.nseg < rows) code = step_lane<true, kMixed>(a, s);
     store_lane(a, s, code < 0 ? (int8_t)EXIT_SUSPENDED : (int8_t)code);
 }
 
@@ -214,8 +215,15 @@ hr_status hr_vr_round(const hr_transport_args *a, int32_t rows, void *stream) {
     HR_REQUIRE(!a->straggle_on || a->stragg_dE, "hr_vr_round: straggling needs stragg_dE");
     HR_REQUIRE(a->seg_count && a->exit_code, "hr_vr_round: null seg_count/exit_code");
     HR_REQUIRE(a->e_count < (1LL << 40), "hr_vr_round: too many particles");
-    hipLaunchKernelGGL(vr_round_kernel, dim3(blocks_for(a->e_count, 256)), dim3(256), 0, (hipStream_t)stream, *a,
-                       rows);
+    HR_REQUIRE(a->precision == 0 || (a->precision == 1 && a->lut_total_rate_f && a->lut_dEds_f &&
+                                      a->lut_inv_beta_f && a->lut_cdf_f && a->lut_alpha_f),
+               "hr_vr_round: precision 0, or 1 with the FP32 table copies");
+    if (a->precision == 1)
+        hipLaunchKernelGGL(vr_round_kernel<true>, dim3(blocks_for(a->e_count, 256)), dim3(256), 0,
+                           (hipStream_t)stream, *a, rows);
+    else
+        hipLaunchKernelGGL(vr_round_kernel<false>, dim3(blocks_for(a->e_count, 256)), dim3(256), 0,
+                           (hipStream_t)stream, *a, rows);
     HR_CHECK_HIP(hipGetLastError());
     return HR_OK;
 }

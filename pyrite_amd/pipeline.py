@@ -38,7 +38,10 @@ def bin_centers(lo_eV, hi_eV, n):
 
 @dataclass
 class CaseConfig:
-    """One plain-method case (defaults: 5 MeV pencil beam on 100 um HOPG, (002))."""
+    """One plain-method case (defaults: 5 MeV pencil beam on 100 um HOPG, (002)).
+
+    ``silicon_1mm_10MeV()`` builds the 1 mm Si (111) case with a Gaussian
+    Twiss beam, 16 reflections and 4096 bins."""
 
     E0_keV: float = 5000.0
     Ne: int = 100_000
@@ -58,9 +61,28 @@ class CaseConfig:
     straggling: bool = True
     max_steps: int = 400_000
     beam: BeamSpec = field(default_factory=BeamSpec)
+    precision: str = "fp64"  # transport precision: "fp64" or "mixed"
+    line_route: str = "dense"  # "dense" sinc^2 over every bin, or "window" (support window + far tail)
+    x_c: float = 100.0  # support cutoff of the windowed route
 
     def grid(self):
         return bin_centers(self.E_lo_eV, self.E_hi_eV, self.n_bins)
+
+    @staticmethod
+    def silicon_1mm_10MeV(Ne=100_000, **kw):
+        """10 MeV Gaussian beam (sigma 1 mm, divergence 0.5 mrad) on 1 mm Si with
+        (111) tilted 1 deg; PXR/CBS over the 16 shortest allowed reflections
+        and brem into 4096 bins over 1-100 keV; mixed transport precision and
+        the windowed line route."""
+        from .spectrum.crystal import allowed_reflections
+
+        base = dict(E0_keV=10_000.0, Ne=Ne, thickness_ang=1.0e7, composition=[("Si", 0.04994)],
+                    crystal="silicon", hkl_list=allowed_reflections("silicon", 16), tilt_deg=1.0,
+                    n_bins=4096, E_lo_eV=1000.0, E_hi_eV=100_000.0,
+                    beam=BeamSpec.gaussian_waist(1.0, 5.0e-4, 10_000.0), precision="mixed",
+                    line_route="window")
+        base.update(kw)
+        return CaseConfig(**base)
 
 
 class CaseTables:
@@ -87,7 +109,8 @@ class CasePipeline(CaseTables):
         dev = self.dev
         self.tcfg = transport_config or PerElectronTransportConfig()
         self.tables = DeviceTables(self.layers, self.lut, dev)
-        self.control = TransportControl(max_steps=cfg.max_steps, energy_model=1, straggle_on=cfg.straggling)
+        self.control = TransportControl(max_steps=cfg.max_steps, energy_model=1, straggle_on=cfg.straggling,
+                                        precision=cfg.precision)
         self.core = make_hip_transport_core(self.tables, self.control)
         self.lines = LineScorer(self.line_tables, self.E_grid, cfg.thickness_ang, self.n_hats, self.weights, dev)
         self.brem = BremScorer(cfg.composition, self.E_grid, cfg.thickness_ang, self.n_hat, dev)
@@ -124,7 +147,13 @@ class CasePipeline(CaseTables):
             "r_mid": rows["seg_mid"], "v_hat": rows["seg_dir"], "L_ang": rows["seg_len"],
             "E_keV": rows["seg_E"], "E_repr_keV": 0.5 * (rows["seg_E"] + rows["seg_E_end"]),
         }, self.dev)
-        self.lines.spectrum(seg, Ne, self.spec_lines)
+        if self.cfg.line_route == "window":
+            pix = self.lines.window_spectra(seg, Ne, x_c=self.cfg.x_c)
+            self.spec_lines += pix.sum(0)
+        elif self.cfg.line_route == "dense":
+            self.lines.spectrum(seg, Ne, self.spec_lines)
+        else:
+            raise ValueError(f"unknown line route {self.cfg.line_route!r}")
         self.brem.spectrum(seg, Ne, self.spec_brem)
         return seg
 

@@ -77,7 +77,7 @@ def campaign(n_cfg=256, seed=2024, scale=1.0):
 class TransportQueue:
     """Device-resident campaign: per-configuration tables, config array, offsets, tallies."""
 
-    def __init__(self, cases, device, max_steps=400_000):
+    def __init__(self, cases, device, max_steps=400_000, precision="fp64"):
         import torch
 
         if not 1 <= len(cases) <= 1024:
@@ -91,7 +91,8 @@ class TransportQueue:
             layers = build_layer_tables([(0.0, q.thickness_ang, list(q.composition))])
             lut = build_transport_lut(q.E_cut_keV, q.E0_keV, layers)
             tabs = DeviceTables(layers, lut, device)
-            ctl = TransportControl(max_steps=max_steps, energy_model=1, straggle_on=q.straggling)
+            ctl = TransportControl(max_steps=max_steps, energy_model=1, straggle_on=q.straggling,
+                                   precision=precision)
             self._keep.append(tabs)
             t = transport_args(tabs, ctl, (0, 0, 0, None), (None,) * 6, {}, (None, None), (q.straggling, None))
             arr[i].t = t
@@ -110,6 +111,7 @@ class TransportQueue:
         self.tally = torch.zeros((len(cases), _lib.QUEUE_TALLY), dtype=torch.float64, device=device)
         self.block_size = 256
         self.per_claim = 1
+        self.precision = 1 if precision == "mixed" else 0
 
     def launch_args(self, offsets=None, total=None, tally=None, block_size=None, per_claim=None, accumulate=0):
         L = _lib.QueueLaunch()
@@ -121,6 +123,7 @@ class TransportQueue:
         L.accumulate = int(accumulate)
         L.block_size = int(self.block_size if block_size is None else block_size)
         L.per_claim = int(self.per_claim if per_claim is None else per_claim)
+        L.precision = int(self.precision)
         return L
 
     def run(self, **kw):

@@ -128,6 +128,32 @@ CRYSTALS = {
 }
 
 
+def structure_factor_abs(crystal: Crystal, hkl):
+    """|sum_j exp(2 pi i h.x_j)| of the basis (unit form factors): the
+    extinction rule of the lattice, not a scattering strength."""
+    h = np.asarray(hkl, dtype=float)
+    ph = [2.0 * np.pi * float(np.dot(h, x)) for _, x in crystal.basis]
+    return float(abs(sum(np.exp(1j * p) for p in ph)))
+
+
+def allowed_reflections(crystal_key, n, h_max=6):
+    """The ``n`` shortest reciprocal vectors with a non-vanishing lattice sum,
+    ordered by (|g|, hkl) so the choice is deterministic."""
+    cr = CRYSTALS[crystal_key]
+    rng = range(-h_max, h_max + 1)
+    cand = []
+    for h in rng:
+        for k in rng:
+            for l in rng:
+                if (h, k, l) == (0, 0, 0):
+                    continue
+                if structure_factor_abs(cr, (h, k, l)) < 1e-6:
+                    continue
+                cand.append((round(cr.g_vector((h, k, l))[1], 9), (h, k, l)))
+    cand.sort()
+    return [hkl for _, hkl in cand[:n]]
+
+
 def debye_waller(g, B_ang2):
     return np.exp(-B_ang2 * (np.asarray(g) / (4.0 * np.pi)) ** 2)
 

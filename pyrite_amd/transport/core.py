@@ -100,6 +100,12 @@ class DeviceTables:
         self.inv_beta = f64(lut.inv_beta)
         self.cdf = f64(lut.cdf)
         self.alpha = f64(lut.alpha)
+        # FP32 copies for the mixed transport precision (transport/_jit_precision.py)
+        self.total_rate_f = self.total_rate.float()
+        self.dEds_f = self.dEds.float()
+        self.inv_beta_f = self.inv_beta.float()
+        self.cdf_f = self.cdf.float()
+        self.alpha_f = self.alpha.float()
 
 
 @dataclass
@@ -113,6 +119,7 @@ class TransportControl:
     finite_footprint: bool = False
     width_ang: float = 0.0
     height_ang: float = 0.0
+    precision: str = "fp64"  # "fp64" or "mixed" (FP32 table lookups, FP64 state)
 
 
 def transport_args(tables: DeviceTables, control: TransportControl, run, state, scratch, out, straggling,
@@ -151,6 +158,11 @@ def transport_args(tables: DeviceTables, control: TransportControl, run, state, 
         setattr(a, name, _ptr(scratch.get(name)))
     a.seg_count, a.exit_code = _ptr(seg_count), _ptr(exit_code)
     a.block_size = int(block_size)
+    if control.precision not in ("fp64", "mixed"):
+        raise ValueError(f"transport precision {control.precision!r} is not 'fp64' or 'mixed'")
+    a.precision = 1 if control.precision == "mixed" else 0
+    a.lut_total_rate_f, a.lut_dEds_f = _ptr(tables.total_rate_f), _ptr(tables.dEds_f)
+    a.lut_inv_beta_f, a.lut_cdf_f, a.lut_alpha_f = _ptr(tables.inv_beta_f), _ptr(tables.cdf_f), _ptr(tables.alpha_f)
     return a
 
 

@@ -53,12 +53,16 @@ class VRConfig:
     rows_per_round: int = 2
     horizon_rows: int = 2
     k_max: float = 16.0  # smallest target weight is 1/k_max
-    ratio_lo: float = 0.5  # weight window band around the target weight
+    # weight-window band around the target weight.  Rouletting copies back up
+    # to a larger weight multiplies the rare grazing / detector-bound rows they
+    # may produce later (the heavy tail of the peak tally), so by default the
+    # window only splits: ratio_lo is tiny and copies keep their weight.
+    ratio_lo: float = 1.0e-6
     ratio_hi: float = 2.0
     tail_fraction: float = 0.01  # below this share of the batch alive, rounds take tail_rows rows
     tail_rows: int = 32
     n_max: int = 32  # copies per split
-    ref_quantile: float = 0.995  # states at this predicted quantile get w_t = 1/k_max
+    ref_quantile: float = 0.98  # states at this predicted quantile get w_t = 1/k_max
     peak_half_width_eV: float = 400.0
     roulette_keep_mass: float = 0.99  # rows at or above the mass carrying this fraction are scored surely
     label_eps_frac: float = 1.0e-3  # floor of the regression target, relative to the mean label

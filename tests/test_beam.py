@@ -75,3 +75,16 @@ def test_device_entry_matches_oracle(device, kind, tilt):
     np.testing.assert_allclose(pos.cpu().numpy(), want_p, rtol=1e-12, atol=1e-9)
     np.testing.assert_allclose(dirs.cpu().numpy(), want_d, rtol=1e-14, atol=1e-15)
     assert torch.all(E == E0) and torch.all(clock == 0)
+
+
+def test_allowed_reflections_follow_the_diamond_rule():
+    from pyrite_amd.spectrum.crystal import CRYSTALS, allowed_reflections, structure_factor_abs
+
+    si = CRYSTALS["silicon"]
+    assert structure_factor_abs(si, (1, 1, 1)) > 1 and structure_factor_abs(si, (2, 2, 0)) > 1
+    assert structure_factor_abs(si, (4, 0, 0)) > 1
+    for forbidden in ((2, 0, 0), (2, 2, 2), (1, 1, 0)):
+        assert structure_factor_abs(si, forbidden) < 1e-9
+    refl = allowed_reflections("silicon", 16)
+    assert len(refl) == 16 and len(set(refl)) == 16
+    assert sum(abs(h) == 1 for h in (r[0] for r in refl)) == 8  # the eight {111}, then {220}

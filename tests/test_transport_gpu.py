@@ -20,7 +20,7 @@ from tests.transport_cases import CASES, HOPG_5MEV, MULTILAYER, SI_30KEV, SI_30K
 pytestmark = pytest.mark.gpu
 
 
-def _gpu_run(case, device, cap=4096, max_steps=400_000, e_start=0, e_count=None, block=0):
+def _gpu_run(case, device, cap=4096, max_steps=400_000, e_start=0, e_count=None, block=0, precision="fp64"):
     import torch
 
     from pyrite_amd import _lib
@@ -40,7 +40,7 @@ def _gpu_run(case, device, cap=4096, max_steps=400_000, e_start=0, e_count=None,
                            max_dE_frac=case.max_dE_frac, straggle_on=case.straggle,
                            finite_footprint=case.finite is not None,
                            width_ang=case.finite[0] if case.finite else 0.0,
-                           height_ang=case.finite[1] if case.finite else 0.0)
+                           height_ang=case.finite[1] if case.finite else 0.0, precision=precision)
     core = make_hip_transport_core(tabs, ctl)
     keys = torch.empty(Ne, dtype=torch.uint64, device=device)
     _lib.call("hr_stream_keys", case.seed, 0, Ne, keys.data_ptr(), None)
@@ -144,6 +144,27 @@ def test_full_histories_agree_in_aggregate(oracle, device, case):
     # most histories agree step for step (host/device log/sin/cos may differ by an ulp)
     same = np.mean(g[1] == o[1])
     assert same > 0.5
+
+
+@pytest.mark.parametrize("case", [SI_30KEV_STRAGGLE, HOPG_5MEV, MULTILAYER], ids=lambda c: c.name)
+def test_mixed_precision_first_step_and_aggregates(oracle, device, case):
+    """The mixed transport precision (FP32 table lookups, FP64 state): the first
+    step within FP32 table rounding of the FP64 oracle, full histories within
+    4 sigma of it."""
+    g_rows, g_cnt, g_code, g_st = _gpu_run(case, device, cap=2, max_steps=1, precision="mixed")
+    o_rows, o_cnt, o_code, o_st = _oracle_run(oracle, case, cap=2, max_steps=1)
+    np.testing.assert_array_equal(g_code, o_code)
+    sel = np.arange(case.Ne) * 2
+    for k in ("seg_len", "seg_E_end", "seg_t_end"):
+        np.testing.assert_allclose(g_rows[k][sel], o_rows[k][sel], rtol=2e-5, atol=1e-300)
+    big = dataclasses.replace(case, Ne=4000)
+    g = _gpu_run(big, device, cap=4096, precision="mixed")
+    o = _oracle_run(oracle, big, cap=4096)
+    go, oo = _observables(g[1], g[2], g[3], big), _observables(o[1], o[2], o[3], big)
+    for k in go:
+        a, b = go[k], oo[k]
+        se = math.sqrt(a.var() / big.Ne + b.var() / big.Ne)
+        assert abs(a.mean() - b.mean()) <= 4.0 * se + 1e-12 * abs(b.mean()), (k, a.mean(), b.mean(), se)
 
 
 def test_simulate_trajectories_api(device):

@@ -263,23 +263,55 @@ def test_rounds_without_splitting_keep_analog_expectation(small_case):
         assert _zs(rnd[:, col], ana[:, col]) < 4.0, name
 
 
-def test_learned_importance_unbiased_and_gains(small_case):
-    from pyrite_amd.vr.importance import LearnedImportance, VRConfig, figure_of_merit
+def test_learned_importance_unbiased(small_case):
+    from pyrite_amd.vr.importance import LearnedImportance, VRConfig
 
     vr = LearnedImportance(small_case, VRConfig(train_rows=400_000, epochs=4))
     x, y = vr.training_set(4, first_batch=10_000)
     hist = vr.fit(x, y)
     assert hist[-1] < hist[0]
     Ne = small_case.cfg.Ne
-    lrn = _weighted_observables(vr, 12, Ne, True, first=300)
-    ana = _analog_observables(small_case, vr, 12, first=400)
-    for col, name in enumerate(("rows", "transmitted", "backscattered", "peak")):
-        if ana[:, col].std() == 0 and lrn[:, col].std() == 0:
-            continue
-        assert _zs(lrn[:, col], ana[:, col]) < 4.0, name
-    vals_a, t_a = vr.measure(12, Ne, "analog", first_batch=500)
-    vals_v, t_v = vr.measure(12, Ne, "vr", first_batch=600)
+    for roulette in (False, True):
+        lrn = []
+        for b in range(12):
+            import torch
+
+            st = {}
+            lines = torch.zeros(small_case.E_grid.size, dtype=torch.float64, device=small_case.dev)
+            brem = torch.zeros_like(lines)
+            vr.run_batch(300 + b, Ne, Ne, lines, brem, use_vr=True, stats=st, roulette=roulette)
+            lrn.append((st["rows"] / Ne, st.get("exit_2", 0.0) / Ne, st.get("exit_1", 0.0) / Ne,
+                        vr.peak_tally(lines.cpu().numpy()), st["rows2"] / Ne))
+        lrn = np.array(lrn)
+        ana = _analog_observables(small_case, vr, 12, first=400)
+        for col, name in enumerate(("rows", "transmitted", "backscattered", "peak")):
+            if ana[:, col].std() == 0 and lrn[:, col].std() == 0:
+                continue
+            assert _zs(lrn[:, col], ana[:, col]) < 4.0, (name, roulette)
+        # the brem weight follows one copy per primary: its weighted rows match the analog rows
+        assert _zs(lrn[:, 4], ana[:, 0]) < 4.0
+
+
+@pytest.fixture(scope="module")
+def hopg_case(device):
+    from pyrite_amd.pipeline import CaseConfig, CasePipeline
+
+    return CasePipeline(CaseConfig(Ne=16384))
+
+
+def test_learned_importance_gains_over_analog(hopg_case):
+    """The trained network raises the PXR-peak figure of merit on the 100 um HOPG case."""
+    from pyrite_amd.vr.importance import LearnedImportance, VRConfig, figure_of_merit
+
+    vr = LearnedImportance(hopg_case, VRConfig(train_rows=600_000, epochs=5))
+    x, y = vr.training_set(6, first_batch=10_000)
+    vr.fit(x, y)
+    Ne = hopg_case.cfg.Ne
+    vr.measure(1, Ne, "analog", first_batch=900)
+    vr.measure(1, Ne, "vr", first_batch=901)
+    vals_a, t_a = vr.measure(16, Ne, "analog", first_batch=500)
+    vals_v, t_v = vr.measure(16, Ne, "vr", first_batch=600)
     fa = figure_of_merit(vals_a, t_a)
     fv = figure_of_merit(vals_v, t_v)
     print(f"analog {fa}  learned {fv}  gain {fv[2] / fa[2]:.2f}")
-    assert fv[2] > fa[2]
+    assert fv[2] > 2.0 * fa[2]
