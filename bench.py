@@ -125,10 +125,10 @@ def main():
     for s in range(args.steps):
         if args.breakdown:
             ta = time.perf_counter()
-            res = pipe.transport(s)
+            res = pipe.transport(s, compact=False)
             torch.cuda.synchronize()
             tb = time.perf_counter()
-            pipe.score(res.rows, cfg.Ne * args.steps)
+            pipe.score_records(pipe.records(res), cfg.Ne * args.steps)
             torch.cuda.synchronize()
             phase["transport"] += tb - ta
             phase["score"] += time.perf_counter() - tb

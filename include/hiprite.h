@@ -346,12 +346,14 @@ hr_status hr_beam_entry(const hr_beam_args *args, void *stream);
  * between transport rounds with weight-preserving decisions. */
 
 /* Advances particles [e_start, e_start+e_count) of a bank by at most `rows`
- * analog steps (1 <= rows <= cap), each from a fresh state: optical depth,
- * flight and substep restart and draws come from stream_key[e] (counter 0).
- * Rows go to slot (e-e_start)*cap + s as in hr_transport_run; exit_code is
- * 6 (suspended) for particles still alive after `rows` steps.  Bank state is
- * updated in place.  alive[] is not read. */
-hr_status hr_vr_round(const hr_transport_args *args, int32_t rows, void *stream);
+ * analog steps (1 <= rows <= cap), or min(rows_each[e], rows) when rows_each
+ * is non-NULL, each from a fresh state: optical depth, flight and substep
+ * restart and draws come from stream_key[e] (counter 0).  Rows go to slot
+ * (e-e_start)*cap + s as in hr_transport_run; exit_code is 6 (suspended) for
+ * particles still alive after their rows.  Bank state is updated in place.
+ * alive[] is not read. */
+hr_status hr_vr_round(const hr_transport_args *args, int32_t rows, const int32_t *rows_each,
+                      void *stream);
 
 /* x[i] = (E/E0, u_x, u_y, u_z, z/thickness, tilt_deg/3) as FP32 [n,6].  With
  * seg_len non-NULL, pos holds row midpoints and z is taken at the row start. */
@@ -373,6 +375,9 @@ typedef struct hr_vr_window_args {
     double *w_child;              /* [n] out: weight of each copy */
     const double *weight2;        /* [n] second tally's weight, or NULL */
     double *w2_child;             /* [n] out: second weight of copy 0 (other copies carry 0), or NULL */
+    int32_t rows_near, rows_far;  /* next round length: rows_far when w I/I_ref < far_ratio */
+    double far_ratio;
+    int32_t *rows_next;           /* [n] out, or NULL */
 } hr_vr_window_args;
 /* r = w/w_t; r > ratio_hi: n = floor(r) + [u < frac(r)], capped at n_max,
  * copies of weight w/n; r < ratio_lo: one copy of weight w_t with probability
@@ -397,6 +402,8 @@ typedef struct hr_vr_spawn_args {
     int64_t *out_prim;
     const double *w2_child;  /* [n] or NULL */
     double *out_weight2;     /* copy 0: w2_child, other copies 0 (NULL with w2_child) */
+    const int32_t *rows_next; /* [n] or NULL: every copy inherits its parent's next round length */
+    int32_t *out_rows;        /* or NULL */
 } hr_vr_spawn_args;
 hr_status hr_vr_spawn(const hr_vr_spawn_args *args, void *stream);
 

@@ -219,6 +219,7 @@ class VrWindowArgs(C.Structure):
         ("n_max", C.c_int32),
         ("count", _dp), ("w_child", _dp),
         ("weight2", _dp), ("w2_child", _dp),
+        ("rows_near", C.c_int32), ("rows_far", C.c_int32), ("far_ratio", C.c_double), ("rows_next", _dp),
     ]
 
 
@@ -233,6 +234,7 @@ class VrSpawnArgs(C.Structure):
         ("out_pos", _dp), ("out_dirs", _dp), ("out_E", _dp), ("out_clock", _dp), ("out_weight", _dp),
         ("out_key", _dp), ("out_prim", _dp),
         ("w2_child", _dp), ("out_weight2", _dp),
+        ("rows_next", _dp), ("out_rows", _dp),
     ]
 
 
@@ -279,7 +281,7 @@ SIGNATURES = {
     "hr_window_reduce": (C.c_int32, [_dp, _dp, _dp, C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double,
                                      _dp, _dp]),
     "hr_window_spectra": (C.c_int32, [C.POINTER(WindowArgs), _dp]),
-    "hr_vr_round": (C.c_int32, [C.POINTER(TransportArgs), C.c_int32, _dp]),
+    "hr_vr_round": (C.c_int32, [C.POINTER(TransportArgs), C.c_int32, _dp, _dp]),
     "hr_vr_features": (C.c_int32, [C.c_int64, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp, _dp]),
     "hr_vr_window": (C.c_int32, [C.POINTER(VrWindowArgs), _dp]),
     "hr_vr_spawn": (C.c_int32, [C.POINTER(VrSpawnArgs), _dp]),

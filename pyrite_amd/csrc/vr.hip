@@ -31,14 +31,20 @@ __host__ __device__ __forceinline__ uint64_t child_key(uint64_t key, int64_t j) 
 
 // One particle per thread: at most `rows` steps, then suspend.
 template <bool kMixed>
-__global__ void __launch_bounds__(256) vr_round_kernel(hr_transport_args a, int32_t rows) {
+__global__ void __launch_bounds__(256) vr_round_kernel(hr_transport_args a, int32_t rows,
+                                                       const int32_t *__restrict__ rows_each) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.e_count) return;
     const int64_t e = a.e_start + i;
+    int32_t mine = rows;
+    if (rows_each) {
+        const int32_t r = rows_each[e];
+        mine = r < 1 ? 1 : (r < rows ? r : rows);
+    }
     Lane s;
     load_lane(a, e, s);
     int code = -1;
-    while (code < 0 && s.nseg < rows) code = step_lane<true, kMixed>(a, s);
+    while (code < 0 && s.nseg < mine) code = step_lane<true, kMixed>(a, s);
     store_lane(a, s, code < 0 ? (int8_t)EXIT_SUSPENDED : (int8_t)code);
 }
 
@@ -97,6 +103,11 @@ __global__ void __launch_bounds__(256) vr_window_kernel(hr_vr_window_args p) {
     p.count[i] = count;
     p.w_child[i] = wc;
     if (p.w2_child) p.w2_child[i] = w2c;
+    if (p.rows_next) {
+        // far below the split threshold: the next round may take rows_far rows
+        const double s = p.weight[i] * exp((double)p.log_importance[i] - p.log_I_ref);
+        p.rows_next[i] = (count > 0 && s < p.far_ratio) ? p.rows_far : p.rows_near;
+    }
 }
 
 // Children of parent i occupy [offset[i], offset[i] + count[i]) of the output bank.
@@ -112,6 +123,7 @@ __global__ void __launch_bounds__(256) vr_spawn_kernel(hr_vr_spawn_args p) {
     const uint64_t key = p.key[i];
     const int64_t prim = p.prim[i];
     const double w2c = p.w2_child ? p.w2_child[i] : 0.0;
+    const int32_t rn = p.rows_next ? p.rows_next[i] : 0;
     for (int32_t j = 0; j < c; ++j) {
         const int64_t k = o + j;
         p.out_pos[3 * k] = px;
@@ -126,6 +138,7 @@ __global__ void __launch_bounds__(256) vr_spawn_kernel(hr_vr_spawn_args p) {
         p.out_key[k] = child_key(key, j);
         p.out_prim[k] = prim;
         if (p.out_weight2) p.out_weight2[k] = j == 0 ? w2c : 0.0;
+        if (p.out_rows) p.out_rows[k] = rn;
     }
 }
 
@@ -199,7 +212,7 @@ unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1
 
 extern "C" {
 
-hr_status hr_vr_round(const hr_transport_args *a, int32_t rows, void *stream) {
+hr_status hr_vr_round(const hr_transport_args *a, int32_t rows, const int32_t *rows_each, void *stream) {
     using namespace hr;
     HR_REQUIRE(a != nullptr, "hr_vr_round: null args");
     HR_REQUIRE(rows >= 1 && rows <= a->cap, "hr_vr_round: rows %d outside [1, cap=%lld]", rows, (long long)a->cap);
@@ -220,10 +233,10 @@ hr_status hr_vr_round(const hr_transport_args *a, int32_t rows, void *stream) {
                "hr_vr_round: precision 0, or 1 with the FP32 table copies");
     if (a->precision == 1)
         hipLaunchKernelGGL(vr_round_kernel<true>, dim3(blocks_for(a->e_count, 256)), dim3(256), 0,
-                           (hipStream_t)stream, *a, rows);
+                           (hipStream_t)stream, *a, rows, rows_each);
     else
         hipLaunchKernelGGL(vr_round_kernel<false>, dim3(blocks_for(a->e_count, 256)), dim3(256), 0,
-                           (hipStream_t)stream, *a, rows);
+                           (hipStream_t)stream, *a, rows, rows_each);
     HR_CHECK_HIP(hipGetLastError());
     return HR_OK;
 }
@@ -251,6 +264,7 @@ hr_status hr_vr_window(const hr_vr_window_args *p, void *stream) {
     HR_REQUIRE(p->ratio_lo > 0.0 && p->ratio_lo <= 1.0 && p->ratio_hi >= 1.0,
                "hr_vr_window: need 0 < ratio_lo <= 1 <= ratio_hi");
     HR_REQUIRE(p->n_max >= 1 && p->n_max <= 1024, "hr_vr_window: n_max %d outside [1, 1024]", p->n_max);
+    HR_REQUIRE(!p->rows_next || (p->rows_near >= 1 && p->rows_far >= 1), "hr_vr_window: round lengths must be >= 1");
     hipLaunchKernelGGL(vr_window_kernel, dim3(blocks_for(p->n, 256)), dim3(256), 0, (hipStream_t)stream, *p);
     HR_CHECK_HIP(hipGetLastError());
     return HR_OK;

@@ -129,9 +129,13 @@ class CasePipeline(CaseTables):
         self.histories = 0
         self.segments = 0
         self.timers = {}
+        self._cap_hint = None
+        self._rec = None
+        self._ones = torch.ones(n, dtype=torch.float64, device=dev)
 
-    def transport(self, batch_index: int):
-        """Transport histories ``[b*Ne, (b+1)*Ne)``; returns the device segment rows."""
+    def transport(self, batch_index: int, compact: bool = True):
+        """Transport histories ``[b*Ne, (b+1)*Ne)``; returns the device segment rows
+        (compacted, or the slot rows of each sub-batch with ``compact=False``)."""
         cfg = self.cfg
         n = cfg.Ne
         _lib.call("hr_stream_keys", int(cfg.seed), int(batch_index) * n, n, self.keys.data_ptr(), None)
@@ -139,22 +143,62 @@ class CasePipeline(CaseTables):
                    self.pos, self.dirs, self.E, self.clock)
         res = drive_batches(self.core, self.keys,
                             (self.alive, self.clock, self.pos, self.dirs, self.E, self.E_cut),
-                            self.stragg, self.control, n, self.dev, self.tcfg)
+                            self.stragg, self.control, n, self.dev, self.tcfg, compact=compact,
+                            cap_hint=self._cap_hint)
+        self._cap_hint = res.capacity
         return res
+
+    def records(self, res):
+        """FP32 scoring records gathered straight from the slot rows of a
+        ``compact=False`` transport (no FP64 compaction pass)."""
+        import ctypes as C
+
+        import torch
+
+        n_rows = int(res.n_segments)
+        if self._rec is None or self._rec["L"].shape[0] < n_rows:
+            m = max(n_rows, 1)
+            self._rec = {"r_mid": torch.empty((m, 3), dtype=torch.float32, device=self.dev),
+                         "v_hat": torch.empty((m, 3), dtype=torch.float32, device=self.dev),
+                         "L": torch.empty(m, dtype=torch.float32, device=self.dev),
+                         "E": torch.empty(m, dtype=torch.float32, device=self.dev),
+                         "w": torch.empty(m, dtype=torch.float32, device=self.dev)}
+        rec = self._rec
+        lib = _lib.load()
+        o = 0
+        for _e0, m, cap, rows, seg_count in res.slots:
+            cnt = seg_count.clamp(max=cap)
+            off = torch.cumsum(cnt, 0) - cnt + o
+            g = _lib.VrGatherArgs()
+            g.n, g.rows = m, cap
+            g.seg_count, g.row_offset = seg_count.data_ptr(), off.data_ptr()
+            g.weight = self._ones[:m].data_ptr()
+            g.seg_mid, g.seg_dir = rows["seg_mid"].data_ptr(), rows["seg_dir"].data_ptr()
+            g.seg_len, g.seg_E, g.seg_E_end = rows["seg_len"].data_ptr(), rows["seg_E"].data_ptr(), \
+                rows["seg_E_end"].data_ptr()
+            g.r_mid, g.v_hat = rec["r_mid"].data_ptr(), rec["v_hat"].data_ptr()
+            g.L, g.E, g.row_weight = rec["L"].data_ptr(), rec["E"].data_ptr(), rec["w"].data_ptr()
+            _lib.check(lib.hr_vr_gather(C.byref(g), C.c_void_p(torch.cuda.current_stream().cuda_stream)),
+                       "hr_vr_gather")
+            o += int(cnt.sum()) if len(res.slots) > 1 else 0
+        return {k: rec[k][:n_rows] for k in ("r_mid", "v_hat", "L", "E")}
+
+    def score_records(self, seg, Ne):
+        """Line and brem tallies of FP32 scoring records."""
+        if self.cfg.line_route == "window":
+            self.spec_lines += self.lines.window_spectra(seg, Ne, x_c=self.cfg.x_c).sum(0)
+        elif self.cfg.line_route == "dense":
+            self.lines.spectrum(seg, Ne, self.spec_lines)
+        else:
+            raise ValueError(f"unknown line route {self.cfg.line_route!r}")
+        self.brem.spectrum(seg, Ne, self.spec_brem)
 
     def score(self, rows, Ne):
         seg = segments_fp32({
             "r_mid": rows["seg_mid"], "v_hat": rows["seg_dir"], "L_ang": rows["seg_len"],
             "E_keV": rows["seg_E"], "E_repr_keV": 0.5 * (rows["seg_E"] + rows["seg_E_end"]),
         }, self.dev)
-        if self.cfg.line_route == "window":
-            pix = self.lines.window_spectra(seg, Ne, x_c=self.cfg.x_c)
-            self.spec_lines += pix.sum(0)
-        elif self.cfg.line_route == "dense":
-            self.lines.spectrum(seg, Ne, self.spec_lines)
-        else:
-            raise ValueError(f"unknown line route {self.cfg.line_route!r}")
-        self.brem.spectrum(seg, Ne, self.spec_brem)
+        self.score_records(seg, Ne)
         return seg
 
     def step(self, batch_index: int, n_batches_total: int = 1):
@@ -163,8 +207,8 @@ class CasePipeline(CaseTables):
         Tallies are per incident electron of the whole run when
         ``n_batches_total`` batches are summed.
         """
-        res = self.transport(batch_index)
-        self.score(res.rows, self.cfg.Ne * n_batches_total)
+        res = self.transport(batch_index, compact=False)
+        self.score_records(self.records(res), self.cfg.Ne * n_batches_total)
         self.histories += self.cfg.Ne
         self.segments += res.n_segments
         return res

@@ -226,26 +226,34 @@ def _batch_electrons(e, cap, Ne, config, row_bytes):
 
 @dataclass
 class DriveResult:
-    rows: dict  # compacted, electron-major / step-minor device tensors
+    rows: dict  # compacted, electron-major / step-minor device tensors (None with compact=False)
     n_segments: int
     counts: dict  # exit-code tallies
     seg_count: object  # per-electron true segment counts (device int64)
     exit_code: object  # per-electron exit codes (device int8)
     replays: int
+    slots: list = None  # compact=False: [(e0, m, cap, slot rows, seg_count)] per sub-batch
+    capacity: int = 0  # capacity learned from the largest count seen (a hint for the next call)
 
 
 def drive_batches(core, keys, state, stragg_dE, control: TransportControl, Ne: int, device,
-                  config: PerElectronTransportConfig = DEFAULT_CONFIG, stream=None) -> DriveResult:
-    """Capacity-replayed batches (``batching.py:477-628``), rows kept on device."""
+                  config: PerElectronTransportConfig = DEFAULT_CONFIG, stream=None, compact=True,
+                  cap_hint=None) -> DriveResult:
+    """Capacity-replayed batches (``batching.py:477-628``), rows kept on device.
+
+    ``compact=False`` skips the electron-major compaction and returns the slot
+    rows of each sub-batch (a scorer can gather them itself); ``cap_hint``
+    starts from a capacity learned by an earlier call."""
     import torch
 
     alive, clock, pos, dirs, E_keV, E_cut = state
     midpoint = control.energy_model == 1
     row_bytes = _ROW_BYTES_MIDPOINT if midpoint else _ROW_BYTES_FROZEN
-    cap = max(1, int(config.seg_capacity))
+    cap = max(1, int(config.seg_capacity if cap_hint is None else cap_hint))
     seen_max = 0
     e = 0
     batches = []
+    slots = []
     counts_all = []
     exit_all = []
     replays = 0
@@ -269,15 +277,20 @@ def drive_batches(core, keys, state, stragg_dE, control: TransportControl, Ne: i
             m = _batch_electrons(e, cap, Ne, config, row_bytes)
             sl = slice(e, e + m)
             snap = tuple(x[:m] for x in snap)
-        keep = (torch.arange(cap, device=device)[None, :] < seg_count[:, None]).reshape(-1)
-        idx = torch.nonzero(keep).squeeze(1)  # one stream compaction shared by every column
-        batches.append({k: v.index_select(0, idx) for k, v in rows.items()})
+        if compact:
+            keep = (torch.arange(cap, device=device)[None, :] < seg_count[:, None]).reshape(-1)
+            idx = torch.nonzero(keep).squeeze(1)  # one stream compaction shared by every column
+            batches.append({k: v.index_select(0, idx) for k, v in rows.items()})
+        else:
+            slots.append((e, m, cap, rows, seg_count))
         counts_all.append(seg_count)
         exit_all.append(exit_code)
         e += m
         if seen_max > 0:
             cap = _capacity_for(seen_max, config)
-    if len(batches) == 1:
+    if not compact:
+        joined = None
+    elif len(batches) == 1:
         joined = batches[0]
     elif batches:
         joined = {k: torch.cat([b[k] for b in batches]) for k in batches[0]}
@@ -294,5 +307,5 @@ def drive_batches(core, keys, state, stragg_dE, control: TransportControl, Ne: i
         "n_step_limited": int(codes[EXIT_STEP_LIMITED]),
         "n_not_entered": int(codes[EXIT_NOT_ENTERED]),
     }
-    n_seg = int(joined["seg_len"].shape[0])
-    return DriveResult(joined, n_seg, counts, seg_count, exit_code, replays)
+    n_seg = int(joined["seg_len"].shape[0]) if compact else int(seg_count.sum())
+    return DriveResult(joined, n_seg, counts, seg_count, exit_code, replays, slots, cap)

@@ -10,7 +10,8 @@ every analog row's lines are reduced to their expected window content
 history.
 
 Production runs transport in rounds (``hr_vr_round``: at most
-``rows_per_round`` rows per particle, then suspend).  Between rounds the
+``rows_per_round`` rows per particle near the split threshold, ``rows_far``
+far below it, then suspend).  Between rounds the
 network scores every suspended particle and a weight window
 (``hr_vr_window``) sets the target weight ``w_t = clamp(I_ref / I(x),
 1/k_max, 1)``: particles heavier than ``w_t`` are split into copies of weight
@@ -50,7 +51,9 @@ def _stream():
 
 @dataclass
 class VRConfig:
-    rows_per_round: int = 2
+    rows_per_round: int = 1  # rows between decisions near the split threshold
+    rows_far: int = 8  # rows between decisions far below it (w I/I_ref < far_ratio)
+    far_ratio: float = 1.0 / 32.0
     horizon_rows: int = 2
     k_max: float = 16.0  # smallest target weight is 1/k_max
     # weight-window band around the target weight.  Rouletting copies back up
@@ -91,6 +94,7 @@ class ParticleBank:
         self.prim = torch.zeros(m, dtype=torch.int64, device=device)
         self.E_cut = torch.full((m,), float(E_cut_keV), dtype=torch.float64, device=device)
         self.stragg = torch.zeros(m, dtype=torch.float64, device=device)
+        self.rows = torch.ones(m, dtype=torch.int32, device=device)  # next round length
 
 
 class LearnedImportance:
@@ -280,7 +284,7 @@ class LearnedImportance:
     def _alloc(self, Ne):
         v = self.vcfg
         cap = int(Ne * min(v.k_max, v.n_max) * 1.25) + 4096
-        R = int(v.rows_per_round)
+        R = max(int(v.rows_per_round), int(v.rows_far))  # slot stride of the round rows
         if self._banks is not None and self._banks[0].capacity >= cap and self._rows_R == R:
             return
         self._rows_R = R
@@ -305,6 +309,7 @@ class LearnedImportance:
         self._row_w = torch.empty(m, dtype=torch.float32, device=dev)
         self._row_w2 = torch.empty(m, dtype=torch.float32, device=dev)
         self._w2c = torch.empty(cap, dtype=torch.float64, device=dev)
+        self._rows_next = torch.empty(cap, dtype=torch.int32, device=dev)
 
     def _window(self, n, bank, use_vr, n_max):
         a = _lib.VrWindowArgs()
@@ -322,6 +327,10 @@ class LearnedImportance:
         a.n_max = int(n_max)
         a.count, a.w_child = self._count.data_ptr(), self._wc.data_ptr()
         a.weight2, a.w2_child = bank.weight2.data_ptr(), self._w2c.data_ptr()
+        if use_vr:
+            a.rows_near, a.rows_far = int(self.vcfg.rows_per_round), int(self.vcfg.rows_far)
+            a.far_ratio = float(self.vcfg.far_ratio)
+            a.rows_next = self._rows_next.data_ptr()
         _lib.check(self.lib.hr_vr_window(C.byref(a), _stream()), "hr_vr_window")
 
     def run_batch(self, batch_index, Ne, Ne_total, spec_lines, brem_acc, use_vr=True, stats=None,
@@ -338,7 +347,7 @@ class LearnedImportance:
         if use_vr and self.log_I_ref is None:
             raise RuntimeError("train the importance network (fit) before a learned-importance run")
         self._alloc(Ne)
-        R = int(v.rows_per_round)
+        R = self._rows_R  # slot stride
         src, dst = self._banks
         cap = src.capacity
         _lib.call("hr_stream_keys", int(p.cfg.seed), int(batch_index) * Ne, Ne, src.key.data_ptr(), None)
@@ -348,6 +357,7 @@ class LearnedImportance:
         src.clock[:Ne].zero_()
         src.weight[:Ne].fill_(1.0)
         src.weight2[:Ne].fill_(1.0)
+        src.rows[:Ne].fill_(int(v.rows_per_round))
         self._row0 = 0
         rkey = _roulette_key(p.cfg.seed, batch_index) if roulette else None
         src.prim[:Ne].copy_(torch.arange(int(batch_index) * Ne, (int(batch_index) + 1) * Ne, device=p.dev))
@@ -357,11 +367,15 @@ class LearnedImportance:
         while n > 0:
             self.rounds += 1
             # the last stragglers take long rounds: fewer launches, decisions rarer where they matter least
-            Rr = R if n > tail_n else max(R, min(int(v.tail_rows), (cap * R) // n))
+            tail = n <= tail_n
+            Rr = R if not tail else max(R, min(int(v.tail_rows), (cap * R) // n))
+            rows_each = src.rows.data_ptr() if (use_vr and not tail) else None
+            if not use_vr and not tail:
+                Rr = int(v.rows_per_round)
             a = transport_args(p.tables, p.control, (0, n, Rr, src.key),
                                (None, src.clock, src.pos, src.dirs, src.E, src.E_cut), rows,
                                (self._seg_count, self._exit), (p.control.straggle_on, src.stragg))
-            _lib.check(self.lib.hr_vr_round(C.byref(a), Rr, _stream()), "hr_vr_round")
+            _lib.check(self.lib.hr_vr_round(C.byref(a), Rr, rows_each, _stream()), "hr_vr_round")
             # rows -> weighted FP32 scoring records
             m = self._seg_count[:n].clamp(max=Rr)
             incl = torch.cumsum(m, 0)
@@ -418,6 +432,8 @@ class LearnedImportance:
                 sp.out_pos, sp.out_dirs, sp.out_E = dst.pos.data_ptr(), dst.dirs.data_ptr(), dst.E.data_ptr()
                 sp.out_clock, sp.out_weight = dst.clock.data_ptr(), dst.weight.data_ptr()
                 sp.out_key, sp.out_prim = dst.key.data_ptr(), dst.prim.data_ptr()
+                if use_vr:
+                    sp.rows_next, sp.out_rows = self._rows_next.data_ptr(), dst.rows.data_ptr()
                 sp.w2_child, sp.out_weight2 = self._w2c.data_ptr(), dst.weight2.data_ptr()
                 _lib.check(self.lib.hr_vr_spawn(C.byref(sp), _stream()), "hr_vr_spawn")
             src, dst = dst, src

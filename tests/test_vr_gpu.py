@@ -169,7 +169,7 @@ def _round_run(case, device, rows, cap):
     code = torch.zeros(Ne, dtype=torch.int8, device=device)
     a = transport_args(tabs, ctl, (0, Ne, cap, keys), (None, clock, st[0], st[1], st[2], st[3]), rws,
                        (cnt, code), (case.straggle, stragg))
-    _lib.check(_lib.load().hr_vr_round(C.byref(a), rows, _stream()), "hr_vr_round")
+    _lib.check(_lib.load().hr_vr_round(C.byref(a), rows, None, _stream()), "hr_vr_round")
     torch.cuda.synchronize()
     return {k: v.cpu().numpy() for k, v in rws.items()}, cnt.cpu().numpy(), code.cpu().numpy()
 
