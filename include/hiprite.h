@@ -121,6 +121,36 @@ typedef struct hr_transport_args {
 hr_status hr_transport_run(const hr_transport_args *args, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Bounded continuation (montecarlo/transport/continuation.py:1-27 and the
+ * round driver _drive_continuation_rounds, :506): a fixed population of
+ * n = e_count lanes, each holding one history, advanced by at most `cap` rows
+ * per launch.  Between launches the caller retires finished lanes and refills
+ * them with pending primaries (fresh record: tau -1, every counter 0,
+ * stragg_dE 0, and the primary's entry state and stream key in the lane).
+ * The record carries every value the step loop holds between steps, so a
+ * history's rows, exit code and final state are bit-identical to one
+ * uninterrupted hr_transport_run, whatever the schedule.
+ * State arrays, stream_key, alive, seg_count and exit_code are indexed by
+ * lane (e_start must be 0); a launch's rows go to slot lane*cap + k for the
+ * lane's k-th row of this launch, seg_id carries history[lane], and
+ * seg_count[lane] counts this launch's rows.  exit_code is 0..5 for a history
+ * that ended in this launch, 6 (suspended) for one still alive after `cap`
+ * rows, and HR_LANE_EMPTY for a lane with history < 0.  alive[lane] is read
+ * only at a history's first launch.
+ * ---------------------------------------------------------------------- */
+#define HR_LANE_EMPTY (-1)
+typedef struct hr_continuation {
+    const int64_t *history; /* [n] global electron id of each lane's history; < 0: empty lane */
+    double *tau;            /* [n] open optical depth; < 0 draws a fresh one at the next step */
+    uint64_t *draws;        /* [n] per-history stream counter */
+    int64_t *nseg;          /* [n] rows the history has emitted before this launch (updated) */
+    int64_t *flight, *substep;
+    int32_t *steps;         /* [n] steps taken (max_steps applies to the whole history) */
+} hr_continuation;
+
+hr_status hr_transport_continue(const hr_transport_args *args, const hr_continuation *cont, void *stream);
+
+/* ------------------------------------------------------------------------
  * Incoherent PXR/CBS line prologue over dumped segments.
  * Replaces the batched line route's per-(segment, reflection) steps 1-7,
  * montecarlo/spectrum/cxr/_batched.py:461-728 (_batched_block,
@@ -437,6 +467,18 @@ hr_status hr_line_mass(const double *E_r, const double *a, const float *w, int64
  * proportional keep probability. */
 hr_status hr_line_roulette(float *w, int64_t n_seg, int32_t n_dir, int32_t n_g, const double *mass,
                            double m_thr, uint64_t key, int64_t row0, void *stream);
+
+/* Per-primary line roulette, the reference's LineRoulette estimator
+ * (src/pyrite/_line_roulette.py:29-80, montecarlo/runner/line_roulette.py).
+ * hr_roulette_uniforms: out[i] = uniform of primary first_id + i keyed only on
+ * (seed, primary id) (roulette_uniforms, line_roulette.py:33-45).
+ * hr_line_screen: for primaries first_id + i with exact line masses mass[i]:
+ * probability 1 and kept = mass > 0, except ids >= pilot with
+ * 0 < mass < threshold, which get probability `floor` and kept = u < floor
+ * (LineScreen._select, line_roulette.py:114-126). */
+hr_status hr_roulette_uniforms(uint64_t seed, int64_t first_id, int64_t n, double *out, void *stream);
+hr_status hr_line_screen(const double *mass, int64_t n, int64_t first_id, int64_t pilot, double threshold,
+                         double floor, uint64_t seed, double *probability, uint8_t *kept, void *stream);
 
 #ifdef __cplusplus
 }

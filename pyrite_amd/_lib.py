@@ -238,6 +238,18 @@ class VrSpawnArgs(C.Structure):
     ]
 
 
+class Continuation(C.Structure):
+    """Mirror of ``hr_continuation``."""
+
+    _fields_ = [
+        ("history", _dp), ("tau", _dp), ("draws", _dp), ("nseg", _dp),
+        ("flight", _dp), ("substep", _dp), ("steps", _dp),
+    ]
+
+
+LANE_EMPTY = -1
+
+
 class VrGatherArgs(C.Structure):
     """Mirror of ``hr_vr_gather_args``."""
 
@@ -264,6 +276,10 @@ SIGNATURES = {
     "hr_device_count": (C.c_int32, []),
     "hr_stream_keys": (C.c_int32, [C.c_uint64, C.c_int64, C.c_int64, _dp, _dp]),
     "hr_transport_run": (C.c_int32, [C.POINTER(TransportArgs), _dp]),
+    "hr_roulette_uniforms": (C.c_int32, [C.c_uint64, C.c_int64, C.c_int64, _dp, _dp]),
+    "hr_line_screen": (C.c_int32, [_dp, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_uint64,
+                                   _dp, _dp, _dp]),
+    "hr_transport_continue": (C.c_int32, [C.POINTER(TransportArgs), C.POINTER(Continuation), _dp]),
     "hr_cxr_lines": (C.c_int32, [C.POINTER(LineArgs), _dp]),
     "hr_sincsq_reduce": (C.c_int32, [C.POINTER(SincsqArgs), _dp]),
     "hr_brem_reduce": (C.c_int32, [C.POINTER(BremArgs), _dp]),

@@ -76,6 +76,7 @@ __global__ void __launch_bounds__(256) queue_kernel(const hr_queue_config *__res
             const int64_t e = id - off[c];
             id += 1;
             s.e = e;
+            s.id = e;
             s.px = 0.0;
             s.py = 0.0;
             s.pz = 0.0;

@@ -82,6 +82,54 @@ __global__ void __launch_bounds__(256) HR_TRANSPORT_OCCUPANCY transport_kernel(h
     }
 }
 
+// Bounded continuation: lane i of the active population advances its history
+// by at most a.cap rows, resuming from the loop-carried record of the previous
+// launch.  The record holds every value step_lane carries between steps (open
+// optical depth, stream counter, flight/substep/step counters; the straggling
+// accumulator stays in stragg_dE), so a history split over any number of
+// launches emits the rows of one uninterrupted run, bit for bit.
+template <bool kMixed>
+__global__ void __launch_bounds__(256) HR_TRANSPORT_OCCUPANCY continue_kernel(hr_transport_args a,
+                                                                               hr_continuation c) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.e_count) return;
+    const int64_t h = c.history[i];
+    if (h < 0) {
+        a.seg_count[i] = 0;
+        a.exit_code[i] = (int8_t)HR_LANE_EMPTY;
+        return;
+    }
+    const bool fresh = c.steps[i] == 0 && c.nseg[i] == 0;
+    if (fresh && !a.alive[i]) {
+        a.seg_count[i] = 0;
+        a.exit_code[i] = EXIT_NOT_ENTERED;
+        if (a.straggle_on) a.stragg_dE[i] = 0.0;
+        return;
+    }
+    Lane s;
+    load_lane(a, i, s);
+    s.id = h;
+    s.tau = c.tau[i];
+    s.draws = c.draws[i];
+    s.flight = c.flight[i];
+    s.substep = c.substep[i];
+    s.steps = c.steps[i];
+    if (a.straggle_on && !fresh) s.stragg = a.stragg_dE[i];
+    int code = -1;
+    if (a.max_steps <= 0) {
+        code = EXIT_STEP_LIMITED;
+    } else {
+        while (code < 0 && s.nseg < a.cap) code = step_lane<true, kMixed>(a, s);
+    }
+    store_lane(a, s, code < 0 ? (int8_t)EXIT_SUSPENDED : (int8_t)code);
+    c.tau[i] = s.tau;
+    c.draws[i] = s.draws;
+    c.flight[i] = s.flight;
+    c.substep[i] = s.substep;
+    c.steps[i] = s.steps;
+    c.nseg[i] += s.nseg;
+}
+
 __global__ void reset_counter(unsigned long long *c) { *c = 0ULL; }
 
 struct DeviceWork {
@@ -156,6 +204,43 @@ extern "C" hr_status hr_transport_run(const hr_transport_args *args, void *strea
         hipLaunchKernelGGL(transport_kernel<true>, dim3((unsigned)grid), dim3(block), 0, s, a, w->counter);
     else
         hipLaunchKernelGGL(transport_kernel<false>, dim3((unsigned)grid), dim3(block), 0, s, a, w->counter);
+    HR_CHECK_HIP(hipGetLastError());
+    return HR_OK;
+}
+
+extern "C" hr_status hr_transport_continue(const hr_transport_args *args, const hr_continuation *cont,
+                                           void *stream) {
+    using namespace hr;
+    HR_REQUIRE(args != nullptr && cont != nullptr, "hr_transport_continue: null args");
+    const hr_transport_args &a = *args;
+    const hr_continuation &c = *cont;
+    HR_REQUIRE(a.e_start == 0 && a.e_count >= 0, "hr_transport_continue: lanes are [0, e_count), e_start must be 0");
+    HR_REQUIRE(a.cap >= 1 && a.cap <= (1 << 30), "hr_transport_continue: cap (rows per launch) %lld outside [1, 2^30]",
+               (long long)a.cap);
+    HR_REQUIRE(a.e_count <= (1LL << 40), "hr_transport_continue: too many lanes");
+    HR_REQUIRE(a.n_layers >= 1 && a.max_el >= 1 && a.lut_n_energy >= 2, "hr_transport_continue: bad table shape");
+    HR_REQUIRE(a.energy_model == 0 || a.energy_model == 1, "hr_transport_continue: energy_model must be 0 or 1");
+    if (a.e_count == 0) return HR_OK;
+    HR_REQUIRE(a.stream_key && a.alive && a.clock && a.pos && a.dirs && a.E_keV && a.E_cut && a.seg_count &&
+                   a.exit_code && a.L_nel && a.L_top && a.L_bot && a.lut_total_rate && a.lut_dEds &&
+                   a.lut_inv_beta && a.lut_cdf && a.lut_alpha,
+               "hr_transport_continue: null state or table pointer");
+    HR_REQUIRE(c.history && c.tau && c.draws && c.nseg && c.flight && c.substep && c.steps,
+               "hr_transport_continue: null continuation record");
+    HR_REQUIRE(!a.straggle_on || (a.stragg_dE && a.L_Z && a.L_J && a.L_k && a.L_coeff && a.L_E_cross),
+               "hr_transport_continue: straggling needs stragg_dE and element tables");
+    HR_REQUIRE(a.seg_dir && a.seg_mid && a.seg_len && a.seg_E && a.seg_t0 && a.seg_id && a.seg_lay,
+               "hr_transport_continue: null row pointer");
+    HR_REQUIRE(a.energy_model == 0 || (a.seg_E_end && a.seg_t_end && a.seg_flight && a.seg_substep && a.seg_event),
+               "hr_transport_continue: midpoint rows need E_end/t_end/flight/substep/event");
+    HR_REQUIRE(a.precision == 0 || (a.precision == 1 && a.lut_total_rate_f && a.lut_dEds_f && a.lut_inv_beta_f &&
+                                    a.lut_cdf_f && a.lut_alpha_f),
+               "hr_transport_continue: precision 0, or 1 with the FP32 table copies");
+    const unsigned grid = (unsigned)((a.e_count + 255) / 256);
+    if (a.precision == 1)
+        hipLaunchKernelGGL(continue_kernel<true>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, c);
+    else
+        hipLaunchKernelGGL(continue_kernel<false>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, c);
     HR_CHECK_HIP(hipGetLastError());
     return HR_OK;
 }

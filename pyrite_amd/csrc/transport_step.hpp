@@ -206,15 +206,19 @@ __device__ double urban_compound(const double *__restrict__ Zr, const double *__
     return loss;
 }
 
+// e indexes the state arrays and the slot rows; id is the history's global
+// electron id written to seg_id (equal to e except under continuation, where e
+// is a lane of the active population).
 struct Lane {
     double px, py, pz, dx, dy, dz, E, clock, tau, E_cut, stragg;
     uint64_t key, draws;
-    int64_t e, nseg, flight, substep;
+    int64_t e, id, nseg, flight, substep;
     int32_t steps;
 };
 
 __device__ __forceinline__ void load_lane(const hr_transport_args &a, int64_t e, Lane &s) {
     s.e = e;
+    s.id = e;
     s.px = a.pos[3 * e];
     s.py = a.pos[3 * e + 1];
     s.pz = a.pos[3 * e + 2];
@@ -404,7 +408,7 @@ __device__ int step_lane(const hr_transport_args &a, Lane &s) {
         a.seg_len[slot] = step;
         a.seg_E[slot] = E;
         a.seg_t0[slot] = s.clock;
-        a.seg_id[slot] = s.e;
+        a.seg_id[slot] = s.id;
         a.seg_lay[slot] = (int16_t)L;
         if (midpoint) {
             a.seg_E_end[slot] = E_end;

@@ -205,6 +205,39 @@ __global__ void __launch_bounds__(256) line_roulette_kernel(float *__restrict__ 
         }
 }
 
+// Per-primary line roulette (montecarlo/runner/line_roulette.py): the draw of
+// primary `id` is counter `id` of the stream keyed by seed ^ "LINEROUL"
+// (:33-45), apart from every transport stream; pilot primaries and those at or
+// above the threshold are scored surely, positive masses below it with
+// probability `floor` (:114-126).
+constexpr uint64_t kLineRouletteSalt = 0x4C494E45524F554CULL;
+
+__global__ void __launch_bounds__(256) roulette_uniforms_kernel(uint64_t seed, int64_t first_id, int64_t n,
+                                                                double *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = stream_uniform(seed ^ kLineRouletteSalt, (uint64_t)(first_id + i));
+}
+
+__global__ void __launch_bounds__(256) line_screen_kernel(const double *__restrict__ mass, int64_t n,
+                                                          int64_t first_id, int64_t pilot, double threshold,
+                                                          double floor_p, uint64_t seed,
+                                                          double *__restrict__ probability,
+                                                          uint8_t *__restrict__ kept) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t id = first_id + i;
+    const double m = mass[i];
+    double p = 1.0;
+    bool k = m > 0.0;
+    if (id >= pilot && m > 0.0 && m < threshold) {
+        p = floor_p;
+        k = stream_uniform(seed ^ kLineRouletteSalt, (uint64_t)id) < floor_p;
+    }
+    probability[i] = p;
+    kept[i] = k ? 1 : 0;
+}
+
 unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 
 }  // namespace
@@ -319,6 +352,30 @@ hr_status hr_line_roulette(float *w, int64_t n_seg, int32_t n_dir, int32_t n_g, 
     HR_REQUIRE(m_thr > 0.0, "hr_line_roulette: m_thr must be positive");
     hipLaunchKernelGGL(line_roulette_kernel, dim3(blocks_for(n_seg, 256)), dim3(256), 0, (hipStream_t)stream, w,
                        n_seg, n_dir, n_g, mass, m_thr, key, row0);
+    HR_CHECK_HIP(hipGetLastError());
+    return HR_OK;
+}
+
+hr_status hr_roulette_uniforms(uint64_t seed, int64_t first_id, int64_t n, double *out, void *stream) {
+    using namespace hr;
+    HR_REQUIRE(n >= 0 && first_id >= 0, "hr_roulette_uniforms: bad range");
+    if (n == 0) return HR_OK;
+    HR_REQUIRE(out != nullptr, "hr_roulette_uniforms: null output");
+    hipLaunchKernelGGL(roulette_uniforms_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, seed,
+                       first_id, n, out);
+    HR_CHECK_HIP(hipGetLastError());
+    return HR_OK;
+}
+
+hr_status hr_line_screen(const double *mass, int64_t n, int64_t first_id, int64_t pilot, double threshold,
+                         double floor_p, uint64_t seed, double *probability, uint8_t *kept, void *stream) {
+    using namespace hr;
+    HR_REQUIRE(n >= 0 && first_id >= 0 && pilot >= 0, "hr_line_screen: bad range");
+    HR_REQUIRE(floor_p > 0.0 && floor_p <= 1.0, "hr_line_screen: floor must lie in (0, 1]");
+    if (n == 0) return HR_OK;
+    HR_REQUIRE(mass && probability && kept, "hr_line_screen: null pointer");
+    hipLaunchKernelGGL(line_screen_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, mass, n,
+                       first_id, pilot, threshold, floor_p, seed, probability, kept);
     HR_CHECK_HIP(hipGetLastError());
     return HR_OK;
 }

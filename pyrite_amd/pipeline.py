@@ -64,6 +64,7 @@ class CaseConfig:
     precision: str = "fp64"  # transport precision: "fp64" or "mixed"
     line_route: str = "dense"  # "dense" sinc^2 over every bin, or "window" (support window + far tail)
     x_c: float = 100.0  # support cutoff of the windowed route
+    line_roulette: dict | None = None  # LineRoulette policy fields (per-primary line roulette), or None
 
     def grid(self):
         return bin_centers(self.E_lo_eV, self.E_hi_eV, self.n_bins)
@@ -148,9 +149,11 @@ class CasePipeline(CaseTables):
         self._cap_hint = res.capacity
         return res
 
-    def records(self, res):
+    def records(self, res, first_id=None):
         """FP32 scoring records gathered straight from the slot rows of a
-        ``compact=False`` transport (no FP64 compaction pass)."""
+        ``compact=False`` transport (no FP64 compaction pass); with
+        ``first_id`` (the batch's first global electron id) also each row's
+        primary id as ``"prim"``."""
         import ctypes as C
 
         import torch
@@ -162,11 +165,12 @@ class CasePipeline(CaseTables):
                          "v_hat": torch.empty((m, 3), dtype=torch.float32, device=self.dev),
                          "L": torch.empty(m, dtype=torch.float32, device=self.dev),
                          "E": torch.empty(m, dtype=torch.float32, device=self.dev),
-                         "w": torch.empty(m, dtype=torch.float32, device=self.dev)}
+                         "w": torch.empty(m, dtype=torch.float32, device=self.dev),
+                         "prim": torch.empty(m, dtype=torch.int64, device=self.dev)}
         rec = self._rec
         lib = _lib.load()
         o = 0
-        for _e0, m, cap, rows, seg_count in res.slots:
+        for e0, m, cap, rows, seg_count in res.slots:
             cnt = seg_count.clamp(max=cap)
             off = torch.cumsum(cnt, 0) - cnt + o
             g = _lib.VrGatherArgs()
@@ -178,20 +182,38 @@ class CasePipeline(CaseTables):
                 rows["seg_E_end"].data_ptr()
             g.r_mid, g.v_hat = rec["r_mid"].data_ptr(), rec["v_hat"].data_ptr()
             g.L, g.E, g.row_weight = rec["L"].data_ptr(), rec["E"].data_ptr(), rec["w"].data_ptr()
+            if first_id is not None:
+                prim = torch.arange(int(first_id) + e0, int(first_id) + e0 + m, dtype=torch.int64, device=self.dev)
+                g.prim, g.row_prim = prim.data_ptr(), rec["prim"].data_ptr()
             _lib.check(lib.hr_vr_gather(C.byref(g), C.c_void_p(torch.cuda.current_stream().cuda_stream)),
                        "hr_vr_gather")
             o += int(cnt.sum()) if len(res.slots) > 1 else 0
-        return {k: rec[k][:n_rows] for k in ("r_mid", "v_hat", "L", "E")}
+        keys = ("r_mid", "v_hat", "L", "E") + (("prim",) if first_id is not None else ())
+        return {k: rec[k][:n_rows] for k in keys}
 
-    def score_records(self, seg, Ne):
-        """Line and brem tallies of FP32 scoring records."""
+    def score_records(self, seg, Ne, line_weight=None):
+        """Line and brem tallies of FP32 scoring records; ``line_weight`` (FP32 per
+        row) weights the line tally only (the line-roulette strata)."""
         if self.cfg.line_route == "window":
-            self.spec_lines += self.lines.window_spectra(seg, Ne, x_c=self.cfg.x_c).sum(0)
+            self.spec_lines += self.lines.window_spectra(seg, Ne, x_c=self.cfg.x_c, row_weight=line_weight).sum(0)
         elif self.cfg.line_route == "dense":
-            self.lines.spectrum(seg, Ne, self.spec_lines)
+            self.lines.spectrum(seg, Ne, self.spec_lines, row_weight=line_weight)
         else:
             raise ValueError(f"unknown line route {self.cfg.line_route!r}")
         self.brem.spectrum(seg, Ne, self.spec_brem)
+
+    def line_screen(self, n_electrons):
+        """The case's per-primary line-roulette screen over ``n_electrons`` primaries, or None."""
+        if self.cfg.line_roulette is None:
+            return None
+        from .line_roulette import LineRoulette, LineScreen
+
+        scr = getattr(self, "_screen", None)
+        if scr is None or scr.n != int(n_electrons):
+            policy = LineRoulette.from_dict(self.cfg.line_roulette)
+            policy.validate_case({"max_dE_frac": self.control.max_dE_frac})
+            scr = self._screen = LineScreen(policy, self.cfg.seed, n_electrons, self.dev)
+        return scr
 
     def score(self, rows, Ne):
         seg = segments_fp32({
@@ -208,8 +230,86 @@ class CasePipeline(CaseTables):
         ``n_batches_total`` batches are summed.
         """
         res = self.transport(batch_index, compact=False)
-        self.score_records(self.records(res), self.cfg.Ne * n_batches_total)
+        n_total = self.cfg.Ne * n_batches_total
+        screen = self.line_screen(n_total)
+        if screen is None:
+            self.score_records(self.records(res), n_total)
+        else:
+            seg = self.records(res, first_id=int(batch_index) * self.cfg.Ne)
+            self.score_records(seg, n_total, screen.row_weights(self.lines, seg, seg["prim"], n_total))
         self.histories += self.cfg.Ne
+        self.segments += res.n_segments
+        return res
+
+    def stream(self, first_id: int, n_histories: int, n_total: int | None = None, lanes: int = 1 << 20,
+               rows_per_round: int = 32, timings: dict | None = None):
+        """Streamed transport -> scoring over electrons ``[first_id, first_id + n_histories)``.
+
+        Continuation rounds (``hr_transport_continue``) keep ``lanes`` histories
+        in flight, refilled from the pending primaries; every round's rows are
+        gathered into FP32 records and scored before the next launch, so device
+        memory holds one round of rows (``lanes x rows_per_round``) however many
+        histories run.  Tallies are per electron of ``n_total`` (default
+        ``n_histories``).  Rows and tallies equal the batched route's up to the
+        FP64 summation order of the tallies.
+        """
+        import ctypes as C
+
+        import torch
+
+        from .transport.continuation import ContinuationDriver
+
+        cfg = self.cfg
+        if cfg.line_roulette is not None:
+            raise NotImplementedError("line roulette screens whole primaries; the streamed route splits them "
+                                      "over rounds (use step())")
+        n_total = int(n_histories if n_total is None else n_total)
+        lanes = max(1, min(int(lanes), int(n_histories)))
+        drv = getattr(self, "_stream_drv", None)
+        if drv is None or drv.lanes != lanes or drv.R != int(rows_per_round):
+            self._stream_drv = None
+            drv = ContinuationDriver(self.tables, self.control, lanes, int(rows_per_round), self.dev,
+                                     cfg.E_cut_keV)
+            self._stream_drv = drv
+            m = lanes * int(rows_per_round)
+            self._srec = {"r_mid": torch.empty((m, 3), dtype=torch.float32, device=self.dev),
+                          "v_hat": torch.empty((m, 3), dtype=torch.float32, device=self.dev),
+                          "L": torch.empty(m, dtype=torch.float32, device=self.dev),
+                          "E": torch.empty(m, dtype=torch.float32, device=self.dev),
+                          "w": torch.empty(m, dtype=torch.float32, device=self.dev)}
+            self._sones = torch.ones(lanes, dtype=torch.float64, device=self.dev)
+        rec, lib = self._srec, _lib.load()
+        tilt = np.deg2rad(cfg.tilt_deg)
+        tim = timings if timings is not None else {}
+
+        def fill(start, n, pos, dirs, E, clock, keys):
+            fill_entry(cfg.beam, cfg.seed, start, n, cfg.E0_keV, tilt, pos, dirs, E, clock)
+            _lib.call("hr_stream_keys", int(cfg.seed), int(start), int(n), keys.data_ptr(), None)
+
+        def consume(rnd):
+            cnt = rnd.seg_count
+            incl = torch.cumsum(cnt, 0)
+            n_rows = int(incl[-1].item()) if rnd.n_lanes else 0
+            if n_rows == 0:
+                return
+            off = incl - cnt
+            g = _lib.VrGatherArgs()
+            g.n, g.rows = rnd.n_lanes, rnd.rows_per_round
+            g.seg_count, g.row_offset = cnt.data_ptr(), off.data_ptr()
+            g.weight = self._sones.data_ptr()
+            rows = rnd.rows
+            g.seg_mid, g.seg_dir = rows["seg_mid"].data_ptr(), rows["seg_dir"].data_ptr()
+            g.seg_len, g.seg_E, g.seg_E_end = rows["seg_len"].data_ptr(), rows["seg_E"].data_ptr(), \
+                rows["seg_E_end"].data_ptr()
+            g.r_mid, g.v_hat = rec["r_mid"].data_ptr(), rec["v_hat"].data_ptr()
+            g.L, g.E, g.row_weight = rec["L"].data_ptr(), rec["E"].data_ptr(), rec["w"].data_ptr()
+            _lib.check(lib.hr_vr_gather(C.byref(g), C.c_void_p(torch.cuda.current_stream().cuda_stream)),
+                       "hr_vr_gather")
+            self.score_records({k: rec[k][:n_rows] for k in ("r_mid", "v_hat", "L", "E")}, n_total)
+            tim["rows"] = tim.get("rows", 0) + n_rows
+
+        res = drv.run(int(first_id), int(n_histories), fill, consume)
+        self.histories += int(n_histories)
         self.segments += res.n_segments
         return res
 

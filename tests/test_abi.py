@@ -61,6 +61,7 @@ STRUCTS = {
     "hr_window_args": "WindowArgs",
     "hr_queue_config": "QueueConfig",
     "hr_queue_launch": "QueueLaunch",
+    "hr_continuation": "Continuation",
 }
 
 
