@@ -98,6 +98,11 @@ def cpu_baseline(case_tables, n_hist):
     }
 
 
+# HBM bytes per sincsq_uniform_kernel launch of the default workload (mean of the
+# step's two launches), measured with rocprofv3 --pmc and a copy calibration.
+SINC_TRAFFIC_BYTES = 384.6e6
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -180,7 +185,9 @@ def main():
             "peak": roofline.PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / roofline.PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": SINC_TRAFFIC_BYTES if (args.ne, args.n_side) == (100_000, 3) else None,
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE (x2.000) and WRITE_SIZE (x1.000), factors from a 1 GiB "
+                              "copy; profiles/sincsq_uniform_hopg_5mev_pmc.md",
             "ops_per_cell": roofline.SINC_CELL_OPS,
             "cells_per_launch": cells / len(ms),
             "avg_launch_ms": avg_ms,

@@ -366,6 +366,24 @@ __global__ void __launch_bounds__(256) sum_partials_kernel(const double *__restr
     out[k] += s;
 }
 
+// First level of the fixed-order partial sum: split s adds rows
+// [s*chunk, min((s+1)*chunk, n_part)) of every bin into level[s][k].
+__global__ void __launch_bounds__(256) sum_partials_split_kernel(const double *__restrict__ partial,
+                                                                 int64_t n_part, int64_t chunk, int64_t n_E,
+                                                                 double *__restrict__ level) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_E) return;
+    const int64_t b0 = (int64_t)blockIdx.y * chunk;
+    const int64_t b1 = b0 + chunk < n_part ? b0 + chunk : n_part;
+    double s = 0.0;
+    for (int64_t b = b0; b < b1; ++b) s += partial[b * n_E + k];
+    level[(int64_t)blockIdx.y * n_E + k] = s;
+}
+
+// Splits of the two-level partial sum (a fixed tree: deterministic, and wide
+// enough that the first level reads the partials at full bandwidth).
+constexpr int64_t kPartialSplits = 64;
+
 constexpr float kMc2f = 510.99895f;
 constexpr float kInvMc2f = (float)(1.0 / 510.99895);
 constexpr double kLog2e = 1.4426950408889634;
@@ -718,6 +736,27 @@ hr_status workspace(size_t need, void *&ptr, int &cu) {
     return HR_OK;
 }
 
+// Workspace bytes for gx partial rows of n_E bins plus the second level.
+size_t partial_bytes(int64_t gx, int64_t n_E) {
+    const int64_t splits = gx < kPartialSplits ? gx : kPartialSplits;
+    return (size_t)(gx + splits) * (size_t)n_E * sizeof(double);
+}
+
+// out[k] += the fixed two-level sum of gx partial rows held at ws.
+hr_status reduce_partials(void *ws, int64_t gx, int64_t n_E, double *out, hipStream_t s) {
+    const double *partial = (const double *)ws;
+    const int64_t splits = gx < kPartialSplits ? gx : kPartialSplits;
+    const int64_t chunk = (gx + splits - 1) / splits;
+    double *level = (double *)ws + gx * n_E;
+    const unsigned bx = (unsigned)((n_E + 255) / 256);
+    hipLaunchKernelGGL(sum_partials_split_kernel, dim3(bx, (unsigned)splits), dim3(256), 0, s, partial, gx, chunk,
+                       n_E, level);
+    HR_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(bx), dim3(256), 0, s, (const double *)level, splits, n_E, out);
+    HR_CHECK_HIP(hipGetLastError());
+    return HR_OK;
+}
+
 // Blocks along the line axis: enough to fill the chip several times over, but
 // never more than there are line tiles.
 int64_t line_blocks(int cu, int64_t n_tiles, int64_t bin_tiles) {
@@ -767,8 +806,7 @@ extern "C" hr_status hr_sincsq_reduce(const hr_sincsq_args *args, void *stream) 
     hr_status st = workspace(0, ws, cu);
     if (st != HR_OK) return st;
     const int64_t gx = line_blocks(cu, n_tiles, bin_tiles);
-    const size_t need = (size_t)gx * (size_t)p.n_E * sizeof(double);
-    st = workspace(need, ws, cu);
+    st = workspace(partial_bytes(gx, p.n_E), ws, cu);
     if (st != HR_OK) return st;
     hipStream_t s = (hipStream_t)stream;
     dim3 grid((unsigned)gx, (unsigned)bin_tiles);
@@ -780,10 +818,7 @@ extern "C" hr_status hr_sincsq_reduce(const hr_sincsq_args *args, void *stream) 
                            p.n_lines, p.E_grid, p.n_E, (double *)ws);
     }
     HR_CHECK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(sum_partials_kernel, dim3((unsigned)((p.n_E + 255) / 256)), dim3(256), 0, s,
-                       (const double *)ws, gx, p.n_E, p.out);
-    HR_CHECK_HIP(hipGetLastError());
-    return HR_OK;
+    return reduce_partials(ws, gx, p.n_E, p.out, s);
 }
 
 extern "C" hr_status hr_brem_reduce(const hr_brem_args *args, void *stream) {
@@ -803,16 +838,13 @@ extern "C" hr_status hr_brem_reduce(const hr_brem_args *args, void *stream) {
     hr_status st = workspace(0, ws, cu);
     if (st != HR_OK) return st;
     const int64_t gx = line_blocks(cu, n_tiles, bin_tiles);
-    st = workspace((size_t)gx * (size_t)p.n_E * sizeof(double), ws, cu);
+    st = workspace(partial_bytes(gx, p.n_E), ws, cu);
     if (st != HR_OK) return st;
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(brem_kernel<BINS>, dim3((unsigned)gx, (unsigned)bin_tiles), dim3(kRedThreads), 0, s, p,
                        (double *)ws);
     HR_CHECK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(sum_partials_kernel, dim3((unsigned)((p.n_E + 255) / 256)), dim3(256), 0, s,
-                       (const double *)ws, gx, p.n_E, p.out);
-    HR_CHECK_HIP(hipGetLastError());
-    return HR_OK;
+    return reduce_partials(ws, gx, p.n_E, p.out, s);
 }
 
 namespace hr {

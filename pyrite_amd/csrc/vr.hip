@@ -144,26 +144,24 @@ __global__ void __launch_bounds__(256) vr_spawn_kernel(hr_vr_spawn_args p) {
 
 // Rows of a round (particle i, slot i*rows + s, s < seg_count[i]) gathered to
 // [row_offset[i], ...) as FP32 scoring records with the particle's weight.
+// One thread per slot (particle i = t / rows, row s = t % rows): consecutive
+// threads read consecutive slot rows and write consecutive records.
 __global__ void __launch_bounds__(256) vr_gather_kernel(hr_vr_gather_args p) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.n) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= p.n * p.rows) return;
+    const int64_t i = t / p.rows, s = t - i * p.rows;
     const int64_t m = p.seg_count[i] < p.rows ? p.seg_count[i] : p.rows;
-    const int64_t o = p.row_offset[i];
-    const float w = (float)p.weight[i];
-    const float w2 = p.weight2 ? (float)p.weight2[i] : 0.0f;
-    const int64_t prim = p.prim ? p.prim[i] : i;
-    for (int64_t s = 0; s < m; ++s) {
-        const int64_t src = i * p.rows + s, dst = o + s;
-        for (int c = 0; c < 3; ++c) {
-            p.r_mid[3 * dst + c] = (float)p.seg_mid[3 * src + c];
-            p.v_hat[3 * dst + c] = (float)p.seg_dir[3 * src + c];
-        }
-        p.L[dst] = (float)p.seg_len[src];
-        p.E[dst] = (float)(0.5 * (p.seg_E[src] + p.seg_E_end[src]));
-        p.row_weight[dst] = w;
-        if (p.row_weight2) p.row_weight2[dst] = w2;
-        if (p.row_prim) p.row_prim[dst] = prim;
+    if (s >= m) return;
+    const int64_t src = t, dst = p.row_offset[i] + s;
+    for (int c = 0; c < 3; ++c) {
+        p.r_mid[3 * dst + c] = (float)p.seg_mid[3 * src + c];
+        p.v_hat[3 * dst + c] = (float)p.seg_dir[3 * src + c];
     }
+    p.L[dst] = (float)p.seg_len[src];
+    p.E[dst] = (float)(0.5 * (p.seg_E[src] + p.seg_E_end[src]));
+    p.row_weight[dst] = (float)p.weight[i];
+    if (p.row_weight2) p.row_weight2[dst] = p.weight2 ? (float)p.weight2[i] : 0.0f;
+    if (p.row_prim) p.row_prim[dst] = p.prim ? p.prim[i] : i;
 }
 
 // Expected window content of each segment's lines: a narrow sinc^2 line of
@@ -325,7 +323,10 @@ hr_status hr_vr_gather(const hr_vr_gather_args *p, void *stream) {
                    p->seg_E_end,
                "hr_vr_gather: null input");
     HR_REQUIRE(p->r_mid && p->v_hat && p->L && p->E && p->row_weight, "hr_vr_gather: null output");
-    hipLaunchKernelGGL(vr_gather_kernel, dim3(blocks_for(p->n, 256)), dim3(256), 0, (hipStream_t)stream, *p);
+    HR_REQUIRE(p->n <= (INT64_MAX / p->rows) && blocks_for(p->n * p->rows, 256) < (1ULL << 31),
+               "hr_vr_gather: %lld slots exceed one launch", (long long)p->n * (long long)p->rows);
+    hipLaunchKernelGGL(vr_gather_kernel, dim3(blocks_for(p->n * p->rows, 256)), dim3(256), 0, (hipStream_t)stream,
+                       *p);
     HR_CHECK_HIP(hipGetLastError());
     return HR_OK;
 }

@@ -68,6 +68,7 @@ class VRConfig:
     ref_quantile: float = 0.98  # states at this predicted quantile get w_t = 1/k_max
     peak_half_width_eV: float = 400.0
     roulette_keep_mass: float = 0.99  # rows at or above the mass carrying this fraction are scored surely
+    roulette_bound: bool = True  # roulette rows on expected mass + peak line weight (a content bound)
     label_eps_frac: float = 1.0e-3  # floor of the regression target, relative to the mean label
     train_rows: int = 2_000_000
     epochs: int = 6
@@ -165,6 +166,11 @@ class LearnedImportance:
         lo, hi = self.window if window else (-1.0e300, 1.0e300)
         _lib.call("hr_line_mass", E_r.data_ptr(), a.data_ptr(), w.data_ptr(), n, lines.n_dir,
                   len(lines.tables.wm), lo, hi, self.dE, out.data_ptr(), _stream())
+        if not window and self.vcfg.roulette_bound:
+            # A line's content on the bins is at most w (its peak) plus about its
+            # expected mass: rouletting on that bound keeps every survivor's
+            # contribution near the threshold, also for narrow lines that hit a bin.
+            out[:n] += w.view(lines.n_dir, n, len(lines.tables.wm)).double().sum(dim=(0, 2))
 
     def score_lines(self, seg, Ne, out, row_weight=None, roulette_key=None):
         """Dense line spectrum of ``seg`` into ``out``; with ``roulette_key`` the
