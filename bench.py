@@ -44,7 +44,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--breakdown", action="store_true", help="print per-phase times to stderr")
     p.add_argument("--no-fom", action="store_true", help="skip the learned-importance figure of merit")
-    p.add_argument("--fom-batches", type=int, default=24)
+    p.add_argument("--fom-batches", type=int, default=96)
     p.add_argument("--copy-calibration", action="store_true",
                    help="also run one 1 GiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     return p.parse_args()
@@ -52,6 +52,7 @@ def parse():
 
 def learned_fom(pipe_cfg, n_batches):
     """PXR-peak figure of merit of the learned importance vs the plain analog run."""
+    import numpy as np
     import torch
 
     from pyrite_amd.pipeline import CaseConfig, CasePipeline
@@ -69,10 +70,18 @@ def learned_fom(pipe_cfg, n_batches):
     vr.measure(1, Ne, "vr", first_batch=901)
     out = {"histories_per_batch": Ne, "batches": n_batches, "training_s": t_train,
            "training_histories": 8 * Ne, "window_eV": list(vr.window)}
+    rng = np.random.default_rng(0)
     for mode, first in (("analog", 0), ("roulette", 5000), ("vr", 1000)):
         vals, t = vr.measure(n_batches, Ne, mode, first_batch=first)
         mean, rel, fom = figure_of_merit(vals, t)
-        out[mode] = {"mean_photons_per_electron": mean, "rel_err": rel, "time_s": t, "fom": fom}
+        boot = []
+        for _ in range(400):  # bootstrap over batches: the peak tally is heavy-tailed
+            s = rng.choice(vals, vals.size)
+            r = s.std(ddof=1) / np.sqrt(s.size) / s.mean()
+            boot.append(1.0 / (r * r * t))
+        out[mode] = {"mean_photons_per_electron": mean, "rel_err": rel, "time_s": t, "fom": fom,
+                     "fom_boot68": [float(x) for x in np.quantile(boot, [0.16, 0.84])],
+                     "max_over_median": float(vals.max() / np.median(vals))}
     out["gain_learned_vs_analog"] = out["vr"]["fom"] / out["analog"]["fom"]
     out["gain_line_roulette_only"] = out["roulette"]["fom"] / out["analog"]["fom"]
     return out

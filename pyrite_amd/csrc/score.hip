@@ -49,6 +49,71 @@ __device__ __forceinline__ int64_t bracket(const float *__restrict__ grid, int64
     return y < 1 ? 1 : (y > n - 1 ? n - 1 : y);
 }
 
+// The same bracket, found from a guess instead of a bisection: the coupling
+// grids are geometric, so index ~ log(x / grid[0]) / log(ratio), and a short
+// walk from the guess lands on the first index with grid[i] >= x -- exactly
+// the bisection's answer for any nondecreasing grid.  A walk longer than
+// kWalk (a grid far from geometric) finishes by bisection.  Two or three
+// dependent loads replace the bisection's dozen.
+struct GridGuess {
+    float g0, inv_lr;
+};
+
+__device__ __forceinline__ GridGuess grid_guess(const float *__restrict__ grid, int64_t n) {
+    GridGuess q;
+    q.g0 = grid[0];
+    const float span = __logf(grid[n - 1] / q.g0);
+    q.inv_lr = span > 0.0f ? (float)(n - 1) / span : 0.0f;
+    return q;
+}
+
+__device__ __forceinline__ int64_t bracket_from(const float *__restrict__ grid, int64_t n, double x,
+                                                const GridGuess &q) {
+    constexpr int kWalk = 16;
+    int64_t y;
+    if (isnan(x)) {
+        y = n;
+    } else if (x > (double)grid[n - 1]) {
+        y = n;
+    } else {
+        const float t = (float)x > q.g0 ? __logf((float)x / q.g0) * q.inv_lr : 0.0f;
+        int64_t m = t < (float)(n - 1) ? (int64_t)t : n - 1;
+        int steps = 0;
+        if ((double)grid[m] < x) {
+            // grid[n-1] >= x, so the walk up ends
+            while ((double)grid[m] < x && steps < kWalk) {
+                ++m;
+                ++steps;
+            }
+            if ((double)grid[m] < x) {  // bisection over (m, n-1]
+                int64_t lo = m + 1, hi = n - 1;
+                while (lo < hi) {
+                    const int64_t c = lo + ((hi - lo) >> 1);
+                    if ((double)grid[c] < x) lo = c + 1;
+                    else hi = c;
+                }
+                m = hi;
+            }
+        } else {
+            while (m > 0 && (double)grid[m - 1] >= x && steps < kWalk) {
+                --m;
+                ++steps;
+            }
+            if (m > 0 && (double)grid[m - 1] >= x) {  // bisection over [0, m)
+                int64_t lo = 0, hi = m - 1;
+                while (lo < hi) {
+                    const int64_t c = lo + ((hi - lo) >> 1);
+                    if ((double)grid[c] < x) lo = c + 1;
+                    else hi = c;
+                }
+                m = hi;
+            }
+        }
+        y = m;
+    }
+    return y < 1 ? 1 : (y > n - 1 ? n - 1 : y);
+}
+
 __device__ __forceinline__ double slab_path(double z, double thickness, double nz) {
     const double inv_nz = 1.0 / fmax(fabs(nz), 1.0e-12);
     const double depth = z < 0.0 ? 0.0 : (z > thickness ? thickness : z);
@@ -91,6 +156,7 @@ __device__ __forceinline__ void segment_lines(const hr_line_args &p, int64_t j, 
     const double inv_E = kHbarc / (gamma * kMe);
     const double h = kHbarc / kMe;
     const int64_t n_tab = p.n_tab;
+    const GridGuess guess = grid_guess(p.E_tab, n_tab);
     for (int g = 0; g < p.n_g; ++g) {
         const double Gx = p.g[3 * g], Gy = p.g[3 * g + 1], Gz = p.g[3 * g + 2];
         const double g2 = Gx * Gx + Gy * Gy + Gz * Gz;
@@ -104,7 +170,7 @@ __device__ __forceinline__ void segment_lines(const hr_line_args &p, int64_t j, 
         double denom = denom_vac, previous = denom, delta = 0.0;
         for (int pass = 0; pass < 3; ++pass) {
             const double Ex = kHbarc * (v_dot_g / denom);
-            const int64_t i = bracket(p.E_tab, n_tab, Ex);
+            const int64_t i = bracket_from(p.E_tab, n_tab, Ex, guess);
             const double x0 = p.E_tab[i - 1];
             const double fr = (Ex - x0) / ((double)p.E_tab[i] - x0);
             const double f0 = p.delta_tab[i - 1];
@@ -132,7 +198,7 @@ __device__ __forceinline__ void segment_lines(const hr_line_args &p, int64_t j, 
         keep = keep && (denom > 0.0) && (oe <= 1.0 - 1.0 / gamma);
         double w = 0.0, a_width = 0.0, Er = 0.0;
         if (keep) {
-            const int64_t i = bracket(p.E_tab, n_tab, E_res);
+            const int64_t i = bracket_from(p.E_tab, n_tab, E_res, guess);
             const double x0 = p.E_tab[i - 1], x1 = p.E_tab[i];
             const double fr = (E_res - x0) / (x1 - x0);
             const bool below = E_res <= (double)p.E_tab[0];

@@ -55,7 +55,7 @@ class VRConfig:
     rows_far: int = 8  # rows between decisions far below it (w I/I_ref < far_ratio)
     far_ratio: float = 1.0 / 32.0
     horizon_rows: int = 2
-    k_max: float = 16.0  # smallest target weight is 1/k_max
+    k_max: float = 32.0  # smallest target weight is 1/k_max
     # weight-window band around the target weight.  Rouletting copies back up
     # to a larger weight multiplies the rare grazing / detector-bound rows they
     # may produce later (the heavy tail of the peak tally), so by default the
@@ -64,7 +64,7 @@ class VRConfig:
     ratio_hi: float = 2.0
     tail_fraction: float = 0.01  # below this share of the batch alive, rounds take tail_rows rows
     tail_rows: int = 32
-    n_max: int = 32  # copies per split
+    n_max: int = 64  # copies per split
     ref_quantile: float = 0.98  # states at this predicted quantile get w_t = 1/k_max
     peak_half_width_eV: float = 400.0
     roulette_keep_mass: float = 0.99  # rows at or above the mass carrying this fraction are scored surely
