@@ -43,6 +43,8 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=8000, help="histories in the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--breakdown", action="store_true", help="print per-phase times to stderr")
+    p.add_argument("--no-overlap", action="store_true", help="run transport and scoring of successive batches "
+                                                              "back to back on one stream")
     p.add_argument("--no-fom", action="store_true", help="skip the learned-importance figure of merit")
     p.add_argument("--fom-batches", type=int, default=96)
     p.add_argument("--copy-calibration", action="store_true",
@@ -136,7 +138,7 @@ def main():
     phase = {"transport": 0.0, "score": 0.0}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for s in range(args.steps):
+    for s in range(args.steps if args.breakdown else 0):
         if args.breakdown:
             ta = time.perf_counter()
             res = pipe.transport(s, compact=False)
@@ -148,8 +150,9 @@ def main():
             phase["score"] += time.perf_counter() - tb
             pipe.histories += cfg.Ne
             pipe.segments += res.n_segments
-        else:
-            pipe.step(s, args.steps)
+    if not args.breakdown:
+        # transport of batch s+1 overlaps the scoring of batch s (two HIP streams)
+        pipe.run_steps(0, args.steps, overlap=not args.no_overlap)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     pipe.lines.record_events = False

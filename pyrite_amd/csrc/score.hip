@@ -459,6 +459,7 @@ constexpr double kLog2e = 1.4426950408889634;
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // BH+Elwert brem over segment tiles (brem.py:396-423, brem_jit_kernel.py:76-153).
 constexpr int kBremTile = 128;
@@ -526,7 +527,20 @@ __global__ void __launch_bounds__(kRedThreads) brem_kernel(hr_brem_args p, doubl
         __syncthreads();
         const int64_t left = p.n_seg - tile * kBremTile;
         const int cnt = left < kBremTile ? (int)left : kBremTile;
+        // cells of kGroup segments are summed in FP32 (8 positive terms: relative
+        // rounding below 5e-7) before each FP64 accumulate
+        constexpr int kGroup = 8;
+        float part[BINS];
+#pragma unroll
+        for (int m = 0; m < BINS; ++m) part[m] = 0.0f;
         for (int q = 0; q < cnt; ++q) {
+            if ((q & (kGroup - 1)) == 0 && q > 0) {
+#pragma unroll
+                for (int m = 0; m < BINS; ++m) {
+                    acc[m] += (double)part[m];
+                    part[m] = 0.0f;
+                }
+            }
             // a zero-weight row (a copy that does not carry this tally) adds nothing
             if (s_pref[q * 4] == 0.0f && s_pref[q * 4 + 1] == 0.0f && s_pref[q * 4 + 2] == 0.0f &&
                 s_pref[q * 4 + 3] == 0.0f)
@@ -538,10 +552,13 @@ __global__ void __launch_bounds__(kRedThreads) brem_kernel(hr_brem_args p, doubl
                 const float Tf = T - keV[m];
                 if (!(Tf > 1.0e-6f) || !(keV[m] > 0.0f)) continue;
                 const float Tm = Tf + 2.0f * kMc2f;
-                const float pf = fast_sqrt(Tf * Tm) * kInvMc2f;
+                // p_f m = sqrt(T_f (T_f + 2m)) and 1/beta_f = (m + T_f) / (p_f m), from one v_rsq_f32
+                const float TT = Tf * Tm;
+                const float rs = fast_rsq(TT);
+                const float pf = TT * rs * kInvMc2f;
                 // beta_f = p_f / (1 + T_f/m);  (p_i + p_f)/(p_i - p_f) with
                 // p_i - p_f = k (T_i + T_f + 2m) / (m^2 (p_i + p_f)) (no cancellation for k << T_i)
-                const float rcp_bf = fast_rcp(pf * kMc2f) * (kMc2f + Tf);
+                const float rcp_bf = rs * (kMc2f + Tf);
                 const float spf = pi_ + pf;
                 const float born = __logf(spf * spf * (kMc2f * kMc2f) * fast_rcp(keV[m] * (T + Tm)));
                 const float base = born * inv_kev[m] * rcp_bf;
@@ -560,9 +577,11 @@ __global__ void __launch_bounds__(kRedThreads) brem_kernel(hr_brem_args p, doubl
                     ratio = 1.0f + d * (-1.0f / 2 + d * (1.0f / 6 + d * (-1.0f / 24 + d * (1.0f / 120 +
                             d * (-1.0f / 720 + d * (1.0f / 5040 + d * (-1.0f / 40320 + d * (1.0f / 362880))))))));
                 }
-                acc[m] += (double)(cell * (fast_exp2(-plo * mu2[m]) * ratio));
+                part[m] += cell * (fast_exp2(-plo * mu2[m]) * ratio);
             }
         }
+#pragma unroll
+        for (int m = 0; m < BINS; ++m) acc[m] += (double)part[m];
         __syncthreads();
     }
     double *out = partial + (int64_t)blockIdx.x * p.n_E;

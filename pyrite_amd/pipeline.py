@@ -137,14 +137,17 @@ class CasePipeline(CaseTables):
     def transport(self, batch_index: int, compact: bool = True):
         """Transport histories ``[b*Ne, (b+1)*Ne)``; returns the device segment rows
         (compacted, or the slot rows of each sub-batch with ``compact=False``)."""
+        import torch
+
         cfg = self.cfg
         n = cfg.Ne
-        _lib.call("hr_stream_keys", int(cfg.seed), int(batch_index) * n, n, self.keys.data_ptr(), None)
+        stream = torch.cuda.current_stream().cuda_stream
+        _lib.call("hr_stream_keys", int(cfg.seed), int(batch_index) * n, n, self.keys.data_ptr(), stream)
         fill_entry(cfg.beam, cfg.seed, int(batch_index) * n, n, cfg.E0_keV, np.deg2rad(cfg.tilt_deg),
-                   self.pos, self.dirs, self.E, self.clock)
+                   self.pos, self.dirs, self.E, self.clock, stream=stream)
         res = drive_batches(self.core, self.keys,
                             (self.alive, self.clock, self.pos, self.dirs, self.E, self.E_cut),
-                            self.stragg, self.control, n, self.dev, self.tcfg, compact=compact,
+                            self.stragg, self.control, n, self.dev, self.tcfg, stream=stream, compact=compact,
                             cap_hint=self._cap_hint)
         self._cap_hint = res.capacity
         return res
@@ -240,6 +243,46 @@ class CasePipeline(CaseTables):
         self.histories += self.cfg.Ne
         self.segments += res.n_segments
         return res
+
+    def run_steps(self, first_batch: int, n_steps: int, n_batches_total: int | None = None, overlap: bool = True):
+        """Batches ``first_batch ... first_batch + n_steps - 1`` through transport and scoring.
+
+        With ``overlap`` the transport of batch s+1 runs on one HIP stream while
+        batch s is scored on another: the transport kernel is latency bound
+        and leaves issue slots that the scoring kernels fill.  Each batch's slot
+        rows are handed to the scoring stream (``record_stream``) and scored in
+        batch order, so tallies are those of ``step()`` run back to back.
+        """
+        import torch
+
+        n_total_b = int(n_steps if n_batches_total is None else n_batches_total)
+        if not overlap or self.cfg.line_roulette is not None:
+            for b in range(int(n_steps)):
+                self.step(int(first_batch) + b, n_total_b)
+            return
+        n_total = self.cfg.Ne * n_total_b
+        main = torch.cuda.current_stream()
+        if getattr(self, "_streams", None) is None:
+            self._streams = (torch.cuda.Stream(device=self.dev), torch.cuda.Stream(device=self.dev))
+        sa, sb = self._streams
+        sa.wait_stream(main)
+        sb.wait_stream(main)
+        for b in range(int(n_steps)):
+            with torch.cuda.stream(sa):
+                res = self.transport(int(first_batch) + b, compact=False)
+                done = torch.cuda.Event()
+                done.record(sa)
+            for _e0, _m, _cap, rows, cnt in res.slots:
+                for t in rows.values():
+                    t.record_stream(sb)
+                cnt.record_stream(sb)
+            with torch.cuda.stream(sb):
+                sb.wait_event(done)
+                self.score_records(self.records(res), n_total)
+            self.histories += self.cfg.Ne
+            self.segments += res.n_segments
+        main.wait_stream(sa)
+        main.wait_stream(sb)
 
     def stream(self, first_id: int, n_histories: int, n_total: int | None = None, lanes: int = 1 << 20,
                rows_per_round: int = 32, timings: dict | None = None):
