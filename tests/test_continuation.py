@@ -117,3 +117,28 @@ def test_refill_gives_each_primary_its_entry_state():
 
     drv.run(first, n, fill)
     assert seen == {h: 10.0 * h for h in range(first, first + n)}
+
+
+def test_empty_run_and_more_lanes_than_histories():
+    drv = ContinuationDriver(None, TransportControl(), 8, 3, "cpu", 5.0)
+    stand = StandIn(drv, 0, np.array([], dtype=int))
+    drv._launch = stand
+    res = drv.run(0, 0, lambda *a: None)
+    assert res.n_segments == 0 and res.rounds == 0 and stand.launches == []
+    T = np.array([5, 0, 2])
+    drv = ContinuationDriver(None, TransportControl(), 8, 3, "cpu", 5.0)
+    stand = StandIn(drv, 10, T)
+    drv._launch = stand
+
+    def fill(start, m, pos, dirs, E, clock, keys):
+        E[:m] = 1.0
+
+    res = drv.run(10, 3, fill)
+    assert res.n_segments == 7 and res.counts["n_not_entered"] == 1
+
+
+def test_driver_refuses_empty_populations():
+    with pytest.raises(ValueError):
+        ContinuationDriver(None, TransportControl(), 0, 3, "cpu", 5.0)
+    with pytest.raises(ValueError):
+        ContinuationDriver(None, TransportControl(), 4, 0, "cpu", 5.0)

@@ -18,7 +18,7 @@ from tests.transport_cases import FRAMED, HOPG_5MEV, MULTILAYER, SI_30KEV_STRAGG
 pytestmark = pytest.mark.gpu
 
 
-def _continued(case, device, lanes, R, precision="fp64"):
+def _continued(case, device, lanes, R, precision="fp64", max_steps=400_000):
     import torch
 
     from pyrite_amd import _lib
@@ -28,7 +28,7 @@ def _continued(case, device, lanes, R, precision="fp64"):
     lt, lut = case.tables()
     pos, dirs, E, E_cut = case.state()
     tabs = DeviceTables(lt, lut, device)
-    ctl = TransportControl(max_steps=400_000, energy_model=case.energy_model, max_dE_frac=case.max_dE_frac,
+    ctl = TransportControl(max_steps=max_steps, energy_model=case.energy_model, max_dE_frac=case.max_dE_frac,
                            straggle_on=case.straggle, finite_footprint=case.finite is not None,
                            width_ang=case.finite[0] if case.finite else 0.0,
                            height_ang=case.finite[1] if case.finite else 0.0, precision=precision)
@@ -61,27 +61,30 @@ def _continued(case, device, lanes, R, precision="fp64"):
     return res, rounds
 
 
-@pytest.mark.parametrize("case,lanes,R,precision", [
-    (SI_30KEV_STRAGGLE, 64, 3, "fp64"),
-    (HOPG_5MEV, 50, 7, "fp64"),
-    (HOPG_5MEV, 96, 5, "mixed"),
-    (MULTILAYER, 40, 2, "fp64"),
-    (FRAMED, 128, 16, "fp64"),
-], ids=["si30s", "hopg5", "hopg5-mixed", "stack", "prism"])
-def test_continuation_is_bitwise_the_uninterrupted_run(device, case, lanes, R, precision):
+@pytest.mark.parametrize("case,lanes,R,precision,max_steps", [
+    (SI_30KEV_STRAGGLE, 64, 3, "fp64", 400_000),
+    (HOPG_5MEV, 50, 7, "fp64", 400_000),
+    (HOPG_5MEV, 96, 5, "mixed", 400_000),
+    (MULTILAYER, 40, 2, "fp64", 400_000),
+    (FRAMED, 128, 16, "fp64", 400_000),
+    (HOPG_5MEV, 37, 4, "fp64", 30),  # step limit reached inside a round
+], ids=["si30s", "hopg5", "hopg5-mixed", "stack", "prism", "hopg5-step-limit"])
+def test_continuation_is_bitwise_the_uninterrupted_run(device, case, lanes, R, precision, max_steps):
     import torch
 
     from pyrite_amd.transport.continuation import ContinuationRound, canonical_rows
 
     cap = 4096
-    ref_rows, ref_cnt, ref_code, ref_st = _gpu_run(case, device, cap=cap, precision=precision)
+    ref_rows, ref_cnt, ref_code, ref_st = _gpu_run(case, device, cap=cap, precision=precision, max_steps=max_steps)
     assert ref_cnt.max() <= cap
+    if max_steps < 1000:
+        assert np.any(ref_code == 4), "fixture must hit the step limit"
     keep = (np.arange(cap)[None, :] < ref_cnt[:, None]).reshape(-1)
     ref = {k: v[keep] for k, v in ref_rows.items()}
     n_rows = int(ref_cnt.sum())
     offsets = torch.as_tensor(np.concatenate([[0], np.cumsum(ref_cnt)[:-1]]), device=device)
 
-    res, rounds = _continued(case, device, lanes, R, precision)
+    res, rounds = _continued(case, device, lanes, R, precision, max_steps)
     assert res.n_segments == n_rows
     assert res.rounds == len(rounds) and len(rounds) > 2
     seen = np.zeros(n_rows, dtype=np.int64)
@@ -131,4 +134,6 @@ def test_streamed_case_matches_the_batched_route(device, route):
     assert sres.n_segments == res.n_segments and sres.counts == res.counts
     assert la.sum() > 0 and ba.sum() > 0
     np.testing.assert_allclose(lb, la, rtol=1e-9, atol=1e-12 * la.max())
-    np.testing.assert_allclose(bb, ba, rtol=1e-9, atol=1e-12 * ba.max())
+    # brem cells are summed in FP32 over groups of 8 rows before each FP64
+    # accumulate, so another row order moves the last bits at about 1e-8
+    np.testing.assert_allclose(bb, ba, rtol=1e-7, atol=1e-10 * ba.max())
