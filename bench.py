@@ -32,6 +32,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
@@ -50,6 +52,13 @@ def parse():
                                                               "back to back on one stream")
     p.add_argument("--no-fom", action="store_true", help="skip the learned-importance figure of merit")
     p.add_argument("--fom-batches", type=int, default=1024)
+    p.add_argument("--workload", choices=("hopg", "si-stream", "pixels"), default="hopg",
+                   help="hopg: the bench line (default); si-stream: the 1 mm Si 10 MeV case streamed through "
+                        "continuation rounds; pixels: windowed scoring of synthetic segments over 32x32 pixels")
+    p.add_argument("--histories", type=int, default=1 << 21, help="si-stream: histories")
+    p.add_argument("--lanes", type=int, default=1 << 19, help="si-stream: lanes in flight")
+    p.add_argument("--rows-per-round", type=int, default=32, help="si-stream: rows per launch")
+    p.add_argument("--log2-segments", type=int, default=20, help="pixels: log2 of the segment count")
     p.add_argument("--copy-calibration", action="store_true",
                    help="also run one 1 GiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     return p.parse_args()
@@ -95,6 +104,72 @@ def learned_fom(pipe_cfg, n_batches):
     return out
 
 
+def si_stream(args):
+    """1 mm Si (111), 10 MeV Twiss beam, 16 reflections x 9 rays, 4096 bins, streamed."""
+    import torch
+
+    from pyrite_amd.pipeline import CaseConfig, CasePipeline
+
+    cfg = CaseConfig.silicon_1mm_10MeV(Ne=65536)
+    pipe = CasePipeline(cfg)
+    pipe.stream(10**9, 65536, lanes=args.lanes, rows_per_round=args.rows_per_round)  # warm-up
+    torch.cuda.synchronize()
+    pipe.spec_lines.zero_()
+    pipe.spec_brem.zero_()
+    t0 = time.perf_counter()
+    res = pipe.stream(0, args.histories, lanes=args.lanes, rows_per_round=args.rows_per_round)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    return {"metric": "histories/s, streamed transport -> scoring", "value": args.histories / t,
+            "unit": "histories/s", "n_gpus": 1, "seconds": t, "rows_per_second": res.n_segments / t,
+            "rows_per_history": res.n_segments / args.histories, "rounds": res.rounds, "counts": res.counts,
+            "dtype": "mixed transport / f64 line phase / f64 tallies", "data": "synthetic (seeded Twiss beam)",
+            "config": {"workload": "1 mm Si (111) tilted 1 deg, 10 MeV Twiss beam (sigma 1 mm, 0.5 mrad), "
+                                   "16 reflections x 9 rays (windowed route) + brem, 4096 bins 1-100 keV",
+                       "histories": args.histories, "lanes": args.lanes, "rows_per_round": args.rows_per_round}}
+
+
+def pixel_scoring(args):
+    """Windowed line scoring of synthetic FP32 segments: 64 reflections, 32x32 pixels, 4096 bins."""
+    import torch
+
+    from pyrite_amd.backend import require_device
+    from pyrite_amd.spectrum.geometry import cone_detector
+    from pyrite_amd.spectrum.lines import LineScorer, synthetic_line_tables
+
+    dev = require_device().torch_device
+    rng = np.random.default_rng(2026)
+    n = 1 << args.log2_segments
+    n_E = 4096
+    E0, dE = 1000.0 + 0.5 * 99000.0 / n_E, 99000.0 / n_E
+    E_grid = E0 + dE * np.arange(n_E)
+    tables = synthetic_line_tables(64, E_grid, rng)
+    n_hats, weights = cone_detector(np.pi / 2, 0.0, 0.01, 32)
+    r = np.stack([rng.uniform(-1e6, 1e6, n), rng.uniform(-1e6, 1e6, n), rng.uniform(0, 5e6, n)], 1)
+    d = np.stack([rng.normal(0, 1e-3, n), rng.normal(0, 1e-3, n), np.ones(n)], 1)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    E = np.exp(rng.uniform(np.log(1000.0), np.log(20000.0), n))
+    L = rng.exponential(1.0e4, n)
+    seg = {k: torch.as_tensor(np.ascontiguousarray(v, dtype=np.float32), device=dev)
+           for k, v in (("r_mid", r), ("v_hat", d), ("L", L), ("E", E))}
+    sc = LineScorer(tables, E_grid, 5.0e6, n_hats, weights, dev)
+    sc.window_spectra({k: v[:4096] for k, v in seg.items()}, 1.0)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = sc.window_spectra(seg, 1.0)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    lines = n * 64 * n_hats.shape[0]
+    o = out.cpu().numpy()
+    return {"metric": "(segment, reflection, pixel) lines/s, windowed route", "value": lines / t,
+            "unit": "lines/s", "n_gpus": 1, "seconds": t, "lines": lines,
+            "full_workload_seconds": (2**26 * 64 * 1024) / (lines / t),
+            "finite": bool(np.isfinite(o).all()), "nonzero_bins": int(np.count_nonzero(o)),
+            "dtype": "f32 records / f64 prologue and phase / f64 tallies", "data": "synthetic segments (seed 2026)",
+            "config": {"workload": "2^%d segments x 64 reflections x 32x32 pixels x 4096 bins 1-100 keV"
+                                   % args.log2_segments}}
+
+
 def cpu_baseline(case_tables, n_hist):
     from oracle import case as ocase
 
@@ -126,6 +201,10 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every box here has one "
               f"GPU and the multi-GPU design is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "hopg":
+        out = si_stream(args) if args.workload == "si-stream" else pixel_scoring(args)
+        print(json.dumps(out))
+        return
     import torch
 
     from pyrite_amd.pipeline import CaseConfig, CasePipeline

@@ -601,7 +601,12 @@ __global__ void __launch_bounds__(kRedThreads) brem_kernel(hr_brem_args p, doubl
 // quarter of the grid or more) and nodes outside [-n, 2n) are summed exactly
 // over the whole grid.  Per-direction spectra and in-grid moments live in LDS
 // (FP64 atomics); out-of-grid moments go to global memory.
-constexpr int kWinThreads = 512;
+// Threads per window workgroup (one workgroup per CU: the spectrum and moments
+// fill its LDS); a tuning build may set HR_WINDOW_THREADS.
+#ifndef HR_WINDOW_THREADS
+#define HR_WINDOW_THREADS 512
+#endif
+constexpr int kWinThreads = HR_WINDOW_THREADS;
 constexpr int kWinMaxBins = 4096;
 
 struct WindowShared {

@@ -346,6 +346,8 @@ class LearnedImportance:
 
         ``stats`` (a dict) accumulates weighted observables: ``rows`` (sum of
         row weights), ``exit_<code>`` (weight ending with each exit code),
+        ``wmass`` (the rows' expected peak-window line content, weighted: a
+        smooth observable free of the bin-hit lottery, for bias checks),
         ``particles`` (particle-rounds) and ``rounds``."""
         import torch
 
@@ -413,6 +415,9 @@ class LearnedImportance:
                 stats["rounds"] = stats.get("rounds", 0) + 1
             if n_rows:
                 seg = {k: t[:n_rows] for k, t in s.items()}
+                if stats is not None:
+                    wm = self.row_masses(seg, Ne_total) * self._row_w[:n_rows].double()
+                    stats["wmass"] = stats.get("wmass", 0.0) + float(wm.sum())
                 self.score_lines(seg, Ne_total, spec_lines, self._row_w[:n_rows], rkey)
                 p.brem.reduce(seg, brem_acc, row_weight=self._row_w2[:n_rows])
             # split / roulette the suspended particles

@@ -246,7 +246,8 @@ def _analog_observables(pipe, vr, n_batches, first):
         pipe.lines.spectrum(seg, Ne, lines)
         pipe.brem.reduce(seg, brem)
         peak = vr.peak_tally(lines.cpu().numpy())
-        out.append((res.n_segments / Ne, (code == 2).mean(), (code == 1).mean(), peak))
+        wmass = float(vr.row_masses(seg, Ne).sum())
+        out.append((res.n_segments / Ne, (code == 2).mean(), (code == 1).mean(), peak, wmass))
     return np.array(out)
 
 
@@ -281,15 +282,15 @@ def test_learned_importance_unbiased(small_case):
             brem = torch.zeros_like(lines)
             vr.run_batch(300 + b, Ne, Ne, lines, brem, use_vr=True, stats=st, roulette=roulette)
             lrn.append((st["rows"] / Ne, st.get("exit_2", 0.0) / Ne, st.get("exit_1", 0.0) / Ne,
-                        vr.peak_tally(lines.cpu().numpy()), st["rows2"] / Ne))
+                        vr.peak_tally(lines.cpu().numpy()), st["wmass"], st["rows2"] / Ne))
         lrn = np.array(lrn)
         ana = _analog_observables(small_case, vr, 12, first=400)
-        for col, name in enumerate(("rows", "transmitted", "backscattered", "peak")):
+        for col, name in enumerate(("rows", "transmitted", "backscattered", "peak", "window mass")):
             if ana[:, col].std() == 0 and lrn[:, col].std() == 0:
                 continue
             assert _zs(lrn[:, col], ana[:, col]) < 4.0, (name, roulette)
         # the brem weight follows one copy per primary: its weighted rows match the analog rows
-        assert _zs(lrn[:, 4], ana[:, 0]) < 4.0
+        assert _zs(lrn[:, 5], ana[:, 0]) < 4.0
 
 
 @pytest.fixture(scope="module")
