@@ -63,7 +63,10 @@ hr_status hr_stream_keys(uint64_t seed, int64_t start, int64_t n, uint64_t *keys
  *    4 step-limited, 5 not-entered};
  *  - draws are counter-addressed by (key[e], counter), so results do not
  *    depend on launch geometry or batch split.
- * All pointers are device pointers.  Rows are structure-of-arrays.
+ * All pointers are device pointers.  Rows are structure-of-arrays.  Lanes claim
+ * electrons from a library-held per-device counter: two hr_transport_run calls
+ * on one device must not overlap (one transport stream beside scoring streams
+ * is fine).
  * ---------------------------------------------------------------------- */
 typedef struct hr_transport_args {
     /* run */

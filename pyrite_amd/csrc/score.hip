@@ -604,7 +604,7 @@ __global__ void __launch_bounds__(kRedThreads) brem_kernel(hr_brem_args p, doubl
 // Threads per window workgroup (one workgroup per CU: the spectrum and moments
 // fill its LDS); a tuning build may set HR_WINDOW_THREADS.
 #ifndef HR_WINDOW_THREADS
-#define HR_WINDOW_THREADS 512
+#define HR_WINDOW_THREADS 1024
 #endif
 constexpr int kWinThreads = HR_WINDOW_THREADS;
 constexpr int kWinMaxBins = 4096;
