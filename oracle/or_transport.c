@@ -40,8 +40,10 @@
 #define LN2 0.6931471805599453
 
 /* events.py:36-131 */
-enum { EV_ELASTIC = 0, EV_SUBSTEP = 1, EV_LAYER = 2, EV_EXIT_TOP = 3, EV_EXIT_BOTTOM = 4,
-       EV_EXIT_SIDE = 5, EV_GROOVE = 6, EV_CUTOFF = 7 };
+/* events.py:36-43: ELASTIC 0, SUBSTEP 1, LAYER_BOUNDARY 2, GROOVE_SURFACE 3 (never
+   emitted on this path), EXIT_TOP 4, EXIT_BOTTOM 5, EXIT_SIDE 6, CUTOFF 7 */
+enum { EV_ELASTIC = 0, EV_SUBSTEP = 1, EV_LAYER = 2, EV_GROOVE = 3, EV_EXIT_TOP = 4, EV_EXIT_BOTTOM = 5,
+       EV_EXIT_SIDE = 6, EV_CUTOFF = 7 };
 /* cores.py:90-95 */
 enum { EXIT_CUTOFF = 0, EXIT_BACK = 1, EXIT_TRANS = 2, EXIT_SIDE = 3, EXIT_STEP_LIMITED = 4,
        EXIT_NOT_ENTERED = 5 };

@@ -24,8 +24,10 @@ constexpr double kRate = 0.55;      // Urban r                  straggling.py:11
 constexpr double kChunk = 64.0;     // Poisson chunk mean       straggling.py:125
 constexpr double kPi = 3.141592653589793;
 
-enum : int8_t { EV_ELASTIC = 0, EV_SUBSTEP = 1, EV_LAYER = 2, EV_EXIT_TOP = 3, EV_EXIT_BOTTOM = 4,
-                EV_EXIT_SIDE = 5, EV_CUTOFF = 7 };
+// Row event codes of montecarlo/transport/events.py:36-43 (3 is the groove
+// surface, which this path never emits).
+enum : int8_t { EV_ELASTIC = 0, EV_SUBSTEP = 1, EV_LAYER = 2, EV_EXIT_TOP = 4, EV_EXIT_BOTTOM = 5,
+                EV_EXIT_SIDE = 6, EV_CUTOFF = 7 };
 enum : int8_t { EXIT_CUTOFF = 0, EXIT_BACK = 1, EXIT_TRANS = 2, EXIT_SIDE = 3, EXIT_STEP_LIMITED = 4,
                 EXIT_NOT_ENTERED = 5, EXIT_SUSPENDED = 6 };
 

@@ -178,3 +178,21 @@ def test_simulate_trajectories_api(device):
     assert n == int(out["seg_count"].sum())
     assert out["n_cutoff_stopped"] + out["n_backscattered"] + out["n_transmitted"] == 500
     np.testing.assert_allclose(out["E_repr_keV"], 0.5 * (out["E_keV"] + out["E_end_keV"]))
+
+
+@pytest.mark.parametrize("case", [c for c in CASES if c.energy_model == 1], ids=lambda c: c.name)
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_device_rows_keep_the_event_contract(device, case, precision):
+    """events.py:156 on the device's own rows: flight/substep sequencing, terminal
+    rows last with the event code the exit code implies, continuity of energy,
+    clock and position between rows."""
+    from tests import event_contract
+
+    rows, cnt, code, _ = _gpu_run(case, device, cap=4096, precision=precision)
+    assert cnt.max() <= 4096
+    keep = (np.arange(4096)[None, :] < cnt[:, None]).reshape(-1)
+    r = {k: v[keep] for k, v in rows.items()}
+    event_contract.check(r)
+    last = np.cumsum(cnt) - 1
+    want = {0: 7, 1: 4, 2: 5, 3: 6}
+    np.testing.assert_array_equal(r["seg_event"][last], [want[int(c)] for c in code])

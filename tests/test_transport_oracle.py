@@ -8,6 +8,7 @@ plus the event contract of transport/events.py:156 check_segment_event_contract.
 import numpy as np
 import pytest
 
+from tests import event_contract
 from tests.transport_cases import CASES, HOPG_5MEV, MULTILAYER, SI_30KEV
 
 
@@ -36,7 +37,7 @@ def test_event_contract(oracle, case):
     ev_last = r["seg_event"][last]
     # exit code <-> last event (events.py:36-131; exit codes _jit_device.py:63-69)
     # (the frozen schema carries no event column: batching.py:398-400)
-    want = {0: 7, 1: 3, 2: 4, 3: 5}
+    want = {0: 7, 1: 4, 2: 5, 3: 6}
     if case.energy_model == 1:
         for c, ev in zip(code, ev_last):
             assert want[int(c)] == int(ev)
@@ -48,6 +49,31 @@ def test_event_contract(oracle, case):
     stopped = code == 0
     if stopped.any():
         np.testing.assert_array_equal(state["E_keV"][stopped], case.E_cut)
+    # the full row-event contract (events.py:156): flight/substep sequencing,
+    # terminal rows last, continuity of energy, clock and position
+    if case.energy_model == 1:
+        event_contract.check(r)
+
+
+def test_event_contract_catches_a_broken_history(oracle):
+    """Planted in the oracle's own rows: a dropped row, a moved row end and a
+    swapped event code each fail the contract."""
+    rows, count, code, state = _run(oracle, SI_30KEV)
+    r = oracle.compact(rows, count, 4096)
+    event_contract.check(r)
+    keep = np.ones(len(r["seg_len"]), dtype=bool)
+    k = int(np.flatnonzero(r["seg_event"] == 0)[3])
+    keep[k] = False  # drop an elastic row from the middle of a history
+    with pytest.raises(AssertionError, match="flight|position|energy|clock"):
+        event_contract.check({key: v[keep] for key, v in r.items()})
+    moved = {key: v.copy() for key, v in r.items()}
+    moved["seg_len"][k] *= 1.5
+    with pytest.raises(AssertionError, match="position"):
+        event_contract.check(moved)
+    swapped = {key: v.copy() for key, v in r.items()}
+    swapped["seg_event"][k] = 1  # an elastic row claimed as a substep
+    with pytest.raises(AssertionError, match="substep|flight"):
+        event_contract.check(swapped)
 
 
 @pytest.mark.parametrize("case", [SI_30KEV, MULTILAYER], ids=lambda c: c.name)
