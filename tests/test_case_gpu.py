@@ -35,11 +35,12 @@ def _gpu_batches(pipe):
 def _cpu_batches(pipe, first):
     from oracle import case as ocase
 
+    n = pipe.cfg.Ne
     lines, brem = [], []
     for b in range(BATCHES):
-        ln, br, _ = ocase.run_case(pipe, first + b * PER_BATCH, PER_BATCH)
-        lines.append(ln / PER_BATCH)
-        brem.append(br / (4.0 * np.pi * PER_BATCH) * pipe.omega)
+        ln, br, _ = ocase.run_case(pipe, first + b * n, n)
+        lines.append(ln / n)
+        brem.append(br / (4.0 * np.pi * n) * pipe.omega)
     return np.array(lines), np.array(brem)
 
 
@@ -65,6 +66,24 @@ def test_hopg_case_spectra_agree_with_the_oracle_per_bin(device):
     pipe = CasePipeline(CaseConfig(Ne=PER_BATCH))
     g_lines, g_brem = _gpu_batches(pipe)
     c_lines, c_brem = _cpu_batches(pipe, first=10_000_000)
+    assert g_lines.sum() > 0 and c_lines.sum() > 0
+    _agree(g_brem, c_brem, "brem")
+    _agree(g_lines, c_lines, "lines")
+
+
+def test_silicon_twiss_case_agrees_with_the_oracle_per_bin(device):
+    """The 1 mm Si (111), 10 MeV Twiss-beam case on its device route (mixed
+    transport precision, windowed line scoring) against the oracle's FP64
+    transport and dense sinc^2, on 4 reflections, one ray and 1024 bins so the
+    CPU side stays small."""
+    from pyrite_amd.pipeline import CaseConfig, CasePipeline
+
+    cfg = CaseConfig.silicon_1mm_10MeV(Ne=120)
+    cfg = CaseConfig.silicon_1mm_10MeV(Ne=120, hkl_list=cfg.hkl_list[:4], n_side=1, n_bins=1024)
+    assert cfg.precision == "mixed" and cfg.line_route == "window"
+    pipe = CasePipeline(cfg)
+    g_lines, g_brem = _gpu_batches(pipe)
+    c_lines, c_brem = _cpu_batches(pipe, first=20_000_000)
     assert g_lines.sum() > 0 and c_lines.sum() > 0
     _agree(g_brem, c_brem, "brem")
     _agree(g_lines, c_lines, "lines")
