@@ -471,6 +471,31 @@ hr_status hr_line_mass(const double *E_r, const double *a, const float *w, int64
 hr_status hr_line_roulette(float *w, int64_t n_seg, int32_t n_dir, int32_t n_g, const double *mass,
                            double m_thr, uint64_t key, int64_t row0, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Depth-dose tally of transport rows: each row's loss E - E_end spread over
+ * the depth bins its flight crosses in proportion to the depth covered there,
+ * plus the residual E_end of a history that ends at the cutoff (event 7,
+ * montecarlo/transport/events.py:43) at the row's end; deposits outside the
+ * grid go to the edge bins.  With the energy of escaping electrons this closes
+ * the energy balance: sum(out)/scale + sum(escaped E) = Ne E0.  The depth-dose
+ * profile is one of the plain method's parity observables; the reference's
+ * transport modules have no single function for it (its runners bin rows).
+ * Rows are slot-major: particle i's rows are slots i*rows + s, s < seg_count[i]
+ * (seg_count NULL: every slot is a row, e.g. rows = 1 for compacted rows).
+ * ---------------------------------------------------------------------- */
+typedef struct hr_dose_args {
+    int64_t n, rows;
+    const int64_t *seg_count; /* [n] or NULL */
+    const double *seg_mid, *seg_dir, *seg_len, *seg_E, *seg_E_end;
+    const int8_t *seg_event;  /* or NULL: no cutoff residual */
+    const double *weight;     /* [n] particle weights, or NULL (= 1) */
+    double scale;             /* multiplies every deposit (1/Ne for per-electron doses) */
+    double z0, dz;            /* top edge of bin 0 and the bin width [A] */
+    int32_t n_bins;           /* 1 .. 8192 */
+    double *out;              /* [n_bins] keV, accumulated */
+} hr_dose_args;
+hr_status hr_depth_dose(const hr_dose_args *args, void *stream);
+
 /* Per-primary line roulette, the reference's LineRoulette estimator
  * (src/pyrite/_line_roulette.py:29-80, montecarlo/runner/line_roulette.py).
  * hr_roulette_uniforms: out[i] = uniform of primary first_id + i keyed only on

@@ -15,7 +15,7 @@ import numpy as np
 from . import oracle
 
 
-def run_case(pipe_like, start, n, threads=0, timings=None):
+def run_case(pipe_like, start, n, threads=0, timings=None, dose_bins=0):
     """Histories ``[start, start+n)`` of the case described by ``pipe_like``.
 
     ``pipe_like`` carries ``cfg``, ``layers``, ``lut``, ``beam_dir``,
@@ -73,4 +73,9 @@ def run_case(pipe_like, start, n, threads=0, timings=None):
         timings["transport"] = timings.get("transport", 0.0) + (t1 - t0)
         timings["lines"] = timings.get("lines", 0.0) + (t2 - t1)
         timings["brem"] = timings.get("brem", 0.0) + (t3 - t2)
+    if dose_bins:
+        from . import tally
+
+        dose = tally.depth_dose(r, 0.0, float(cfg.thickness_ang) / dose_bins, dose_bins)
+        return lines, brem, len(seg["L"]), dose
     return lines, brem, len(seg["L"])

@@ -238,6 +238,17 @@ class VrSpawnArgs(C.Structure):
     ]
 
 
+class DoseArgs(C.Structure):
+    """Mirror of ``hr_dose_args``."""
+
+    _fields_ = [
+        ("n", C.c_int64), ("rows", C.c_int64), ("seg_count", _dp),
+        ("seg_mid", _dp), ("seg_dir", _dp), ("seg_len", _dp), ("seg_E", _dp), ("seg_E_end", _dp),
+        ("seg_event", _dp), ("weight", _dp), ("scale", C.c_double), ("z0", C.c_double), ("dz", C.c_double),
+        ("n_bins", C.c_int32), ("out", _dp),
+    ]
+
+
 class Continuation(C.Structure):
     """Mirror of ``hr_continuation``."""
 
@@ -279,6 +290,7 @@ SIGNATURES = {
     "hr_roulette_uniforms": (C.c_int32, [C.c_uint64, C.c_int64, C.c_int64, _dp, _dp]),
     "hr_line_screen": (C.c_int32, [_dp, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_uint64,
                                    _dp, _dp, _dp]),
+    "hr_depth_dose": (C.c_int32, [C.POINTER(DoseArgs), _dp]),
     "hr_transport_continue": (C.c_int32, [C.POINTER(TransportArgs), C.POINTER(Continuation), _dp]),
     "hr_cxr_lines": (C.c_int32, [C.POINTER(LineArgs), _dp]),
     "hr_sincsq_reduce": (C.c_int32, [C.POINTER(SincsqArgs), _dp]),

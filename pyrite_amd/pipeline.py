@@ -65,6 +65,7 @@ class CaseConfig:
     line_route: str = "dense"  # "dense" sinc^2 over every bin, or "window" (support window + far tail)
     x_c: float = 100.0  # support cutoff of the windowed route
     line_roulette: dict | None = None  # LineRoulette policy fields (per-primary line roulette), or None
+    depth_bins: int = 0  # > 0: also tally the depth-dose profile over this many bins of the slab
 
     def grid(self):
         return bin_centers(self.E_lo_eV, self.E_hi_eV, self.n_bins)
@@ -127,6 +128,7 @@ class CasePipeline(CaseTables):
         self.beam_d = torch.as_tensor(self.beam_dir, device=dev)
         self.spec_lines = torch.zeros(self.E_grid.size, dtype=torch.float64, device=dev)
         self.spec_brem = torch.zeros(self.E_grid.size, dtype=torch.float64, device=dev)
+        self.spec_dose = torch.zeros(max(int(cfg.depth_bins), 1), dtype=torch.float64, device=dev)
         self.histories = 0
         self.segments = 0
         self.timers = {}
@@ -194,6 +196,18 @@ class CasePipeline(CaseTables):
         keys = ("r_mid", "v_hat", "L", "E") + (("prim",) if first_id is not None else ())
         return {k: rec[k][:n_rows] for k in keys}
 
+    def score_dose(self, res, Ne):
+        """Depth-dose of a ``compact=False`` transport's slot rows into ``spec_dose``
+        (keV per electron of ``Ne`` per bin)."""
+        from .transport.tally import depth_dose
+
+        nb = int(self.cfg.depth_bins)
+        if nb <= 0:
+            return
+        dz = float(self.cfg.thickness_ang) / nb
+        for _e0, m, cap, rows, seg_count in res.slots:
+            depth_dose(self.spec_dose, 0.0, dz, rows, m, slots=cap, seg_count=seg_count, scale=1.0 / float(Ne))
+
     def score_records(self, seg, Ne, line_weight=None):
         """Line and brem tallies of FP32 scoring records; ``line_weight`` (FP32 per
         row) weights the line tally only (the line-roulette strata)."""
@@ -234,6 +248,7 @@ class CasePipeline(CaseTables):
         """
         res = self.transport(batch_index, compact=False)
         n_total = self.cfg.Ne * n_batches_total
+        self.score_dose(res, n_total)
         screen = self.line_screen(n_total)
         if screen is None:
             self.score_records(self.records(res), n_total)
@@ -278,6 +293,7 @@ class CasePipeline(CaseTables):
                 cnt.record_stream(sb)
             with torch.cuda.stream(sb):
                 sb.wait_event(done)
+                self.score_dose(res, n_total)
                 self.score_records(self.records(res), n_total)
             self.histories += self.cfg.Ne
             self.segments += res.n_segments
@@ -349,6 +365,11 @@ class CasePipeline(CaseTables):
             _lib.check(lib.hr_vr_gather(C.byref(g), C.c_void_p(torch.cuda.current_stream().cuda_stream)),
                        "hr_vr_gather")
             self.score_records({k: rec[k][:n_rows] for k in ("r_mid", "v_hat", "L", "E")}, n_total)
+            if cfg.depth_bins > 0:
+                from .transport.tally import depth_dose
+
+                depth_dose(self.spec_dose, 0.0, float(cfg.thickness_ang) / cfg.depth_bins, rnd.rows, rnd.n_lanes,
+                           slots=rnd.rows_per_round, seg_count=cnt, scale=1.0 / float(n_total))
             tim["rows"] = tim.get("rows", 0) + n_rows
 
         res = drv.run(int(first_id), int(n_histories), fill, consume)
@@ -359,3 +380,7 @@ class CasePipeline(CaseTables):
     def spectra(self):
         """``(lines [photons/eV/electron], brem [photons/eV/electron] over the cone)``."""
         return self.spec_lines.cpu().numpy(), self.spec_brem.cpu().numpy() * self.omega
+
+    def depth_dose(self):
+        """Deposited energy per depth bin [keV/electron] (``depth_bins`` uniform bins over the slab)."""
+        return self.spec_dose.cpu().numpy()[: max(int(self.cfg.depth_bins), 0)]

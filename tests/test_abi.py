@@ -62,6 +62,7 @@ STRUCTS = {
     "hr_queue_config": "QueueConfig",
     "hr_queue_launch": "QueueLaunch",
     "hr_continuation": "Continuation",
+    "hr_dose_args": "DoseArgs",
 }
 
 

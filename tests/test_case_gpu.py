@@ -7,7 +7,7 @@ on the device and by the CPU oracle over disjoint histories, each in 32
 independent batches; per bin, |mean_gpu - mean_cpu| <= 3 sqrt(se_gpu^2 +
 se_cpu^2) may fail only as often as chance allows (binomial count of tripped
 bins), and the integrated line and brem yields agree within 4 standard
-errors.  Bins share histories, so the integral, not a sign count over bins,
+errors; the depth-dose profile is held to the same bar.  Bins share histories, so the integral, not a sign count over bins,
 is the whole-spectrum check.
 """
 
@@ -20,36 +20,43 @@ pytestmark = pytest.mark.gpu
 BATCHES, PER_BATCH = 32, 1500
 
 
-def _gpu_batches(pipe):
+def _gpu_batches(pipe, dose=None):
     lines, brem = [], []
     for b in range(BATCHES):
         pipe.spec_lines.zero_()
         pipe.spec_brem.zero_()
+        pipe.spec_dose.zero_()
         pipe.step(b, 1)
         ln, br = pipe.spectra()
         lines.append(ln)
         brem.append(br)
+        if dose is not None:
+            dose.append(pipe.depth_dose())
     return np.array(lines), np.array(brem)
 
 
-def _cpu_batches(pipe, first):
+def _cpu_batches(pipe, first, dose=None):
     from oracle import case as ocase
 
     n = pipe.cfg.Ne
     lines, brem = [], []
     for b in range(BATCHES):
-        ln, br, _ = ocase.run_case(pipe, first + b * n, n)
+        if dose is None:
+            ln, br, _ = ocase.run_case(pipe, first + b * n, n)
+        else:
+            ln, br, _, dd = ocase.run_case(pipe, first + b * n, n, dose_bins=pipe.cfg.depth_bins)
+            dose.append(dd / n)
         lines.append(ln / n)
         brem.append(br / (4.0 * np.pi * n) * pipe.omega)
     return np.array(lines), np.array(brem)
 
 
-def _agree(a, b, name, p_trip=0.0027):
+def _agree(a, b, name, p_trip=0.0027, min_bins=100):
     ma, mb = a.mean(0), b.mean(0)
     se = np.sqrt(a.var(0, ddof=1) / len(a) + b.var(0, ddof=1) / len(b))
     live = se > 0
     n = int(live.sum())
-    assert n > 100, name
+    assert n >= min_bins, name
     z = (ma - mb)[live] / se[live]
     trips = int((np.abs(z) > 3.0).sum())
     allowed = int(binom.ppf(1.0 - 1e-4, n, p_trip))
@@ -63,12 +70,14 @@ def _agree(a, b, name, p_trip=0.0027):
 def test_hopg_case_spectra_agree_with_the_oracle_per_bin(device):
     from pyrite_amd.pipeline import CaseConfig, CasePipeline
 
-    pipe = CasePipeline(CaseConfig(Ne=PER_BATCH))
-    g_lines, g_brem = _gpu_batches(pipe)
-    c_lines, c_brem = _cpu_batches(pipe, first=10_000_000)
+    pipe = CasePipeline(CaseConfig(Ne=PER_BATCH, depth_bins=100))
+    g_dose, c_dose = [], []
+    g_lines, g_brem = _gpu_batches(pipe, g_dose)
+    c_lines, c_brem = _cpu_batches(pipe, first=10_000_000, dose=c_dose)
     assert g_lines.sum() > 0 and c_lines.sum() > 0
     _agree(g_brem, c_brem, "brem")
     _agree(g_lines, c_lines, "lines")
+    _agree(np.array(g_dose), np.array(c_dose), "depth dose", min_bins=50)
 
 
 def test_silicon_twiss_case_agrees_with_the_oracle_per_bin(device):
