@@ -45,6 +45,9 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--ne", type=int, default=100_000)
     p.add_argument("--n-side", type=int, default=3)
+    p.add_argument("--line-route", choices=("dense", "window"), default="dense",
+                   help="dense: every (line, bin) pair, the reference's device default for per-row lines; "
+                        "window: the support-windowed route with x_c = 100 (the reference's NumPy default)")
     p.add_argument("--cpu-sample", type=int, default=8000, help="histories in the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--breakdown", action="store_true", help="print per-phase times to stderr")
@@ -212,7 +215,7 @@ def main():
     from pyrite_amd.pipeline import CaseConfig, CasePipeline
     from pyrite_amd import roofline
 
-    cfg = CaseConfig(Ne=args.ne, n_side=args.n_side)
+    cfg = CaseConfig(Ne=args.ne, n_side=args.n_side, line_route=args.line_route)
     pipe = CasePipeline(cfg)
     for w in range(args.warmup):
         pipe.step(1000 + w)
@@ -247,7 +250,7 @@ def main():
     hist = pipe.histories
     value = hist / dt
     # dominant kernel: sinc^2 reduction, event-timed per launch
-    ms = [ev[0].elapsed_time(ev[1]) for ev, _ in pipe.lines.events]
+    ms = [ev[0].elapsed_time(ev[1]) for ev, _ in pipe.lines.events] or [float("nan")]
     cells = sum(c for _, c in pipe.lines.events)
     avg_ms = sum(ms) / len(ms)
     flops_per_launch = roofline.SINC_CELL_OPS * cells / len(ms)
@@ -284,7 +287,7 @@ def main():
             "peak": roofline.PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / roofline.PEAK_FP32_TFLOPS,
-            "traffic": SINC_TRAFFIC_BYTES if (args.ne, args.n_side) == (100_000, 3) else None,
+            "traffic": SINC_TRAFFIC_BYTES if (args.ne, args.n_side, args.line_route) == (100_000, 3, "dense") else None,
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE (x2.000) and WRITE_SIZE (x1.000), factors from a 1 GiB "
                               "copy; profiles/sincsq_uniform_hopg_5mev_pmc.md",
             "ops_per_cell": roofline.SINC_CELL_OPS,
