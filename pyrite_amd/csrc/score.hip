@@ -460,16 +460,21 @@ __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcp
 __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float fast_log2(float x) { return __builtin_amdgcn_logf(x); }
+constexpr float kLn2f = 0.69314718055994531f;
 
 // BH+Elwert brem over segment tiles (brem.py:396-423, brem_jit_kernel.py:76-153).
 constexpr int kBremTile = 128;
-template <int BINS>
+// NEL: element count of the composition (1..4), a template parameter so the
+// per-element loop unrolls with its constants in registers.
+template <int BINS, int NEL>
 __global__ void __launch_bounds__(kRedThreads) brem_kernel(hr_brem_args p, double *__restrict__ partial) {
     __shared__ float s_T[kBremTile], s_pi[kBremTile], s_p0[kBremTile], s_p1[kBremTile];
     __shared__ float s_pref[kBremTile * 4];
+    __shared__ int s_live[kBremTile];
     const int t = threadIdx.x;
     const int64_t bin0 = (int64_t)blockIdx.y * kRedThreads * BINS;
-    const int n_el = p.n_el;
+    constexpr int n_el = NEL;
     double acc[BINS];
     float kev[BINS], keV[BINS], mu[BINS];
 #pragma unroll
@@ -516,12 +521,15 @@ __global__ void __launch_bounds__(kRedThreads) brem_kernel(hr_brem_args p, doubl
                     }
                     s_pref[t * 4 + el] = pref;
                 }
+                s_live[t] = (s_pref[t * 4] != 0.0f) | (s_pref[t * 4 + 1] != 0.0f) | (s_pref[t * 4 + 2] != 0.0f) |
+                            (s_pref[t * 4 + 3] != 0.0f);
             } else {
                 s_T[t] = 0.0f;
                 s_pi[t] = 0.0f;
                 s_p0[t] = 0.0f;
                 s_p1[t] = 0.0f;
                 for (int el = 0; el < 4; ++el) s_pref[t * 4 + el] = 0.0f;
+                s_live[t] = 0;
             }
         }
         __syncthreads();
@@ -542,9 +550,7 @@ __global__ void __launch_bounds__(kRedThreads) brem_kernel(hr_brem_args p, doubl
                 }
             }
             // a zero-weight row (a copy that does not carry this tally) adds nothing
-            if (s_pref[q * 4] == 0.0f && s_pref[q * 4 + 1] == 0.0f && s_pref[q * 4 + 2] == 0.0f &&
-                s_pref[q * 4 + 3] == 0.0f)
-                continue;
+            if (!s_live[q]) continue;
             const float T = s_T[q], pi_ = s_pi[q], p0 = s_p0[q], p1 = s_p1[q];
             const float plo = fminf(p0, p1), dp01 = fabsf(p1 - p0);
 #pragma unroll
@@ -560,9 +566,11 @@ __global__ void __launch_bounds__(kRedThreads) brem_kernel(hr_brem_args p, doubl
                 // p_i - p_f = k (T_i + T_f + 2m) / (m^2 (p_i + p_f)) (no cancellation for k << T_i)
                 const float rcp_bf = rs * (kMc2f + Tf);
                 const float spf = pi_ + pf;
-                const float born = __logf(spf * spf * (kMc2f * kMc2f) * fast_rcp(keV[m] * (T + Tm)));
+                // ln via the hardware log2 (the argument is >= 1 up to rounding: no denormal path)
+                const float born = kLn2f * fast_log2(spf * spf * (kMc2f * kMc2f) * fast_rcp(keV[m] * (T + Tm)));
                 const float base = born * inv_kev[m] * rcp_bf;
                 float cell = 0.0f;
+#pragma unroll
                 for (int el = 0; el < n_el; ++el) {
                     const float den_f = 1.0f - fast_exp2(-zi2[el] * rcp_bf);
                     cell += s_pref[q * 4 + el] * base * fast_rcp(den_f);
@@ -931,8 +939,13 @@ extern "C" hr_status hr_brem_reduce(const hr_brem_args *args, void *stream) {
     st = workspace(partial_bytes(gx, p.n_E), ws, cu);
     if (st != HR_OK) return st;
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(brem_kernel<BINS>, dim3((unsigned)gx, (unsigned)bin_tiles), dim3(kRedThreads), 0, s, p,
-                       (double *)ws);
+    const dim3 grid((unsigned)gx, (unsigned)bin_tiles);
+    switch (p.n_el) {
+    case 1: hipLaunchKernelGGL((brem_kernel<BINS, 1>), grid, dim3(kRedThreads), 0, s, p, (double *)ws); break;
+    case 2: hipLaunchKernelGGL((brem_kernel<BINS, 2>), grid, dim3(kRedThreads), 0, s, p, (double *)ws); break;
+    case 3: hipLaunchKernelGGL((brem_kernel<BINS, 3>), grid, dim3(kRedThreads), 0, s, p, (double *)ws); break;
+    default: hipLaunchKernelGGL((brem_kernel<BINS, 4>), grid, dim3(kRedThreads), 0, s, p, (double *)ws); break;
+    }
     HR_CHECK_HIP(hipGetLastError());
     return reduce_partials(ws, gx, p.n_E, p.out, s);
 }
