@@ -347,6 +347,61 @@ typedef struct hr_window_args {
  * materialising the lines (n_seg * n_dir * n_g of them). */
 hr_status hr_window_spectra(const hr_window_args *args, void *stream);
 
+/* ---- coherent line route: phased fields, squared once every batch is summed
+ * Replaces the coherent stream pipeline: the prologue
+ * (src/pyrite/montecarlo/spectrum/coherent_prologue_jit_kernel.py, CPU twin
+ * cxr/_batched.py:561-690 _batched_coherent_block), the field accumulation
+ * (coherent_stream_jit_kernel.py:523 run_coherent_field_accumulation_kernel)
+ * and the finalize (:649 finalize_coherent_fields; CPU twin
+ * cxr/_batched_finalize.py:201-235).  Reflections and directions stay
+ * incoherent (one field row each); the flat term only (no inter-electron
+ * decoherence blend).
+ *
+ * Line records, per (direction d, reflection g, segment j), 13 FP64 planes
+ * rec[f][d*n_g + g][j]: f = 0 E_c (exact vacuum centre, eV), 1 a (sinc width,
+ * 1/eV), 2 phase slope (t_mid - n.r + d_shift)/hbar c (1/eV), 3 lattice phase
+ * g.r + g_shift, 4-7 c_s re/im, c_p re/im with c = sqrt(alpha omega/(4 pi^2
+ * hbar c)) t_L A, 8 L_esc (mean escape path), 9 half_dL (half its change
+ * along the segment), 10-12 apb, bma, q (formation-factor attenuation).
+ * Rejected lines carry zero coefficients (and zeros elsewhere). */
+typedef struct hr_coherent_args {
+    const hr_line_args *lines; /* crystal, tables, n_hat/n_dir and keep window; its segment
+                                  inputs, scale, dir_weight, row_weight and outputs are not used */
+    int64_t n, rows;           /* transport slot rows: particle i, row s at i*rows + s */
+    const int64_t *seg_count;  /* [n] rows written per particle (clamped to rows) */
+    const int64_t *row_offset; /* [n] exclusive prefix sum of the clamped counts */
+    const double *seg_mid;     /* [n*rows,3] midpoint [A] */
+    const double *seg_dir;     /* [n*rows,3] */
+    const double *seg_len;     /* [n*rows] [A] */
+    const double *seg_E, *seg_E_end; /* [n*rows] keV; the row's energy is their mean */
+    const double *seg_t0;      /* [n*rows] age at the segment start [A] (c = 1) */
+    int64_t first;             /* record column of the first row written */
+    int64_t ld;                /* column stride of the record planes (>= first + rows written) */
+    double *rec;               /* [13][n_dir*n_g][ld] */
+} hr_coherent_args;
+hr_status hr_coherent_lines(const hr_coherent_args *args, void *stream);
+
+typedef struct hr_coherent_field_args {
+    const double *rec;  /* record planes of hr_coherent_lines */
+    int64_t ld;         /* their column stride */
+    int64_t n_lines;    /* columns [0, n_lines) are summed */
+    int32_t n_rows;     /* n_dir*n_g field rows */
+    int64_t n_E;
+    const double *E_grid; /* [n_E] eV */
+    const double *dom;    /* [n_E] delta(E) E / hbar c [1/A]: the escape-path phase rate */
+    double *field;      /* [n_rows][4][n_E] FP64 (sigma re, sigma im, pi re, pi im), accumulated */
+} hr_coherent_field_args;
+/* field[r][.][k] += sum_j c_j F(v_jk) exp(i phi_jk), v = a (E - E_c) -
+ * half_dL dom, phi = phase_slope E - g_phase - L_esc dom, F = exp(-tau_c/2)
+ * sinh(w)/w with w = i v - q (cxr/_formation.py:86-115).  FP64 phase and
+ * sums, FP32 profile; each field cell has one writer per call (fixed-order
+ * partial sums, no atomics).  Uses a library-held per-device workspace:
+ * concurrent calls on one device must not overlap. */
+hr_status hr_coherent_field(const hr_coherent_field_args *args, void *stream);
+/* out[k] += scale sum_d dir_weight[d] sum_g wm[g] (|F_s|^2 + |F_p|^2) */
+hr_status hr_coherent_finalize(const double *field, int32_t n_dir, int32_t n_g, const double *dir_weight,
+                               const double *wm, int64_t n_E, double scale, double *out, void *stream);
+
 /* ---- incident states on the entrance face
  * Replaces initial_beam_positions + primary_directions for an ungrooved
  * analytic beam (src/pyrite/montecarlo/transport/beam_entry.py:23-126,

@@ -15,7 +15,7 @@ import numpy as np
 from . import oracle
 
 
-def run_case(pipe_like, start, n, threads=0, timings=None, dose_bins=0):
+def run_case(pipe_like, start, n, threads=0, timings=None, dose_bins=0, coherent=False):
     """Histories ``[start, start+n)`` of the case described by ``pipe_like``.
 
     ``pipe_like`` carries ``cfg``, ``layers``, ``lut``, ``beam_dir``,
@@ -23,7 +23,10 @@ def run_case(pipe_like, start, n, threads=0, timings=None, dose_bins=0):
     ``CasePipeline`` or ``CaseTables``).  Returns ``(lines, brem_sum, n_rows)``:
     the solid-angle line spectrum summed over these histories (not divided by
     Ne) and the raw brem sum ``sum n L dsigma/dk T`` at the central direction
-    (divide by ``4 pi Ne`` and multiply by Omega for the cone).
+    (divide by ``4 pi Ne`` and multiply by Omega for the cone).  With
+    ``coherent`` the line spectrum is the coherent one: per direction and
+    reflection the fields of all these histories' segments are summed, then
+    squared (``sum_d w_d sum_g wm_g |F|^2``, again not divided by Ne).
     """
     cfg = pipe_like.cfg
     t0 = time.perf_counter()
@@ -54,8 +57,21 @@ def run_case(pipe_like, start, n, threads=0, timings=None, dose_bins=0):
 
     tab = pipe_like.line_tables.as_dict()
     lines = np.zeros(pipe_like.E_grid.size)
+    if coherent:
+        from pyrite_amd.spectrum.crystal import CRYSTALS, decrement
+
+        seg64 = {"r_mid": r["seg_mid"], "v_hat": r["seg_dir"], "L": r["seg_len"],
+                 "E": 0.5 * (r["seg_E"] + r["seg_E_end"]), "t0": r["seg_t0"]}
+        E_grid = pipe_like.E_grid
+        dom = decrement(CRYSTALS[cfg.crystal], E_grid) * E_grid / 1973.269804
     for d, n_hat in enumerate(pipe_like.n_hats):
         es, ep = polarization_pairs(n_hat, tab["g"])
+        if coherent:
+            rec, _good = oracle.coherent_lines(seg64, cfg.thickness_ang, n_hat, tab, es, ep, threads=threads)
+            for g in range(rec.shape[1]):
+                F = oracle.coherent_field(rec[:, g, :], E_grid, dom, threads=threads)
+                lines += pipe_like.weights[d] * tab["wm"][g] * (F**2).sum(0)
+            continue
         E_r, a, w, good = oracle.cxr_lines(seg, cfg.thickness_ang, n_hat, tab, es, ep, threads=threads)
         w = np.where(good, w * pipe_like.weights[d], 0.0)
         oracle.sincsq_reduce(E_r, a, w, pipe_like.E_grid, out=lines, threads=threads)

@@ -140,6 +140,16 @@ void or_quantum_line_factors(double h, double gamma, double vdg, double k_dot_v,
  * rejected pairs get w = 0 and good = 0. */
 void or_cxr_lines(const or_line_args *p, double *E_r, double *a, double *w, uint8_t *good,
                   int threads);
+/* Coherent line records per (segment j, reflection g) at 13*(j*n_g+g): E_c, a,
+ * phase slope, g phase, cs_re, cs_im, cp_re, cp_im, L_esc, half_dL, apb, bma, q
+ * (definition in or_score.c); FP64 segment inputs, tables from p. */
+void or_coherent_lines(const or_line_args *p, const double *r_mid, const double *v_hat, const double *L,
+                       const double *E, const double *t_start, double *out, uint8_t *good, int threads);
+/* exp(-tau_c/2) sinh(w)/w, w = i v - q (formation factor of a line piece) */
+void or_formation_factor(double v, double apb, double bma, double q, double *re, double *im);
+/* F[4][n_E] += phased field of one row's coherent records (13 per line) */
+void or_coherent_field(const double *rec, int64_t n_lines, const double *E, const double *dom, int64_t n_E,
+                       double *F, double *bound, int threads);
 /* out[k] += sum_j w_j [sin x/x]^2, x = a_j (E_k - E_r,j) (x = 0 -> 1e-20) */
 void or_sincsq_reduce(const double *E_r, const double *a, const double *w, int64_t n,
                       const double *E_grid, int64_t n_E, double *out, int threads);

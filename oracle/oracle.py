@@ -130,6 +130,12 @@ def lib():
         L.or_brem_reduce.restype = None
         L.or_brem_reduce.argtypes = [_dp, _dp, _dp, _dp, C.c_int64, C.c_double, _dp, _dp, _dp,
                                      C.c_int, _dp, _dp, C.c_int64, _dp, C.c_int]
+        L.or_coherent_lines.restype = None
+        L.or_coherent_lines.argtypes = [C.POINTER(LineArgs), _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int]
+        L.or_formation_factor.restype = None
+        L.or_formation_factor.argtypes = [C.c_double] * 4 + [C.POINTER(C.c_double)] * 2
+        L.or_coherent_field.restype = None
+        L.or_coherent_field.argtypes = [_dp, C.c_int64, _dp, _dp, C.c_int64, _dp, _dp, C.c_int]
         _LIB = L
     return _LIB
 
@@ -314,6 +320,67 @@ def cxr_lines(seg, thickness, n_hat, tables, e_s, e_p, threads=0):
     good = np.zeros(m, dtype=np.uint8)
     lib().or_cxr_lines(C.byref(a), _p(E_r), _p(aw), _p(w), _p(good), int(threads))
     return E_r, aw, w, good.astype(bool)
+
+
+def _line_args(n_seg, thickness, n_hat, tables, e_s, e_p, hold):
+    hold.update({
+        "g": _f64(tables["g"]), "wm": _f64(tables["wm"]),
+        "row": np.ascontiguousarray(tables["row"], dtype=np.int32),
+        "e_s": _f64(e_s), "e_p": _f64(e_p),
+        "E_tab": _f32(tables["E_tab"]), "delta_tab": _f32(tables["delta_tab"]),
+        "chi_re": _f32(tables["chi_re"]), "chi_im": _f32(tables["chi_im"]),
+        "u_re": _f32(tables["u_re"]), "u_im": _f32(tables["u_im"]), "log_mu": _f32(tables["log_mu"]),
+    })
+    a = LineArgs()
+    a.n_seg = int(n_seg)
+    for k in ("g", "wm", "row", "e_s", "e_p", "E_tab", "delta_tab", "chi_re", "chi_im", "u_re", "u_im",
+              "log_mu"):
+        setattr(a, k, _p(hold[k]))
+    a.thickness = float(thickness)
+    a.n_hat = (C.c_double * 3)(*[float(x) for x in n_hat])
+    a.n_g = len(hold["wm"])
+    a.n_tab = len(hold["E_tab"])
+    a.n_mu = hold["log_mu"].reshape(-1, a.n_tab).shape[0]
+    a.lo_keep, a.hi_keep = float(tables["lo_keep"]), float(tables["hi_keep"])
+    return a
+
+
+COHERENT_FIELDS = ("E_c", "a", "phase_slope", "g_phase", "cs_re", "cs_im", "cp_re", "cp_im", "L_esc",
+                   "half_dL", "apb", "bma", "q")
+
+
+def coherent_lines(seg, thickness, n_hat, tables, e_s, e_p, threads=0):
+    """Coherent line records for one direction: ``(rec [n_seg, n_g, 13], good [n_seg, n_g])``.
+
+    ``seg`` holds FP64 r_mid [n,3], v_hat [n,3], L, E (representative energy,
+    keV) and t0 (segment start age, Angstrom); record fields are
+    ``COHERENT_FIELDS``."""
+    hold = {k: _f64(seg[k]) for k in ("r_mid", "v_hat", "L", "E", "t0")}
+    n = len(hold["L"])
+    a = _line_args(n, thickness, n_hat, tables, e_s, e_p, hold)
+    out = np.zeros((n, a.n_g, 13))
+    good = np.zeros((n, a.n_g), dtype=np.uint8)
+    lib().or_coherent_lines(C.byref(a), _p(hold["r_mid"]), _p(hold["v_hat"]), _p(hold["L"]), _p(hold["E"]),
+                            _p(hold["t0"]), _p(out), _p(good), int(threads))
+    return out, good.astype(bool)
+
+
+def formation_factor(v, apb, bma, q):
+    re, im = C.c_double(), C.c_double()
+    lib().or_formation_factor(float(v), float(apb), float(bma), float(q), C.byref(re), C.byref(im))
+    return complex(re.value, im.value)
+
+
+def coherent_field(rec, E_grid, dom, F=None, bound=None, threads=0):
+    """Add one row's phased field to ``F`` ([4, n_E]: sigma re/im, pi re/im);
+    ``rec`` is [n_lines, 13]; ``bound`` (optional [n_E]) gets the L1 size."""
+    rec = _f64(np.asarray(rec).reshape(-1, 13))
+    E_grid, dom = _f64(E_grid), _f64(dom)
+    F = np.zeros((4, len(E_grid))) if F is None else F
+    assert F.flags.c_contiguous and F.dtype == np.float64 and F.shape == (4, len(E_grid))
+    lib().or_coherent_field(_p(rec), rec.shape[0], _p(E_grid), _p(dom), len(E_grid), _p(F),
+                            None if bound is None else _p(bound), int(threads))
+    return F
 
 
 def sincsq_reduce(E_r, a, w, E_grid, out=None, threads=0):

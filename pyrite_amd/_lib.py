@@ -191,6 +191,29 @@ class WindowArgs(C.Structure):
     ]
 
 
+class CoherentArgs(C.Structure):
+    """Mirror of ``hr_coherent_args``."""
+
+    _fields_ = [
+        ("lines", C.POINTER(LineArgs)),
+        ("n", C.c_int64), ("rows", C.c_int64),
+        ("seg_count", _dp), ("row_offset", _dp),
+        ("seg_mid", _dp), ("seg_dir", _dp), ("seg_len", _dp),
+        ("seg_E", _dp), ("seg_E_end", _dp), ("seg_t0", _dp),
+        ("first", C.c_int64), ("ld", C.c_int64),
+        ("rec", _dp),
+    ]
+
+
+class CoherentFieldArgs(C.Structure):
+    """Mirror of ``hr_coherent_field_args``."""
+
+    _fields_ = [
+        ("rec", _dp), ("ld", C.c_int64), ("n_lines", C.c_int64), ("n_rows", C.c_int32),
+        ("n_E", C.c_int64), ("E_grid", _dp), ("dom", _dp), ("field", _dp),
+    ]
+
+
 class BeamArgs(C.Structure):
     """Mirror of ``hr_beam_args``."""
 
@@ -309,6 +332,9 @@ SIGNATURES = {
     "hr_window_reduce": (C.c_int32, [_dp, _dp, _dp, C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double,
                                      _dp, _dp]),
     "hr_window_spectra": (C.c_int32, [C.POINTER(WindowArgs), _dp]),
+    "hr_coherent_lines": (C.c_int32, [C.POINTER(CoherentArgs), _dp]),
+    "hr_coherent_field": (C.c_int32, [C.POINTER(CoherentFieldArgs), _dp]),
+    "hr_coherent_finalize": (C.c_int32, [_dp, C.c_int32, C.c_int32, _dp, _dp, C.c_int64, C.c_double, _dp, _dp]),
     "hr_vr_round": (C.c_int32, [C.POINTER(TransportArgs), C.c_int32, _dp, _dp]),
     "hr_vr_features": (C.c_int32, [C.c_int64, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp, _dp]),
     "hr_vr_window": (C.c_int32, [C.POINTER(VrWindowArgs), _dp]),

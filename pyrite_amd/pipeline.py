@@ -66,6 +66,7 @@ class CaseConfig:
     x_c: float = 100.0  # support cutoff of the windowed route
     line_roulette: dict | None = None  # LineRoulette policy fields (per-primary line roulette), or None
     depth_bins: int = 0  # > 0: also tally the depth-dose profile over this many bins of the slab
+    coherent: bool = False  # sum line fields before squaring (the coherent route); brem stays incoherent
 
     def grid(self):
         return bin_centers(self.E_lo_eV, self.E_hi_eV, self.n_bins)
@@ -116,6 +117,14 @@ class CasePipeline(CaseTables):
         self.core = make_hip_transport_core(self.tables, self.control)
         self.lines = LineScorer(self.line_tables, self.E_grid, cfg.thickness_ang, self.n_hats, self.weights, dev)
         self.brem = BremScorer(cfg.composition, self.E_grid, cfg.thickness_ang, self.n_hat, dev)
+        self.coherent = None
+        if cfg.coherent:
+            if cfg.line_roulette is not None:
+                raise ValueError("line roulette weights line intensities; the coherent route sums fields")
+            from .spectrum.coherent import CoherentScorer
+
+            self.coherent = CoherentScorer(self.line_tables, cfg.crystal, self.E_grid, cfg.thickness_ang,
+                                           self.n_hats, self.weights, dev)
         n = cfg.Ne
         self.keys = torch.empty(n, dtype=torch.uint64, device=dev)
         self.pos = torch.empty((n, 3), dtype=torch.float64, device=dev)
@@ -250,7 +259,10 @@ class CasePipeline(CaseTables):
         n_total = self.cfg.Ne * n_batches_total
         self.score_dose(res, n_total)
         screen = self.line_screen(n_total)
-        if screen is None:
+        if self.coherent is not None:
+            self.coherent.add_transport(res, self.cfg.Ne)
+            self.brem.spectrum(self.records(res), n_total, self.spec_brem)
+        elif screen is None:
             self.score_records(self.records(res), n_total)
         else:
             seg = self.records(res, first_id=int(batch_index) * self.cfg.Ne)
@@ -271,7 +283,7 @@ class CasePipeline(CaseTables):
         import torch
 
         n_total_b = int(n_steps if n_batches_total is None else n_batches_total)
-        if not overlap or self.cfg.line_roulette is not None:
+        if not overlap or self.cfg.line_roulette is not None or self.coherent is not None:
             for b in range(int(n_steps)):
                 self.step(int(first_batch) + b, n_total_b)
             return
@@ -322,6 +334,8 @@ class CasePipeline(CaseTables):
         if cfg.line_roulette is not None:
             raise NotImplementedError("line roulette screens whole primaries; the streamed route splits them "
                                       "over rounds (use step())")
+        if self.coherent is not None:
+            raise NotImplementedError("the coherent route is fed by step(); streamed rounds are not wired to it")
         n_total = int(n_histories if n_total is None else n_total)
         lanes = max(1, min(int(lanes), int(n_histories)))
         drv = getattr(self, "_stream_drv", None)
@@ -378,8 +392,20 @@ class CasePipeline(CaseTables):
         return res
 
     def spectra(self):
-        """``(lines [photons/eV/electron], brem [photons/eV/electron] over the cone)``."""
-        return self.spec_lines.cpu().numpy(), self.spec_brem.cpu().numpy() * self.omega
+        """``(lines [photons/eV/electron], brem [photons/eV/electron] over the cone)``.
+
+        On the coherent route the lines are the squared fields of every
+        history added since ``reset_coherent()``, per electron of them."""
+        if self.coherent is not None:
+            lines = self.coherent.spectrum().cpu().numpy()
+        else:
+            lines = self.spec_lines.cpu().numpy()
+        return lines, self.spec_brem.cpu().numpy() * self.omega
+
+    def reset_coherent(self):
+        """Clear the coherent field planes (a new independent estimate starts)."""
+        if self.coherent is not None:
+            self.coherent.reset()
 
     def depth_dose(self):
         """Deposited energy per depth bin [keV/electron] (``depth_bins`` uniform bins over the slab)."""

@@ -1,0 +1,194 @@
+"""Coherent PXR/CBS line spectrum on the hip backend.
+
+The reference's ``coherent=True`` route of ``mc_cxr_spectrum``
+(``montecarlo/spectrum/cxr/_spectrum.py:163-164``: "Sum segment fields
+coherently instead of segment intensities"), streamed the way its coherent
+stream pipeline is (``coherent_stream_jit_kernel.py:1-22``): every batch of
+transport rows becomes per-line complex records (``hr_coherent_lines``), which
+are summed into persistent FP64 field planes, one per (direction, reflection)
+row (``hr_coherent_field``); the planes are squared only when the spectrum is
+read (``hr_coherent_finalize``), so the fields of all batches add before the
+square.  Reflections, mosaic orientations and detector directions stay
+incoherent with one another.
+
+Only the flat term is built: the inter-electron decoherence blend
+(``physical_electrons``, ``longitudinal_rms_fs``, ``phase_retention``), layered
+absorbers and the coherent ``sinc_cutoff`` window are refused, as the
+reference refuses layered absorbers and cutoffs on this route.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .. import _lib
+from .crystal import CRYSTALS, decrement
+from .lines import LineScorer, LineTables, line_tables
+
+HBARC = 1973.269804  # eV Angstrom (materials/crystal.py:48)
+N_FIELDS = 13
+# Record bytes per device pass (13 FP64 planes per line and row).
+RECORD_BYTES = 1 << 30
+
+
+def _stream():
+    import torch
+
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def escape_phase_rate(crystal_key, E_grid_eV):
+    """``delta(E) E / hbar c`` [1/A] on the output grid: the in-medium
+    propagation phase per unit escape path (``cxr/_setup.py:1030-1045``)."""
+    E = np.asarray(E_grid_eV, dtype=np.float64)
+    return decrement(CRYSTALS[crystal_key], E) * E / HBARC
+
+
+class CoherentScorer:
+    """Persistent coherent field planes of one case, fed batch by batch."""
+
+    def __init__(self, tables: LineTables, crystal_key, E_grid_eV, thickness_ang, n_hats, weights, device):
+        import torch
+
+        self.lines = LineScorer(tables, E_grid_eV, thickness_ang, n_hats, weights, device)
+        self.device = device
+        self.n_dir = self.lines.n_dir
+        self.n_g = len(tables.wm)
+        self.n_rows = self.n_dir * self.n_g
+        self.n_E = int(np.asarray(E_grid_eV).size)
+        self.dom = torch.as_tensor(escape_phase_rate(crystal_key, E_grid_eV), device=device)
+        self.field = torch.zeros((self.n_rows, 4, self.n_E), dtype=torch.float64, device=device)
+        self.n_electrons = 0
+        self._largs = self.lines.line_args({"L": torch.empty(0, device=device),
+                                            "r_mid": torch.empty((0, 3), device=device),
+                                            "v_hat": torch.empty((0, 3), device=device),
+                                            "E": torch.empty(0, device=device)}, 1.0)
+        self._rec = None
+        self.lib = _lib.load()
+
+    def reset(self):
+        self.field.zero_()
+        self.n_electrons = 0
+
+    def _records(self, n_lines):
+        import torch
+
+        need = N_FIELDS * self.n_rows * max(int(n_lines), 1)
+        if self._rec is None or self._rec.numel() < need:
+            self._rec = torch.empty(need, dtype=torch.float64, device=self.device)
+        return self._rec
+
+    def records(self, rows, seg_count, n, slots):
+        """Record planes ``[13, n_rows, n_lines]`` of slot rows (for inspection)."""
+        import torch
+
+        cnt = seg_count[:n].clamp(max=slots)
+        off = torch.cumsum(cnt, 0) - cnt
+        n_lines = int(cnt.sum().item()) if n else 0
+        rec = self._records(n_lines)[: N_FIELDS * self.n_rows * max(n_lines, 1)]
+        self._launch_lines(rows, seg_count, off, 0, n, slots, rec, n_lines)
+        return rec.view(N_FIELDS, self.n_rows, max(n_lines, 1))[:, :, :n_lines]
+
+    def _launch_lines(self, rows, seg_count, row_offset, i0, m, slots, rec, ld):
+        a = _lib.CoherentArgs()
+        a.lines = C.pointer(self._largs)
+        a.n, a.rows = int(m), int(slots)
+        a.seg_count = seg_count.data_ptr() + 8 * i0
+        a.row_offset = row_offset.data_ptr()  # indexed from particle i0
+        base = i0 * slots
+        a.seg_mid = rows["seg_mid"].data_ptr() + 24 * base
+        a.seg_dir = rows["seg_dir"].data_ptr() + 24 * base
+        for k in ("seg_len", "seg_E", "seg_E_end", "seg_t0"):
+            setattr(a, k, rows[k].data_ptr() + 8 * base)
+        a.first, a.ld = 0, int(ld)
+        a.rec = rec.data_ptr()
+        _lib.check(self.lib.hr_coherent_lines(C.byref(a), _stream()), "hr_coherent_lines")
+
+    def _launch_field(self, rec, n_lines):
+        f = _lib.CoherentFieldArgs()
+        f.rec, f.ld, f.n_lines, f.n_rows = rec.data_ptr(), int(n_lines), int(n_lines), self.n_rows
+        f.n_E, f.E_grid, f.dom = self.n_E, self.lines.E_grid_d.data_ptr(), self.dom.data_ptr()
+        f.field = self.field.data_ptr()
+        _lib.check(self.lib.hr_coherent_field(C.byref(f), _stream()), "hr_coherent_field")
+
+    def add_rows(self, rows, seg_count, n, slots):
+        """Add the fields of ``n`` particles' slot rows (``slots`` rows each,
+        ``seg_count`` written) to the planes, in passes of at most
+        ``RECORD_BYTES`` of records."""
+        import torch
+
+        if n == 0:
+            return
+        cnt = seg_count[:n].clamp(max=slots)
+        incl = torch.cumsum(cnt, 0)
+        per_line = N_FIELDS * self.n_rows * 8
+        max_lines = max(1, RECORD_BYTES // per_line)
+        incl_h = incl.cpu().numpy()
+        i0 = 0
+        while i0 < n:
+            base = int(incl_h[i0 - 1]) if i0 else 0
+            i1 = int(np.searchsorted(incl_h, base + max_lines, side="right"))
+            i1 = min(max(i1, i0 + 1), n)
+            n_lines = int(incl_h[i1 - 1]) - base
+            if n_lines > 0:
+                off = incl[i0:i1] - cnt[i0:i1] - base
+                rec = self._records(n_lines)
+                self._launch_lines(rows, seg_count, off, i0, i1 - i0, slots, rec, n_lines)
+                self._launch_field(rec, n_lines)
+                del off
+            i0 = i1
+
+    def add_transport(self, res, n_electrons):
+        """Fields of a ``compact=False`` transport result's slot rows."""
+        for _e0, m, cap, rows, seg_count in res.slots:
+            self.add_rows(rows, seg_count, m, cap)
+        self.n_electrons += int(n_electrons)
+
+    def spectrum(self, Ne=None, out=None):
+        """``sum_d w_d sum_g wm_g (|F_s|^2 + |F_p|^2) / Ne`` [photons/eV/electron] (device FP64)."""
+        import torch
+
+        Ne = self.n_electrons if Ne is None else Ne
+        if out is None:
+            out = torch.zeros(self.n_E, dtype=torch.float64, device=self.device)
+        if Ne <= 0:
+            return out
+        t = self.lines.dtab
+        _lib.check(self.lib.hr_coherent_finalize(self.field.data_ptr(), self.n_dir, self.n_g,
+                                                 self.lines.weights.data_ptr(), t.wm.data_ptr(), self.n_E,
+                                                 1.0 / float(Ne), out.data_ptr(), _stream()),
+                   "hr_coherent_finalize")
+        return out
+
+
+def segment_rows(segments, device):
+    """Slot rows (one row per segment) of a reference-style segments dict:
+    r_mid, v_hat, L_ang, E_repr_keV (or E_keV) and t_ang (segment start age)."""
+    import torch
+
+    def f64(a):
+        return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=device)
+
+    E = segments.get("E_repr_keV", segments["E_keV"])
+    n = int(np.asarray(segments["L_ang"]).shape[0])
+    rows = {"seg_mid": f64(segments["r_mid"]).reshape(-1, 3), "seg_dir": f64(segments["v_hat"]).reshape(-1, 3),
+            "seg_len": f64(segments["L_ang"]), "seg_E": f64(E), "seg_E_end": f64(E),
+            "seg_t0": f64(segments.get("t_ang", np.zeros(n)))}
+    return rows, torch.ones(n, dtype=torch.int64, device=device), n
+
+
+def mc_cxr_coherent_spectrum(segments, E_grid_eV, crystal, hkl_list, *, n_hats, weights, composition=None,
+                             electron_limit=None):
+    """Coherent ``sum_i weights_i * mc_cxr_spectrum(n_hat=n_hats_i, coherent=True)``
+    [photons/eV/electron] of a segments dict (host FP64 result)."""
+    from ..backend import require_device
+
+    dev = require_device().torch_device
+    tables = line_tables(crystal, hkl_list, E_grid_eV, composition)
+    sc = CoherentScorer(tables, crystal, E_grid_eV, segments["thickness_ang"], n_hats, weights, dev)
+    rows, cnt, n = segment_rows(segments, dev)
+    sc.add_rows(rows, cnt, n, 1)
+    Ne = segments["Ne"] if electron_limit is None else electron_limit
+    return sc.spectrum(Ne).cpu().numpy()

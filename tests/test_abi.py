@@ -59,6 +59,8 @@ STRUCTS = {
     "hr_vr_gather_args": "VrGatherArgs",
     "hr_beam_args": "BeamArgs",
     "hr_window_args": "WindowArgs",
+    "hr_coherent_args": "CoherentArgs",
+    "hr_coherent_field_args": "CoherentFieldArgs",
     "hr_queue_config": "QueueConfig",
     "hr_queue_launch": "QueueLaunch",
     "hr_continuation": "Continuation",

@@ -87,3 +87,29 @@ def test_python_layer_raises_typed_errors(lib):
     with pytest.raises(_lib.HipriteError) as exc:
         _lib.call("hr_stream_keys", 1, -1, 4, None, None)
     assert "negative" in str(exc.value)
+
+
+def test_coherent_route_validates_before_launching(lib):
+    assert lib.hr_coherent_lines(None, None) == HR_ERR_INVALID
+    assert "null args" in _err(lib)
+    a, la = _lib.CoherentArgs(), _lib.LineArgs()
+    a.lines = C.pointer(la)
+    a.n, a.rows = 4, 0
+    assert lib.hr_coherent_lines(C.byref(a), None) == HR_ERR_INVALID
+    assert "bad sizes" in _err(lib)
+    a.rows, la.n_dir, la.n_g, la.n_tab, la.n_mu = 8, 1, 1, 16, 1
+    a.first, a.ld = 5, 2
+    assert lib.hr_coherent_lines(C.byref(a), None) == HR_ERR_INVALID
+    assert "record columns" in _err(lib)
+    a.first, a.ld = 0, 32
+    assert lib.hr_coherent_lines(C.byref(a), None) == HR_ERR_INVALID
+    assert "null row" in _err(lib)
+    a.n = 0
+    assert lib.hr_coherent_lines(C.byref(a), None) == HR_OK
+    f = _lib.CoherentFieldArgs()
+    f.n_lines, f.ld, f.n_rows, f.n_E = 10, 4, 1, 8
+    assert lib.hr_coherent_field(C.byref(f), None) == HR_ERR_INVALID
+    f.n_lines = 0
+    assert lib.hr_coherent_field(C.byref(f), None) == HR_OK
+    assert lib.hr_coherent_finalize(None, 0, 1, None, None, 8, 1.0, None, None) == HR_ERR_INVALID
+    assert lib.hr_coherent_finalize(None, 1, 1, None, None, 0, 1.0, None, None) == HR_OK
