@@ -55,9 +55,10 @@ def parse():
                                                               "back to back on one stream")
     p.add_argument("--no-fom", action="store_true", help="skip the learned-importance figure of merit")
     p.add_argument("--fom-batches", type=int, default=1024)
-    p.add_argument("--workload", choices=("hopg", "si-stream", "pixels"), default="hopg",
+    p.add_argument("--workload", choices=("hopg", "si-stream", "pixels", "coherent"), default="hopg",
                    help="hopg: the bench line (default); si-stream: the 1 mm Si 10 MeV case streamed through "
-                        "continuation rounds; pixels: windowed scoring of synthetic segments over 32x32 pixels")
+                        "continuation rounds; pixels: windowed scoring of synthetic segments over 32x32 pixels; "
+                        "coherent: the HOPG case on the coherent line route")
     p.add_argument("--histories", type=int, default=1 << 21, help="si-stream: histories")
     p.add_argument("--lanes", type=int, default=1 << 19, help="si-stream: lanes in flight")
     p.add_argument("--rows-per-round", type=int, default=32, help="si-stream: rows per launch")
@@ -175,6 +176,74 @@ def pixel_scoring(args):
                                    % args.log2_segments}}
 
 
+def coherent_case(args):
+    """The HOPG case with coherent line scoring (fields summed over every history, squared at the end)."""
+    import torch
+
+    from pyrite_amd import roofline
+    from pyrite_amd.pipeline import CaseConfig, CasePipeline
+
+    cfg = CaseConfig(Ne=args.ne, n_side=args.n_side, coherent=True)
+    pipe = CasePipeline(cfg)
+    for w in range(args.warmup):
+        pipe.step(1000 + w)
+    torch.cuda.synchronize()
+    pipe.reset_coherent()
+    pipe.spec_brem.zero_()
+    pipe.histories = pipe.segments = 0
+    sc = pipe.coherent
+    sc.events = []
+    sc.record_events = True
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        pipe.step(s, args.steps)
+    lines = sc.spectrum()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    sc.record_events = False
+    ms = [ev[0].elapsed_time(ev[1]) for ev, _ in sc.events]
+    cells = [float(c) for _, c in sc.events]
+    avg_ms = sum(ms) / len(ms)
+    achieved = roofline.COHERENT_CELL_OPS * (sum(cells) / len(cells)) / (avg_ms * 1e-3) / 1e12
+    out = {
+        "metric": "histories/s, coherent line route", "value": pipe.histories / dt, "unit": "histories/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 transport / f64 records and phase + f32 profile / f64 fields",
+        "data": "synthetic (seeded pencil beam; analytic tables)",
+        "config": {"workload": "HOPG 100 um, 5 MeV pencil, (002) coherent PXR/CBS fields over a 10 mrad cone "
+                               "(3x3 rays) + BH brem, 1024 bins 1-50 keV",
+                   "histories_per_step": cfg.Ne, "segments_per_history": pipe.segments / max(pipe.histories, 1)},
+        "roofline": {
+            "kernel": "hr_coherent_field: coherent_field_kernel + field_partials_kernel",
+            "bound": "mfma", "pipe": "vector ALU, FP32 peak (FP64 phase reduction counted as FP32 work)",
+            "achieved": achieved, "peak": roofline.PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / roofline.PEAK_FP32_TFLOPS, "traffic": None,
+            "ops_per_cell": roofline.COHERENT_CELL_OPS, "live_cells_per_launch": sum(cells) / len(cells),
+            "avg_launch_ms": avg_ms, "launches": len(ms),
+            "algorithmic_bytes_per_launch": (sum(cells) / len(cells)) / cfg.n_bins * 104 + 9 * 4 * cfg.n_bins * 8,
+        },
+        "spectrum_integral": float(lines.sum().item()),
+    }
+    if not args.no_cpu:
+        from oracle import case as ocase
+
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        n = args.cpu_sample
+        ocase.run_case(pipe, 10_000_000, 32, threads=threads, coherent=True)  # warm-up
+        tim = {}
+        t0 = time.perf_counter()
+        ocase.run_case(pipe, 0, n, threads=threads, timings=tim, coherent=True)
+        dtc = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": n / dtc, "unit": "histories/s", "cores": threads, "kind": "port",
+            "sample": f"{n} histories of the same case on the coherent route, FP64 C oracle, {threads} OpenMP "
+                      f"threads, {dtc:.2f} s; transport {tim['transport']:.2f} s, lines {tim['lines']:.2f} s, "
+                      f"brem {tim['brem']:.2f} s"}
+        out["speedup_vs_cpu_baseline"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def cpu_baseline(case_tables, n_hist):
     from oracle import case as ocase
 
@@ -207,8 +276,8 @@ def main():
               f"GPU and the multi-GPU design is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
     if args.workload != "hopg":
-        out = si_stream(args) if args.workload == "si-stream" else pixel_scoring(args)
-        print(json.dumps(out))
+        run = {"si-stream": si_stream, "pixels": pixel_scoring, "coherent": coherent_case}[args.workload]
+        print(json.dumps(run(args)))
         return
     import torch
 

@@ -20,7 +20,12 @@ instruction stream):
   straggling): about 90 FP64 ops and 6 transcendentals (3 log, sqrt, sin, cos)
   -> ``90 + 6*40 = 330`` FP64-equivalent operations, plus the Urban draws;
 * line prologue, one (segment, reflection, direction): about 220 FP64 ops and
-  8 transcendentals -> ``220 + 8*40 = 540`` FP64-equivalent operations.
+  8 transcendentals -> ``220 + 8*40 = 540`` FP64-equivalent operations;
+* coherent field cell, one (line, bin) of the formation-factor route
+  (``coherent_stream_jit_kernel.py:153-187``): formation argument (3), the
+  factor's ``w^2`` and real/imaginary parts (12), propagation phase (4), the
+  phase rotation (6) and two complex multiply-adds (16), plus sin and cos of
+  both arguments and one division -> ``41 + 5*20 = 141``.
 """
 
 TRANSCENDENTAL_FP32 = 20
@@ -30,6 +35,7 @@ SINC_CELL_OPS = 8 + 2 * TRANSCENDENTAL_FP32
 BREM_CELL_OPS = 25 + 5 * TRANSCENDENTAL_FP32
 TRANSPORT_STEP_OPS_FP64 = 90 + 6 * TRANSCENDENTAL_FP64
 LINE_PROLOGUE_OPS_FP64 = 220 + 8 * TRANSCENDENTAL_FP64
+COHERENT_CELL_OPS = 41 + 5 * TRANSCENDENTAL_FP32
 
 # MI355X (gfx950) published peaks.
 PEAK_FP32_TFLOPS = 157.3  # vector FP32 (also the dense FP32 matrix figure)

@@ -67,6 +67,8 @@ class CoherentScorer:
                                             "E": torch.empty(0, device=device)}, 1.0)
         self._rec = None
         self.lib = _lib.load()
+        self.record_events = False
+        self.events = []  # ((start, end), live cells) per field launch when recording
 
     def reset(self):
         self.field.zero_()
@@ -111,7 +113,17 @@ class CoherentScorer:
         f.rec, f.ld, f.n_lines, f.n_rows = rec.data_ptr(), int(n_lines), int(n_lines), self.n_rows
         f.n_E, f.E_grid, f.dom = self.n_E, self.lines.E_grid_d.data_ptr(), self.dom.data_ptr()
         f.field = self.field.data_ptr()
+        ev = None
+        if self.record_events:
+            import torch
+
+            ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            ev[0].record()
         _lib.check(self.lib.hr_coherent_field(C.byref(f), _stream()), "hr_coherent_field")
+        if ev is not None:
+            ev[1].record()
+            coeff = rec[: N_FIELDS * self.n_rows * n_lines].view(N_FIELDS, self.n_rows, n_lines)[4:8]
+            self.events.append((ev, (coeff != 0).any(0).sum() * self.n_E))
 
     def add_rows(self, rows, seg_count, n, slots):
         """Add the fields of ``n`` particles' slot rows (``slots`` rows each,
