@@ -10,7 +10,9 @@
 //    a in FP64 keeps the phase a(E - E_r) exact to ~1e-11 turns;
 //  * the reduction cell evaluates the phase u = a(E - E_r)/pi in FP64 (one FMA
 //    on a uniform grid), reduces it to r = u - rint(u) in FP64, and takes
-//    sin(pi r)^2 / (pi u)^2 in FP32; every cell is accumulated in FP64;
+//    [sin(pi r) / (pi u)]^2 in FP32 (Taylor sine, hardware reciprocal); cells
+//    are summed in FP32 over groups of 4 lines (all terms are positive) and in
+//    FP64 beyond;
 //  * the brem cell is FP32 (as the reference's), its sums FP64.
 // Reductions never use atomics: each workgroup writes a partial spectrum and a
 // second pass adds the partials in a fixed order, so results are bitwise
@@ -94,8 +96,17 @@ __device__ __forceinline__ float sinpi_half(float r) {
     return s * r;
 }
 
+// [sin(pi u)/(pi u)]^2 from r = u - rint(u) (|r| <= 1/2) and u in FP32: the
+// Taylor sin(pi r) and one hardware reciprocal (v_rcp_f32, 1 ulp); u == 0 is
+// x == 0, where the reference substitutes x = 1e-20 and the profile is 1.
+__device__ __forceinline__ float sinc2_cell(float r, float uf) {
+    const float x = sinpi_half(r) * __builtin_amdgcn_rcpf((float)kPi * uf);
+    return uf == 0.0f ? 1.0f : x * x;
+}
+
 constexpr int kRedThreads = 256;
 constexpr int kLineTile = 256;  // lines staged in LDS per step
+constexpr int kSincGroup = 4;   // cells summed in FP32 before each FP64 add (all terms >= 0)
 
 // Dense sum over a uniform grid E_k = E0 + k dE.  Block b takes line tiles
 // b, b + gridDim.x, ...; thread t owns bins t + kRedThreads*m (m < BINS_PER_THREAD)
@@ -131,22 +142,23 @@ __global__ void __launch_bounds__(kRedThreads) sincsq_uniform_kernel(
             s_w[t] = 0.0f;
         }
         __syncthreads();
-        for (int q = 0; q < kLineTile; ++q) {
-            const float wq = s_w[q];
-            if (wq == 0.0f) continue;  // uniform across the block
-            const double u0 = s_u0[q], du = s_du[q];
+        for (int q0 = 0; q0 < kLineTile; q0 += kSincGroup) {
+            float g[BINS];
 #pragma unroll
-            for (int m = 0; m < BINS; ++m) {
-                const double u = fma(kd[m], du, u0);
-                const double rn = rint(u);
-                const float r = (float)(u - rn);
-                const float uf = (float)u;
-                const float s = sinpi_half(r);
-                const float denom = (float)(kPi * kPi) * uf * uf;
-                // u == 0 is x == 0: the reference substitutes x = 1e-20, sinc -> 1
-                const float cell = uf == 0.0f ? wq : wq * (s * s) / denom;
-                acc[m] += (double)cell;
+            for (int m = 0; m < BINS; ++m) g[m] = 0.0f;
+            for (int q = q0; q < q0 + kSincGroup; ++q) {
+                const float wq = s_w[q];
+                if (wq == 0.0f) continue;  // uniform across the block
+                const double u0 = s_u0[q], du = s_du[q];
+#pragma unroll
+                for (int m = 0; m < BINS; ++m) {
+                    const double u = fma(kd[m], du, u0);
+                    const float r = (float)(u - rint(u));
+                    g[m] = fmaf(wq, sinc2_cell(r, (float)u), g[m]);
+                }
             }
+#pragma unroll
+            for (int m = 0; m < BINS; ++m) acc[m] += (double)g[m];
         }
         __syncthreads();
     }
@@ -183,21 +195,23 @@ __global__ void __launch_bounds__(kRedThreads) sincsq_general_kernel(
         s_a[t] = l < n_lines ? a[l] * inv_pi : 0.0;
         s_w[t] = l < n_lines ? w[l] : 0.0f;
         __syncthreads();
-        for (int q = 0; q < kLineTile; ++q) {
-            const float wq = s_w[q];
-            if (wq == 0.0f) continue;
-            const double er = s_er[q], aq = s_a[q];
+        for (int q0 = 0; q0 < kLineTile; q0 += kSincGroup) {
+            float g[BINS];
 #pragma unroll
-            for (int m = 0; m < BINS; ++m) {
-                const double u = aq * (Ek[m] - er);
-                const double rn = rint(u);
-                const float r = (float)(u - rn);
-                const float uf = (float)u;
-                const float s = sinpi_half(r);
-                const float denom = (float)(kPi * kPi) * uf * uf;
-                const float cell = uf == 0.0f ? wq : wq * (s * s) / denom;
-                acc[m] += (double)cell;
+            for (int m = 0; m < BINS; ++m) g[m] = 0.0f;
+            for (int q = q0; q < q0 + kSincGroup; ++q) {
+                const float wq = s_w[q];
+                if (wq == 0.0f) continue;
+                const double er = s_er[q], aq = s_a[q];
+#pragma unroll
+                for (int m = 0; m < BINS; ++m) {
+                    const double u = aq * (Ek[m] - er);
+                    const float r = (float)(u - rint(u));
+                    g[m] = fmaf(wq, sinc2_cell(r, (float)u), g[m]);
+                }
             }
+#pragma unroll
+            for (int m = 0; m < BINS; ++m) acc[m] += (double)g[m];
         }
         __syncthreads();
     }
@@ -418,12 +432,7 @@ __device__ __forceinline__ double tail_value(double m2, double m3, double m4, do
 
 // Exact cell of the dense reducer at u = a (E - E_r) / pi (FP64 phase, FP32 profile).
 __device__ __forceinline__ double sincsq_cell(double u) {
-    const double rn = rint(u);
-    const float r = (float)(u - rn);
-    const float uf = (float)u;
-    const float sv = sinpi_half(r);
-    const float denom = (float)(kPi * kPi) * uf * uf;
-    return uf == 0.0f ? 1.0 : (double)((sv * sv) / denom);
+    return (double)sinc2_cell((float)(u - rint(u)), (float)u);
 }
 
 struct WindowGrid {

@@ -283,7 +283,7 @@ class CasePipeline(CaseTables):
         import torch
 
         n_total_b = int(n_steps if n_batches_total is None else n_batches_total)
-        if not overlap or self.cfg.line_roulette is not None or self.coherent is not None:
+        if not overlap or self.cfg.line_roulette is not None:
             for b in range(int(n_steps)):
                 self.step(int(first_batch) + b, n_total_b)
             return
@@ -306,7 +306,11 @@ class CasePipeline(CaseTables):
             with torch.cuda.stream(sb):
                 sb.wait_event(done)
                 self.score_dose(res, n_total)
-                self.score_records(self.records(res), n_total)
+                if self.coherent is not None:
+                    self.coherent.add_transport(res, self.cfg.Ne)
+                    self.brem.spectrum(self.records(res), n_total, self.spec_brem)
+                else:
+                    self.score_records(self.records(res), n_total)
             self.histories += self.cfg.Ne
             self.segments += res.n_segments
         main.wait_stream(sa)
@@ -334,8 +338,6 @@ class CasePipeline(CaseTables):
         if cfg.line_roulette is not None:
             raise NotImplementedError("line roulette screens whole primaries; the streamed route splits them "
                                       "over rounds (use step())")
-        if self.coherent is not None:
-            raise NotImplementedError("the coherent route is fed by step(); streamed rounds are not wired to it")
         n_total = int(n_histories if n_total is None else n_total)
         lanes = max(1, min(int(lanes), int(n_histories)))
         drv = getattr(self, "_stream_drv", None)
@@ -378,7 +380,12 @@ class CasePipeline(CaseTables):
             g.L, g.E, g.row_weight = rec["L"].data_ptr(), rec["E"].data_ptr(), rec["w"].data_ptr()
             _lib.check(lib.hr_vr_gather(C.byref(g), C.c_void_p(torch.cuda.current_stream().cuda_stream)),
                        "hr_vr_gather")
-            self.score_records({k: rec[k][:n_rows] for k in ("r_mid", "v_hat", "L", "E")}, n_total)
+            seg = {k: rec[k][:n_rows] for k in ("r_mid", "v_hat", "L", "E")}
+            if self.coherent is not None:  # fields add over rounds; squared when the spectrum is read
+                self.coherent.add_rows(rnd.rows, cnt, rnd.n_lanes, rnd.rows_per_round)
+                self.brem.spectrum(seg, n_total, self.spec_brem)
+            else:
+                self.score_records(seg, n_total)
             if cfg.depth_bins > 0:
                 from .transport.tally import depth_dose
 
@@ -387,6 +394,8 @@ class CasePipeline(CaseTables):
             tim["rows"] = tim.get("rows", 0) + n_rows
 
         res = drv.run(int(first_id), int(n_histories), fill, consume)
+        if self.coherent is not None:
+            self.coherent.n_electrons += int(n_histories)
         self.histories += int(n_histories)
         self.segments += res.n_segments
         return res

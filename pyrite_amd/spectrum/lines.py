@@ -260,7 +260,8 @@ class LineScorer:
         _lib.check(self.lib.hr_sincsq_reduce(C.byref(args), _stream()), "hr_sincsq_reduce")
         if ev is not None:
             ev[1].record()
-            self.events.append((ev, args.n_lines * args.n_E))
+            # live cells: lines of weight zero (rejected resonances) are skipped by the kernel
+            self.events.append((ev, (w != 0).sum() * args.n_E))
 
     def spectrum(self, seg, Ne, out=None, row_weight=None):
         """Solid-angle-integrated per-electron spectrum [photons/eV/electron] (device FP64)."""
