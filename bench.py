@@ -185,8 +185,7 @@ def coherent_case(args):
 
     cfg = CaseConfig(Ne=args.ne, n_side=args.n_side, coherent=True)
     pipe = CasePipeline(cfg)
-    for w in range(args.warmup):
-        pipe.step(1000 + w)
+    pipe.run_steps(1000, args.warmup, overlap=not args.no_overlap)  # warm-up through the timed path
     torch.cuda.synchronize()
     pipe.reset_coherent()
     pipe.spec_brem.zero_()
@@ -201,7 +200,15 @@ def coherent_case(args):
     dt = time.perf_counter() - t0
     sc.record_events = False
     ms = [ev[0].elapsed_time(ev[1]) for ev, _ in sc.events]
-    cells = [float(c) for _, c in sc.events]
+    # live (line, row) pairs of the timed field launches, recomputed after the
+    # timed region (the records are deterministic per batch) so that counting
+    # adds no work inside it
+    counts = []
+    for s in range(args.steps):
+        for _e0, m, cap, rows, seg_count in pipe.transport(s, compact=False).slots:
+            sc.add_rows(rows, seg_count, m, cap, live_counts=counts)
+    assert len(counts) == len(ms), (len(counts), len(ms))
+    cells = [float(c) * cfg.n_bins for c in counts]
     avg_ms = sum(ms) / len(ms)
     achieved = roofline.COHERENT_CELL_OPS * (sum(cells) / len(cells)) / (avg_ms * 1e-3) / 1e12
     out = {
@@ -263,6 +270,36 @@ def cpu_baseline(case_tables, n_hist):
     }
 
 
+def live_cells(pipe, steps, launches):
+    """Live (nonzero-weight) line x bin cells of the timed sinc^2 launches.
+
+    Lines of weight zero (rejected resonances: grazing rays whose in-medium root
+    does not settle) are skipped by the kernel, so only live cells are work.
+    Counting them inside the timed region would add tensor work there, so the
+    timed batches are recomputed afterwards (transport and line prologue are
+    deterministic per batch) and their lines counted block by block, in the
+    order the reduction launched them."""
+    import torch
+
+    from pyrite_amd.spectrum.lines import LINE_BLOCK
+
+    sc = pipe.lines
+    per_seg = sc.n_dir * len(sc.tables.wm)
+    step = max(1, LINE_BLOCK // per_seg)
+    total = 0.0
+    n = 0
+    for s in range(steps):
+        seg = pipe.records(pipe.transport(s, compact=False))
+        n_seg = int(seg["L"].shape[0])
+        for j0 in range(0, n_seg, step):
+            _E_r, _a, w = sc.lines(seg, 1.0, j0, min(step, n_seg - j0))
+            total += float((w != 0).sum().item()) * sc.E_grid.size
+            n += 1
+        torch.cuda.synchronize()
+    assert n == launches, (n, launches)
+    return total
+
+
 # HBM bytes per sincsq_uniform_kernel launch of the default workload (mean of the
 # step's two launches), measured with rocprofv3 --pmc and a copy calibration.
 SINC_TRAFFIC_BYTES = 384.6e6
@@ -285,8 +322,8 @@ def main():
 
     cfg = CaseConfig(Ne=args.ne, n_side=args.n_side, line_route=args.line_route)
     pipe = CasePipeline(cfg)
-    for w in range(args.warmup):
-        pipe.step(1000 + w)
+    # warm-up through the timed path itself (its two streams and their allocator pools)
+    pipe.run_steps(1000, args.warmup, overlap=not args.no_overlap)
     torch.cuda.synchronize()
     pipe.spec_lines.zero_()
     pipe.spec_brem.zero_()
@@ -319,7 +356,7 @@ def main():
     value = hist / dt
     # dominant kernel: sinc^2 reduction, event-timed per launch
     ms = [ev[0].elapsed_time(ev[1]) for ev, _ in pipe.lines.events] or [float("nan")]
-    cells = sum(float(c) for _, c in pipe.lines.events)  # live (nonzero-weight) lines x bins
+    cells = live_cells(pipe, args.steps, len(ms))  # live (nonzero-weight) lines x bins of the timed launches
     avg_ms = sum(ms) / len(ms)
     flops_per_launch = roofline.SINC_CELL_OPS * cells / len(ms)
     achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12

@@ -68,7 +68,7 @@ class CoherentScorer:
         self._rec = None
         self.lib = _lib.load()
         self.record_events = False
-        self.events = []  # ((start, end), live cells) per field launch when recording
+        self.events = []  # ((start, end), cells) per field launch when recording
 
     def reset(self):
         self.field.zero_()
@@ -122,13 +122,14 @@ class CoherentScorer:
         _lib.check(self.lib.hr_coherent_field(C.byref(f), _stream()), "hr_coherent_field")
         if ev is not None:
             ev[1].record()
-            coeff = rec[: N_FIELDS * self.n_rows * n_lines].view(N_FIELDS, self.n_rows, n_lines)[4:8]
-            self.events.append((ev, (coeff != 0).any(0).sum() * self.n_E))
+            self.events.append((ev, n_lines * self.n_rows * self.n_E))
 
-    def add_rows(self, rows, seg_count, n, slots):
+    def add_rows(self, rows, seg_count, n, slots, live_counts=None):
         """Add the fields of ``n`` particles' slot rows (``slots`` rows each,
         ``seg_count`` written) to the planes, in passes of at most
-        ``RECORD_BYTES`` of records."""
+        ``RECORD_BYTES`` of records.  With a ``live_counts`` list the records
+        are only built, and each pass's live (line, row) count is appended
+        instead of adding fields."""
         import torch
 
         if n == 0:
@@ -148,7 +149,11 @@ class CoherentScorer:
                 off = incl[i0:i1] - cnt[i0:i1] - base
                 rec = self._records(n_lines)
                 self._launch_lines(rows, seg_count, off, i0, i1 - i0, slots, rec, n_lines)
-                self._launch_field(rec, n_lines)
+                if live_counts is None:
+                    self._launch_field(rec, n_lines)
+                else:
+                    coeff = rec[: N_FIELDS * self.n_rows * n_lines].view(N_FIELDS, self.n_rows, n_lines)[4:8]
+                    live_counts.append(int((coeff != 0).any(0).sum().item()))
                 del off
             i0 = i1
 

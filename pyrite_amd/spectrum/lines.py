@@ -193,7 +193,8 @@ class LineScorer:
         self.e_p = torch.as_tensor(ep, device=device)
         self.lib = _lib.load()
         self.record_events = False
-        self.events = []  # ((start, end), cells) per reduction launch when recording
+        self.events = []  # ((start, end), cells) per reduction launch when recording (nothing else
+        #                   is computed, so recording does not disturb the timed work)
 
     def line_args(self, seg, scale, j0=0, n=None, row_weight=None):
         """``hr_line_args`` for segments ``[j0, j0+n)`` of ``seg`` (outputs unset)."""
@@ -260,8 +261,7 @@ class LineScorer:
         _lib.check(self.lib.hr_sincsq_reduce(C.byref(args), _stream()), "hr_sincsq_reduce")
         if ev is not None:
             ev[1].record()
-            # live cells: lines of weight zero (rejected resonances) are skipped by the kernel
-            self.events.append((ev, (w != 0).sum() * args.n_E))
+            self.events.append((ev, args.n_lines * args.n_E))
 
     def spectrum(self, seg, Ne, out=None, row_weight=None):
         """Solid-angle-integrated per-electron spectrum [photons/eV/electron] (device FP64)."""

@@ -121,7 +121,7 @@ def test_continuation_is_bitwise_the_uninterrupted_run(device, case, lanes, R, p
 @pytest.mark.parametrize("route", ["dense", "window"])
 def test_streamed_case_matches_the_batched_route(device, route):
     """Streamed transport -> scoring gives the batched route's tallies (same rows;
-    only the FP64 summation order of the tallies differs)."""
+    only the summation order of the tallies differs)."""
     from pyrite_amd.pipeline import CaseConfig, CasePipeline
 
     cfg = CaseConfig(Ne=3000, line_route=route, E0_keV=2000.0, thickness_ang=2.0e5)
@@ -133,7 +133,8 @@ def test_streamed_case_matches_the_batched_route(device, route):
     lb, bb = b.spectra()
     assert sres.n_segments == res.n_segments and sres.counts == res.counts
     assert la.sum() > 0 and ba.sum() > 0
-    np.testing.assert_allclose(lb, la, rtol=1e-9, atol=1e-12 * la.max())
-    # brem cells are summed in FP32 over groups of 8 rows before each FP64
-    # accumulate, so another row order moves the last bits at about 1e-8
+    # sinc^2 cells are summed in FP32 over groups of 4 lines and brem cells
+    # over groups of 8 rows before each FP64 accumulate, so another row order
+    # moves the last bits at about 1e-8
+    np.testing.assert_allclose(lb, la, rtol=1e-7, atol=1e-12 * la.max())
     np.testing.assert_allclose(bb, ba, rtol=1e-7, atol=1e-10 * ba.max())
