@@ -110,7 +110,12 @@ class CasePipeline(CaseTables):
         super().__init__(cfg)
         self.dev = require_device().torch_device
         dev = self.dev
-        self.tcfg = transport_config or PerElectronTransportConfig()
+        # Row slots: 2048 per history from the start (a 1e5-history batch of
+        # slots is 23 GB, two in flight), so capacity replays are rare and the
+        # slot arrays keep one size from batch to batch (the allocator reuses
+        # them).  No field of the config changes results.
+        self.tcfg = transport_config or PerElectronTransportConfig(seg_capacity=2048,
+                                                                   scratch_budget_bytes=48 << 30)
         self.tables = DeviceTables(self.layers, self.lut, dev)
         self.control = TransportControl(max_steps=cfg.max_steps, energy_model=1, straggle_on=cfg.straggling,
                                         precision=cfg.precision)
@@ -142,16 +147,43 @@ class CasePipeline(CaseTables):
         self.segments = 0
         self.timers = {}
         self._cap_hint = None
+        self._row_pool = [None, None]
+        self._pool_turn = 0
+        self._pool_last = 0
+        self._pool_free = [None, None]
         self._rec = None
         self._ones = torch.ones(n, dtype=torch.float64, device=dev)
 
+    def _pool_rows(self, idx, e, m, cap, midpoint, device):
+        """Slot rows of a batch's first sub-batch from persistent buffer ``idx``
+        (grown when a batch needs more slots); later sub-batches get fresh rows."""
+        from .transport.core import alloc_rows
+
+        if e != 0:
+            return alloc_rows(m, cap, midpoint, device)
+        need = m * cap
+        buf = self._row_pool[idx]
+        if buf is None or buf["seg_len"].shape[0] < need:
+            self._row_pool[idx] = None
+            buf = self._row_pool[idx] = alloc_rows(m, cap, midpoint, device)
+        return {k: v[:need] for k, v in buf.items()}
+
     def transport(self, batch_index: int, compact: bool = True):
         """Transport histories ``[b*Ne, (b+1)*Ne)``; returns the device segment rows
-        (compacted, or the slot rows of each sub-batch with ``compact=False``)."""
+        (compacted, or the slot rows of each sub-batch with ``compact=False``).
+
+        Slot rows live in two persistent buffers used in turn, so those of one
+        call stay valid until the second call after it; a call waits (on the
+        device) until the scoring that last read its buffer has finished."""
         import torch
 
         cfg = self.cfg
         n = cfg.Ne
+        idx = self._pool_turn
+        self._pool_turn ^= 1
+        if self._pool_free[idx] is not None:
+            torch.cuda.current_stream().wait_event(self._pool_free[idx])
+        self._pool_last = idx
         stream = torch.cuda.current_stream().cuda_stream
         _lib.call("hr_stream_keys", int(cfg.seed), int(batch_index) * n, n, self.keys.data_ptr(), stream)
         fill_entry(cfg.beam, cfg.seed, int(batch_index) * n, n, cfg.E0_keV, np.deg2rad(cfg.tilt_deg),
@@ -159,9 +191,19 @@ class CasePipeline(CaseTables):
         res = drive_batches(self.core, self.keys,
                             (self.alive, self.clock, self.pos, self.dirs, self.E, self.E_cut),
                             self.stragg, self.control, n, self.dev, self.tcfg, stream=stream, compact=compact,
-                            cap_hint=self._cap_hint)
+                            cap_hint=None if self._cap_hint is None else max(self._cap_hint, self.tcfg.seg_capacity),
+                            alloc=lambda e, m, cap, mid, dev: self._pool_rows(idx, e, m, cap, mid, dev))
         self._cap_hint = res.capacity
         return res
+
+    def _scored(self):
+        """Mark the rows of the last transport as read by the work queued on the
+        current stream (their buffer may be refilled after it)."""
+        import torch
+
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream())
+        self._pool_free[self._pool_last] = ev
 
     def records(self, res, first_id=None):
         """FP32 scoring records gathered straight from the slot rows of a
@@ -267,6 +309,7 @@ class CasePipeline(CaseTables):
         else:
             seg = self.records(res, first_id=int(batch_index) * self.cfg.Ne)
             self.score_records(seg, n_total, screen.row_weights(self.lines, seg, seg["prim"], n_total))
+        self._scored()
         self.histories += self.cfg.Ne
         self.segments += res.n_segments
         return res
@@ -311,6 +354,7 @@ class CasePipeline(CaseTables):
                     self.brem.spectrum(self.records(res), n_total, self.spec_brem)
                 else:
                     self.score_records(self.records(res), n_total)
+                self._scored()
             self.histories += self.cfg.Ne
             self.segments += res.n_segments
         main.wait_stream(sa)

@@ -238,12 +238,14 @@ class DriveResult:
 
 def drive_batches(core, keys, state, stragg_dE, control: TransportControl, Ne: int, device,
                   config: PerElectronTransportConfig = DEFAULT_CONFIG, stream=None, compact=True,
-                  cap_hint=None) -> DriveResult:
+                  cap_hint=None, alloc=None) -> DriveResult:
     """Capacity-replayed batches (``batching.py:477-628``), rows kept on device.
 
     ``compact=False`` skips the electron-major compaction and returns the slot
     rows of each sub-batch (a scorer can gather them itself); ``cap_hint``
-    starts from a capacity learned by an earlier call."""
+    starts from a capacity learned by an earlier call; ``alloc(e, m, cap,
+    midpoint, device)`` (default: fresh tensors) supplies the slot rows of the
+    sub-batch starting at electron ``e``, e.g. from a persistent pool."""
     import torch
 
     alive, clock, pos, dirs, E_keV, E_cut = state
@@ -262,7 +264,7 @@ def drive_batches(core, keys, state, stragg_dE, control: TransportControl, Ne: i
         sl = slice(e, e + m)
         snap = (pos[sl].clone(), dirs[sl].clone(), E_keV[sl].clone(), clock[sl].clone())
         while True:
-            rows = alloc_rows(m, cap, midpoint, device)
+            rows = alloc_rows(m, cap, midpoint, device) if alloc is None else alloc(e, m, cap, midpoint, device)
             seg_count = torch.zeros(m, dtype=torch.int64, device=device)
             exit_code = torch.zeros(m, dtype=torch.int8, device=device)
             core((e, m, cap, keys), state, rows, (seg_count, exit_code),
