@@ -378,6 +378,8 @@ typedef struct hr_coherent_args {
     int64_t first;             /* record column of the first row written */
     int64_t ld;                /* column stride of the record planes (>= first + rows written) */
     double *rec;               /* [13][n_dir*n_g][ld] */
+    int64_t n_lines;           /* rows written (sum of the clamped counts): one thread per row;
+                                  0 = one thread per slot */
 } hr_coherent_args;
 hr_status hr_coherent_lines(const hr_coherent_args *args, void *stream);
 
@@ -507,6 +509,8 @@ typedef struct hr_vr_gather_args {
     int64_t *row_prim;             /* out or NULL */
     const double *weight2;         /* [n] or NULL */
     float *row_weight2;            /* out, or NULL */
+    int64_t n_out;                 /* rows to write (sum of the clamped counts): one thread per row;
+                                      0 = one thread per slot */
 } hr_vr_gather_args;
 hr_status hr_vr_gather(const hr_vr_gather_args *args, void *stream);
 

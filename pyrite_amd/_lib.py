@@ -202,6 +202,7 @@ class CoherentArgs(C.Structure):
         ("seg_E", _dp), ("seg_E_end", _dp), ("seg_t0", _dp),
         ("first", C.c_int64), ("ld", C.c_int64),
         ("rec", _dp),
+        ("n_lines", C.c_int64),
     ]
 
 
@@ -294,6 +295,7 @@ class VrGatherArgs(C.Structure):
         ("r_mid", _dp), ("v_hat", _dp), ("L", _dp), ("E", _dp),
         ("row_weight", _dp), ("row_prim", _dp),
         ("weight2", _dp), ("row_weight2", _dp),
+        ("n_out", C.c_int64),
     ]
 
 

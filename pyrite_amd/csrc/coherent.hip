@@ -39,14 +39,26 @@ constexpr double kInv2Pi = 0.15915494309189535;
 inline int64_t blocks_of(int64_t n, int t) { return (n + t - 1) / t; }
 
 // ---------------------------------------------------------------- prologue
-// One thread per slot row; for each direction and reflection it writes the
-// 13 record planes at column first + row_offset[i] + s (coalesced along s).
+// One thread per written row (its particle found by binary search over the
+// row offsets; with n_lines = 0, one thread per slot); for each direction
+// and reflection it writes the 13 record planes at column
+// first + row_offset[i] + s (coalesced along the rows).
 __global__ void __launch_bounds__(256) coherent_lines_kernel(hr_coherent_args a, hr_line_args p) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.n * a.rows) return;
-    const int64_t i = t / a.rows, s = t - i * a.rows;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i, s;
+    if (a.n_lines > 0) {
+        if (tid >= a.n_lines) return;
+        const int64_t r = a.row_offset[0] + tid;
+        i = row_owner(a.row_offset, a.n, r);
+        s = r - a.row_offset[i];
+    } else {
+        if (tid >= a.n * a.rows) return;
+        i = tid / a.rows;
+        s = tid - i * a.rows;
+    }
     const int64_t m = a.seg_count[i] < a.rows ? a.seg_count[i] : a.rows;
     if (s >= m) return;
+    const int64_t t = i * a.rows + s;
     const int64_t col = a.first + a.row_offset[i] + s;
     const int64_t plane = (int64_t)p.n_dir * p.n_g * a.ld;
     const double rx = a.seg_mid[3 * t], ry = a.seg_mid[3 * t + 1], rz = a.seg_mid[3 * t + 2];
@@ -352,7 +364,9 @@ extern "C" hr_status hr_coherent_lines(const hr_coherent_args *args, void *strea
     HR_REQUIRE(p.n_hat && p.g && p.row && p.e_s && p.e_p && p.E_tab && p.delta_tab && p.chi_re && p.chi_im &&
                    p.u_re && p.u_im && p.log_mu,
                "hr_coherent_lines: null table pointer");
-    const int64_t total = a.n * a.rows;
+    HR_REQUIRE(a.n_lines >= 0 && a.n_lines <= a.n * a.rows, "hr_coherent_lines: n_lines %lld outside [0, %lld]",
+               (long long)a.n_lines, (long long)(a.n * a.rows));
+    const int64_t total = a.n_lines > 0 ? a.n_lines : a.n * a.rows;
     hipLaunchKernelGGL(coherent_lines_kernel, dim3((unsigned)blocks_of(total, 256)), dim3(256), 0,
                        (hipStream_t)stream, a, p);
     HR_CHECK_HIP(hipGetLastError());

@@ -53,6 +53,20 @@ __host__ __device__ __forceinline__ double stream_uniform(uint64_t key, uint64_t
     return (double)(z >> 11) * kU53;
 }
 
+// Particle owning compact row r of slot rows laid out by an exclusive prefix
+// sum of per-particle row counts: the last i with row_offset[i] <= r
+// (particles without rows share the next one's offset and are passed over).
+// row_offset[0] <= r < row_offset[n-1] + count[n-1] is the caller's range.
+__device__ __forceinline__ int64_t row_owner(const int64_t *__restrict__ row_offset, int64_t n, int64_t r) {
+    int64_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (row_offset[mid] <= r) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 inline int pick_block(int requested, int fallback) {
     if (requested <= 0) return fallback;
     if (requested > 1024) return 1024;

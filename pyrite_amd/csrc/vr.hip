@@ -148,11 +148,20 @@ __global__ void __launch_bounds__(256) vr_spawn_kernel(hr_vr_spawn_args p) {
 // threads read consecutive slot rows and write consecutive records.
 __global__ void __launch_bounds__(256) vr_gather_kernel(hr_vr_gather_args p) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= p.n * p.rows) return;
-    const int64_t i = t / p.rows, s = t - i * p.rows;
+    int64_t i, s;
+    if (p.n_out > 0) {  // one thread per written row
+        if (t >= p.n_out) return;
+        const int64_t r = p.row_offset[0] + t;
+        i = row_owner(p.row_offset, p.n, r);
+        s = r - p.row_offset[i];
+    } else {  // one thread per slot
+        if (t >= p.n * p.rows) return;
+        i = t / p.rows;
+        s = t - i * p.rows;
+    }
     const int64_t m = p.seg_count[i] < p.rows ? p.seg_count[i] : p.rows;
     if (s >= m) return;
-    const int64_t src = t, dst = p.row_offset[i] + s;
+    const int64_t src = i * p.rows + s, dst = p.row_offset[i] + s;
     for (int c = 0; c < 3; ++c) {
         p.r_mid[3 * dst + c] = (float)p.seg_mid[3 * src + c];
         p.v_hat[3 * dst + c] = (float)p.seg_dir[3 * src + c];
@@ -325,8 +334,10 @@ hr_status hr_vr_gather(const hr_vr_gather_args *p, void *stream) {
     HR_REQUIRE(p->r_mid && p->v_hat && p->L && p->E && p->row_weight, "hr_vr_gather: null output");
     HR_REQUIRE(p->n <= (INT64_MAX / p->rows) && blocks_for(p->n * p->rows, 256) < (1ULL << 31),
                "hr_vr_gather: %lld slots exceed one launch", (long long)p->n * (long long)p->rows);
-    hipLaunchKernelGGL(vr_gather_kernel, dim3(blocks_for(p->n * p->rows, 256)), dim3(256), 0, (hipStream_t)stream,
-                       *p);
+    HR_REQUIRE(p->n_out >= 0 && p->n_out <= p->n * p->rows, "hr_vr_gather: n_out %lld outside [0, %lld]",
+               (long long)p->n_out, (long long)(p->n * p->rows));
+    const int64_t threads = p->n_out > 0 ? p->n_out : p->n * p->rows;
+    hipLaunchKernelGGL(vr_gather_kernel, dim3(blocks_for(threads, 256)), dim3(256), 0, (hipStream_t)stream, *p);
     HR_CHECK_HIP(hipGetLastError());
     return HR_OK;
 }

@@ -231,6 +231,7 @@ class CasePipeline(CaseTables):
             off = torch.cumsum(cnt, 0) - cnt + o
             g = _lib.VrGatherArgs()
             g.n, g.rows = m, cap
+            g.n_out = n_rows if len(res.slots) == 1 else int(cnt.sum())  # one thread per written row
             g.seg_count, g.row_offset = seg_count.data_ptr(), off.data_ptr()
             g.weight = self._ones[:m].data_ptr()
             g.seg_mid, g.seg_dir = rows["seg_mid"].data_ptr(), rows["seg_dir"].data_ptr()
@@ -413,7 +414,7 @@ class CasePipeline(CaseTables):
                 return
             off = incl - cnt
             g = _lib.VrGatherArgs()
-            g.n, g.rows = rnd.n_lanes, rnd.rows_per_round
+            g.n, g.rows, g.n_out = rnd.n_lanes, rnd.rows_per_round, n_rows
             g.seg_count, g.row_offset = cnt.data_ptr(), off.data_ptr()
             g.weight = self._sones.data_ptr()
             rows = rnd.rows

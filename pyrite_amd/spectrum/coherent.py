@@ -105,6 +105,7 @@ class CoherentScorer:
         for k in ("seg_len", "seg_E", "seg_E_end", "seg_t0"):
             setattr(a, k, rows[k].data_ptr() + 8 * base)
         a.first, a.ld = 0, int(ld)
+        a.n_lines = int(ld)  # the pass writes columns [0, ld): one thread per row
         a.rec = rec.data_ptr()
         _lib.check(self.lib.hr_coherent_lines(C.byref(a), _stream()), "hr_coherent_lines")
 

@@ -390,7 +390,7 @@ class LearnedImportance:
             n_rows = int(incl[-1])
             off = incl - m
             g = _lib.VrGatherArgs()
-            g.n, g.rows = n, Rr
+            g.n, g.rows, g.n_out = n, Rr, n_rows
             g.seg_count, g.row_offset = self._seg_count.data_ptr(), off.data_ptr()
             g.weight, g.prim = src.weight.data_ptr(), None
             g.seg_mid, g.seg_dir = rows["seg_mid"].data_ptr(), rows["seg_dir"].data_ptr()
