@@ -18,8 +18,8 @@ region; `cpu_baseline` times the FP64 C oracle (the CPU restatement of the
 reference path, OpenMP over histories and bins) on a bounded sample of the same
 workload.  `fom` trains the importance network on analog tallies of the same
 case and compares the PXR-peak figure of merit 1/(rel_err^2 t) of the learned
-splitting/roulette run with the plain analog run (1024 VR and 3072 analog
-batches of 65536 histories, disjoint, with bootstrap intervals because the
+splitting/roulette run with the plain analog run (256 VR and 768 analog
+batches of 262,144 histories, disjoint, with bootstrap intervals because the
 peak tally is heavy-tailed; after the timed region, not part of `value`).
 """
 
@@ -54,7 +54,8 @@ def parse():
     p.add_argument("--no-overlap", action="store_true", help="run transport and scoring of successive batches "
                                                               "back to back on one stream")
     p.add_argument("--no-fom", action="store_true", help="skip the learned-importance figure of merit")
-    p.add_argument("--fom-batches", type=int, default=1024)
+    p.add_argument("--fom-batches", type=int, default=256, help="learned-VR batches (analog: three times as many)")
+    p.add_argument("--fom-histories", type=int, default=262144, help="histories per FOM batch")
     p.add_argument("--workload", choices=("hopg", "si-stream", "pixels", "coherent"), default="hopg",
                    help="hopg: the bench line (default); si-stream: the 1 mm Si 10 MeV case streamed through "
                         "continuation rounds; pixels: windowed scoring of synthetic segments over 32x32 pixels; "
@@ -68,7 +69,7 @@ def parse():
     return p.parse_args()
 
 
-def learned_fom(pipe_cfg, n_batches):
+def learned_fom(pipe_cfg, n_batches, Ne=262144):
     """PXR-peak figure of merit of the learned importance vs the plain analog run."""
     import numpy as np
     import torch
@@ -76,24 +77,31 @@ def learned_fom(pipe_cfg, n_batches):
     from pyrite_amd.pipeline import CaseConfig, CasePipeline
     from pyrite_amd.vr.importance import LearnedImportance, VRConfig, figure_of_merit
 
-    Ne = 65536
     pipe = CasePipeline(CaseConfig(**{**pipe_cfg.__dict__, "Ne": Ne}))
     vr = LearnedImportance(pipe, VRConfig())
     t0 = time.perf_counter()
-    x, y = vr.training_set(8, first_batch=10_000)
+    n_train = max(1, (8 * 65536) // Ne)  # 524,288 training histories
+    x, y = vr.training_set(n_train, first_batch=10_000)
     vr.fit(x, y)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     vr.measure(1, Ne, "analog", first_batch=900)
     vr.measure(1, Ne, "vr", first_batch=901)
     out = {"histories_per_batch": Ne, "training_s": t_train,
-           "training_histories": 8 * Ne, "window_eV": list(vr.window)}
+           "training_histories": n_train * Ne, "window_eV": list(vr.window)}
     rng = np.random.default_rng(0)
     boots = {}
     # the analog estimate of the variance is the weak one (its tail is the heavier),
     # so it gets three times the batches
     for mode, first, nb in (("analog", 0, 3 * n_batches), ("vr", 20_000, n_batches)):
-        vals, t = vr.measure(nb, Ne, mode, first_batch=first)
+        parts, t = [], 0.0
+        chunk = max(1, (512 * 65536) // Ne)
+        for c0 in range(0, nb, chunk):  # chunks, with a progress line each
+            v, tc = vr.measure(min(chunk, nb - c0), Ne, mode, first_batch=first + c0)
+            parts.append(np.asarray(v))
+            t += tc
+            print(f"fom: {mode} {c0 + len(parts[-1])}/{nb} batches, {t:.1f} s", file=sys.stderr, flush=True)
+        vals = np.concatenate(parts)
         mean, rel, fom = figure_of_merit(vals, t)
         boot = []
         for _ in range(1000):  # bootstrap over batches: the peak tally is heavy-tailed
@@ -410,7 +418,7 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
     if not args.no_fom:
-        out["fom"] = learned_fom(cfg, args.fom_batches)
+        out["fom"] = learned_fom(cfg, args.fom_batches, args.fom_histories)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pipe, args.cpu_sample)
         out["speedup_vs_cpu_baseline"] = value / out["cpu_baseline"]["value"]

@@ -119,6 +119,8 @@ class LearnedImportance:
         self.m_thr_unit = None  # line-roulette threshold for a line scale of 1 (per-electron masses x Ne)
         self.train_history = []
         self._banks = None
+        self.overlap = True  # score each round on a second stream beside the next round's transport
+        self._score_stream = None
         self._rows_R = 0
         self.rounds = 0
 
@@ -306,14 +308,17 @@ class LearnedImportance:
         self._count = torch.empty(cap, dtype=torch.int32, device=dev)
         self._wc = torch.empty(cap, dtype=torch.float64, device=dev)
         m = cap * R
-        self._seg = {
-            "r_mid": torch.empty((m, 3), dtype=torch.float32, device=dev),
-            "v_hat": torch.empty((m, 3), dtype=torch.float32, device=dev),
-            "L": torch.empty(m, dtype=torch.float32, device=dev),
-            "E": torch.empty(m, dtype=torch.float32, device=dev),
-        }
-        self._row_w = torch.empty(m, dtype=torch.float32, device=dev)
-        self._row_w2 = torch.empty(m, dtype=torch.float32, device=dev)
+        # two record sets used in turn: round r+1 gathers into one while the
+        # scoring stream still reads round r's
+        self._sets = [{
+            "seg": {"r_mid": torch.empty((m, 3), dtype=torch.float32, device=dev),
+                    "v_hat": torch.empty((m, 3), dtype=torch.float32, device=dev),
+                    "L": torch.empty(m, dtype=torch.float32, device=dev),
+                    "E": torch.empty(m, dtype=torch.float32, device=dev)},
+            "w": torch.empty(m, dtype=torch.float32, device=dev),
+            "w2": torch.empty(m, dtype=torch.float32, device=dev),
+        } for _ in range(2)]
+        self._set_free = [None, None]
         self._w2c = torch.empty(cap, dtype=torch.float64, device=dev)
         self._rows_next = torch.empty(cap, dtype=torch.int32, device=dev)
 
@@ -372,6 +377,17 @@ class LearnedImportance:
         n = Ne
         rows = self._rows
         tail_n = max(64, int(v.tail_fraction * Ne))
+        # rounds are scored on a second stream while the next round is
+        # transported; scoring keeps its order, so tallies are those of the
+        # one-stream schedule bit for bit (stats need synchronous reductions)
+        overlap = self.overlap and stats is None
+        main = torch.cuda.current_stream()
+        if overlap:
+            if self._score_stream is None:
+                self._score_stream = torch.cuda.Stream(device=p.dev)
+            sb = self._score_stream
+            sb.wait_stream(main)
+        k_set = 0
         while n > 0:
             self.rounds += 1
             # the last stragglers take long rounds: fewer launches, decisions rarer where they matter least
@@ -384,7 +400,10 @@ class LearnedImportance:
                                (None, src.clock, src.pos, src.dirs, src.E, src.E_cut), rows,
                                (self._seg_count, self._exit), (p.control.straggle_on, src.stragg))
             _lib.check(self.lib.hr_vr_round(C.byref(a), Rr, rows_each, _stream()), "hr_vr_round")
-            # rows -> weighted FP32 scoring records
+            # rows -> weighted FP32 scoring records (record set k_set, once its last reader is done)
+            rec = self._sets[k_set]
+            if overlap and self._set_free[k_set] is not None:
+                main.wait_event(self._set_free[k_set])
             m = self._seg_count[:n].clamp(max=Rr)
             incl = torch.cumsum(m, 0)
             n_rows = int(incl[-1])
@@ -396,11 +415,11 @@ class LearnedImportance:
             g.seg_mid, g.seg_dir = rows["seg_mid"].data_ptr(), rows["seg_dir"].data_ptr()
             g.seg_len, g.seg_E, g.seg_E_end = rows["seg_len"].data_ptr(), rows["seg_E"].data_ptr(), \
                 rows["seg_E_end"].data_ptr()
-            s = self._seg
+            s = rec["seg"]
             g.r_mid, g.v_hat, g.L, g.E = s["r_mid"].data_ptr(), s["v_hat"].data_ptr(), s["L"].data_ptr(), \
                 s["E"].data_ptr()
-            g.row_weight, g.row_prim = self._row_w.data_ptr(), None
-            g.weight2, g.row_weight2 = src.weight2.data_ptr(), self._row_w2.data_ptr()
+            g.row_weight, g.row_prim = rec["w"].data_ptr(), None
+            g.weight2, g.row_weight2 = src.weight2.data_ptr(), rec["w2"].data_ptr()
             _lib.check(self.lib.hr_vr_gather(C.byref(g), _stream()), "hr_vr_gather")
             if stats is not None:
                 stats["rows2"] = stats.get("rows2", 0.0) + float((m.to(torch.float64) * src.weight2[:n]).sum())
@@ -416,10 +435,22 @@ class LearnedImportance:
             if n_rows:
                 seg = {k: t[:n_rows] for k, t in s.items()}
                 if stats is not None:
-                    wm = self.row_masses(seg, Ne_total) * self._row_w[:n_rows].double()
+                    wm = self.row_masses(seg, Ne_total) * rec["w"][:n_rows].double()
                     stats["wmass"] = stats.get("wmass", 0.0) + float(wm.sum())
-                self.score_lines(seg, Ne_total, spec_lines, self._row_w[:n_rows], rkey)
-                p.brem.reduce(seg, brem_acc, row_weight=self._row_w2[:n_rows])
+                if overlap:
+                    ready = torch.cuda.Event()
+                    ready.record(main)
+                    with torch.cuda.stream(sb):
+                        sb.wait_event(ready)
+                        self.score_lines(seg, Ne_total, spec_lines, rec["w"][:n_rows], rkey)
+                        p.brem.reduce(seg, brem_acc, row_weight=rec["w2"][:n_rows])
+                        done = torch.cuda.Event()
+                        done.record(sb)
+                    self._set_free[k_set] = done
+                    k_set ^= 1
+                else:
+                    self.score_lines(seg, Ne_total, spec_lines, rec["w"][:n_rows], rkey)
+                    p.brem.reduce(seg, brem_acc, row_weight=rec["w2"][:n_rows])
             # split / roulette the suspended particles
             if use_vr:
                 self._features(n, src.pos, src.dirs, src.E, None, self._x)
@@ -449,6 +480,8 @@ class LearnedImportance:
                 _lib.check(self.lib.hr_vr_spawn(C.byref(sp), _stream()), "hr_vr_spawn")
             src, dst = dst, src
             n = total
+        if overlap:
+            main.wait_stream(sb)
         self._banks = [src, dst]
 
     def measure(self, n_batches, Ne, mode="vr", first_batch=0):

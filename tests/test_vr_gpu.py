@@ -300,6 +300,32 @@ def hopg_case(device):
     return CasePipeline(CaseConfig(Ne=16384))
 
 
+def test_overlapped_rounds_give_the_one_stream_tallies(small_case):
+    """Scoring each round on a second stream beside the next round's transport
+    keeps the scoring order: line and brem tallies equal the one-stream
+    schedule's bit for bit."""
+    import torch
+
+    from pyrite_amd.vr.importance import LearnedImportance, VRConfig
+
+    vr = LearnedImportance(small_case, VRConfig(train_rows=200_000, epochs=2))
+    x, y = vr.training_set(2, first_batch=30_000)
+    vr.fit(x, y)
+    Ne, p = small_case.cfg.Ne, small_case
+    out = []
+    for overlap in (False, True):
+        vr.overlap = overlap
+        lines = torch.zeros(p.E_grid.size, dtype=torch.float64, device=p.dev)
+        brem = torch.zeros_like(lines)
+        for b in range(3):
+            vr.run_batch(40_000 + b, Ne, 3 * Ne, lines, brem, use_vr=True, roulette=True)
+        out.append((lines.cpu().numpy(), brem.cpu().numpy()))
+    vr.overlap = True
+    assert out[0][0].sum() > 0 and out[0][1].sum() > 0
+    np.testing.assert_array_equal(out[1][0], out[0][0])
+    np.testing.assert_array_equal(out[1][1], out[0][1])
+
+
 def test_learned_importance_gains_over_analog(hopg_case):
     """The trained network raises the PXR-peak figure of merit on the 100 um HOPG case."""
     from pyrite_amd.vr.importance import LearnedImportance, VRConfig, figure_of_merit
