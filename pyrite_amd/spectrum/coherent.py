@@ -61,10 +61,14 @@ class CoherentScorer:
         self.dom = torch.as_tensor(escape_phase_rate(crystal_key, E_grid_eV), device=device)
         self.field = torch.zeros((self.n_rows, 4, self.n_E), dtype=torch.float64, device=device)
         self.n_electrons = 0
+        # crystal, tables and detector rays of the incoherent prologue's args; the
+        # coherent kernels read the FP64 slot rows instead of its FP32 segments
         self._largs = self.lines.line_args({"L": torch.empty(0, device=device),
                                             "r_mid": torch.empty((0, 3), device=device),
                                             "v_hat": torch.empty((0, 3), device=device),
                                             "E": torch.empty(0, device=device)}, 1.0)
+        self._largs.n_seg = 0
+        self._largs.r_mid = self._largs.v_hat = self._largs.L = self._largs.E = None
         self._rec = None
         self.lib = _lib.load()
         self.record_events = False
