@@ -56,10 +56,12 @@ def parse():
     p.add_argument("--no-fom", action="store_true", help="skip the learned-importance figure of merit")
     p.add_argument("--fom-batches", type=int, default=256, help="learned-VR batches (analog: three times as many)")
     p.add_argument("--fom-histories", type=int, default=262144, help="histories per FOM batch")
-    p.add_argument("--workload", choices=("hopg", "si-stream", "pixels", "coherent"), default="hopg",
+    p.add_argument("--workload", choices=("hopg", "si-stream", "pixels", "coherent", "queue"), default="hopg",
                    help="hopg: the bench line (default); si-stream: the 1 mm Si 10 MeV case streamed through "
                         "continuation rounds; pixels: windowed scoring of synthetic segments over 32x32 pixels; "
-                        "coherent: the HOPG case on the coherent line route")
+                        "coherent: the HOPG case on the coherent line route; queue: 256 uneven configurations "
+                        "in one graph-replayed work-queue kernel")
+    p.add_argument("--queue-scale", type=float, default=1e-3, help="queue: history-count scale of the campaign")
     p.add_argument("--histories", type=int, default=1 << 21, help="si-stream: histories")
     p.add_argument("--lanes", type=int, default=1 << 19, help="si-stream: lanes in flight")
     p.add_argument("--rows-per-round", type=int, default=32, help="si-stream: rows per launch")
@@ -258,6 +260,57 @@ def coherent_case(args):
     return out
 
 
+def queue_campaign(args):
+    """256 configurations of uneven size (histories log-uniform 1e4-1e7 times the
+    scale, thickness 10 um-2 mm, 1-30 MeV, graphite/Si/Ge/W, tilt 0-3 deg) as one
+    graph-replayed work-queue launch, against one launch per configuration."""
+    import ctypes as C
+
+    import torch
+
+    from pyrite_amd import _lib
+    from pyrite_amd.backend import require_device
+    from pyrite_amd.queue import TransportQueue, campaign
+
+    dev = require_device().torch_device
+    cases = campaign(256, seed=2024, scale=args.queue_scale)
+    tq = TransportQueue(cases, dev)
+    tuned = tq.autotune(fraction=0.02)
+    g = tq.graph()
+    g.replay()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.replay()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    t_q = min(ts)
+    tally = tq.tally.cpu().numpy()
+    rows = float(tally[:, 7].sum())
+    offs = tq.offsets_host
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(len(cases)):  # one launch per configuration, same kernel
+        sub = torch.as_tensor(np.array([0, offs[i + 1] - offs[i]], dtype=np.int64), device=dev)
+        L = tq.launch_args(offsets=sub, total=int(offs[i + 1] - offs[i]), tally=tq.tally[i:i + 1])
+        L.configs = tq.configs.data_ptr() + i * C.sizeof(_lib.QueueConfig)
+        L.n_cfg = 1
+        _lib.check(tq.lib.hr_queue_run(C.byref(L), C.c_void_p(torch.cuda.current_stream().cuda_stream)),
+                   "hr_queue_run")
+    torch.cuda.synchronize()
+    t_s = time.perf_counter() - t0
+    same = bool(np.array_equal(tq.tally.cpu().numpy()[:, :9], tally[:, :9]))
+    return {"metric": "histories/s, 256-configuration campaign in one work-queue graph", "value": tq.total / t_q,
+            "unit": "histories/s", "n_gpus": 1, "seconds": t_q, "histories": int(tq.total), "rows": rows,
+            "rows_per_second": rows / t_q, "serial_seconds": t_s, "speedup_vs_serial_launches": t_s / t_q,
+            "serial_counts_equal": same, "autotune": tuned, "dtype": "f64 transport (analog, no rows)",
+            "data": "synthetic campaign (seed 2024)",
+            "config": {"workload": "256 configurations, histories log-uniform 1e4-1e7 x %g, thickness 10 um-2 mm, "
+                                   "1-30 MeV, graphite/Si/Ge/W, tilt 0-3 deg" % args.queue_scale}}
+
+
 def cpu_baseline(case_tables, n_hist):
     from oracle import case as ocase
 
@@ -320,7 +373,8 @@ def main():
               f"GPU and the multi-GPU design is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
     if args.workload != "hopg":
-        run = {"si-stream": si_stream, "pixels": pixel_scoring, "coherent": coherent_case}[args.workload]
+        run = {"si-stream": si_stream, "pixels": pixel_scoring, "coherent": coherent_case,
+               "queue": queue_campaign}[args.workload]
         print(json.dumps(run(args)))
         return
     import torch
