@@ -369,8 +369,9 @@ SINC_TRAFFIC_BYTES = 384.6e6
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every box here has one "
-              f"GPU and the multi-GPU design is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): several GPUs would split each "
+              f"batch by global electron id and add their FP64 tallies once, which is not launched here",
+              file=sys.stderr)
         sys.exit(2)
     if args.workload != "hopg":
         run = {"si-stream": si_stream, "pixels": pixel_scoring, "coherent": coherent_case,
