@@ -83,7 +83,7 @@ def _probe_hip() -> HipBackend:
 def select_backend(requested: str | None = None) -> HipBackend:
     """Resolve ``requested`` (default ``$PYRITE_MC_BACKEND`` or ``"auto"``).
 
-    ``"hip"`` (and ``"rocm"``, which names the same device on this build)
+    ``"hip"`` (and ``"rocm"``, which names the same device under ROCm PyTorch)
     must resolve to the device or raise; ``"auto"`` resolves to the device
     when present and otherwise raises too, since this package has no host
     path of its own.

@@ -8,7 +8,7 @@ here so the builder and any reviewer compute the same fraction:
   ``TRANSCENDENTAL_FP32 = 20`` FP32 operations (``TRANSCENDENTAL_FP64 = 40``);
 * an FMA counts as 2 operations.
 
-Per-unit counts (estimates of the reference algorithm, not of this build's
+Per-unit counts (estimates of the reference algorithm, not of these kernels'
 instruction stream):
 
 * sinc^2 cell, one (line, bin): phase ``a (E - E_r)`` (2), reduction to
