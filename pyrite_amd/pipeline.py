@@ -362,7 +362,7 @@ class CasePipeline(CaseTables):
         main.wait_stream(sb)
 
     def stream(self, first_id: int, n_histories: int, n_total: int | None = None, lanes: int = 1 << 20,
-               rows_per_round: int = 32, timings: dict | None = None):
+               rows_per_round: int = 32, timings: dict | None = None, sort_lanes: bool = False):
         """Streamed transport -> scoring over electrons ``[first_id, first_id + n_histories)``.
 
         Continuation rounds (``hr_transport_continue``) keep ``lanes`` histories
@@ -386,10 +386,10 @@ class CasePipeline(CaseTables):
         n_total = int(n_histories if n_total is None else n_total)
         lanes = max(1, min(int(lanes), int(n_histories)))
         drv = getattr(self, "_stream_drv", None)
-        if drv is None or drv.lanes != lanes or drv.R != int(rows_per_round):
+        if drv is None or drv.lanes != lanes or drv.R != int(rows_per_round) or drv.sort_lanes != bool(sort_lanes):
             self._stream_drv = None
             drv = ContinuationDriver(self.tables, self.control, lanes, int(rows_per_round), self.dev,
-                                     cfg.E_cut_keV)
+                                     cfg.E_cut_keV, sort_lanes=sort_lanes)
             self._stream_drv = drv
             m = lanes * int(rows_per_round)
             self._srec = {"r_mid": torch.empty((m, 3), dtype=torch.float32, device=self.dev),

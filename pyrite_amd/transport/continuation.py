@@ -83,7 +83,8 @@ class ContinuationDriver:
     """
 
     def __init__(self, tables: DeviceTables | None, control: TransportControl, lanes: int, rows_per_round: int,
-                 device, E_cut_keV: float, launch=None, tail_fraction: float = 0.25, max_tail_rows: int = 4096):
+                 device, E_cut_keV: float, launch=None, tail_fraction: float = 0.25, max_tail_rows: int = 4096,
+                 sort_lanes: bool = False):
         import torch
 
         if lanes < 1 or rows_per_round < 1:
@@ -93,6 +94,9 @@ class ContinuationDriver:
         self.dev = device
         self.tail_fraction = float(tail_fraction)
         self.max_tail_rows = max(int(max_tail_rows), self.R)
+        # order the active lanes by energy before every launch, so a wavefront
+        # holds histories of similar step cost (empty lanes last)
+        self.sort_lanes = bool(sort_lanes)
         n = self.lanes
         f64 = dict(dtype=torch.float64, device=device)
         i64 = dict(dtype=torch.int64, device=device)
@@ -174,6 +178,21 @@ class ContinuationDriver:
             self._staging = st
         return st
 
+    _LANE_STATE = ("key", "pos", "dirs", "E", "clock", "E_cut", "alive", "stragg", "history", "tau", "draws",
+                   "nseg", "flight", "substep", "steps")
+
+    def _sort(self, n_active):
+        """Permute the first ``n_active`` lanes into descending energy, empty lanes last."""
+        import torch
+
+        E = self.E[:n_active].clone()
+        E.masked_fill_(self.history[:n_active] < 0, -1.0)
+        order = torch.argsort(E, descending=True, stable=True)
+        for name in self._LANE_STATE:
+            t = getattr(self, name)
+            v = t.view(torch.int64) if t.dtype == torch.uint64 else t
+            v[:n_active] = v[:n_active].index_select(0, order)
+
     def _pack(self, n_active):
         """Move the live lanes among the first ``n_active`` to the front; returns their count."""
         import torch
@@ -182,8 +201,7 @@ class ContinuationDriver:
         n_live = int(live.numel())
         if n_live == n_active:
             return n_live
-        for name in ("key", "pos", "dirs", "E", "clock", "E_cut", "alive", "stragg", "history", "tau", "draws",
-                     "nseg", "flight", "substep", "steps"):
+        for name in self._LANE_STATE:
             t = getattr(self, name)
             v = t.view(torch.int64) if t.dtype == torch.uint64 else t
             v[:n_live] = v.index_select(0, live)
@@ -211,6 +229,8 @@ class ContinuationDriver:
                 if n_live <= self.tail_fraction * n_active:
                     n_active = self._pack(n_active)
                     R = max(self.R, min(self.max_tail_rows, (self.lanes * self.R) // n_active))
+            if self.sort_lanes:
+                self._sort(n_active)
             self._launch(n_active, R)
             cnt = self.seg_count[:n_active]
             rnd = ContinuationRound(rounds, n_active, R, self.rows, cnt, self.history[:n_active],

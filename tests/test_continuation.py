@@ -53,12 +53,13 @@ class StandIn:
             d.exit_code[i] = 6 if left > R else _exit_for(h, T)
 
 
-@pytest.mark.parametrize("lanes,R", [(16, 5), (7, 64), (64, 1)])
-def test_rows_land_once_at_electron_major_positions(lanes, R):
+@pytest.mark.parametrize("lanes,R,sort_lanes", [(16, 5, False), (7, 64, False), (64, 1, False), (16, 5, True),
+                                                (64, 1, True)])
+def test_rows_land_once_at_electron_major_positions(lanes, R, sort_lanes):
     first, n = 1000, 300
     T = _lengths(first, n)
     offsets = torch.as_tensor(np.concatenate([[0], np.cumsum(T)[:-1]]), dtype=torch.int64)
-    drv = ContinuationDriver(None, TransportControl(), lanes, R, "cpu", 5.0)
+    drv = ContinuationDriver(None, TransportControl(), lanes, R, "cpu", 5.0, sort_lanes=sort_lanes)
     stand = StandIn(drv, first, T)
     drv._launch = stand
     seen = np.zeros(T.sum(), dtype=np.int64)

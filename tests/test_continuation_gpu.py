@@ -18,7 +18,7 @@ from tests.transport_cases import FRAMED, HOPG_5MEV, MULTILAYER, SI_30KEV_STRAGG
 pytestmark = pytest.mark.gpu
 
 
-def _continued(case, device, lanes, R, precision="fp64", max_steps=400_000):
+def _continued(case, device, lanes, R, precision="fp64", max_steps=400_000, sort_lanes=False):
     import torch
 
     from pyrite_amd import _lib
@@ -32,7 +32,7 @@ def _continued(case, device, lanes, R, precision="fp64", max_steps=400_000):
                            straggle_on=case.straggle, finite_footprint=case.finite is not None,
                            width_ang=case.finite[0] if case.finite else 0.0,
                            height_ang=case.finite[1] if case.finite else 0.0, precision=precision)
-    drv = ContinuationDriver(tabs, ctl, lanes, R, device, float(E_cut[0]))
+    drv = ContinuationDriver(tabs, ctl, lanes, R, device, float(E_cut[0]), sort_lanes=sort_lanes)
     pos_d, dirs_d, E_d = (torch.as_tensor(x, device=device) for x in (pos, dirs, E))
 
     def fill(start, n, p, d, e, clock, keys):
@@ -61,15 +61,16 @@ def _continued(case, device, lanes, R, precision="fp64", max_steps=400_000):
     return res, rounds
 
 
-@pytest.mark.parametrize("case,lanes,R,precision,max_steps", [
-    (SI_30KEV_STRAGGLE, 64, 3, "fp64", 400_000),
-    (HOPG_5MEV, 50, 7, "fp64", 400_000),
-    (HOPG_5MEV, 96, 5, "mixed", 400_000),
-    (MULTILAYER, 40, 2, "fp64", 400_000),
-    (FRAMED, 128, 16, "fp64", 400_000),
-    (HOPG_5MEV, 37, 4, "fp64", 30),  # step limit reached inside a round
-], ids=["si30s", "hopg5", "hopg5-mixed", "stack", "prism", "hopg5-step-limit"])
-def test_continuation_is_bitwise_the_uninterrupted_run(device, case, lanes, R, precision, max_steps):
+@pytest.mark.parametrize("case,lanes,R,precision,max_steps,sort_lanes", [
+    (SI_30KEV_STRAGGLE, 64, 3, "fp64", 400_000, False),
+    (HOPG_5MEV, 50, 7, "fp64", 400_000, False),
+    (HOPG_5MEV, 96, 5, "mixed", 400_000, False),
+    (MULTILAYER, 40, 2, "fp64", 400_000, False),
+    (FRAMED, 128, 16, "fp64", 400_000, False),
+    (HOPG_5MEV, 37, 4, "fp64", 30, False),  # step limit reached inside a round
+    (SI_30KEV_STRAGGLE, 64, 3, "mixed", 400_000, True),  # lanes re-ordered by energy every launch
+], ids=["si30s", "hopg5", "hopg5-mixed", "stack", "prism", "hopg5-step-limit", "si30s-mixed-sorted"])
+def test_continuation_is_bitwise_the_uninterrupted_run(device, case, lanes, R, precision, max_steps, sort_lanes):
     import torch
 
     from pyrite_amd.transport.continuation import ContinuationRound, canonical_rows
@@ -84,7 +85,7 @@ def test_continuation_is_bitwise_the_uninterrupted_run(device, case, lanes, R, p
     n_rows = int(ref_cnt.sum())
     offsets = torch.as_tensor(np.concatenate([[0], np.cumsum(ref_cnt)[:-1]]), device=device)
 
-    res, rounds = _continued(case, device, lanes, R, precision, max_steps)
+    res, rounds = _continued(case, device, lanes, R, precision, max_steps, sort_lanes)
     assert res.n_segments == n_rows
     assert res.rounds == len(rounds) and len(rounds) > 2
     seen = np.zeros(n_rows, dtype=np.int64)
