@@ -55,7 +55,14 @@ class VRConfig:
     rows_far: int = 8  # rows between decisions far below it (w I/I_ref < far_ratio)
     far_ratio: float = 1.0 / 32.0
     horizon_rows: int = 2
-    k_max: float = 32.0  # smallest target weight is 1/k_max
+    # "second": the label is the square root of the rows' second moment over
+    # the bin offset (narrow lines that hit a bin centre deposit their peak
+    # weight, so a row's variance grows as w^2/(a dE) while its mean grows as
+    # w/(a dE); splitting in proportion to the root of the second moment per
+    # unit cost minimises variance x cost).  "mean": their expected
+    # peak-window content.
+    label: str = "second"
+    k_max: float = 16.0  # smallest target weight is 1/k_max
     # weight-window band around the target weight.  Rouletting copies back up
     # to a larger weight multiplies the rare grazing / detector-bound rows they
     # may produce later (the heavy tail of the peak tally), so by default the
@@ -75,6 +82,15 @@ class VRConfig:
     batch: int = 4096
     lr: float = 2.0e-3
     seed: int = 7
+
+
+def line_second_moment(w, a_dE):
+    """Offset-averaged second moment of the content a line of peak ``w`` and
+    width ``a_dE = a dE`` (in bins) deposits on a uniform grid: the larger of
+    the narrow-line limit ``w^2 (2 pi/3)/(a dE)`` and the wide-line limit
+    ``(w pi/(a dE))^2``.  Works on numpy arrays and torch tensors."""
+    return w * w * np.maximum((np.pi / a_dE) ** 2, (2.0 * np.pi / 3.0) / a_dE) if isinstance(a_dE, np.ndarray) \
+        else w * w * ((np.pi / a_dE) ** 2).maximum((2.0 * np.pi / 3.0) / a_dE)
 
 
 class ParticleBank:
@@ -163,6 +179,35 @@ class LearnedImportance:
             self._mass(E_r, a, w, n, mass[j0:], window=window)
         return mass
 
+    def row_second_moments(self, seg, Ne):
+        """Second moment over the bin offset of every row's peak-window line
+        content (device FP64 ``[n_rows]``, cross terms between lines left out).
+
+        A line of peak ``w`` and width parameter ``a`` on bins of width ``dE``
+        deposits ``X = w sum_k sinc^2(a (E_k - E_r))``.  With ``E_r`` uniform
+        over a bin, ``E[X] = w pi/(a dE)`` and, for a line narrower than a bin,
+        ``E[X^2] = w^2 (2 pi/3)/(a dE)`` (the integral of sinc^4 is 2 pi/3);
+        a line wider than a few bins deposits its mean, ``E[X^2] = E[X]^2``.
+        The larger of the two is taken."""
+        import torch
+
+        lines = self.pipe.lines
+        n_total = int(seg["L"].shape[0])
+        n_dir, n_g = lines.n_dir, len(lines.tables.wm)
+        lo, hi = self.window
+        out = torch.empty(n_total, dtype=torch.float64, device=self.pipe.dev)
+        step = max(1, LINE_BLOCK // (n_dir * n_g))
+        for j0 in range(0, n_total, step):
+            n = min(step, n_total - j0)
+            E_r, a, w = lines.lines(seg, 1.0 / float(Ne), j0, n)
+            E_r = E_r[: n_dir * n * n_g].view(n_dir, n, n_g)
+            a = a[: n_dir * n * n_g].view(n_dir, n, n_g)
+            w = w[: n_dir * n * n_g].view(n_dir, n, n_g).double()
+            live = (w != 0) & (a > 0) & (E_r >= lo) & (E_r < hi)
+            m2 = line_second_moment(w, torch.where(live, torch.clamp(a * float(self.dE), min=1.0e-6), torch.ones_like(a)))
+            out[j0:j0 + n] = torch.where(live, m2, torch.zeros_like(m2)).sum(dim=(0, 2))
+        return out
+
     def _mass(self, E_r, a, w, n, out, window=True):
         lines = self.pipe.lines
         lo, hi = self.window if window else (-1.0e300, 1.0e300)
@@ -206,7 +251,7 @@ class LearnedImportance:
             "E_keV": rows["seg_E"], "E_repr_keV": 0.5 * (rows["seg_E"] + rows["seg_E_end"]),
         }, dev)
         n = int(seg["L"].shape[0])
-        mass = self.row_masses(seg, Ne)
+        mass = self.row_masses(seg, Ne) if self.vcfg.label == "mean" else self.row_second_moments(seg, Ne)
         self._last_full_mass = self.row_masses(seg, Ne, window=False) * float(Ne)
         x = torch.empty((n, 6), dtype=torch.float32, device=dev)
         self._features(n, rows["seg_mid"], rows["seg_dir"], rows["seg_E"], rows["seg_len"], x)
@@ -220,6 +265,9 @@ class LearnedImportance:
         csum = torch.zeros(n + 1, dtype=torch.float64, device=dev)
         csum[1:] = torch.cumsum(mass, 0)
         y = csum[hi] - csum[j]
+        if self.vcfg.label != "mean":
+            # the difference of two running sums can fall a rounding below zero
+            y = torch.sqrt(torch.clamp(y, min=0.0))
         return x, y
 
     def training_set(self, n_batches, first_batch=10_000):
