@@ -48,7 +48,10 @@ def parse():
     p.add_argument("--line-route", choices=("dense", "window"), default="dense",
                    help="dense: every (line, bin) pair, the reference's device default for per-row lines; "
                         "window: the support-windowed route with x_c = 100 (the reference's NumPy default)")
-    p.add_argument("--cpu-sample", type=int, default=8000, help="histories in the CPU baseline sample")
+    p.add_argument("--cpu-sample", type=int, default=8000,
+                   help="histories per timed run of the all-core CPU baseline (median of 3)")
+    p.add_argument("--cpu-sample-one", type=int, default=500,
+                   help="histories per timed run of the single-thread CPU baseline (median of 3)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--breakdown", action="store_true", help="print per-phase times to stderr")
     p.add_argument("--no-overlap", action="store_true", help="run transport and scoring of successive batches "
@@ -215,8 +218,8 @@ def coherent_case(args):
     # adds no work inside it
     counts = []
     for s in range(args.steps):
-        for _e0, m, cap, rows, seg_count in pipe.transport(s, compact=False).slots:
-            sc.add_rows(rows, seg_count, m, cap, live_counts=counts)
+        pipe.transport(s, compact=False, consume=lambda e0, m, cap, rows, cnt: (
+            sc.add_rows(rows, cnt, m, cap, live_counts=counts), pipe._scored()))
     assert len(counts) == len(ms), (len(counts), len(ms))
     cells = [float(c) * cfg.n_bins for c in counts]
     avg_ms = sum(ms) / len(ms)
@@ -232,7 +235,7 @@ def coherent_case(args):
                    "histories_per_step": cfg.Ne, "segments_per_history": pipe.segments / max(pipe.histories, 1)},
         "roofline": {
             "kernel": "hr_coherent_field: coherent_field_kernel + field_partials_kernel",
-            "bound": "mfma", "pipe": "vector ALU, FP32 peak (FP64 phase reduction counted as FP32 work)",
+            "bound": "valu", "pipe": "vector ALU, FP32 peak (FP64 phase reduction counted as FP32 work)",
             "achieved": achieved, "peak": roofline.PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / roofline.PEAK_FP32_TFLOPS, "traffic": None,
             "ops_per_cell": roofline.COHERENT_CELL_OPS, "live_cells_per_launch": sum(cells) / len(cells),
@@ -311,23 +314,58 @@ def queue_campaign(args):
                                    "1-30 MeV, graphite/Si/Ge/W, tilt 0-3 deg" % args.queue_scale}}
 
 
-def cpu_baseline(case_tables, n_hist):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(case_tables, n_all, n_one, repeats=3):
+    """The FP64 C oracle (the CPU restatement of the reference path) on the
+    host cores: ``repeats`` timed runs after one warm-up, median reported, both
+    on every core the process may use (OpenMP over histories and bins) and on
+    one thread, each on a bounded sample of the bench workload's histories."""
     from oracle import case as ocase
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    ocase.run_case(case_tables, 10_000_000, 64, threads=threads)  # warm-up (page-in, thread pool)
-    tim = {}
-    t0 = time.perf_counter()
-    ocase.run_case(case_tables, 0, n_hist, threads=threads, timings=tim)
-    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
+
+    def timed(n_hist, nthr, first):
+        ocase.run_case(case_tables, 10_000_000, 32, threads=nthr)  # warm-up (page-in, thread pool)
+        runs, tims = [], []
+        for r in range(repeats):
+            tim = {}
+            t0 = time.perf_counter()
+            ocase.run_case(case_tables, first + r * n_hist, n_hist, threads=nthr, timings=tim)
+            runs.append(time.perf_counter() - t0)
+            tims.append(tim)
+        k = int(np.argsort(runs)[len(runs) // 2])
+        return n_hist / runs[k], runs, tims[k]
+
+    v_all, runs_all, tim = timed(n_all, threads, 0)
+    v_one, runs_one, tim1 = timed(n_one, 1, 50_000)
     return {
-        "value": n_hist / dt,
+        "value": v_all,
         "unit": "histories/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{n_hist} histories of the same case (transport + lines over {len(case_tables.n_hats)} "
-                  f"rays + brem, 1024 bins), FP64 C oracle, {threads} OpenMP threads, {dt:.2f} s; "
-                  f"transport {tim['transport']:.2f} s, lines {tim['lines']:.2f} s, brem {tim['brem']:.2f} s",
+        "sample": f"median of {repeats} warm runs of {n_all} histories each (disjoint ids) of the same case "
+                  f"(transport + lines over {len(case_tables.n_hats)} rays + brem, {case_tables.E_grid.size} bins), "
+                  f"FP64 C oracle, {threads} OpenMP threads: {', '.join(f'{t:.2f}' for t in runs_all)} s; median "
+                  f"run: transport {tim['transport']:.2f} s, lines {tim['lines']:.2f} s, brem {tim['brem']:.2f} s",
+        "single_thread": {"value": v_one, "unit": "histories/s", "cores": 1,
+                          "sample": f"median of {repeats} warm runs of {n_one} histories: "
+                                    f"{', '.join(f'{t:.2f}' for t in runs_one)} s; median run: transport "
+                                    f"{tim1['transport']:.2f} s, lines {tim1['lines']:.2f} s, brem "
+                                    f"{tim1['brem']:.2f} s",
+                          "us_per_history": 1e6 / v_one},
+        "cpu_model": _cpu_model(),
+        "nproc": os.cpu_count(),
+        "affinity_cpus": len(os.sched_getaffinity(0)),
     }
 
 
@@ -349,13 +387,18 @@ def live_cells(pipe, steps, launches):
     step = max(1, LINE_BLOCK // per_seg)
     total = 0.0
     n = 0
-    for s in range(steps):
-        seg = pipe.records(pipe.transport(s, compact=False))
+    def consume(e0, m, cap, rows, seg_count):
+        nonlocal total, n
+        seg = pipe.slot_records([(e0, m, cap, rows, seg_count)], int(seg_count.clamp(max=cap).sum()))
         n_seg = int(seg["L"].shape[0])
         for j0 in range(0, n_seg, step):
             _E_r, _a, w = sc.lines(seg, 1.0, j0, min(step, n_seg - j0))
             total += float((w != 0).sum().item()) * sc.E_grid.size
             n += 1
+        pipe._scored()
+
+    for s in range(steps):
+        pipe.transport(s, compact=False, consume=consume)
         torch.cuda.synchronize()
     assert n == launches, (n, launches)
     return total
@@ -449,7 +492,7 @@ def main():
         },
         "roofline": {
             "kernel": "hr_sincsq_reduce: sincsq_uniform_kernel + sum_partials_kernel (dense sinc^2 lines)",
-            "bound": "mfma",
+            "bound": "valu",
             "pipe": "vector ALU, FP32 peak (the listed fp32 dense figure equals the VALU FP32 peak)",
             "achieved": achieved,
             "peak": roofline.PEAK_FP32_TFLOPS,
@@ -472,11 +515,20 @@ def main():
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
+    # the bench pipeline's arenas go before the figure-of-merit leg builds its own
+    from pyrite_amd.pipeline import CaseTables
+
+    host_tables = CaseTables(cfg)
+    pipe.release()
+    del pipe
+    torch.cuda.empty_cache()
     if not args.no_fom:
         out["fom"] = learned_fom(cfg, args.fom_batches, args.fom_histories)
+        torch.cuda.empty_cache()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(pipe, args.cpu_sample)
+        out["cpu_baseline"] = cpu_baseline(host_tables, args.cpu_sample, args.cpu_sample_one)
         out["speedup_vs_cpu_baseline"] = value / out["cpu_baseline"]["value"]
+        out["speedup_vs_cpu_single_thread"] = value / out["cpu_baseline"]["single_thread"]["value"]
     print(json.dumps(out))
 
 

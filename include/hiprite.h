@@ -35,6 +35,18 @@ const char *hr_version(void);
 /* Number of visible HIP devices (0 when none); never launches a kernel. */
 int32_t hr_device_count(void);
 
+/* Caller-owned device scratch of one call (partial sums, counters).  The
+ * library allocates nothing on its own: each entry point that needs scratch
+ * takes one of these, 256-byte aligned and at least as large as its
+ * *_workspace_bytes query says for the same arguments on the current device
+ * (the answer depends on the device's compute-unit count).  A workspace is
+ * written by the launches of the call it is passed to, so calls that may run
+ * at the same time (different streams) need different workspaces. */
+typedef struct hr_workspace {
+    void *ptr;    /* device memory */
+    size_t bytes; /* its size */
+} hr_workspace;
+
 /* ------------------------------------------------------------------------
  * Counter-addressed SplitMix64 stream keys.
  * Replaces montecarlo/transport/kinematics.py:226 stream_keys(seed, Ne, start=)
@@ -64,9 +76,9 @@ hr_status hr_stream_keys(uint64_t seed, int64_t start, int64_t n, uint64_t *keys
  *  - draws are counter-addressed by (key[e], counter), so results do not
  *    depend on launch geometry or batch split.
  * All pointers are device pointers.  Rows are structure-of-arrays.  Lanes claim
- * electrons from a library-held per-device counter: two hr_transport_run calls
- * on one device must not overlap (one transport stream beside scoring streams
- * is fine).
+ * electrons from the caller's claim counter `work` (8 bytes of device memory,
+ * reset by the call itself with a kernel): calls that may run at the same time
+ * need different counters.  The call allocates nothing.
  * ---------------------------------------------------------------------- */
 typedef struct hr_transport_args {
     /* run */
@@ -119,6 +131,9 @@ typedef struct hr_transport_args {
      * sqrt(1 - cos^2) and azimuth sin/cos, FP64 state, draws and straggling */
     int32_t precision;
     const float *lut_total_rate_f, *lut_dEds_f, *lut_inv_beta_f, *lut_cdf_f, *lut_alpha_f;
+    /* claim counter of hr_transport_run (8 bytes, caller-owned; unused by the
+     * continuation and round entry points, which run one lane per thread) */
+    unsigned long long *work;
 } hr_transport_args;
 
 hr_status hr_transport_run(const hr_transport_args *args, void *stream);
@@ -205,10 +220,9 @@ hr_status hr_cxr_lines(const hr_line_args *args, void *stream);
  * x = a_j (E_k - E_r,j), x = 0 -> 1.
  * Replaces montecarlo/spectrum/cxr_jit_kernel.py:561 run_reduction_kernel
  * (and the CPU twin cxr/_cpu_reduce.py:26 _sincsq_reduce).  FP64 phase,
- * FP32 profile, FP64 accumulation; no atomics (fixed-order partial sums).
+ * FP32 profile, FP64 accumulation; no atomics (fixed-order partial sums in
+ * the caller's workspace, hr_sincsq_workspace_bytes).
  * `uniform` selects the grid E_k = E0 + k*dE (E_grid may then be NULL).
- * Uses a library-held per-device workspace: concurrent calls on different
- * streams of one device must not overlap.
  * ---------------------------------------------------------------------- */
 typedef struct hr_sincsq_args {
     int64_t n_lines;
@@ -220,9 +234,12 @@ typedef struct hr_sincsq_args {
     int32_t uniform;
     double E0, dE;
     double *out; /* [n_E] FP64, accumulated */
+    hr_workspace ws;
 } hr_sincsq_args;
 
 hr_status hr_sincsq_reduce(const hr_sincsq_args *args, void *stream);
+/* Workspace bytes hr_sincsq_reduce needs for these sizes on the current device. */
+hr_status hr_sincsq_workspace_bytes(const hr_sincsq_args *args, size_t *bytes);
 
 /* ------------------------------------------------------------------------
  * Bethe-Heitler + Elwert bremsstrahlung track-length reduction:
@@ -230,7 +247,8 @@ hr_status hr_sincsq_reduce(const hr_sincsq_args *args, void *stream);
  * Replaces montecarlo/spectrum/brem_jit_kernel.py:779 run_brem_reduction_kernel
  * (cell :76-153; incident prefactor brem.py:396-423); the caller divides by
  * 4 pi Ne (brem.py:1169).  FP32 cells, FP64 sums, single absorber layer,
- * 1..4 elements.  Same workspace rule as hr_sincsq_reduce.
+ * 1..4 elements.  Fixed-order partial sums in the caller's workspace
+ * (hr_brem_workspace_bytes).
  * ---------------------------------------------------------------------- */
 typedef struct hr_brem_args {
     int64_t n_seg;
@@ -248,9 +266,11 @@ typedef struct hr_brem_args {
     const float *mu;         /* [n_E] 1/A total attenuation */
     const float *row_weight; /* [n_seg] statistical weight of each row, or NULL (= 1) */
     double *out;             /* [n_E] FP64, accumulated */
+    hr_workspace ws;
 } hr_brem_args;
 
 hr_status hr_brem_reduce(const hr_brem_args *args, void *stream);
+hr_status hr_brem_workspace_bytes(const hr_brem_args *args, size_t *bytes);
 
 /* ------------------------------------------------------------------------
  * Importance network (learned variance reduction).
@@ -308,6 +328,7 @@ typedef struct hr_queue_config {
 } hr_queue_config;
 typedef struct hr_queue_launch {
     const hr_queue_config *configs; /* device [n_cfg], n_cfg <= 1024 */
+    unsigned long long *counter;    /* caller-owned claim counter (8 bytes; reset by the launch) */
     int32_t n_cfg;
     const int64_t *offsets;         /* device [n_cfg + 1] */
     int64_t total;                  /* offsets[n_cfg] */
@@ -319,7 +340,9 @@ typedef struct hr_queue_launch {
 } hr_queue_launch;
 hr_status hr_queue_run(const hr_queue_launch *launch, void *stream);
 /* The same launch captured once into a hipGraph (reset + queue kernel) and
- * replayed by hr_queue_graph_launch; the graph owns its own claim counter. */
+ * replayed by hr_queue_graph_launch; the graph uses launch->counter, which must
+ * outlive it (replays of one graph, or of graphs sharing a counter, must not
+ * overlap). */
 hr_status hr_queue_graph_create(const hr_queue_launch *launch, void **graph);
 hr_status hr_queue_graph_launch(void *graph, void *stream);
 hr_status hr_queue_graph_destroy(void *graph);
@@ -331,9 +354,12 @@ hr_status hr_queue_graph_destroy(void *graph);
  * bins within |a (E - E_r)| <= x_c of each line (the reference default is
  * x_c = 100, DEFAULT_NODE_SUPPORT_CUTOFF), the tail m2/x^2 + m3/x^3 + m4/x^4
  * elsewhere; broad windows (a quarter of the grid) are exact everywhere.
- * Uniform grids E0 + k dE with n_E <= 4096. */
+ * Uniform grids E0 + k dE with n_E <= 4096.  Partials and node moments live in
+ * the caller's workspace (hr_window_reduce_workspace_bytes). */
 hr_status hr_window_reduce(const double *E_r, const double *a, const float *w, int64_t n_lines,
-                           int64_t n_E, double E0, double dE, double x_c, double *out, void *stream);
+                           int64_t n_E, double E0, double dE, double x_c, double *out, hr_workspace ws,
+                           void *stream);
+hr_status hr_window_reduce_workspace_bytes(int64_t n_lines, int64_t n_E, size_t *bytes);
 
 typedef struct hr_window_args {
     const hr_line_args *lines; /* prologue inputs; its E_r/a/w outputs are not used */
@@ -341,11 +367,13 @@ typedef struct hr_window_args {
     double E0, dE;             /* uniform grid [eV] */
     double x_c;                /* exact support |a (E - E_r)| <= x_c */
     double *out;               /* [n_dir, n_E] FP64 per-direction (pixel) spectra, accumulated */
+    hr_workspace ws;
 } hr_window_args;
 /* Per-direction spectra: the line prologue of hr_cxr_lines fused with the
  * windowed reduction, one spectrum per detector direction (pixel), without
  * materialising the lines (n_seg * n_dir * n_g of them). */
 hr_status hr_window_spectra(const hr_window_args *args, void *stream);
+hr_status hr_window_spectra_workspace_bytes(const hr_window_args *args, size_t *bytes);
 
 /* ---- coherent line route: phased fields, squared once every batch is summed
  * Replaces the coherent stream pipeline: the prologue
@@ -392,14 +420,15 @@ typedef struct hr_coherent_field_args {
     const double *E_grid; /* [n_E] eV */
     const double *dom;    /* [n_E] delta(E) E / hbar c [1/A]: the escape-path phase rate */
     double *field;      /* [n_rows][4][n_E] FP64 (sigma re, sigma im, pi re, pi im), accumulated */
+    hr_workspace ws;
 } hr_coherent_field_args;
 /* field[r][.][k] += sum_j c_j F(v_jk) exp(i phi_jk), v = a (E - E_c) -
  * half_dL dom, phi = phase_slope E - g_phase - L_esc dom, F = exp(-tau_c/2)
  * sinh(w)/w with w = i v - q (cxr/_formation.py:86-115).  FP64 phase and
  * sums, FP32 profile; each field cell has one writer per call (fixed-order
- * partial sums, no atomics).  Uses a library-held per-device workspace:
- * concurrent calls on one device must not overlap. */
+ * partial sums in the caller's workspace, no atomics). */
 hr_status hr_coherent_field(const hr_coherent_field_args *args, void *stream);
+hr_status hr_coherent_field_workspace_bytes(const hr_coherent_field_args *args, size_t *bytes);
 /* out[k] += scale sum_d dir_weight[d] sum_g wm[g] (|F_s|^2 + |F_p|^2) */
 hr_status hr_coherent_finalize(const double *field, int32_t n_dir, int32_t n_g, const double *dir_weight,
                                const double *wm, int64_t n_E, double scale, double *out, void *stream);

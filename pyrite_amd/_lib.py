@@ -37,6 +37,45 @@ class HipriteOOM(HipriteError, MemoryError):
     """Device allocation failed inside libhiprite."""
 
 
+class Workspace(C.Structure):
+    """Mirror of ``hr_workspace`` (caller-owned device scratch of one call)."""
+
+    _fields_ = [("ptr", _dp), ("bytes", C.c_size_t)]
+
+
+class DeviceScratch:
+    """Caller-owned device workspace of one scorer, grown to what a call needs.
+
+    One instance serves the calls of one stream (calls on different streams
+    that may overlap need different instances); growth frees the old buffer
+    through torch's stream-ordered allocator."""
+
+    def __init__(self, device):
+        self.device = device
+        self.buf = None
+
+    def get(self, nbytes: int) -> "Workspace":
+        import torch
+
+        nbytes = int(nbytes)
+        ws = Workspace()
+        if nbytes <= 0:
+            ws.ptr, ws.bytes = None, 0
+            return ws
+        if self.buf is None or self.buf.numel() < nbytes:
+            self.buf = None
+            self.buf = torch.empty(max(nbytes, 1 << 20), dtype=torch.uint8, device=self.device)
+        ws.ptr, ws.bytes = self.buf.data_ptr(), self.buf.numel()
+        return ws
+
+
+def workspace_bytes(name: str, *args) -> int:
+    """Bytes an entry point's ``*_workspace_bytes`` query asks for."""
+    n = C.c_size_t(0)
+    check(getattr(load(), name)(*args, C.byref(n)), name)
+    return int(n.value)
+
+
 class TransportArgs(C.Structure):
     """Mirror of ``hr_transport_args`` (include/hiprite.h)."""
 
@@ -97,6 +136,7 @@ class TransportArgs(C.Structure):
         ("precision", C.c_int32),
         ("lut_total_rate_f", _dp), ("lut_dEds_f", _dp), ("lut_inv_beta_f", _dp), ("lut_cdf_f", _dp),
         ("lut_alpha_f", _dp),
+        ("work", _dp),
     ]
 
 
@@ -134,6 +174,7 @@ class SincsqArgs(C.Structure):
         ("uniform", C.c_int32),
         ("E0", C.c_double), ("dE", C.c_double),
         ("out", _dp),
+        ("ws", Workspace),
     ]
 
 
@@ -150,6 +191,7 @@ class BremArgs(C.Structure):
         ("E_grid_eV", _dp), ("mu", _dp),
         ("row_weight", _dp),
         ("out", _dp),
+        ("ws", Workspace),
     ]
 
 
@@ -171,7 +213,7 @@ class QueueLaunch(C.Structure):
     """Mirror of ``hr_queue_launch``."""
 
     _fields_ = [
-        ("configs", _dp), ("n_cfg", C.c_int32),
+        ("configs", _dp), ("counter", _dp), ("n_cfg", C.c_int32),
         ("offsets", _dp), ("total", C.c_int64),
         ("tally", _dp),
         ("accumulate", C.c_int32), ("block_size", C.c_int32), ("per_claim", C.c_int32),
@@ -188,6 +230,7 @@ class WindowArgs(C.Structure):
         ("E0", C.c_double), ("dE", C.c_double),
         ("x_c", C.c_double),
         ("out", _dp),
+        ("ws", Workspace),
     ]
 
 
@@ -212,6 +255,7 @@ class CoherentFieldArgs(C.Structure):
     _fields_ = [
         ("rec", _dp), ("ld", C.c_int64), ("n_lines", C.c_int64), ("n_rows", C.c_int32),
         ("n_E", C.c_int64), ("E_grid", _dp), ("dom", _dp), ("field", _dp),
+        ("ws", Workspace),
     ]
 
 
@@ -319,7 +363,9 @@ SIGNATURES = {
     "hr_transport_continue": (C.c_int32, [C.POINTER(TransportArgs), C.POINTER(Continuation), _dp]),
     "hr_cxr_lines": (C.c_int32, [C.POINTER(LineArgs), _dp]),
     "hr_sincsq_reduce": (C.c_int32, [C.POINTER(SincsqArgs), _dp]),
+    "hr_sincsq_workspace_bytes": (C.c_int32, [C.POINTER(SincsqArgs), C.POINTER(C.c_size_t)]),
     "hr_brem_reduce": (C.c_int32, [C.POINTER(BremArgs), _dp]),
+    "hr_brem_workspace_bytes": (C.c_int32, [C.POINTER(BremArgs), C.POINTER(C.c_size_t)]),
     "hr_mlp_param_count": (C.c_int32, []),
     "hr_mlp_workspace_bytes": (C.c_size_t, [C.c_int64]),
     "hr_mlp_shadow": (C.c_int32, [_dp, _dp, _dp, _dp]),
@@ -332,10 +378,13 @@ SIGNATURES = {
     "hr_queue_graph_launch": (C.c_int32, [_dp, _dp]),
     "hr_queue_graph_destroy": (C.c_int32, [_dp]),
     "hr_window_reduce": (C.c_int32, [_dp, _dp, _dp, C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double,
-                                     _dp, _dp]),
+                                     _dp, Workspace, _dp]),
+    "hr_window_reduce_workspace_bytes": (C.c_int32, [C.c_int64, C.c_int64, C.POINTER(C.c_size_t)]),
     "hr_window_spectra": (C.c_int32, [C.POINTER(WindowArgs), _dp]),
+    "hr_window_spectra_workspace_bytes": (C.c_int32, [C.POINTER(WindowArgs), C.POINTER(C.c_size_t)]),
     "hr_coherent_lines": (C.c_int32, [C.POINTER(CoherentArgs), _dp]),
     "hr_coherent_field": (C.c_int32, [C.POINTER(CoherentFieldArgs), _dp]),
+    "hr_coherent_field_workspace_bytes": (C.c_int32, [C.POINTER(CoherentFieldArgs), C.POINTER(C.c_size_t)]),
     "hr_coherent_finalize": (C.c_int32, [_dp, C.c_int32, C.c_int32, _dp, _dp, C.c_int64, C.c_double, _dp, _dp]),
     "hr_vr_round": (C.c_int32, [C.POINTER(TransportArgs), C.c_int32, _dp, _dp]),
     "hr_vr_features": (C.c_int32, [C.c_int64, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp, _dp]),

@@ -47,17 +47,28 @@ class HipBackend:
 
     def is_oom_error(self, error: BaseException) -> bool:
         """OOM from torch's allocator or from libhiprite (``_backend.py:179-205``)."""
-        import torch
-
-        if isinstance(error, (_lib.HipriteOOM, torch.cuda.OutOfMemoryError, MemoryError)):
-            return True
-        msg = str(error).lower()
-        return "out of memory" in msg or "hiperroroutofmemory" in msg.replace("_", "")
+        return is_oom_error(error)
 
     def synchronize(self) -> None:
         import torch
 
         torch.cuda.synchronize(self.torch_device)
+
+
+def is_oom_error(error: BaseException) -> bool:
+    """Whether ``error`` is a device out-of-memory failure (``_backend.py:179-205``):
+    torch's allocator error, libhiprite's ``HR_ERR_OOM``, or a message naming it."""
+    if isinstance(error, (_lib.HipriteOOM, MemoryError)):
+        return True
+    try:
+        import torch
+
+        if isinstance(error, torch.cuda.OutOfMemoryError):
+            return True
+    except ImportError:  # pragma: no cover
+        pass
+    msg = str(error).lower()
+    return "out of memory" in msg or "hiperroroutofmemory" in msg.replace("_", "")
 
 
 def _probe_hip() -> HipBackend:

@@ -309,37 +309,34 @@ __global__ void __launch_bounds__(256) coherent_finalize_kernel(const double *__
     out[k] += scale * total;
 }
 
-struct FieldWorkspace {
-    void *ptr = nullptr;
-    size_t bytes = 0;
-    int cu = 0;
-};
-FieldWorkspace g_fws[64];
-
-hr_status field_workspace(size_t need, void *&ptr, int &cu) {
-    int dev = 0;
-    HR_CHECK_HIP(hipGetDevice(&dev));
-    HR_REQUIRE(dev >= 0 && dev < 64, "device index %d out of range", dev);
-    FieldWorkspace &w = g_fws[dev];
-    if (w.cu == 0) {
-        hipDeviceProp_t prop;
-        HR_CHECK_HIP(hipGetDeviceProperties(&prop, dev));
-        w.cu = prop.multiProcessorCount;
-    }
-    if (need > 0 && w.bytes < need) {
-        if (w.ptr) HR_CHECK_HIP(hipFree(w.ptr));
-        w.ptr = nullptr;
-        w.bytes = 0;
-        HR_CHECK_HIP(hipMalloc(&w.ptr, need));
-        w.bytes = need;
-    }
-    ptr = w.ptr;
-    cu = w.cu;
-    return HR_OK;
-}
-
 constexpr int kFieldBins = 4;
 constexpr size_t kMaxPartialBytes = size_t(1) << 30;
+
+// Launch shape of hr_coherent_field: S splits along the line axis (fill the
+// chip ~8 workgroups per CU, never more than there are tiles, partials under
+// kMaxPartialBytes) over n_rows*W row windows; workspace bytes for S > 1.
+struct FieldPlan {
+    int64_t W = 0, S = 1;
+    size_t bytes = 0;
+};
+hr_status field_plan(const hr_coherent_field_args &p, FieldPlan &P) {
+    int cu = 0;
+    hr_status st = device_cu(cu);
+    if (st != HR_OK) return st;
+    const int64_t per_block = (int64_t)kFieldThreads * kFieldBins;
+    P.W = (p.n_E + per_block - 1) / per_block;
+    const int64_t n_tiles = (p.n_lines + kFieldTile - 1) / kFieldTile;
+    const int64_t cells = (int64_t)p.n_rows * 4 * p.n_E;
+    int64_t S = (int64_t)cu * 8 / ((int64_t)p.n_rows * P.W);
+    if (S < 1) S = 1;
+    if (S > n_tiles) S = n_tiles;
+    const int64_t s_cap = (int64_t)(kMaxPartialBytes / ((size_t)cells * sizeof(double)));
+    if (S > s_cap) S = s_cap > 1 ? s_cap : 1;
+    if (S < 1) S = 1;
+    P.S = S;
+    P.bytes = (S > 1 && p.n_lines > 0) ? (size_t)S * (size_t)cells * sizeof(double) : 0;
+    return HR_OK;
+}
 
 }  // namespace
 }  // namespace hr
@@ -373,6 +370,18 @@ extern "C" hr_status hr_coherent_lines(const hr_coherent_args *args, void *strea
     return HR_OK;
 }
 
+extern "C" hr_status hr_coherent_field_workspace_bytes(const hr_coherent_field_args *args, size_t *bytes) {
+    using namespace hr;
+    HR_REQUIRE(args != nullptr && bytes != nullptr, "hr_coherent_field_workspace_bytes: null argument");
+    HR_REQUIRE(args->n_lines >= 0 && args->n_rows >= 1 && args->n_E >= 1,
+               "hr_coherent_field_workspace_bytes: bad sizes");
+    FieldPlan P;
+    hr_status st = field_plan(*args, P);
+    if (st != HR_OK) return st;
+    *bytes = P.bytes;
+    return HR_OK;
+}
+
 extern "C" hr_status hr_coherent_field(const hr_coherent_field_args *args, void *stream) {
     using namespace hr;
     HR_REQUIRE(args != nullptr, "hr_coherent_field: null args");
@@ -382,30 +391,18 @@ extern "C" hr_status hr_coherent_field(const hr_coherent_field_args *args, void 
                (long long)p.ld, p.n_rows, (long long)p.n_E);
     if (p.n_lines == 0) return HR_OK;
     HR_REQUIRE(p.rec && p.E_grid && p.dom && p.field, "hr_coherent_field: null pointer");
-    const int64_t per_block = (int64_t)kFieldThreads * kFieldBins;
-    const int64_t W = (p.n_E + per_block - 1) / per_block;
+    FieldPlan P;
+    hr_status st = field_plan(p, P);
+    if (st != HR_OK) return st;
+    const int64_t W = P.W, S = P.S;
+    const int64_t cells = (int64_t)p.n_rows * 4 * p.n_E;
     HR_REQUIRE((int64_t)p.n_rows * W <= 65535, "hr_coherent_field: %lld row windows exceed the grid",
                (long long)(p.n_rows * W));
-    void *ws = nullptr;
-    int cu = 0;
-    hr_status st = field_workspace(0, ws, cu);
-    if (st != HR_OK) return st;
-    // splits along the line axis: fill the chip ~8 workgroups per CU, never
-    // more than there are tiles, and keep the partials under kMaxPartialBytes
-    const int64_t n_tiles = (p.n_lines + kFieldTile - 1) / kFieldTile;
-    const int64_t cells = (int64_t)p.n_rows * 4 * p.n_E;
-    int64_t S = (int64_t)cu * 8 / ((int64_t)p.n_rows * W);
-    if (S < 1) S = 1;
-    if (S > n_tiles) S = n_tiles;
-    const int64_t s_cap = (int64_t)(kMaxPartialBytes / ((size_t)cells * sizeof(double)));
-    if (S > s_cap) S = s_cap > 1 ? s_cap : 1;
+    HR_REQUIRE_WS(p.ws, P.bytes, "hr_coherent_field");
+    void *ws = p.ws.ptr;
     hipStream_t s = (hipStream_t)stream;
     double *out = p.field;
-    if (S > 1) {
-        st = field_workspace((size_t)S * (size_t)cells * sizeof(double), ws, cu);
-        if (st != HR_OK) return st;
-        out = (double *)ws;
-    }
+    if (S > 1) out = (double *)ws;
     hipLaunchKernelGGL(coherent_field_kernel<kFieldBins>, dim3((unsigned)S, (unsigned)(p.n_rows * W)),
                        dim3(kFieldThreads), 0, s, p, (int)W, (int)S, out);
     HR_CHECK_HIP(hipGetLastError());

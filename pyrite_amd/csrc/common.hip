@@ -19,6 +19,24 @@ void set_error(const char *fmt, ...) {
 const char *last_error() { return g_error; }
 
 namespace {
+int g_cu[64];  // compute units per device, filled on first use
+}
+
+hr_status device_cu(int &cu) {
+    int dev = 0;
+    HR_CHECK_HIP(hipGetDevice(&dev));
+    HR_REQUIRE(dev >= 0 && dev < 64, "device index %d out of range", dev);
+    if (g_cu[dev] == 0) {
+        int n = 0;
+        HR_CHECK_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+        HR_REQUIRE(n > 0, "device %d reports %d compute units", dev, n);
+        g_cu[dev] = n;
+    }
+    cu = g_cu[dev];
+    return HR_OK;
+}
+
+namespace {
 // key[e] = splitmix64(seed + PHI*(start + e + 1))  (kinematics.py:226-245)
 __global__ void __launch_bounds__(256) stream_keys_kernel(uint64_t seed, int64_t start, int64_t n,
                                                           uint64_t *__restrict__ keys) {

@@ -67,6 +67,18 @@ __device__ __forceinline__ int64_t row_owner(const int64_t *__restrict__ row_off
     return lo;
 }
 
+// Compute units of the current device (a host-side property cache; no
+// allocation, no launch).
+hr_status device_cu(int &cu);
+
+// A caller-owned workspace holds at least `need` bytes, 256-byte aligned.
+#define HR_REQUIRE_WS(ws, need, who)                                                              \
+    HR_REQUIRE((need) == 0 || ((ws).ptr != nullptr && (ws).bytes >= (size_t)(need) &&             \
+                               ((uintptr_t)(ws).ptr & 255u) == 0),                                \
+               "%s: workspace of %zu bytes at %p is smaller than the %zu it needs or misaligned "  \
+               "(query the *_workspace_bytes entry point)",                                      \
+               who, (size_t)(ws).bytes, (ws).ptr, (size_t)(need))
+
 inline int pick_block(int requested, int fallback) {
     if (requested <= 0) return fallback;
     if (requested > 1024) return 1024;

@@ -114,31 +114,11 @@ __global__ void queue_reset_kernel(unsigned long long *next, double *__restrict_
     if (i < n) tally[i] = 0.0;
 }
 
-struct QueueDevice {
-    unsigned long long *counter = nullptr;
-    int cu = 0;
-};
-QueueDevice g_queue[64];
-
-hr_status queue_device(QueueDevice *&w) {
-    int dev = 0;
-    HR_CHECK_HIP(hipGetDevice(&dev));
-    HR_REQUIRE(dev >= 0 && dev < 64, "device index %d out of range", dev);
-    w = &g_queue[dev];
-    if (w->counter == nullptr) {
-        HR_CHECK_HIP(hipMalloc(&w->counter, 256));
-        hipDeviceProp_t prop;
-        HR_CHECK_HIP(hipGetDeviceProperties(&prop, dev));
-        w->cu = prop.multiProcessorCount;
-    }
-    return HR_OK;
-}
-
 hr_status queue_check(const hr_queue_launch *L) {
     HR_REQUIRE(L != nullptr, "queue: null launch");
     HR_REQUIRE(L->n_cfg >= 1 && L->n_cfg <= kQueueMaxCfg, "queue: n_cfg %d outside [1, %d]", L->n_cfg,
                kQueueMaxCfg);
-    HR_REQUIRE(L->configs && L->offsets && L->tally, "queue: null pointer");
+    HR_REQUIRE(L->configs && L->offsets && L->tally && L->counter, "queue: null pointer");
     HR_REQUIRE(L->total >= 0, "queue: negative total");
     HR_REQUIRE(L->per_claim >= 1 && L->per_claim <= 1024, "queue: per_claim %d outside [1, 1024]", L->per_claim);
     HR_REQUIRE(L->block_size == 0 || (L->block_size % 64 == 0 && L->block_size >= 64 && L->block_size <= 256),
@@ -173,7 +153,6 @@ hr_status queue_enqueue(const hr_queue_launch *L, unsigned long long *counter, i
 
 struct QueueGraph {
     hipGraphExec_t exec = nullptr;
-    unsigned long long *counter = nullptr;
 };
 
 }  // namespace
@@ -185,10 +164,10 @@ hr_status hr_queue_run(const hr_queue_launch *launch, void *stream) {
     using namespace hr;
     hr_status st = queue_check(launch);
     if (st != HR_OK) return st;
-    QueueDevice *w = nullptr;
-    st = queue_device(w);
+    int cu = 0;
+    st = device_cu(cu);
     if (st != HR_OK) return st;
-    return queue_enqueue(launch, w->counter, w->cu, (hipStream_t)stream);
+    return queue_enqueue(launch, launch->counter, cu, (hipStream_t)stream);
 }
 
 hr_status hr_queue_graph_create(const hr_queue_launch *launch, void **graph) {
@@ -197,26 +176,20 @@ hr_status hr_queue_graph_create(const hr_queue_launch *launch, void **graph) {
     *graph = nullptr;
     hr_status st = queue_check(launch);
     if (st != HR_OK) return st;
-    QueueDevice *w = nullptr;
-    st = queue_device(w);
+    int cu = 0;
+    st = device_cu(cu);
     if (st != HR_OK) return st;
     QueueGraph *g = new QueueGraph();
-    hipError_t e = hipMalloc(&g->counter, 256);
-    if (e != hipSuccess) {
-        delete g;
-        HR_CHECK_HIP(e);
-    }
     hipStream_t cs = nullptr;
-    e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
     if (e != hipSuccess) {
-        (void)hipFree(g->counter);
         delete g;
         HR_CHECK_HIP(e);
     }
     hipGraph_t graph_t = nullptr;
     e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-        st = queue_enqueue(launch, g->counter, w->cu, cs);
+        st = queue_enqueue(launch, launch->counter, cu, cs);
         const hipError_t e2 = hipStreamEndCapture(cs, &graph_t);
         if (st == HR_OK && e2 != hipSuccess) e = e2;
     }
@@ -224,7 +197,6 @@ hr_status hr_queue_graph_create(const hr_queue_launch *launch, void **graph) {
     if (graph_t) (void)hipGraphDestroy(graph_t);
     (void)hipStreamDestroy(cs);
     if (e != hipSuccess || st != HR_OK) {
-        (void)hipFree(g->counter);
         delete g;
         if (st != HR_OK) return st;
         HR_CHECK_HIP(e);
@@ -246,7 +218,6 @@ hr_status hr_queue_graph_destroy(void *graph) {
     if (graph == nullptr) return HR_OK;
     QueueGraph *g = (QueueGraph *)graph;
     if (g->exec) HR_CHECK_HIP(hipGraphExecDestroy(g->exec));
-    if (g->counter) HR_CHECK_HIP(hipFree(g->counter));
     delete g;
     return HR_OK;
 }

@@ -598,35 +598,6 @@ __global__ void fill_zero_kernel(double *__restrict__ p, int64_t n) {
     if (i < n) p[i] = 0.0;
 }
 
-struct Workspace {
-    void *ptr = nullptr;
-    size_t bytes = 0;
-    int cu = 0;
-};
-Workspace g_ws[64];
-
-hr_status workspace(size_t need, void *&ptr, int &cu) {
-    int dev = 0;
-    HR_CHECK_HIP(hipGetDevice(&dev));
-    HR_REQUIRE(dev >= 0 && dev < 64, "device index %d out of range", dev);
-    Workspace &w = g_ws[dev];
-    if (w.cu == 0) {
-        hipDeviceProp_t prop;
-        HR_CHECK_HIP(hipGetDeviceProperties(&prop, dev));
-        w.cu = prop.multiProcessorCount;
-    }
-    if (w.bytes < need) {
-        if (w.ptr) HR_CHECK_HIP(hipFree(w.ptr));
-        w.ptr = nullptr;
-        w.bytes = 0;
-        HR_CHECK_HIP(hipMalloc(&w.ptr, need));
-        w.bytes = need;
-    }
-    ptr = w.ptr;
-    cu = w.cu;
-    return HR_OK;
-}
-
 // Workspace bytes for gx partial rows of n_E bins plus the second level.
 size_t partial_bytes(int64_t gx, int64_t n_E) {
     const int64_t splits = gx < kPartialSplits ? gx : kPartialSplits;
@@ -654,6 +625,41 @@ int64_t line_blocks(int cu, int64_t n_tiles, int64_t bin_tiles) {
     int64_t want = (int64_t)cu * 8 / (bin_tiles > 0 ? bin_tiles : 1);
     if (want < 1) want = 1;
     return n_tiles < want ? (n_tiles > 0 ? n_tiles : 1) : want;
+}
+
+constexpr int kDenseBins = 4;  // bins per thread of the dense reducers
+
+// Launch shape and workspace bytes of a dense reducer over n_items items in
+// tiles of `tile`, on a grid of n_E bins, for the current device.
+struct DensePlan {
+    int64_t gx = 0, bin_tiles = 0;
+    size_t bytes = 0;
+};
+hr_status dense_plan(int64_t n_items, int64_t tile, int64_t n_E, DensePlan &P) {
+    int cu = 0;
+    hr_status st = device_cu(cu);
+    if (st != HR_OK) return st;
+    P.bin_tiles = (n_E + kRedThreads * kDenseBins - 1) / (kRedThreads * kDenseBins);
+    const int64_t n_tiles = (n_items + tile - 1) / tile;
+    P.gx = line_blocks(cu, n_tiles, P.bin_tiles);
+    P.bytes = (n_items == 0 || n_E == 0) ? 0 : partial_bytes(P.gx, n_E);
+    return HR_OK;
+}
+
+// Blocks of the standalone windowed reduction and its workspace bytes:
+// [partials blocks*4*n | spec n | node moments 9n].
+int64_t window_reduce_blocks(int cu, int64_t n_lines) {
+    int64_t blocks = (n_lines + kWinThreads - 1) / kWinThreads;
+    return blocks < 2 * (int64_t)cu ? blocks : 2 * (int64_t)cu;
+}
+
+// Line-axis chunks per direction of the fused pixel route (about two resident
+// blocks per CU in total; one block fills a CU's LDS).
+int64_t window_chunks(int cu, int64_t n_seg, int32_t n_dir) {
+    int64_t chunks = (2 * (int64_t)cu + n_dir - 1) / n_dir;
+    const int64_t max_chunks = (n_seg + kWinThreads - 1) / kWinThreads;
+    chunks = chunks < max_chunks ? chunks : max_chunks;
+    return chunks < 1 ? 1 : chunks;
 }
 
 }  // namespace
@@ -689,18 +695,15 @@ extern "C" hr_status hr_sincsq_reduce(const hr_sincsq_args *args, void *stream) 
     if (p.n_lines == 0 || p.n_E == 0) return HR_OK;
     HR_REQUIRE(p.E_r && p.a && p.w && p.out, "hr_sincsq_reduce: null pointer");
     HR_REQUIRE(p.uniform || p.E_grid, "hr_sincsq_reduce: a non-uniform grid needs E_grid");
-    constexpr int BINS = 4;
-    const int64_t bin_tiles = (p.n_E + kRedThreads * BINS - 1) / (kRedThreads * BINS);
-    const int64_t n_tiles = (p.n_lines + kLineTile - 1) / kLineTile;
-    void *ws = nullptr;
-    int cu = 0;
-    hr_status st = workspace(0, ws, cu);
+    constexpr int BINS = kDenseBins;
+    DensePlan P;
+    hr_status st = dense_plan(p.n_lines, kLineTile, p.n_E, P);
     if (st != HR_OK) return st;
-    const int64_t gx = line_blocks(cu, n_tiles, bin_tiles);
-    st = workspace(partial_bytes(gx, p.n_E), ws, cu);
-    if (st != HR_OK) return st;
+    HR_REQUIRE_WS(p.ws, P.bytes, "hr_sincsq_reduce");
+    const int64_t gx = P.gx;
+    void *ws = p.ws.ptr;
     hipStream_t s = (hipStream_t)stream;
-    dim3 grid((unsigned)gx, (unsigned)bin_tiles);
+    dim3 grid((unsigned)gx, (unsigned)P.bin_tiles);
     if (p.uniform) {
         hipLaunchKernelGGL(sincsq_uniform_kernel<BINS>, grid, dim3(kRedThreads), 0, s, p.E_r, p.a, p.w,
                            p.n_lines, p.E0, p.dE, p.n_E, (double *)ws);
@@ -712,6 +715,28 @@ extern "C" hr_status hr_sincsq_reduce(const hr_sincsq_args *args, void *stream) 
     return reduce_partials(ws, gx, p.n_E, p.out, s);
 }
 
+extern "C" hr_status hr_sincsq_workspace_bytes(const hr_sincsq_args *args, size_t *bytes) {
+    using namespace hr;
+    HR_REQUIRE(args != nullptr && bytes != nullptr, "hr_sincsq_workspace_bytes: null argument");
+    HR_REQUIRE(args->n_lines >= 0 && args->n_E >= 0, "hr_sincsq_workspace_bytes: negative size");
+    DensePlan P;
+    hr_status st = dense_plan(args->n_lines, kLineTile, args->n_E, P);
+    if (st != HR_OK) return st;
+    *bytes = P.bytes;
+    return HR_OK;
+}
+
+extern "C" hr_status hr_brem_workspace_bytes(const hr_brem_args *args, size_t *bytes) {
+    using namespace hr;
+    HR_REQUIRE(args != nullptr && bytes != nullptr, "hr_brem_workspace_bytes: null argument");
+    HR_REQUIRE(args->n_seg >= 0 && args->n_E >= 0, "hr_brem_workspace_bytes: negative size");
+    DensePlan P;
+    hr_status st = dense_plan(args->n_seg, kBremTile, args->n_E, P);
+    if (st != HR_OK) return st;
+    *bytes = P.bytes;
+    return HR_OK;
+}
+
 extern "C" hr_status hr_brem_reduce(const hr_brem_args *args, void *stream) {
     using namespace hr;
     HR_REQUIRE(args != nullptr, "hr_brem_reduce: null args");
@@ -721,18 +746,15 @@ extern "C" hr_status hr_brem_reduce(const hr_brem_args *args, void *stream) {
     if (p.n_seg == 0 || p.n_E == 0) return HR_OK;
     HR_REQUIRE(p.r_mid && p.v_hat && p.L && p.E_keV && p.E_grid_eV && p.mu && p.out,
                "hr_brem_reduce: null pointer");
-    constexpr int BINS = 4;
-    const int64_t bin_tiles = (p.n_E + kRedThreads * BINS - 1) / (kRedThreads * BINS);
-    const int64_t n_tiles = (p.n_seg + kBremTile - 1) / kBremTile;
-    void *ws = nullptr;
-    int cu = 0;
-    hr_status st = workspace(0, ws, cu);
+    constexpr int BINS = kDenseBins;
+    DensePlan P;
+    hr_status st = dense_plan(p.n_seg, kBremTile, p.n_E, P);
     if (st != HR_OK) return st;
-    const int64_t gx = line_blocks(cu, n_tiles, bin_tiles);
-    st = workspace(partial_bytes(gx, p.n_E), ws, cu);
-    if (st != HR_OK) return st;
+    HR_REQUIRE_WS(p.ws, P.bytes, "hr_brem_reduce");
+    const int64_t gx = P.gx;
+    void *ws = p.ws.ptr;
     hipStream_t s = (hipStream_t)stream;
-    const dim3 grid((unsigned)gx, (unsigned)bin_tiles);
+    const dim3 grid((unsigned)gx, (unsigned)P.bin_tiles);
     switch (p.n_el) {
     case 1: hipLaunchKernelGGL((brem_kernel<BINS, 1>), grid, dim3(kRedThreads), 0, s, p, (double *)ws); break;
     case 2: hipLaunchKernelGGL((brem_kernel<BINS, 2>), grid, dim3(kRedThreads), 0, s, p, (double *)ws); break;
@@ -776,8 +798,22 @@ hr_status window_finish(double *partial, int64_t blocks, int n_dir, const Window
 }  // namespace
 }  // namespace hr
 
+extern "C" hr_status hr_window_reduce_workspace_bytes(int64_t n_lines, int64_t n_E, size_t *bytes) {
+    using namespace hr;
+    HR_REQUIRE(bytes != nullptr && n_lines >= 0 && n_E >= 0, "hr_window_reduce_workspace_bytes: bad arguments");
+    *bytes = 0;
+    if (n_lines == 0 || n_E == 0) return HR_OK;
+    int cu = 0;
+    hr_status st = device_cu(cu);
+    if (st != HR_OK) return st;
+    const int64_t blocks = window_reduce_blocks(cu, n_lines);
+    *bytes = (size_t)(blocks * 4 * n_E + n_E + 9 * n_E) * sizeof(double);
+    return HR_OK;
+}
+
 extern "C" hr_status hr_window_reduce(const double *E_r, const double *a, const float *w, int64_t n_lines,
-                                      int64_t n_E, double E0, double dE, double x_c, double *out, void *stream) {
+                                      int64_t n_E, double E0, double dE, double x_c, double *out, hr_workspace ws,
+                                      void *stream) {
     using namespace hr;
     HR_REQUIRE(n_lines >= 0, "hr_window_reduce: negative n_lines");
     if (n_lines == 0 || n_E == 0) return HR_OK;
@@ -785,17 +821,14 @@ extern "C" hr_status hr_window_reduce(const double *E_r, const double *a, const 
     WindowGrid G;
     hr_status st = window_setup(n_E, E0, dE, x_c, G);
     if (st != HR_OK) return st;
-    void *ws = nullptr;
     int cu = 0;
-    st = workspace(0, ws, cu);
+    st = device_cu(cu);
     if (st != HR_OK) return st;
-    int64_t blocks = (n_lines + kWinThreads - 1) / kWinThreads;
-    blocks = blocks < 2 * (int64_t)cu ? blocks : 2 * (int64_t)cu;
+    const int64_t blocks = window_reduce_blocks(cu, n_lines);
     const int64_t n = n_E;
     const size_t need = (size_t)(blocks * 4 * n + n + 9 * n) * sizeof(double);
-    st = workspace(need, ws, cu);
-    if (st != HR_OK) return st;
-    double *partial = (double *)ws, *spec = partial + blocks * 4 * n, *dep = spec + n;
+    HR_REQUIRE_WS(ws, need, "hr_window_reduce");
+    double *partial = (double *)ws.ptr, *spec = partial + blocks * 4 * n, *dep = spec + n;
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((9 * n + 255) / 256)), dim3(256), 0, s, dep, 9 * n);
     HR_CHECK_HIP(hipGetLastError());
@@ -803,6 +836,23 @@ extern "C" hr_status hr_window_reduce(const double *E_r, const double *a, const 
                        partial, dep);
     HR_CHECK_HIP(hipGetLastError());
     return window_finish(partial, blocks, 1, G, spec, dep, 0, 3 * n, out, s);
+}
+
+extern "C" hr_status hr_window_spectra_workspace_bytes(const hr_window_args *args, size_t *bytes) {
+    using namespace hr;
+    HR_REQUIRE(args != nullptr && args->lines != nullptr && bytes != nullptr,
+               "hr_window_spectra_workspace_bytes: null argument");
+    const hr_line_args &p = *args->lines;
+    HR_REQUIRE(p.n_seg >= 0 && p.n_dir >= 1 && args->n_E >= 0, "hr_window_spectra_workspace_bytes: bad sizes");
+    *bytes = 0;
+    if (p.n_seg == 0 || args->n_E == 0) return HR_OK;
+    int cu = 0;
+    hr_status st = device_cu(cu);
+    if (st != HR_OK) return st;
+    const int64_t n = args->n_E;
+    const int64_t chunks = window_chunks(cu, p.n_seg, p.n_dir);
+    *bytes = (size_t)((int64_t)p.n_dir * (chunks * 4 * n + n + 9 * n)) * sizeof(double);
+    return HR_OK;
 }
 
 extern "C" hr_status hr_window_spectra(const hr_window_args *args, void *stream) {
@@ -820,20 +870,14 @@ extern "C" hr_status hr_window_spectra(const hr_window_args *args, void *stream)
     WindowGrid G;
     hr_status st = window_setup(args->n_E, args->E0, args->dE, args->x_c, G);
     if (st != HR_OK) return st;
-    void *ws = nullptr;
     int cu = 0;
-    st = workspace(0, ws, cu);
+    st = device_cu(cu);
     if (st != HR_OK) return st;
     const int64_t n = G.n;
-    // about two resident blocks per CU in total (one block fills a CU's LDS)
-    int64_t chunks = (2 * (int64_t)cu + p.n_dir - 1) / p.n_dir;
-    const int64_t max_chunks = (p.n_seg + kWinThreads - 1) / kWinThreads;
-    chunks = chunks < max_chunks ? chunks : max_chunks;
-    chunks = chunks < 1 ? 1 : chunks;
+    const int64_t chunks = window_chunks(cu, p.n_seg, p.n_dir);
     const size_t need = (size_t)((int64_t)p.n_dir * (chunks * 4 * n + n + 9 * n)) * sizeof(double);
-    st = workspace(need, ws, cu);
-    if (st != HR_OK) return st;
-    double *partial = (double *)ws;
+    HR_REQUIRE_WS(args->ws, need, "hr_window_spectra");
+    double *partial = (double *)args->ws.ptr;
     double *spec = partial + (int64_t)p.n_dir * chunks * 4 * n;
     double *dep = spec + (int64_t)p.n_dir * n;
     hipStream_t s = (hipStream_t)stream;

@@ -132,27 +132,20 @@ __global__ void __launch_bounds__(256) HR_TRANSPORT_OCCUPANCY continue_kernel(hr
 
 __global__ void reset_counter(unsigned long long *c) { *c = 0ULL; }
 
-struct DeviceWork {
-    unsigned long long *counter = nullptr;
-    int cu = 0;
-    int occupancy = 0;
-};
-DeviceWork g_work[64];
+int g_occupancy[64];  // resident 256-lane blocks per CU of transport_kernel, per device
 
-hr_status device_work(DeviceWork *&w) {
+hr_status transport_occupancy(int block, int &occupancy) {
     int dev = 0;
     HR_CHECK_HIP(hipGetDevice(&dev));
     HR_REQUIRE(dev >= 0 && dev < 64, "device index %d out of range", dev);
-    w = &g_work[dev];
-    if (w->counter == nullptr) {
-        HR_CHECK_HIP(hipMalloc(&w->counter, 256));
-        hipDeviceProp_t prop;
-        HR_CHECK_HIP(hipGetDeviceProperties(&prop, dev));
-        w->cu = prop.multiProcessorCount;
-        int blocks = 0;
-        HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, transport_kernel<false>, 256, 0));
-        w->occupancy = blocks > 0 ? blocks : 1;
+    if (block == 256 && g_occupancy[dev] > 0) {
+        occupancy = g_occupancy[dev];
+        return HR_OK;
     }
+    int blocks = 0;
+    HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, transport_kernel<false>, block, 0));
+    occupancy = blocks > 0 ? blocks : 1;
+    if (block == 256) g_occupancy[dev] = occupancy;
     return HR_OK;
 }
 
@@ -183,27 +176,24 @@ extern "C" hr_status hr_transport_run(const hr_transport_args *args, void *strea
     HR_REQUIRE(a.precision == 0 || (a.lut_total_rate_f && a.lut_dEds_f && a.lut_inv_beta_f && a.lut_cdf_f &&
                                     a.lut_alpha_f),
                "hr_transport_run: mixed precision needs the FP32 table copies");
-    DeviceWork *w = nullptr;
-    hr_status st = device_work(w);
+    HR_REQUIRE(a.work != nullptr, "hr_transport_run: null claim counter (work)");
+    int cu = 0, occupancy = 0;
+    hr_status st = device_cu(cu);
     if (st != HR_OK) return st;
-    hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, w->counter);
-    HR_CHECK_HIP(hipGetLastError());
     // Whole wave64s, at most the kernel's launch bound of 256 lanes.
     const int block = a.block_size <= 0 ? 256 : (a.block_size >= 256 ? 256 : pick_block(a.block_size, 256));
-    int occupancy = w->occupancy;
-    if (block != 256) {
-        int blocks = 0;
-        HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, transport_kernel<false>, block, 0));
-        occupancy = blocks > 0 ? blocks : 1;
-    }
+    st = transport_occupancy(block, occupancy);
+    if (st != HR_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, a.work);
+    HR_CHECK_HIP(hipGetLastError());
     const int64_t need = (a.e_count + block - 1) / block;
-    const int64_t resident = (int64_t)w->cu * occupancy;
+    const int64_t resident = (int64_t)cu * occupancy;
     const int64_t grid = need < resident ? need : resident;
     if (a.precision == 1)
-        hipLaunchKernelGGL(transport_kernel<true>, dim3((unsigned)grid), dim3(block), 0, s, a, w->counter);
+        hipLaunchKernelGGL(transport_kernel<true>, dim3((unsigned)grid), dim3(block), 0, s, a, a.work);
     else
-        hipLaunchKernelGGL(transport_kernel<false>, dim3((unsigned)grid), dim3(block), 0, s, a, w->counter);
+        hipLaunchKernelGGL(transport_kernel<false>, dim3((unsigned)grid), dim3(block), 0, s, a, a.work);
     HR_CHECK_HIP(hipGetLastError());
     return HR_OK;
 }

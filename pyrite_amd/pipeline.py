@@ -103,19 +103,41 @@ class CaseTables:
         self.line_tables = line_tables(cfg.crystal, cfg.hkl_list, self.E_grid, cfg.composition)
 
 
+# Device bytes all live pipelines may hold in slot-row arenas together, as a
+# fraction of the device's memory; each pipeline's two arenas take at most
+# ROW_ARENA_MAX_BYTES each.
+ROW_ARENA_DEVICE_FRACTION = 0.30
+ROW_ARENA_MAX_BYTES = 24 << 30
+_ARENA_LIVE = {"bytes": 0}
+
+
+def _arena_release(nbytes):
+    _ARENA_LIVE["bytes"] = max(0, _ARENA_LIVE["bytes"] - int(nbytes))
+
+
 class CasePipeline(CaseTables):
-    def __init__(self, cfg: CaseConfig, transport_config: PerElectronTransportConfig | None = None):
+    def __init__(self, cfg: CaseConfig, transport_config: PerElectronTransportConfig | None = None,
+                 arena_bytes: int | None = None):
         import torch
 
         super().__init__(cfg)
         self.dev = require_device().torch_device
         dev = self.dev
-        # Row slots: 2048 per history from the start (a 1e5-history batch of
-        # slots is 23 GB, two in flight), so capacity replays are rare and the
-        # slot arrays keep one size from batch to batch (the allocator reuses
-        # them).  No field of the config changes results.
+        # Slot rows come from two byte arenas used in turn, one sub-batch each:
+        # 2048 slots per history to start with (a 1e5-history batch of slots
+        # is 23.6 GB), so capacity replays are rare, and a batch whose grid
+        # outgrows an arena is cut into sub-batches, each scored before the
+        # arena it sits in is refilled.  Every live pipeline's arenas together
+        # stay under ROW_ARENA_DEVICE_FRACTION of the device's memory.  No
+        # field of the config changes results.
+        total = int(torch.cuda.get_device_properties(dev).total_memory)
+        room = int(ROW_ARENA_DEVICE_FRACTION * total) - _ARENA_LIVE["bytes"]
+        want = ROW_ARENA_MAX_BYTES if arena_bytes is None else int(arena_bytes)
+        self.arena_bytes = max(256 << 20, min(want, room // 2))
+        self._reserved = 2 * self.arena_bytes  # held against the device-wide arena budget until release()
+        _ARENA_LIVE["bytes"] += self._reserved
         self.tcfg = transport_config or PerElectronTransportConfig(seg_capacity=2048,
-                                                                   scratch_budget_bytes=48 << 30)
+                                                                   scratch_budget_bytes=self.arena_bytes)
         self.tables = DeviceTables(self.layers, self.lut, dev)
         self.control = TransportControl(max_steps=cfg.max_steps, energy_model=1, straggle_on=cfg.straggling,
                                         precision=cfg.precision)
@@ -147,74 +169,126 @@ class CasePipeline(CaseTables):
         self.segments = 0
         self.timers = {}
         self._cap_hint = None
-        self._row_pool = [None, None]
-        self._pool_turn = 0
-        self._pool_last = 0
-        self._pool_free = [None, None]
+        self._arenas = [None, None]
+        self._arena_turn = 0
+        self._arena_cur = 0
+        self._arena_e = None
+        self._arena_free = [None, None]
         self._rec = None
         self._ones = torch.ones(n, dtype=torch.float64, device=dev)
 
-    def _pool_rows(self, idx, e, m, cap, midpoint, device):
-        """Slot rows of a batch's first sub-batch from persistent buffer ``idx``
-        (grown when a batch needs more slots); later sub-batches get fresh rows."""
-        from .transport.core import alloc_rows
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
 
-        if e != 0:
-            return alloc_rows(m, cap, midpoint, device)
-        need = m * cap
-        buf = self._row_pool[idx]
-        if buf is None or buf["seg_len"].shape[0] < need:
-            self._row_pool[idx] = None
-            buf = self._row_pool[idx] = alloc_rows(m, cap, midpoint, device)
-        return {k: v[:need] for k, v in buf.items()}
+    def release(self):
+        """Drop the slot-row arenas (their bytes return to the other pipelines' budget)."""
+        for k in range(len(getattr(self, "_arenas", ()))):
+            self._arenas[k] = None
+        _arena_release(getattr(self, "_reserved", 0))
+        self._reserved = 0
 
-    def transport(self, batch_index: int, compact: bool = True):
-        """Transport histories ``[b*Ne, (b+1)*Ne)``; returns the device segment rows
-        (compacted, or the slot rows of each sub-batch with ``compact=False``).
+    def _arena(self, idx):
+        from .backend import is_oom_error
+        from .transport.core import RowArena
 
-        Slot rows live in two persistent buffers used in turn, so those of one
-        call stay valid until the second call after it; a call waits (on the
-        device) until the scoring that last read its buffer has finished."""
+        a = self._arenas[idx]
+        if a is None:
+            nbytes = self.arena_bytes
+            while True:
+                try:
+                    a = RowArena(nbytes, self.dev)
+                    break
+                except Exception as exc:  # noqa: BLE001 - halve on device OOM only
+                    if not is_oom_error(exc) or nbytes <= (256 << 20):
+                        raise
+                    import torch
+
+                    torch.cuda.empty_cache()
+                    nbytes //= 2
+            self._arenas[idx] = a
+        return a
+
+    def _arena_rows(self, e, m, cap, midpoint, device):
+        """Slot rows of the sub-batch starting at electron ``e`` of the current call,
+        carved from the next arena in turn (a replay of the same sub-batch reuses
+        its arena); waits, on the device, until the scoring that last read that
+        arena has finished."""
         import torch
 
+        if e != self._arena_e:
+            self._arena_e = e
+            self._arena_cur = self._arena_turn
+            self._arena_turn ^= 1
+            ev = self._arena_free[self._arena_cur]
+            if ev is not None:
+                torch.cuda.current_stream().wait_event(ev)
+        return self._arena(self._arena_cur).rows(m, cap, midpoint)
+
+    def transport(self, batch_index: int, compact: bool = True, consume=None):
+        """Transport histories ``[b*Ne, (b+1)*Ne)``; returns the device segment rows
+        (compacted; with ``compact=False`` the slot rows of each sub-batch, or,
+        with ``consume``, nothing: ``consume(e0, m, cap, rows, seg_count)`` takes
+        each sub-batch as it is done, and must queue its readers of ``rows``
+        before it returns, then call ``_scored()``).
+
+        Slot rows live in two arenas used in turn, one sub-batch each, so the
+        rows of a sub-batch stay valid until the second sub-batch after it is
+        transported; a sub-batch waits (on the device) until the scoring that
+        last read its arena has finished.  Without ``consume`` a
+        ``compact=False`` call may use at most two sub-batches (both stay
+        referenced in the result): a batch whose grid needs more raises."""
         cfg = self.cfg
         n = cfg.Ne
-        idx = self._pool_turn
-        self._pool_turn ^= 1
-        if self._pool_free[idx] is not None:
-            torch.cuda.current_stream().wait_event(self._pool_free[idx])
-        self._pool_last = idx
-        stream = torch.cuda.current_stream().cuda_stream
+        stream = __import__("torch").cuda.current_stream().cuda_stream
         _lib.call("hr_stream_keys", int(cfg.seed), int(batch_index) * n, n, self.keys.data_ptr(), stream)
         fill_entry(cfg.beam, cfg.seed, int(batch_index) * n, n, cfg.E0_keV, np.deg2rad(cfg.tilt_deg),
                    self.pos, self.dirs, self.E, self.clock, stream=stream)
+        self._arena_e = None
+        held = []
+
+        def alloc(e, m, cap, mid, dev):
+            if not compact and consume is None and e != self._arena_e:
+                held.append(e)
+                if len(held) > 2:
+                    raise RuntimeError(f"batch of {n} histories needs more than two slot-row sub-batches at "
+                                       f"capacity {cap}; pass consume= (or compact=True)")
+            return self._arena_rows(e, m, cap, mid, dev)
+
         res = drive_batches(self.core, self.keys,
                             (self.alive, self.clock, self.pos, self.dirs, self.E, self.E_cut),
                             self.stragg, self.control, n, self.dev, self.tcfg, stream=stream, compact=compact,
                             cap_hint=None if self._cap_hint is None else max(self._cap_hint, self.tcfg.seg_capacity),
-                            alloc=lambda e, m, cap, mid, dev: self._pool_rows(idx, e, m, cap, mid, dev))
+                            alloc=alloc, consume=consume)
         self._cap_hint = res.capacity
         return res
 
     def _scored(self):
-        """Mark the rows of the last transport as read by the work queued on the
-        current stream (their buffer may be refilled after it)."""
+        """Mark the rows of the current arena as read by the work queued on the
+        current stream (the arena may be refilled after it)."""
         import torch
 
         ev = torch.cuda.Event()
         ev.record(torch.cuda.current_stream())
-        self._pool_free[self._pool_last] = ev
+        self._arena_free[self._arena_cur] = ev
 
     def records(self, res, first_id=None):
         """FP32 scoring records gathered straight from the slot rows of a
         ``compact=False`` transport (no FP64 compaction pass); with
         ``first_id`` (the batch's first global electron id) also each row's
         primary id as ``"prim"``."""
+        return self.slot_records(res.slots, int(res.n_segments), first_id)
+
+    def slot_records(self, slots, n_rows, first_id=None):
+        """FP32 scoring records of slot rows ``[(e0, m, cap, rows, seg_count)]``
+        holding ``n_rows`` written rows in total."""
         import ctypes as C
 
         import torch
 
-        n_rows = int(res.n_segments)
+        n_rows = int(n_rows)
         if self._rec is None or self._rec["L"].shape[0] < n_rows:
             m = max(n_rows, 1)
             self._rec = {"r_mid": torch.empty((m, 3), dtype=torch.float32, device=self.dev),
@@ -226,12 +300,12 @@ class CasePipeline(CaseTables):
         rec = self._rec
         lib = _lib.load()
         o = 0
-        for e0, m, cap, rows, seg_count in res.slots:
+        for e0, m, cap, rows, seg_count in slots:
             cnt = seg_count.clamp(max=cap)
             off = torch.cumsum(cnt, 0) - cnt + o
             g = _lib.VrGatherArgs()
             g.n, g.rows = m, cap
-            g.n_out = n_rows if len(res.slots) == 1 else int(cnt.sum())  # one thread per written row
+            g.n_out = n_rows if len(slots) == 1 else int(cnt.sum())  # one thread per written row
             g.seg_count, g.row_offset = seg_count.data_ptr(), off.data_ptr()
             g.weight = self._ones[:m].data_ptr()
             g.seg_mid, g.seg_dir = rows["seg_mid"].data_ptr(), rows["seg_dir"].data_ptr()
@@ -244,7 +318,7 @@ class CasePipeline(CaseTables):
                 g.prim, g.row_prim = prim.data_ptr(), rec["prim"].data_ptr()
             _lib.check(lib.hr_vr_gather(C.byref(g), C.c_void_p(torch.cuda.current_stream().cuda_stream)),
                        "hr_vr_gather")
-            o += int(cnt.sum()) if len(res.slots) > 1 else 0
+            o += int(cnt.sum()) if len(slots) > 1 else 0
         keys = ("r_mid", "v_hat", "L", "E") + (("prim",) if first_id is not None else ())
         return {k: rec[k][:n_rows] for k in keys}
 
@@ -292,27 +366,41 @@ class CasePipeline(CaseTables):
         self.score_records(seg, Ne)
         return seg
 
+    def _score_slot(self, e0, m, cap, rows, seg_count, batch_index, n_total):
+        """Every tally of one sub-batch of slot rows (queued on the current stream)."""
+        slot = [(e0, m, cap, rows, seg_count)]
+        n_rows = int(seg_count.clamp(max=cap).sum())
+        nb = int(self.cfg.depth_bins)
+        if nb > 0:
+            from .transport.tally import depth_dose
+
+            depth_dose(self.spec_dose, 0.0, float(self.cfg.thickness_ang) / nb, rows, m, slots=cap,
+                       seg_count=seg_count, scale=1.0 / float(n_total))
+        screen = self.line_screen(n_total)
+        if self.coherent is not None:
+            self.coherent.add_slots(slot, self.cfg.Ne if e0 == 0 else 0)
+            self.brem.spectrum(self.slot_records(slot, n_rows), n_total, self.spec_brem)
+        elif screen is None:
+            self.score_records(self.slot_records(slot, n_rows), n_total)
+        else:
+            seg = self.slot_records(slot, n_rows, first_id=int(batch_index) * self.cfg.Ne)
+            self.score_records(seg, n_total, screen.row_weights(self.lines, seg, seg["prim"], n_total))
+        self._scored()
+        self.segments += n_rows
+
     def step(self, batch_index: int, n_batches_total: int = 1):
         """One batch of ``Ne`` histories through transport and scoring.
 
         Tallies are per incident electron of the whole run when
         ``n_batches_total`` batches are summed.
         """
-        res = self.transport(batch_index, compact=False)
         n_total = self.cfg.Ne * n_batches_total
-        self.score_dose(res, n_total)
-        screen = self.line_screen(n_total)
-        if self.coherent is not None:
-            self.coherent.add_transport(res, self.cfg.Ne)
-            self.brem.spectrum(self.records(res), n_total, self.spec_brem)
-        elif screen is None:
-            self.score_records(self.records(res), n_total)
-        else:
-            seg = self.records(res, first_id=int(batch_index) * self.cfg.Ne)
-            self.score_records(seg, n_total, screen.row_weights(self.lines, seg, seg["prim"], n_total))
-        self._scored()
+
+        def consume(e0, m, cap, rows, seg_count):
+            self._score_slot(e0, m, cap, rows, seg_count, batch_index, n_total)
+
+        res = self.transport(batch_index, compact=False, consume=consume)
         self.histories += self.cfg.Ne
-        self.segments += res.n_segments
         return res
 
     def run_steps(self, first_batch: int, n_steps: int, n_batches_total: int | None = None, overlap: bool = True):
@@ -320,9 +408,9 @@ class CasePipeline(CaseTables):
 
         With ``overlap`` the transport of batch s+1 runs on one HIP stream while
         batch s is scored on another: the transport kernel is latency bound
-        and leaves issue slots that the scoring kernels fill.  Each batch's slot
-        rows are handed to the scoring stream (``record_stream``) and scored in
-        batch order, so tallies are those of ``step()`` run back to back.
+        and leaves issue slots that the scoring kernels fill.  Each sub-batch's
+        slot rows are scored in order on the scoring stream, so tallies are
+        those of ``step()`` run back to back.
         """
         import torch
 
@@ -339,25 +427,19 @@ class CasePipeline(CaseTables):
         sa.wait_stream(main)
         sb.wait_stream(main)
         for b in range(int(n_steps)):
-            with torch.cuda.stream(sa):
-                res = self.transport(int(first_batch) + b, compact=False)
+            batch = int(first_batch) + b
+
+            def consume(e0, m, cap, rows, seg_count, batch=batch):
                 done = torch.cuda.Event()
                 done.record(sa)
-            for _e0, _m, _cap, rows, cnt in res.slots:
-                for t in rows.values():
-                    t.record_stream(sb)
-                cnt.record_stream(sb)
-            with torch.cuda.stream(sb):
-                sb.wait_event(done)
-                self.score_dose(res, n_total)
-                if self.coherent is not None:
-                    self.coherent.add_transport(res, self.cfg.Ne)
-                    self.brem.spectrum(self.records(res), n_total, self.spec_brem)
-                else:
-                    self.score_records(self.records(res), n_total)
-                self._scored()
+                seg_count.record_stream(sb)
+                with torch.cuda.stream(sb):
+                    sb.wait_event(done)
+                    self._score_slot(e0, m, cap, rows, seg_count, batch, n_total)
+
+            with torch.cuda.stream(sa):
+                self.transport(batch, compact=False, consume=consume)
             self.histories += self.cfg.Ne
-            self.segments += res.n_segments
         main.wait_stream(sa)
         main.wait_stream(sb)
 

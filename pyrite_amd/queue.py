@@ -112,10 +112,13 @@ class TransportQueue:
         self.block_size = 256
         self.per_claim = 1
         self.precision = 1 if precision == "mixed" else 0
+        self.counter = torch.zeros(1, dtype=torch.int64, device=device)  # claim counter of eager launches
 
-    def launch_args(self, offsets=None, total=None, tally=None, block_size=None, per_claim=None, accumulate=0):
+    def launch_args(self, offsets=None, total=None, tally=None, block_size=None, per_claim=None, accumulate=0,
+                    counter=None):
         L = _lib.QueueLaunch()
         L.configs = self.configs.data_ptr()
+        L.counter = (self.counter if counter is None else counter).data_ptr()
         L.n_cfg = len(self.cases)
         L.offsets = (self.offsets if offsets is None else offsets).data_ptr()
         L.total = self.total if total is None else int(total)
@@ -136,8 +139,12 @@ class TransportQueue:
         return self.tally
 
     def graph(self, **kw):
-        """A captured hipGraph of the whole campaign (reset + queue kernel)."""
-        return QueueGraph(self, self.launch_args(**kw))
+        """A captured hipGraph of the whole campaign (reset + queue kernel) with its
+        own claim counter, so a replay may run beside an eager launch."""
+        import torch
+
+        counter = torch.zeros(1, dtype=torch.int64, device=self.device)
+        return QueueGraph(self, self.launch_args(counter=counter, **kw), counter)
 
     def sample_offsets(self, fraction, minimum=64):
         """Offsets of the queue's leading ``fraction`` of each configuration."""
@@ -183,9 +190,10 @@ class TransportQueue:
 
 
 class QueueGraph:
-    def __init__(self, queue: TransportQueue, launch):
+    def __init__(self, queue: TransportQueue, launch, counter):
         self.queue = queue
         self.launch_struct = launch
+        self.counter = counter  # the graph's claim counter; held as long as the graph
         self.handle = C.c_void_p()
         _lib.check(queue.lib.hr_queue_graph_create(C.byref(launch), C.byref(self.handle)), "hr_queue_graph_create")
 

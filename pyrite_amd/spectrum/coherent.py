@@ -71,6 +71,7 @@ class CoherentScorer:
         self._largs.r_mid = self._largs.v_hat = self._largs.L = self._largs.E = None
         self._rec = None
         self.lib = _lib.load()
+        self.scratch = _lib.DeviceScratch(device)  # field partials of this scorer's calls
         self.record_events = False
         self.events = []  # ((start, end), cells) per field launch when recording
 
@@ -118,6 +119,7 @@ class CoherentScorer:
         f.rec, f.ld, f.n_lines, f.n_rows = rec.data_ptr(), int(n_lines), int(n_lines), self.n_rows
         f.n_E, f.E_grid, f.dom = self.n_E, self.lines.E_grid_d.data_ptr(), self.dom.data_ptr()
         f.field = self.field.data_ptr()
+        f.ws = self.scratch.get(_lib.workspace_bytes("hr_coherent_field_workspace_bytes", C.byref(f)))
         ev = None
         if self.record_events:
             import torch
@@ -164,8 +166,13 @@ class CoherentScorer:
 
     def add_transport(self, res, n_electrons):
         """Fields of a ``compact=False`` transport result's slot rows."""
-        for _e0, m, cap, rows, seg_count in res.slots:
-            self.add_rows(rows, seg_count, m, cap)
+        self.add_slots(res.slots, n_electrons)
+
+    def add_slots(self, slots, n_electrons, live_counts=None):
+        """Fields of slot rows ``[(e0, m, cap, rows, seg_count)]``; ``n_electrons``
+        histories join the count the spectrum is divided by."""
+        for _e0, m, cap, rows, seg_count in slots:
+            self.add_rows(rows, seg_count, m, cap, live_counts=live_counts)
         self.n_electrons += int(n_electrons)
 
     def spectrum(self, Ne=None, out=None):

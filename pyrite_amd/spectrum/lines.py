@@ -192,6 +192,7 @@ class LineScorer:
         self.e_s = torch.as_tensor(es, device=device)
         self.e_p = torch.as_tensor(ep, device=device)
         self.lib = _lib.load()
+        self.scratch = _lib.DeviceScratch(device)  # caller-owned workspace of this scorer's calls
         self.record_events = False
         self.events = []  # ((start, end), cells) per reduction launch when recording (nothing else
         #                   is computed, so recording does not disturb the timed work)
@@ -252,6 +253,7 @@ class LineScorer:
         args.uniform = int(self.uniform)
         args.E0, args.dE = self.E0, self.dE
         args.out = out.data_ptr()
+        args.ws = self.scratch.get(_lib.workspace_bytes("hr_sincsq_workspace_bytes", C.byref(args)))
         ev = None
         if self.record_events:
             import torch
@@ -294,6 +296,7 @@ class LineScorer:
         wa.lines = C.pointer(largs)
         wa.n_E, wa.E0, wa.dE, wa.x_c = int(self.E_grid.size), self.E0, self.dE, float(x_c)
         wa.out = out.data_ptr()
+        wa.ws = self.scratch.get(_lib.workspace_bytes("hr_window_spectra_workspace_bytes", C.byref(wa)))
         _lib.check(self.lib.hr_window_spectra(C.byref(wa), _stream()), "hr_window_spectra")
         return out
 
@@ -328,6 +331,7 @@ class BremScorer:
         self.n_hat = np.asarray(n_hat, dtype=float)
         self.device = device
         self.lib = _lib.load()
+        self.scratch = _lib.DeviceScratch(device)
 
     def reduce(self, seg, out, row_weight=None):
         args = _lib.BremArgs()
@@ -345,6 +349,7 @@ class BremScorer:
         args.mu = self.mu_d.data_ptr()
         args.row_weight = None if row_weight is None else row_weight.data_ptr()
         args.out = out.data_ptr()
+        args.ws = self.scratch.get(_lib.workspace_bytes("hr_brem_workspace_bytes", C.byref(args)))
         _lib.check(self.lib.hr_brem_reduce(C.byref(args), _stream()), "hr_brem_reduce")
 
     def spectrum(self, seg, Ne, out=None, row_weight=None):

@@ -156,6 +156,7 @@ def transport_args(tables: DeviceTables, control: TransportControl, run, state, 
     a.E_keV, a.E_cut, a.stragg_dE = _ptr(E_keV), _ptr(E_cut), _ptr(stragg_dE)
     for name in ROW_FIELDS:
         setattr(a, name, _ptr(scratch.get(name)))
+    a.work = _ptr(scratch.get("work"))  # the launch's claim counter (driver-owned scratch)
     a.seg_count, a.exit_code = _ptr(seg_count), _ptr(exit_code)
     a.block_size = int(block_size)
     if control.precision not in ("fp64", "mixed"):
@@ -171,13 +172,22 @@ def make_hip_transport_core(tables: DeviceTables, control: TransportControl):
 
     ``run = (e_start, e_count, cap, keys)``; ``state = (alive, clock, pos,
     dirs, E_keV, E_cut)`` device tensors indexed by global electron id;
-    ``scratch`` a dict of row tensors sized ``e_count*cap`` (``ROW_FIELDS``);
+    ``scratch`` a dict of row tensors sized ``e_count*cap`` (``ROW_FIELDS``)
+    plus ``"work"``, the launch's 8-byte claim counter (the driver owns every
+    buffer; the core allocates nothing);
     ``out = (seg_count, exit_code)``; ``straggling = (straggle_on, stragg_dE)``.
     """
     lib = _lib.load()
+    own = {}
 
     def core(run, state, scratch, out, straggling, stream=None, block_size=0):
         a = transport_args(tables, control, run, state, scratch, out, straggling, block_size)
+        if not a.work:  # a scratch dict without a counter: the core's own (calls of one core run in order)
+            if "work" not in own:
+                import torch
+
+                own["work"] = torch.zeros(1, dtype=torch.int64, device=tables.total_rate.device)
+            a.work = own["work"].data_ptr()
         _lib.check(lib.hr_transport_run(C.byref(a), C.c_void_p(stream)), "hr_transport_run")
 
     return core
@@ -208,20 +218,75 @@ def alloc_rows(m: int, cap: int, midpoint: bool, device):
     return rows
 
 
+_COLUMN_SPEC = (  # (name, dtype name, width) in the reference's column dtypes (batching.py:398-425)
+    ("seg_dir", "float64", 3), ("seg_mid", "float64", 3), ("seg_len", "float64", 1), ("seg_E", "float64", 1),
+    ("seg_t0", "float64", 1), ("seg_id", "int64", 1), ("seg_lay", "int16", 1),
+)
+_MIDPOINT_SPEC = (
+    ("seg_E_end", "float64", 1), ("seg_t_end", "float64", 1), ("seg_flight", "int64", 1),
+    ("seg_substep", "int64", 1), ("seg_event", "int8", 1),
+)
+_ALIGN = 256
+# alignment padding an arena reserves on top of the grid bytes (one pad per column)
+ARENA_SLACK = _ALIGN * (len(_COLUMN_SPEC) + len(_MIDPOINT_SPEC))
+
+
+class RowArena:
+    """One flat device byte buffer that every slot-row grid of a driver is carved from.
+
+    The grid of a sub-batch, ``(m, cap)`` slots in the reference's column
+    dtypes, is a set of views into the buffer (columns 256-byte aligned), so
+    a driver that owns two arenas and uses them in turn holds exactly their
+    bytes however long its histories grow: a longer capacity means fewer
+    electrons per sub-batch, never a larger allocation.  ``budget_bytes`` is
+    what ``_batch_electrons`` may plan against."""
+
+    def __init__(self, nbytes: int, device):
+        import torch
+
+        self.nbytes = int(nbytes)
+        self.buf = torch.empty(self.nbytes, dtype=torch.uint8, device=device)
+
+    @property
+    def budget_bytes(self) -> int:
+        return self.nbytes - ARENA_SLACK
+
+    def rows(self, m: int, cap: int, midpoint: bool):
+        import torch
+
+        n = int(m) * int(cap)
+        spec = _COLUMN_SPEC + (_MIDPOINT_SPEC if midpoint else ())
+        out, off = {}, 0
+        for name, dt, width in spec:
+            dtype = getattr(torch, dt)
+            size = torch.empty(0, dtype=dtype).element_size()
+            nb = n * width * size
+            if off + nb > self.nbytes:
+                raise ValueError(f"row grid of {m} x {cap} slots does not fit a {self.nbytes}-byte arena")
+            col = self.buf[off:off + nb].view(dtype)
+            out[name] = col.view(n, width) if width > 1 else col
+            off += (nb + _ALIGN - 1) // _ALIGN * _ALIGN
+        return out
+
+
 def _capacity_for(seen_max, config):
     """``batching.py:462-469``: never below ``seen_max``."""
     return max(1, seen_max, int(np.ceil(seen_max * config.capacity_headroom)))
 
 
-def _batch_electrons(e, cap, Ne, config, row_bytes):
+def _batch_electrons(e, cap, Ne, config, row_bytes, budget=None):
     """``batching.py:431-480``: electrons whose ``(batch, cap)`` grid fits the budget."""
     per = cap * int(row_bytes) + _STATUS_BYTES
-    n = int(config.scratch_budget_bytes // per)
+    n = int((config.scratch_budget_bytes if budget is None else budget) // per)
     limit = (1 << 62) if config.max_batch is None else int(config.max_batch)
     n = min(max(1, min(n, limit)), Ne - e)
     if e == 0 and config.probe_electrons > 0:
         n = min(n, int(config.probe_electrons))
     return n
+
+
+# Smallest scratch budget an out-of-memory replay halves down to.
+MIN_SCRATCH_BUDGET = 64 << 20
 
 
 @dataclass
@@ -234,24 +299,38 @@ class DriveResult:
     replays: int
     slots: list = None  # compact=False: [(e0, m, cap, slot rows, seg_count)] per sub-batch
     capacity: int = 0  # capacity learned from the largest count seen (a hint for the next call)
+    oom_replays: int = 0  # sub-batches replayed at half the scratch budget after a device OOM
 
 
 def drive_batches(core, keys, state, stragg_dE, control: TransportControl, Ne: int, device,
                   config: PerElectronTransportConfig = DEFAULT_CONFIG, stream=None, compact=True,
-                  cap_hint=None, alloc=None) -> DriveResult:
+                  cap_hint=None, alloc=None, consume=None) -> DriveResult:
     """Capacity-replayed batches (``batching.py:477-628``), rows kept on device.
 
     ``compact=False`` skips the electron-major compaction and returns the slot
     rows of each sub-batch (a scorer can gather them itself); ``cap_hint``
     starts from a capacity learned by an earlier call; ``alloc(e, m, cap,
     midpoint, device)`` (default: fresh tensors) supplies the slot rows of the
-    sub-batch starting at electron ``e``, e.g. from a persistent pool."""
+    sub-batch starting at electron ``e``, e.g. from a persistent arena.
+    ``consume(e, m, cap, rows, seg_count)`` (with ``compact=False``) takes each
+    sub-batch as soon as it is transported, before the next one is allocated,
+    and the result then holds no slot rows: a driver that recycles two arenas
+    keeps its memory bounded whatever the batch size and capacity.
+
+    A device out-of-memory error while a sub-batch is allocated or launched
+    (``backend.is_oom_error``) is replayed at half the scratch budget, the
+    reference's OOM replay (``_backend.py:179-205``); below
+    ``MIN_SCRATCH_BUDGET`` it is raised."""
     import torch
+
+    from ..backend import is_oom_error
 
     alive, clock, pos, dirs, E_keV, E_cut = state
     midpoint = control.energy_model == 1
     row_bytes = _ROW_BYTES_MIDPOINT if midpoint else _ROW_BYTES_FROZEN
     cap = max(1, int(config.seg_capacity if cap_hint is None else cap_hint))
+    budget = int(config.scratch_budget_bytes)
+    work = torch.zeros(1, dtype=torch.int64, device=device)  # the core's claim counter (this call's own)
     seen_max = 0
     e = 0
     batches = []
@@ -259,32 +338,53 @@ def drive_batches(core, keys, state, stragg_dE, control: TransportControl, Ne: i
     counts_all = []
     exit_all = []
     replays = 0
+    oom_replays = 0
     while e < Ne:
-        m = _batch_electrons(e, cap, Ne, config, row_bytes)
+        m = _batch_electrons(e, cap, Ne, config, row_bytes, budget)
         sl = slice(e, e + m)
         snap = (pos[sl].clone(), dirs[sl].clone(), E_keV[sl].clone(), clock[sl].clone())
+        rows = None
         while True:
-            rows = alloc_rows(m, cap, midpoint, device) if alloc is None else alloc(e, m, cap, midpoint, device)
-            seg_count = torch.zeros(m, dtype=torch.int64, device=device)
-            exit_code = torch.zeros(m, dtype=torch.int8, device=device)
-            core((e, m, cap, keys), state, rows, (seg_count, exit_code),
-                 (control.straggle_on, stragg_dE), stream=stream, block_size=config.kernel_threads)
-            needed = int(seg_count.max().item())  # the one device sync per batch
+            rows = None  # drop a replayed grid before its successor is allocated
+            try:
+                rows = alloc_rows(m, cap, midpoint, device) if alloc is None else alloc(e, m, cap, midpoint, device)
+                rows.setdefault("work", work)
+                seg_count = torch.zeros(m, dtype=torch.int64, device=device)
+                exit_code = torch.zeros(m, dtype=torch.int8, device=device)
+                core((e, m, cap, keys), state, rows, (seg_count, exit_code),
+                     (control.straggle_on, stragg_dE), stream=stream, block_size=config.kernel_threads)
+                needed = int(seg_count.max().item())  # the one device sync per batch
+            except Exception as exc:  # noqa: BLE001 - only OOM is replayed
+                if not is_oom_error(exc) or budget // 2 < MIN_SCRATCH_BUDGET:
+                    raise
+                rows = None
+                if torch.cuda.is_available():
+                    torch.cuda.empty_cache()
+                budget //= 2
+                oom_replays += 1
+                pos[sl], dirs[sl], E_keV[sl], clock[sl] = snap
+                m = _batch_electrons(e, cap, Ne, config, row_bytes, budget)
+                sl = slice(e, e + m)
+                snap = tuple(x[:m] for x in snap)
+                continue
             seen_max = max(seen_max, needed)
             if needed <= cap:
                 break
             replays += 1
             cap = _capacity_for(seen_max, config)
             pos[sl], dirs[sl], E_keV[sl], clock[sl] = snap
-            m = _batch_electrons(e, cap, Ne, config, row_bytes)
+            m = _batch_electrons(e, cap, Ne, config, row_bytes, budget)
             sl = slice(e, e + m)
             snap = tuple(x[:m] for x in snap)
         if compact:
             keep = (torch.arange(cap, device=device)[None, :] < seg_count[:, None]).reshape(-1)
             idx = torch.nonzero(keep).squeeze(1)  # one stream compaction shared by every column
-            batches.append({k: v.index_select(0, idx) for k, v in rows.items()})
+            batches.append({k: v.index_select(0, idx) for k, v in rows.items() if k != "work"})
+        elif consume is not None:
+            consume(e, m, cap, rows, seg_count)
         else:
             slots.append((e, m, cap, rows, seg_count))
+        rows = None
         counts_all.append(seg_count)
         exit_all.append(exit_code)
         e += m
@@ -310,4 +410,5 @@ def drive_batches(core, keys, state, stragg_dE, control: TransportControl, Ne: i
         "n_not_entered": int(codes[EXIT_NOT_ENTERED]),
     }
     n_seg = int(joined["seg_len"].shape[0]) if compact else int(seg_count.sum())
-    return DriveResult(joined, n_seg, counts, seg_count, exit_code, replays, slots, cap)
+    return DriveResult(joined, n_seg, counts, seg_count, exit_code, replays,
+                       None if (consume is not None and not compact) else slots, cap, oom_replays)

@@ -65,6 +65,7 @@ STRUCTS = {
     "hr_queue_launch": "QueueLaunch",
     "hr_continuation": "Continuation",
     "hr_dose_args": "DoseArgs",
+    "hr_workspace": "Workspace",
 }
 
 

@@ -34,7 +34,7 @@ def test_null_and_negative_arguments_are_invalid(lib):
     assert lib.hr_stream_keys(1, 0, -5, None, None) == HR_ERR_INVALID
     assert lib.hr_roulette_uniforms(7, 0, -1, None, None) == HR_ERR_INVALID
     assert lib.hr_sincsq_reduce(None, None) == HR_ERR_INVALID
-    assert lib.hr_window_reduce(None, None, None, -1, 16, 0.0, 1.0, 100.0, None, None) == HR_ERR_INVALID
+    assert lib.hr_window_reduce(None, None, None, -1, 16, 0.0, 1.0, 100.0, None, _lib.Workspace(), None) == HR_ERR_INVALID
     assert lib.hr_vr_gather(None, None) == HR_ERR_INVALID
 
 

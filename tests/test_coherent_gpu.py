@@ -138,6 +138,7 @@ def test_worked_two_energy_block_on_the_device(device):
     f = _lib.CoherentFieldArgs()
     f.rec, f.ld, f.n_lines, f.n_rows, f.n_E = rec_d.data_ptr(), 2, 2, 1, 2
     f.E_grid, f.dom, f.field = E.data_ptr(), dom.data_ptr(), field.data_ptr()
+    f.ws = _lib.DeviceScratch(dev).get(_lib.workspace_bytes("hr_coherent_field_workspace_bytes", C.byref(f)))
     _lib.call("hr_coherent_field", C.byref(f), None)
     out = torch.zeros(2, dtype=torch.float64, device=dev)
     one = torch.ones(1, dtype=torch.float64, device=dev)

@@ -116,6 +116,16 @@ def test_sincsq_worked_block_on_device(device):
     args.n_lines, args.E_r, args.a, args.w = n, E_r.data_ptr(), a.data_ptr(), w.data_ptr()
     args.n_E, args.E_grid, args.uniform, args.E0, args.dE = 3, None, 1, 0.0, 1.0
     args.out = out.data_ptr()
+    need = _lib.workspace_bytes("hr_sincsq_workspace_bytes", C.byref(args))
+    assert need > 0
+    scratch = _lib.DeviceScratch(device)
+    # a workspace smaller than the query's answer is refused before anything launches
+    small = scratch.get(need)
+    small.bytes = need - 8
+    args.ws = small
+    assert _lib.load().hr_sincsq_reduce(C.byref(args), None) == _lib.HR_ERR_INVALID
+    assert "workspace" in _lib.load().hr_last_error().decode()
+    args.ws = scratch.get(need)
     _lib.check(_lib.load().hr_sincsq_reduce(C.byref(args), None), "hr_sincsq_reduce")
     pi2 = math.pi**2
     np.testing.assert_allclose(out.cpu().numpy(), [5 + 8 / pi2, 2 + 16 / pi2, 3 + 8 / pi2], rtol=1e-7)

@@ -42,8 +42,9 @@ def test_window_reduce_matches_oracle(oracle, device, x_c):
     want = oracle.window_reduce(E_r, a, w, E0, dE, n, x_c)
     t = [torch.as_tensor(v, device=device) for v in (E_r, a, w)]
     out = torch.zeros(n, dtype=torch.float64, device=device)
+    ws = _lib.DeviceScratch(device).get(_lib.workspace_bytes("hr_window_reduce_workspace_bytes", n_lines, n))
     _lib.call("hr_window_reduce", t[0].data_ptr(), t[1].data_ptr(), t[2].data_ptr(), n_lines, n, E0, dE, x_c,
-              out.data_ptr(), _stream())
+              out.data_ptr(), ws, _stream())
     _close(out.cpu().numpy(), want)
 
 
