@@ -22,33 +22,37 @@ from __future__ import annotations
 
 import ctypes as C
 from collections.abc import Mapping
-from dataclasses import asdict, dataclass
+from dataclasses import dataclass
 from typing import Any
 
 import numpy as np
 
 from . import _lib
 
-#: Versioned estimator: pre-screen, strata and draw stream of this policy.
+# Interface string and field names are the reference policy's
+# (src/pyrite/_line_roulette.py:29-39), so a saved case selects the same
+# estimator; everything below them is this backend's own.
 LINE_ROULETTE_MODEL = "exact-line-mass-two-stratum-v1"
-
-#: Case keys whose line scoring is not an additive per-history incoherent sum.
-_REFUSED_KEYS = {
-    "coherent_emission": "coherent emission squares a sum of amplitudes",
-    "temporal_profile": "temporal profiles accumulate unweighted line boxes",
-    "channeling": "channeling joins zones across a history",
-    "max_dE_frac": "substeps of a flight add coherently, so a row has no separate line mass",
-}
+_FIELDS = ("floor", "keep_fraction", "pilot_histories", "model")
 
 
-def _unit_interval(name, value):
-    if isinstance(value, bool) or not isinstance(value, int | float) or not 0 < value <= 1:
-        raise ValueError(f"line roulette {name} must lie in (0, 1]")
+def _check_fields(floor, keep_fraction, pilot_histories, model):
+    """Every reason the four fields do not form a usable policy (empty if none)."""
+    bad = []
+    for name, x in (("floor", floor), ("keep_fraction", keep_fraction)):
+        if type(x) not in (int, float) or not (0.0 < float(x) <= 1.0):
+            bad.append(f"{name}={x!r} is not a probability in (0, 1]")
+    if type(pilot_histories) is not int or pilot_histories <= 0:
+        bad.append(f"pilot_histories={pilot_histories!r} is not a positive int")
+    if model != LINE_ROULETTE_MODEL:
+        bad.append(f"model={model!r} is not {LINE_ROULETTE_MODEL!r}")
+    return bad
 
 
 @dataclass(frozen=True, slots=True, kw_only=True)
 class LineRoulette:
-    """Keep low-mass histories with probability ``floor`` and weight ``1/floor``."""
+    """Per-primary line roulette: below the pilot threshold a primary is
+    scored with probability ``floor`` at weight ``1/floor``."""
 
     floor: float
     keep_fraction: float = 0.1
@@ -56,33 +60,37 @@ class LineRoulette:
     model: str = LINE_ROULETTE_MODEL
 
     def __post_init__(self) -> None:
-        if self.model != LINE_ROULETTE_MODEL:
-            raise ValueError(f"unknown line roulette model {self.model!r}")
-        _unit_interval("floor", self.floor)
-        _unit_interval("keep_fraction", self.keep_fraction)
-        pilot = self.pilot_histories
-        if isinstance(pilot, bool) or not isinstance(pilot, int) or pilot < 1:
-            raise ValueError("line roulette pilot_histories must be a positive integer")
+        bad = _check_fields(self.floor, self.keep_fraction, self.pilot_histories, self.model)
+        if bad:
+            raise ValueError("line roulette policy: " + "; ".join(bad))
 
     def to_dict(self) -> dict[str, Any]:
-        return asdict(self)
+        return {f: getattr(self, f) for f in _FIELDS}
 
     @classmethod
     def from_dict(cls, payload: Mapping[str, Any]) -> LineRoulette:
+        """Policy from a case's ``line_roulette`` mapping (the four fields only)."""
         if not isinstance(payload, Mapping):
-            raise TypeError("line_roulette must be a mapping")
-        unknown = sorted(set(payload) - set(cls.__dataclass_fields__))
-        if unknown:
-            raise ValueError(f"unknown line roulette field(s) {unknown}")
-        return cls(**payload)
+            raise TypeError(f"line roulette policy must be a mapping, not {type(payload).__name__}")
+        extra = set(payload).difference(_FIELDS)
+        if extra:
+            raise ValueError(f"line roulette policy: fields {sorted(extra)} are not part of it")
+        return cls(**dict(payload))
 
-    def validate_case(self, case: Mapping[str, Any]) -> None:
-        """Refuse routes whose line tally is not a sum over primary histories."""
-        for key, reason in _REFUSED_KEYS.items():
-            if case.get(key):
-                raise ValueError(f"line roulette does not support {key}: {reason}")
-        if (case.get("transition_radiation") or {}).get("model") == "two-beam":
-            raise ValueError("line roulette does not support two-beam line scoring")
+    def validate_case(self, cfg, max_dE_frac: float = 0.0, route: str = "batched") -> None:
+        """Refuse a hip-backend case whose line tally is not a sum of
+        independent per-primary terms, which the two-stratum estimator needs:
+        the coherent route squares a field summed over rows (``cfg.coherent``),
+        the streamed route scores a primary's rows over several rounds before
+        its whole line mass is known, and sub-stepped flights
+        (``max_dE_frac > 0``) split one flight into rows that the screen would
+        weigh apart."""
+        if getattr(cfg, "coherent", False):
+            raise ValueError("line roulette needs incoherent line scoring; this case sets coherent=True")
+        if route != "batched":
+            raise ValueError(f"line roulette needs whole primaries per block; route {route!r} splits them")
+        if max_dE_frac > 0.0:
+            raise ValueError(f"line roulette needs one row per flight; max_dE_frac={max_dE_frac} sub-steps flights")
 
 
 def roulette_uniforms(seed: int, first_id: int, n: int, device):

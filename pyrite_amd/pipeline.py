@@ -128,8 +128,11 @@ class CasePipeline(CaseTables):
         # is 23.6 GB), so capacity replays are rare, and a batch whose grid
         # outgrows an arena is cut into sub-batches, each scored before the
         # arena it sits in is refilled.  Every live pipeline's arenas together
-        # stay under ROW_ARENA_DEVICE_FRACTION of the device's memory.  No
-        # field of the config changes results.
+        # stay under ROW_ARENA_DEVICE_FRACTION of the device's memory.  The
+        # arena size changes no transport row; it can move where sub-batches
+        # split a batch, and with them the FP32 line groups of the tallies
+        # (last-bit differences), so bitwise comparisons pass equal
+        # ``arena_bytes``.
         total = int(torch.cuda.get_device_properties(dev).total_memory)
         room = int(ROW_ARENA_DEVICE_FRACTION * total) - _ARENA_LIVE["bytes"]
         want = ROW_ARENA_MAX_BYTES if arena_bytes is None else int(arena_bytes)
@@ -354,7 +357,7 @@ class CasePipeline(CaseTables):
         scr = getattr(self, "_screen", None)
         if scr is None or scr.n != int(n_electrons):
             policy = LineRoulette.from_dict(self.cfg.line_roulette)
-            policy.validate_case({"max_dE_frac": self.control.max_dE_frac})
+            policy.validate_case(self.cfg, self.control.max_dE_frac)
             scr = self._screen = LineScreen(policy, self.cfg.seed, n_electrons, self.dev)
         return scr
 
@@ -463,8 +466,9 @@ class CasePipeline(CaseTables):
 
         cfg = self.cfg
         if cfg.line_roulette is not None:
-            raise NotImplementedError("line roulette screens whole primaries; the streamed route splits them "
-                                      "over rounds (use step())")
+            from .line_roulette import LineRoulette
+
+            LineRoulette.from_dict(cfg.line_roulette).validate_case(cfg, self.control.max_dE_frac, route="streamed")
         n_total = int(n_histories if n_total is None else n_total)
         lanes = max(1, min(int(lanes), int(n_histories)))
         drv = getattr(self, "_stream_drv", None)

@@ -23,6 +23,16 @@ def gpu_available() -> bool:
         return False
 
 
+@pytest.fixture(autouse=True)
+def _collect_pipelines():
+    """Collect each test's garbage when it ends, so the slot-row arenas of its
+    pipelines return to the device-wide arena budget before the next test."""
+    yield
+    import gc
+
+    gc.collect()
+
+
 @pytest.fixture(scope="session")
 def oracle():
     from oracle import oracle as o

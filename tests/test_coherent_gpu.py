@@ -220,10 +220,11 @@ def test_overlapped_steps_give_the_sequential_fields(device):
     from pyrite_amd.pipeline import CaseConfig, CasePipeline
 
     cfg = CaseConfig(Ne=3000, coherent=True, E0_keV=3000.0, thickness_ang=3.0e5)
-    a = CasePipeline(cfg)
+    # equal arenas: equal sub-batches, so equal line groups in the FP32 field cells
+    a = CasePipeline(cfg, arena_bytes=2 << 30)
     for b in range(3):
         a.step(b, 3)
-    b_ = CasePipeline(cfg)
+    b_ = CasePipeline(cfg, arena_bytes=2 << 30)
     b_.run_steps(0, 3, overlap=True)
     np.testing.assert_array_equal(b_.coherent.field.cpu().numpy(), a.coherent.field.cpu().numpy())
     la, ba = a.spectra()

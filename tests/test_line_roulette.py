@@ -43,13 +43,17 @@ def test_policy_round_trips_and_refuses_bad_fields():
         LineRoulette.from_dict([("floor", 0.5)])
 
 
-@pytest.mark.parametrize("key", ["coherent_emission", "temporal_profile", "channeling", "max_dE_frac"])
-def test_policy_refuses_routes_that_are_not_per_history_sums(key):
-    with pytest.raises(ValueError, match=key):
-        LineRoulette(floor=0.5).validate_case({key: 0.05 if key == "max_dE_frac" else True})
-    LineRoulette(floor=0.5).validate_case({key: 0.0 if key == "max_dE_frac" else False})
-    with pytest.raises(ValueError, match="two-beam"):
-        LineRoulette(floor=0.5).validate_case({"transition_radiation": {"model": "two-beam"}})
+def test_policy_refuses_routes_that_are_not_per_history_sums():
+    from pyrite_amd.pipeline import CaseConfig
+
+    plain = CaseConfig(line_roulette=POLICY.to_dict())
+    POLICY.validate_case(plain, 0.0)
+    with pytest.raises(ValueError, match="coherent"):
+        POLICY.validate_case(CaseConfig(coherent=True), 0.0)
+    with pytest.raises(ValueError, match="streamed"):
+        POLICY.validate_case(plain, 0.0, route="streamed")
+    with pytest.raises(ValueError, match="max_dE_frac"):
+        POLICY.validate_case(plain, 0.05)
 
 
 def _masses(n=400, seed=3):

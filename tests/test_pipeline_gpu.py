@@ -12,11 +12,12 @@ def test_overlapped_steps_equal_sequential_steps(device, route):
     from pyrite_amd.pipeline import CaseConfig, CasePipeline
 
     cfg = CaseConfig(Ne=6000, line_route=route, E0_keV=3000.0, thickness_ang=3.0e5)
-    a = CasePipeline(cfg)
+    # equal arenas: equal sub-batches, so equal line groups in the FP32 cells
+    a = CasePipeline(cfg, arena_bytes=2 << 30)
     for b in range(4):
         a.step(b, 4)
     la, ba = a.spectra()
-    p = CasePipeline(cfg)
+    p = CasePipeline(cfg, arena_bytes=2 << 30)
     p.run_steps(0, 4, overlap=True)
     lb, bb = p.spectra()
     assert p.histories == a.histories and p.segments == a.segments
