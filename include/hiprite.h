@@ -241,6 +241,19 @@ hr_status hr_sincsq_reduce(const hr_sincsq_args *args, void *stream);
 /* Workspace bytes hr_sincsq_reduce needs for these sizes on the current device. */
 hr_status hr_sincsq_workspace_bytes(const hr_sincsq_args *args, size_t *bytes);
 
+/* hr_sincsq_reduce without a workspace and without any allocation: the
+ * partial spectra go to a static device array of
+ * HR_SINCSQ_STATIC_SCRATCH_BYTES that each device holds until the process
+ * ends (args->ws is ignored).  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops).  Sizes whose partials exceed the array are refused
+ * with HR_ERR_INVALID (at most about 1M bins on a 256-CU device).  Every call
+ * on one device shares the array: calls that overlap on one device (two
+ * streams, or two host threads) race.  Results equal hr_sincsq_reduce's bit
+ * for bit. */
+#define HR_SINCSQ_STATIC_SCRATCH_BYTES (32u << 20)
+hr_status hr_sincsq_reduce_static(const hr_sincsq_args *args, void *stream);
+
 /* ------------------------------------------------------------------------
  * Bethe-Heitler + Elwert bremsstrahlung track-length reduction:
  * out[k] += sum_seg sum_el n_el L dsigma/dk(T, k) * <exp(-mu_k path)>.

@@ -364,6 +364,7 @@ SIGNATURES = {
     "hr_cxr_lines": (C.c_int32, [C.POINTER(LineArgs), _dp]),
     "hr_sincsq_reduce": (C.c_int32, [C.POINTER(SincsqArgs), _dp]),
     "hr_sincsq_workspace_bytes": (C.c_int32, [C.POINTER(SincsqArgs), C.POINTER(C.c_size_t)]),
+    "hr_sincsq_reduce_static": (C.c_int32, [C.POINTER(SincsqArgs), _dp]),
     "hr_brem_reduce": (C.c_int32, [C.POINTER(BremArgs), _dp]),
     "hr_brem_workspace_bytes": (C.c_int32, [C.POINTER(BremArgs), C.POINTER(C.c_size_t)]),
     "hr_mlp_param_count": (C.c_int32, []),

@@ -19,6 +19,8 @@
 // reproducible for a fixed launch shape.
 #include <math.h>
 
+#include <mutex>
+
 #include "hr_common.hpp"
 #include "line_core.hpp"
 
@@ -713,6 +715,55 @@ extern "C" hr_status hr_sincsq_reduce(const hr_sincsq_args *args, void *stream) 
     }
     HR_CHECK_HIP(hipGetLastError());
     return reduce_partials(ws, gx, p.n_E, p.out, s);
+}
+
+// Scratch of hr_sincsq_reduce_static: the partial spectra of one call live in
+// this file-scope device array instead of a caller workspace.  Each device
+// holds its own copy at its own address, found once per device with
+// hipGetSymbolAddress; host code uses that address, never the symbol itself.
+static __device__ __align__(256) unsigned char scratch[HR_SINCSQ_STATIC_SCRATCH_BYTES];
+
+namespace hr {
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[64];
+
+hr_status static_scratch(void **out) {
+    int dev = 0;
+    HR_CHECK_HIP(hipGetDevice(&dev));
+    HR_REQUIRE(dev >= 0 && dev < 64, "device index %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *p = nullptr;
+        HR_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        HR_REQUIRE(p != nullptr && ((uintptr_t)p & 255u) == 0, "static scratch at %p is null or misaligned", p);
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return HR_OK;
+}
+}  // namespace
+}  // namespace hr
+
+extern "C" hr_status hr_sincsq_reduce_static(const hr_sincsq_args *args, void *stream) {
+    using namespace hr;
+    HR_REQUIRE(args != nullptr, "hr_sincsq_reduce_static: null args");
+    HR_REQUIRE(args->n_lines >= 0 && args->n_E >= 0, "hr_sincsq_reduce_static: negative size");
+    if (args->n_lines == 0 || args->n_E == 0) return HR_OK;
+    DensePlan P;
+    hr_status st = dense_plan(args->n_lines, kLineTile, args->n_E, P);
+    if (st != HR_OK) return st;
+    HR_REQUIRE(P.bytes <= (size_t)HR_SINCSQ_STATIC_SCRATCH_BYTES,
+               "hr_sincsq_reduce_static: %lld bins need %zu bytes of partials, more than the %zu-byte static "
+               "scratch (use hr_sincsq_reduce with a workspace)",
+               (long long)args->n_E, P.bytes, (size_t)HR_SINCSQ_STATIC_SCRATCH_BYTES);
+    void *ptr = nullptr;
+    st = static_scratch(&ptr);
+    if (st != HR_OK) return st;
+    hr_sincsq_args q = *args;
+    q.ws.ptr = ptr;
+    q.ws.bytes = (size_t)HR_SINCSQ_STATIC_SCRATCH_BYTES;
+    return hr_sincsq_reduce(&q, stream);
 }
 
 extern "C" hr_status hr_sincsq_workspace_bytes(const hr_sincsq_args *args, size_t *bytes) {
