@@ -80,6 +80,81 @@ hr_status hr_stream_keys(uint64_t seed, int64_t start, int64_t n, uint64_t *keys
  * reset by the call itself with a kernel): calls that may run at the same time
  * need different counters.  The call allocates nothing.
  * ---------------------------------------------------------------------- */
+/* ------------------------------------------------------------------------
+ * Hard-event modes of the transport step (both switches 0: the plain step).
+ *
+ * rad_on -- coupled radiative mode (montecarlo/transport/_jit_kernel.py:253-378,
+ * 554-558, 866-905; cores.py:465-480, 880-906; device helpers
+ * _jit_radiative_device.py:40-481): bremsstrahlung below rad_cutoff_eV is a
+ * continuous loss (lut_rad_soft, subtracted from dEds and added to each
+ * straggled row as its mean); photons above it are explicit events on their
+ * own optical depth and key (rad_key = stream keys of seed ^
+ * 0xA54FF53A5F1D36F1, _jit_radiative.py:17-22), scheduled at the row-start
+ * rate (lut_rad_rate) and sampled, element and photon energy, from the exact
+ * per-element moments at the row-end energy; the electron keeps its direction.
+ * Row event 9 (HARD_RADIATIVE), photon energy seg_rad_k [eV] and emitting Z
+ * seg_rad_Z (0 on other rows).
+ *
+ * shell_on -- shell soft/hard inelastic mode (cores.py:456-463, 616-650,
+ * 850-880; hard_inelastic.py:120-554): lut_dEds holds the soft stopping,
+ * soft losses of a straggled row come from the two-moment sampler (mean
+ * -dEds s, variance lut_sh_omega2 s) on the straggling key domain, and hard
+ * collisions are explicit events on their own optical depth and key
+ * (splitmix64(stream_key ^ 0x3C6EF372FE94F82B), hard_inelastic.py:40,77-80):
+ * channel by cumulative rate at the row start, loss by inverting the
+ * channel's restricted CDF, primary deflection from its recoil kinematics.
+ * Row event 8 (HARD_INELASTIC), loss seg_hard_W [keV] and channel code
+ * seg_hard_ch (-1 on other rows).
+ *
+ * Both modes need the midpoint energy model.  Tables: radiative slot
+ * s = layer*max_el + i (the layer's element order); shell channel c of layer
+ * L at L*sh_max_ch + c; LUT columns on the transport LUT's ln-E grid.
+ * ---------------------------------------------------------------------- */
+typedef struct hr_hard_events {
+    int32_t rad_on, shell_on;
+    /* radiative */
+    const uint64_t *rad_key;     /* [Ne] by global electron id */
+    const double *lut_rad_soft;  /* [n_layers, n_E] keV/A */
+    const double *lut_rad_rate;  /* [n_layers, n_E] 1/A */
+    const double *rad_incident;  /* [slots, rad_max_t] keV */
+    const double *rad_nominal;   /* [slots, rad_n_red] reduced photon energies */
+    const double *rad_top;       /* [slots, rad_max_t] tip of each incident row */
+    const double *rad_chi;       /* [slots, rad_max_t, rad_n_red] scaled SDCS, mb */
+    const double *rad_z;         /* [slots] */
+    const double *rad_ncm3;      /* [slots] atoms/cm^3 */
+    const int32_t *rad_nt;       /* [slots] incident rows in use */
+    const double *rad_plasma;    /* [n_layers] plasma energy, eV (0: no dielectric suppression) */
+    int32_t rad_max_t, rad_n_red;
+    double rad_cutoff_eV;
+    /* shell */
+    const double *lut_sh_rate;    /* [n_layers, n_E] total hard rate 1/A */
+    const double *lut_sh_omega2;  /* [n_layers, n_E] soft variance per length keV^2/A */
+    const double *lut_sh_ch_rate; /* [n_layers, sh_max_ch, n_E] 1/A */
+    const double *sh_U, *sh_W;    /* [n_layers, sh_max_ch] ionisation and resonance energies, eV */
+    const int32_t *sh_branch;     /* [n_layers, sh_max_ch] 0 distant longitudinal, 1 distant transverse, 2 close */
+    const int16_t *sh_code;       /* [n_layers, sh_max_ch] channel code written to seg_hard_ch */
+    const int32_t *sh_nch;        /* [n_layers] */
+    int32_t sh_max_ch;
+    double sh_cutoff_eV;
+    /* mode rows (NULL: not written) */
+    double *seg_hard_W;   /* keV */
+    int16_t *seg_hard_ch;
+    double *seg_rad_k;    /* eV */
+    int16_t *seg_rad_Z;
+} hr_hard_events;
+
+/* Radiative LUT columns: soft loss [keV/A] and hard photon rate [1/A] of each
+ * layer at each node of E_grid [keV] (row-major [n_layers, n_E]), from the
+ * exact per-element moments (the reference's prepared moment lookup,
+ * _jit_moment_lookup_device.py:10-70, on the transport LUT's grid). */
+hr_status hr_radiative_lut(const hr_hard_events *h, int32_t n_layers, int32_t max_el, const int32_t *L_nel,
+                           const double *E_grid_keV, int64_t n_E, double *soft_out, double *rate_out, void *stream);
+/* Hard photon energies [eV] of element slot `slot` at energies E_eV[i] and
+ * uniforms u[i], with the plasma energy of the slot's layer (the event
+ * sampler, _jit_radiative_device.py:359-481). */
+hr_status hr_radiative_sample(const hr_hard_events *h, int32_t slot, double plasma_eV, int64_t n, const double *E_eV,
+                              const double *u, double *k_out, void *stream);
+
 typedef struct hr_transport_args {
     /* run */
     int64_t e_start, e_count, cap;
@@ -134,6 +209,8 @@ typedef struct hr_transport_args {
     /* claim counter of hr_transport_run (8 bytes, caller-owned; unused by the
      * continuation and round entry points, which run one lane per thread) */
     unsigned long long *work;
+    /* hard-event modes (hr_hard_events; both switches 0 = the plain step) */
+    hr_hard_events hard;
 } hr_transport_args;
 
 hr_status hr_transport_run(const hr_transport_args *args, void *stream);

@@ -54,6 +54,51 @@ void or_rotate_direction(double *d, double cos_t, double phi);
 
 /* ---- per-electron LUT transport core (transport/cores.py:365-1250,
  *      transport/_jit_lut_kernel.py:64-574) */
+/* ---- hard-event modes (coupled radiative, shell soft/hard) ---------------- */
+typedef struct or_rad_tables {
+    const double *incident, *nominal, *top, *chi, *z, *ncm3; /* slot-major, as hr_hard_events */
+    const int32_t *nt;
+    const double *plasma; /* [n_layers] eV */
+    int32_t max_t, n_red;
+    double cutoff_eV;
+} or_rad_tables;
+
+typedef struct or_hard_modes {
+    int32_t rad_on, shell_on;
+    or_rad_tables rad;
+    const uint64_t *rad_key;     /* [Ne] */
+    const double *lut_rad_soft;  /* [n_layers, n_E] keV/A */
+    const double *lut_rad_rate;  /* [n_layers, n_E] 1/A */
+    const double *lut_sh_rate, *lut_sh_omega2, *lut_sh_ch_rate; /* [L,nE], [L,nE], [L,max_ch,nE] */
+    const double *sh_U, *sh_W;
+    const int32_t *sh_branch;
+    const int16_t *sh_code;
+    const int32_t *sh_nch;
+    int32_t sh_max_ch;
+    double sh_cutoff_eV;
+    double *seg_hard_W, *seg_rad_k; /* rows (NULL: not written) */
+    int16_t *seg_hard_ch, *seg_rad_Z;
+} or_hard_modes;
+
+void or_rad_moments(const or_rad_tables *t, int slot, double E_eV, double cutoff_eV, double plasma_eV, double *soft,
+                    double *hard_rate, double *hard_first);
+double or_rad_sample_photon(const or_rad_tables *t, int slot, double E_eV, double cutoff_eV, double u,
+                            double plasma_eV);
+void or_rad_layer_moments(const or_rad_tables *t, int layer, int n_el, int max_el, double E_eV, double *soft,
+                          double *hard, double *rates);
+void or_rad_layer_lut(const or_rad_tables *t, int n_layers, const int32_t *nel, int max_el, const double *E_keV,
+                      int64_t n_E, double *soft_keV_per_ang, double *rate_per_ang);
+void or_shell_bounds(double E, double U, double W, int branch, double Wc, double *lower, double *upper,
+                     double *peak_end);
+double or_moller_j0(double E, double prime, double w);
+double or_triangle_cdf(double peak_end, double lower, double w);
+double or_shell_sample_transfer(double E, double U, double W, int branch, double Wc, double u);
+double or_shell_sample_transfer_bisection(double E, double U, double W, int branch, double Wc, double u);
+double or_qmin(double E, double W);
+double or_shell_primary_cosine(double E, double U, double W, int branch, double W_eV, double u);
+double or_shell_secondary_cosine(double E, double U, double W, int branch, double W_eV, double u);
+double or_shell_soft_loss(double mean, double var, uint64_t key, uint64_t *counter);
+
 typedef struct or_transport_args {
     /* run */
     int64_t e_start, e_count, cap;
@@ -97,6 +142,8 @@ typedef struct or_transport_args {
     /* per local electron */
     int64_t *seg_count;
     int8_t *exit_code;
+    /* hard-event modes (zeroed: off) */
+    or_hard_modes hard;
 } or_transport_args;
 
 /* Runs electrons [e_start, e_start+e_count) with `threads` OpenMP threads
@@ -132,6 +179,8 @@ typedef struct or_line_args {
     double lo_keep, hi_keep;
 } or_line_args;
 
+/* hbar*c of the line-recoil terms (default hbar*c; 0 = the classical limit, _recoil.py:14-16). */
+void or_set_recoil_hbarc(double hbarc);
 /* Exact second-order (pxr_num, f_cbs) of one polarization (cxr/_recoil.py:35-76). */
 void or_quantum_line_factors(double h, double gamma, double vdg, double k_dot_v, double k_dot_g,
                              double g2, double k2, double v_dot_e, double g_dot_e, double k_dot_e,

@@ -92,6 +92,12 @@ static double mean_transmission(double tau0, double tau1) {
     return exp(-lo) * ratio;
 }
 
+/* hbar*c in the line-recoil terms only (_recoil.py:14-16): zero is the
+ * reference's hbar -> 0 hook, under which every line reduces to its classical
+ * root, width and amplitudes.  Set by or_set_recoil_hbarc (tests only). */
+static double g_recoil_hbarc = HBARC;
+void or_set_recoil_hbarc(double hbarc) { g_recoil_hbarc = hbarc; }
+
 /* _recoil.py:35-76 */
 static void quantum_line_factors(double h, double gamma, double vdg, double k_dot_v, double k_dot_g,
                                  double g2, double k2, double v_dot_e, double g_dot_e,
@@ -151,7 +157,7 @@ static void segment_kinematics(const or_line_args *p, double rz, double vhx, dou
     const double half = 0.5 * L;
     k->path0 = slab_path(rz - half * vhz, p->thickness, nz);
     k->path1 = slab_path(rz + half * vhz, p->thickness, nz);
-    k->inv_E = HBARC / (k->gamma * M_E_EV); /* _recoil.py:17-23 */
+    k->inv_E = g_recoil_hbarc / (k->gamma * M_E_EV); /* _recoil.py:17-23 */
 }
 
 /* One (segment, reflection) line at the in-medium root: the resonance, the
@@ -163,7 +169,7 @@ typedef struct line_core {
 
 static int reflection_line(const or_line_args *p, const seg_kin *k, int g, line_core *o) {
     const double nx = p->n_hat[0], ny = p->n_hat[1], nz = p->n_hat[2];
-    const double h = HBARC / M_E_EV; /* _recoil.py:26-32 */
+    const double h = g_recoil_hbarc / M_E_EV; /* _recoil.py:26-32 */
     const int64_t n_tab = p->n_tab;
     const double Gx = p->g[3 * g], Gy = p->g[3 * g + 1], Gz = p->g[3 * g + 2];
     const double g2 = Gx * Gx + Gy * Gy + Gz * Gz;

@@ -41,9 +41,12 @@
 
 /* events.py:36-131 */
 /* events.py:36-43: ELASTIC 0, SUBSTEP 1, LAYER_BOUNDARY 2, GROOVE_SURFACE 3 (never
-   emitted on this path), EXIT_TOP 4, EXIT_BOTTOM 5, EXIT_SIDE 6, CUTOFF 7 */
+   emitted on this path), EXIT_TOP 4, EXIT_BOTTOM 5, EXIT_SIDE 6, CUTOFF 7,
+   HARD_INELASTIC 8, HARD_RADIATIVE 9 */
 enum { EV_ELASTIC = 0, EV_SUBSTEP = 1, EV_LAYER = 2, EV_GROOVE = 3, EV_EXIT_TOP = 4, EV_EXIT_BOTTOM = 5,
-       EV_EXIT_SIDE = 6, EV_CUTOFF = 7 };
+       EV_EXIT_SIDE = 6, EV_CUTOFF = 7, EV_HARD_INELASTIC = 8, EV_HARD_RADIATIVE = 9 };
+/* hard_inelastic.py:40 */
+#define HARD_SALT 0x3C6EF372FE94F82BULL
 /* cores.py:90-95 */
 enum { EXIT_CUTOFF = 0, EXIT_BACK = 1, EXIT_TRANS = 2, EXIT_SIDE = 3, EXIT_STEP_LIMITED = 4,
        EXIT_NOT_ENTERED = 5 };
@@ -323,8 +326,11 @@ static int searchsorted_right(const double *b, double x, int n) {
     return lo;
 }
 
-/* One electron: cores.py:408-1239 with per_electron=True, lut=True, every
- * optional mode (groove, shell, radiative, mixed, cone split) off. */
+/* One electron: cores.py:408-1239 with per_electron=True, lut=True; the shell
+ * (inelastic) and coupled radiative modes by a->hard, every other optional
+ * mode (groove, mixed, cone split) off.  The radiative mode reads its rates
+ * from LUT columns on the transport grid (the reference's table moment mode,
+ * cores.py:467-470). */
 static void transport_one(const or_transport_args *a, int64_t e) {
     const int64_t i = e - a->e_start;
     const int nE = a->lut_n_energy;
@@ -344,6 +350,11 @@ static void transport_one(const or_transport_args *a, int64_t e) {
     int32_t steps = 0;
     double *pos = a->pos + 3 * e;
     double *dir = a->dirs + 3 * e;
+    const or_hard_modes *h = &a->hard;
+    const uint64_t hard_key = h->shell_on ? or_splitmix64(key ^ HARD_SALT) : 0;
+    const uint64_t rad_key = h->rad_on ? h->rad_key[e] : 0;
+    uint64_t hard_draws = 0, rad_draws = 0;
+    double tau_h = -1.0, tau_rad = -1.0;
 
     while (steps < a->max_steps) {
         int L = a->n_layers == 1 ? 0 : searchsorted_right(a->internal_bounds, pos[2], a->n_layers - 1);
@@ -360,6 +371,32 @@ static void transport_one(const or_transport_args *a, int64_t e) {
             draws += 1ULL;
         }
         double step = tau_left * lam;
+        int hard_j = 0, rad_j = 0;
+        double mu_hard = 0.0, mu_rad = 0.0, rad_soft = 0.0;
+        if (h->shell_on) {
+            mu_hard = lerp_row(h->lut_sh_rate + (size_t)L * nE, li, lf);
+            if (tau_h < 0.0) {
+                tau_h = -log(or_stream_uniform(hard_key, hard_draws));
+                hard_draws += 1ULL;
+            }
+            if (mu_hard > 0.0 && tau_h / mu_hard < step) {
+                step = tau_h / mu_hard;
+                hard_j = 1;
+            }
+        }
+        if (h->rad_on) {
+            rad_soft = lerp_row(h->lut_rad_soft + (size_t)L * nE, li, lf);
+            mu_rad = lerp_row(h->lut_rad_rate + (size_t)L * nE, li, lf);
+            if (tau_rad < 0.0) {
+                tau_rad = -log(or_stream_uniform(rad_key, rad_draws));
+                rad_draws += 1ULL;
+            }
+            if (mu_rad > 0.0 && tau_rad / mu_rad < step) {
+                step = tau_rad / mu_rad;
+                rad_j = 1;
+                hard_j = 0;
+            }
+        }
 
         double dx = dir[0], dy = dir[1], dz = dir[2];
         double px = pos[0], py = pos[1], pz = pos[2];
@@ -392,6 +429,12 @@ static void transport_one(const or_transport_args *a, int64_t e) {
 
         double dEds = lerp_row(a->lut_dEds + (size_t)L * nE, li, lf);
         double inv_beta_j = lerp_row(a->lut_inv_beta, li, lf);
+        /* soft radiative stopping joins as a mean loss (cores.py:605-611) */
+        double rad_dEds = 0.0;
+        if (h->rad_on) {
+            rad_dEds = rad_soft;
+            dEds -= rad_dEds;
+        }
         int cutoff = 0, limited = 0;
         int geometry_event = cross_up || cross_dn || exit_side;
         double E_end, t_end;
@@ -410,9 +453,15 @@ static void transport_one(const or_transport_args *a, int64_t e) {
             uint64_t flight_key = or_urban_flight_key(urban_key, flight, substep);
             uint64_t counter = 0;
             const size_t row = (size_t)L * max_el;
-            double loss = or_urban_sample_compound(a->L_Z + row, a->L_J + row, a->L_k + row,
-                                                   a->L_coeff + row, a->L_E_cross + row, n_el, E,
-                                                   step, 1.0, flight_key, &counter);
+            double loss;
+            if (h->shell_on) { /* cores.py:616-640: the two-moment soft sampler */
+                double omega2 = lerp_row(h->lut_sh_omega2 + (size_t)L * nE, li, lf);
+                loss = or_shell_soft_loss(-(dEds + rad_dEds) * step, omega2 * step, flight_key, &counter);
+            } else {
+                loss = or_urban_sample_compound(a->L_Z + row, a->L_J + row, a->L_k + row, a->L_coeff + row,
+                                                a->L_E_cross + row, n_el, E, step, 1.0, flight_key, &counter);
+            }
+            loss += rad_dEds * step;
             a->stragg_dE[e] += loss;
             double delta_cut = E - E_cut;
             if (loss > delta_cut || (loss == delta_cut && !geometry_event)) {
@@ -440,6 +489,7 @@ static void transport_one(const or_transport_args *a, int64_t e) {
                 double cf;
                 or_lut_index_frac(E_mid, a->lut_log_E_min, a->lut_inv_dlogE, nE, &ci, &cf);
                 double cutoff_rate = lerp_row(a->lut_dEds + (size_t)L * nE, ci, cf);
+                if (h->rad_on) cutoff_rate -= lerp_row(h->lut_rad_soft + (size_t)L * nE, ci, cf);
                 cutoff_distance = (E_cut - E) / cutoff_rate;
             } else {
                 cutoff_distance = (E_cut - E) / dEds;
@@ -467,6 +517,7 @@ static void transport_one(const or_transport_args *a, int64_t e) {
                     double mf;
                     or_lut_index_frac(E_mid, a->lut_log_E_min, a->lut_inv_dlogE, nE, &mi, &mf);
                     double mid_rate = lerp_row(a->lut_dEds + (size_t)L * nE, mi, mf);
+                    if (h->rad_on) mid_rate -= lerp_row(h->lut_rad_soft + (size_t)L * nE, mi, mf);
                     E_end = E + step * mid_rate;
                 }
                 int ci;
@@ -486,7 +537,57 @@ static void transport_one(const or_transport_args *a, int64_t e) {
         else if (exit_bot) event = EV_EXIT_BOTTOM;
         else if (exit_side) event = EV_EXIT_SIDE;
         else if (cross_up || cross_dn) event = EV_LAYER;
+        else if (h->shell_on && hard_j) event = EV_HARD_INELASTIC;
+        else if (h->rad_on && rad_j) event = EV_HARD_RADIATIVE;
         else event = EV_ELASTIC;
+
+        double hard_W_keV = 0.0, hard_cos = 1.0, hard_phi = 0.0, rad_k_eV = 0.0;
+        int16_t hard_code = -1, rad_Z = 0;
+        if (event == EV_HARD_INELASTIC) { /* cores.py:850-880, _sample_hard_collision */
+            double u_ch = or_stream_uniform(hard_key, hard_draws);
+            double u_w = or_stream_uniform(hard_key, hard_draws + 1ULL);
+            double u_q = or_stream_uniform(hard_key, hard_draws + 2ULL);
+            double u_phi = or_stream_uniform(hard_key, hard_draws + 3ULL);
+            hard_draws += 4ULL;
+            double target = u_ch * mu_hard;
+            int nch = h->sh_nch[L], ch = nch - 1;
+            double cum = 0.0;
+            for (int k = 0; k < nch; ++k) {
+                cum += lerp_row(h->lut_sh_ch_rate + ((size_t)L * h->sh_max_ch + k) * nE, li, lf);
+                if (cum > target) { ch = k; break; }
+            }
+            size_t c = (size_t)L * h->sh_max_ch + ch;
+            double E_eV = E * 1.0e3;
+            double W_eV = or_shell_sample_transfer(E_eV, h->sh_U[c], h->sh_W[c], h->sh_branch[c], h->sh_cutoff_eV, u_w);
+            hard_W_keV = W_eV * 1.0e-3;
+            hard_code = h->sh_code[c];
+            if (E_end - hard_W_keV <= E_cut) {
+                event = EV_CUTOFF;
+                cutoff = 1;
+            } else {
+                hard_cos = or_shell_primary_cosine(E_eV, h->sh_U[c], h->sh_W[c], h->sh_branch[c], W_eV, u_q);
+                hard_phi = 2.0 * M_PI * u_phi;
+            }
+        } else if (event == EV_HARD_RADIATIVE) { /* cores.py:880-906, _sample_hard_photon */
+            double rates[16], soft_e, mu_event;
+            or_rad_layer_moments(&h->rad, L, n_el, max_el, E_end * 1e3, &soft_e, &mu_event, rates);
+            double u_el = or_stream_uniform(rad_key, rad_draws);
+            double u_k = or_stream_uniform(rad_key, rad_draws + 1ULL);
+            rad_draws += 2ULL;
+            double target = u_el * mu_event, cum = 0.0;
+            int sel = n_el - 1;
+            for (int k = 0; k < n_el; ++k) {
+                cum += rates[k];
+                if (cum > target) { sel = k; break; }
+            }
+            int slot = L * max_el + sel;
+            rad_k_eV = or_rad_sample_photon(&h->rad, slot, E_end * 1e3, h->rad.cutoff_eV, u_k, h->rad.plasma[L]);
+            rad_Z = (int16_t)h->rad.z[slot];
+            if (E_end - rad_k_eV * 1e-3 <= E_cut) {
+                event = EV_CUTOFF;
+                cutoff = 1;
+            }
+        }
 
         if (nseg < a->cap) {
             const int64_t slot = i * a->cap + nseg;
@@ -507,6 +608,14 @@ static void transport_one(const or_transport_args *a, int64_t e) {
                 a->seg_flight[slot] = flight;
                 a->seg_substep[slot] = substep;
                 a->seg_event[slot] = event;
+                if (h->seg_hard_W) {
+                    h->seg_hard_W[slot] = hard_W_keV;
+                    h->seg_hard_ch[slot] = hard_code;
+                }
+                if (h->seg_rad_k) {
+                    h->seg_rad_k[slot] = rad_k_eV;
+                    h->seg_rad_Z[slot] = rad_Z;
+                }
             }
         }
         nseg += 1;
@@ -517,6 +626,14 @@ static void transport_one(const or_transport_args *a, int64_t e) {
         a->clock[e] = t_end;
         tau_left -= step / lam;
         if (tau_left < 0.0) tau_left = 0.0;
+        if (h->shell_on) {
+            tau_h -= step * mu_hard;
+            if (tau_h < 0.0) tau_h = 0.0;
+        }
+        if (h->rad_on) {
+            tau_rad -= step * mu_rad;
+            if (tau_rad < 0.0) tau_rad = 0.0;
+        }
 
         if (limited) {
             substep += 1;
@@ -530,8 +647,21 @@ static void transport_one(const or_transport_args *a, int64_t e) {
         flight += 1;
         substep = 0;
         tau_left = -1.0;
+        tau_h = -1.0;
+        tau_rad = -1.0;
         if (cross_up || cross_dn) {
             pos[2] += (dir[2] > 0.0 ? 1.0 : -1.0) * 1e-6;
+            steps += 1;
+            continue;
+        }
+        if (event == EV_HARD_RADIATIVE) { /* a photon: debit, no deflection */
+            a->E_keV[e] -= rad_k_eV * 1e-3;
+            steps += 1;
+            continue;
+        }
+        if (event == EV_HARD_INELASTIC) { /* the loss and recoil deflection replace the elastic draw */
+            a->E_keV[e] -= hard_W_keV;
+            rotate_direction(&dir[0], &dir[1], &dir[2], hard_cos, hard_phi);
             steps += 1;
             continue;
         }

@@ -27,6 +27,24 @@ def build() -> Path:
     return LIB_PATH
 
 
+class RadTables(C.Structure):
+    _fields_ = [
+        ("incident", _dp), ("nominal", _dp), ("top", _dp), ("chi", _dp), ("z", _dp), ("ncm3", _dp),
+        ("nt", _dp), ("plasma", _dp), ("max_t", C.c_int32), ("n_red", C.c_int32), ("cutoff_eV", C.c_double),
+    ]
+
+
+class HardModes(C.Structure):
+    _fields_ = [
+        ("rad_on", C.c_int32), ("shell_on", C.c_int32), ("rad", RadTables),
+        ("rad_key", _dp), ("lut_rad_soft", _dp), ("lut_rad_rate", _dp),
+        ("lut_sh_rate", _dp), ("lut_sh_omega2", _dp), ("lut_sh_ch_rate", _dp),
+        ("sh_U", _dp), ("sh_W", _dp), ("sh_branch", _dp), ("sh_code", _dp), ("sh_nch", _dp),
+        ("sh_max_ch", C.c_int32), ("sh_cutoff_eV", C.c_double),
+        ("seg_hard_W", _dp), ("seg_rad_k", _dp), ("seg_hard_ch", _dp), ("seg_rad_Z", _dp),
+    ]
+
+
 class TransportArgs(C.Structure):
     _fields_ = [
         ("e_start", C.c_int64), ("e_count", C.c_int64), ("cap", C.c_int64),
@@ -47,6 +65,7 @@ class TransportArgs(C.Structure):
         ("seg_id", _dp), ("seg_lay", _dp), ("seg_E_end", _dp), ("seg_t_end", _dp),
         ("seg_flight", _dp), ("seg_substep", _dp), ("seg_event", _dp),
         ("seg_count", _dp), ("exit_code", _dp),
+        ("hard", HardModes),
     ]
 
 
@@ -136,6 +155,32 @@ def lib():
         L.or_formation_factor.argtypes = [C.c_double] * 4 + [C.POINTER(C.c_double)] * 2
         L.or_coherent_field.restype = None
         L.or_coherent_field.argtypes = [_dp, C.c_int64, _dp, _dp, C.c_int64, _dp, _dp, C.c_int]
+        d = C.c_double
+        L.or_set_recoil_hbarc.restype = None
+        L.or_set_recoil_hbarc.argtypes = [d]
+        L.or_rad_moments.restype = None
+        L.or_rad_moments.argtypes = [C.POINTER(RadTables), C.c_int, d, d, d, C.POINTER(d), C.POINTER(d),
+                                     C.POINTER(d)]
+        L.or_rad_sample_photon.restype = d
+        L.or_rad_sample_photon.argtypes = [C.POINTER(RadTables), C.c_int, d, d, d, d]
+        L.or_rad_layer_lut.restype = None
+        L.or_rad_layer_lut.argtypes = [C.POINTER(RadTables), C.c_int, _dp, C.c_int, _dp, C.c_int64, _dp, _dp]
+        L.or_shell_bounds.restype = None
+        L.or_shell_bounds.argtypes = [d, d, d, C.c_int, d, C.POINTER(d), C.POINTER(d), C.POINTER(d)]
+        for name in ("or_shell_sample_transfer", "or_shell_sample_transfer_bisection"):
+            getattr(L, name).restype = d
+            getattr(L, name).argtypes = [d, d, d, C.c_int, d, d]
+        L.or_moller_j0.restype = d
+        L.or_moller_j0.argtypes = [d, d, d]
+        L.or_triangle_cdf.restype = d
+        L.or_triangle_cdf.argtypes = [d, d, d]
+        L.or_qmin.restype = d
+        L.or_qmin.argtypes = [d, d]
+        for name in ("or_shell_primary_cosine", "or_shell_secondary_cosine"):
+            getattr(L, name).restype = d
+            getattr(L, name).argtypes = [d, d, d, C.c_int, d, d]
+        L.or_shell_soft_loss.restype = d
+        L.or_shell_soft_loss.argtypes = [d, d, C.c_uint64, C.POINTER(C.c_uint64)]
         _LIB = L
     return _LIB
 
@@ -203,13 +248,87 @@ def lut_index_frac(E, log_E_min, inv_dlogE, n_energy):
     return i.value, f.value
 
 
+# ---- hard-event modes ---------------------------------------------------------
+class RadOracle:
+    """Oracle view of a radiative table set (the product's host arrays)."""
+
+    def __init__(self, rt):
+        self.rt = rt
+        self.hold = {k: np.ascontiguousarray(getattr(rt, k), dtype=np.float64)
+                     for k in ("incident", "nominal", "top", "chi", "z", "ncm3", "plasma")}
+        self.hold["nt"] = np.ascontiguousarray(rt.nt, dtype=np.int32)
+        t = RadTables()
+        for k, v in self.hold.items():
+            setattr(t, k, _p(v))
+        t.max_t, t.n_red, t.cutoff_eV = int(rt.max_t), int(rt.n_red), float(rt.cutoff_eV)
+        self.t = t
+
+    def moments(self, slot, E_eV, cutoff_eV=None, plasma_eV=0.0):
+        """(soft first [eV cm^2], hard rate [cm^2], hard first [eV cm^2]) per atom."""
+        s, r, f = C.c_double(), C.c_double(), C.c_double()
+        c = self.rt.cutoff_eV if cutoff_eV is None else cutoff_eV
+        lib().or_rad_moments(C.byref(self.t), int(slot), float(E_eV), float(c), float(plasma_eV), C.byref(s),
+                             C.byref(r), C.byref(f))
+        return s.value, r.value, f.value
+
+    def sample(self, slot, E_eV, u, plasma_eV=0.0, cutoff_eV=None):
+        c = self.rt.cutoff_eV if cutoff_eV is None else cutoff_eV
+        return float(lib().or_rad_sample_photon(C.byref(self.t), int(slot), float(E_eV), float(c), float(u),
+                                                float(plasma_eV)))
+
+    def layer_lut(self, layers, E_keV):
+        """(soft keV/A, rate 1/A) columns ``[n_layers, n_E]`` on ``E_keV``."""
+        E = np.ascontiguousarray(E_keV, dtype=np.float64)
+        nel = np.ascontiguousarray(layers.L_nel, dtype=np.int32)
+        soft = np.zeros((layers.n_layers, E.size))
+        rate = np.zeros((layers.n_layers, E.size))
+        lib().or_rad_layer_lut(C.byref(self.t), int(layers.n_layers), _p(nel), int(layers.max_el), _p(E), E.size,
+                               _p(soft), _p(rate))
+        return soft, rate
+
+
+def shell_bounds(E, U, W, branch, Wc):
+    lo, hi, pe = C.c_double(), C.c_double(), C.c_double()
+    lib().or_shell_bounds(E, U, W, int(branch), Wc, C.byref(lo), C.byref(hi), C.byref(pe))
+    return lo.value, hi.value, pe.value
+
+
+def shell_sample_transfer(E, U, W, branch, Wc, u, bisection=False):
+    f = lib().or_shell_sample_transfer_bisection if bisection else lib().or_shell_sample_transfer
+    return float(f(E, U, W, int(branch), Wc, u))
+
+
+def shell_cosines(E, U, W, branch, W_eV, u):
+    return (float(lib().or_shell_primary_cosine(E, U, W, int(branch), W_eV, u)),
+            float(lib().or_shell_secondary_cosine(E, U, W, int(branch), W_eV, u)))
+
+
+def qmin(E, W):
+    return float(lib().or_qmin(E, W))
+
+
+def moller_j0(E, prime, w):
+    return float(lib().or_moller_j0(E, prime, w))
+
+
+def shell_soft_loss(mean, var, key, counter=0):
+    c = C.c_uint64(counter)
+    loss = lib().or_shell_soft_loss(mean, var, key, C.byref(c))
+    return float(loss), int(c.value)
+
+
 def run_transport(layers, lut, keys, pos, dirs, E_keV, E_cut, *, cap, e_start=0, e_count=None,
                   alive=None, clock=None, max_steps=400_000, energy_model=1, max_dE_frac=0.0,
-                  straggle_on=False, finite=None, threads=0):
+                  straggle_on=False, finite=None, threads=0, rad=None, shell=None):
     """Run the oracle core on host arrays; returns (rows dict, seg_count, exit_code, state).
 
     ``layers``/``lut`` are the product's host tables (the kernel inputs);
     ``pos``/``dirs``/``E_keV``/``clock`` are copied, never mutated.
+    ``rad = (RadiativeTables, rad_keys, soft_col, rate_col)`` switches on the
+    coupled radiative mode (LUT columns ``[n_layers, n_E]``); ``shell =
+    ShellTables`` the shell mode (the caller's ``lut.dEds`` must then be the
+    soft stopping).  Their rows add ``seg_rad_k``/``seg_rad_Z`` and
+    ``seg_hard_W``/``seg_hard_ch``.
     """
     Ne = len(E_keV)
     if e_count is None:
@@ -265,6 +384,35 @@ def run_transport(layers, lut, keys, pos, dirs, E_keV, E_cut, *, cap, e_start=0,
     for k, v in rows.items():
         setattr(a, k, _p(v))
     a.seg_count, a.exit_code = _p(seg_count), _p(exit_code)
+    if rad is not None:
+        rt, rkeys, soft_col, rate_col = rad
+        ro = RadOracle(rt)
+        hold["rad"] = ro
+        hold["rkeys"] = np.ascontiguousarray(rkeys, dtype=np.uint64)
+        hold["rsoft"] = np.ascontiguousarray(soft_col, dtype=np.float64)
+        hold["rrate"] = np.ascontiguousarray(rate_col, dtype=np.float64)
+        rows["seg_rad_k"] = np.zeros(n)
+        rows["seg_rad_Z"] = np.zeros(n, dtype=np.int16)
+        a.hard.rad_on, a.hard.rad = 1, ro.t
+        a.hard.rad_key, a.hard.lut_rad_soft, a.hard.lut_rad_rate = _p(hold["rkeys"]), _p(hold["rsoft"]), \
+            _p(hold["rrate"])
+        a.hard.seg_rad_k, a.hard.seg_rad_Z = _p(rows["seg_rad_k"]), _p(rows["seg_rad_Z"])
+    if shell is not None:
+        for k in ("rate", "omega2", "ch_rate", "U", "W"):
+            hold["sh_" + k] = np.ascontiguousarray(getattr(shell, k), dtype=np.float64)
+        hold["sh_branch"] = np.ascontiguousarray(shell.branch, dtype=np.int32)
+        hold["sh_code"] = np.ascontiguousarray(shell.code, dtype=np.int16)
+        hold["sh_nch"] = np.ascontiguousarray(shell.nch, dtype=np.int32)
+        rows["seg_hard_W"] = np.zeros(n)
+        rows["seg_hard_ch"] = np.zeros(n, dtype=np.int16)
+        a.hard.shell_on = 1
+        a.hard.lut_sh_rate, a.hard.lut_sh_omega2 = _p(hold["sh_rate"]), _p(hold["sh_omega2"])
+        a.hard.lut_sh_ch_rate = _p(hold["sh_ch_rate"])
+        a.hard.sh_U, a.hard.sh_W = _p(hold["sh_U"]), _p(hold["sh_W"])
+        a.hard.sh_branch, a.hard.sh_code, a.hard.sh_nch = _p(hold["sh_branch"]), _p(hold["sh_code"]), \
+            _p(hold["sh_nch"])
+        a.hard.sh_max_ch, a.hard.sh_cutoff_eV = int(shell.max_ch), float(shell.cutoff_eV)
+        a.hard.seg_hard_W, a.hard.seg_hard_ch = _p(rows["seg_hard_W"]), _p(rows["seg_hard_ch"])
     lib().or_transport_lut(C.byref(a), int(threads))
     state = {"pos": pos, "dirs": dirs, "E_keV": E_keV, "clock": clock, "stragg_dE": stragg}
     return rows, seg_count, exit_code, state
@@ -286,6 +434,16 @@ def _f32(a):
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class classical_recoil:
+    """Context: the line-recoil hbar*c set to zero (the classical limit), restored on exit."""
+
+    def __enter__(self):
+        lib().or_set_recoil_hbarc(0.0)
+
+    def __exit__(self, *exc):
+        lib().or_set_recoil_hbarc(1973.269804)
 
 
 def cxr_lines(seg, thickness, n_hat, tables, e_s, e_p, threads=0):

@@ -76,6 +76,24 @@ def workspace_bytes(name: str, *args) -> int:
     return int(n.value)
 
 
+class HardEvents(C.Structure):
+    """Mirror of ``hr_hard_events`` (the hard-event modes of the transport step)."""
+
+    _fields_ = [
+        ("rad_on", C.c_int32), ("shell_on", C.c_int32),
+        ("rad_key", _dp), ("lut_rad_soft", _dp), ("lut_rad_rate", _dp),
+        ("rad_incident", _dp), ("rad_nominal", _dp), ("rad_top", _dp), ("rad_chi", _dp), ("rad_z", _dp),
+        ("rad_ncm3", _dp), ("rad_nt", _dp), ("rad_plasma", _dp),
+        ("rad_max_t", C.c_int32), ("rad_n_red", C.c_int32),
+        ("rad_cutoff_eV", C.c_double),
+        ("lut_sh_rate", _dp), ("lut_sh_omega2", _dp), ("lut_sh_ch_rate", _dp),
+        ("sh_U", _dp), ("sh_W", _dp), ("sh_branch", _dp), ("sh_code", _dp), ("sh_nch", _dp),
+        ("sh_max_ch", C.c_int32),
+        ("sh_cutoff_eV", C.c_double),
+        ("seg_hard_W", _dp), ("seg_hard_ch", _dp), ("seg_rad_k", _dp), ("seg_rad_Z", _dp),
+    ]
+
+
 class TransportArgs(C.Structure):
     """Mirror of ``hr_transport_args`` (include/hiprite.h)."""
 
@@ -137,6 +155,7 @@ class TransportArgs(C.Structure):
         ("lut_total_rate_f", _dp), ("lut_dEds_f", _dp), ("lut_inv_beta_f", _dp), ("lut_cdf_f", _dp),
         ("lut_alpha_f", _dp),
         ("work", _dp),
+        ("hard", HardEvents),
     ]
 
 
@@ -356,6 +375,10 @@ SIGNATURES = {
     "hr_device_count": (C.c_int32, []),
     "hr_stream_keys": (C.c_int32, [C.c_uint64, C.c_int64, C.c_int64, _dp, _dp]),
     "hr_transport_run": (C.c_int32, [C.POINTER(TransportArgs), _dp]),
+    "hr_radiative_lut": (C.c_int32, [C.POINTER(HardEvents), C.c_int32, C.c_int32, _dp, _dp, C.c_int64, _dp, _dp,
+                                     _dp]),
+    "hr_radiative_sample": (C.c_int32, [C.POINTER(HardEvents), C.c_int32, C.c_double, C.c_int64, _dp, _dp, _dp,
+                                        _dp]),
     "hr_roulette_uniforms": (C.c_int32, [C.c_uint64, C.c_int64, C.c_int64, _dp, _dp]),
     "hr_line_screen": (C.c_int32, [_dp, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_uint64,
                                    _dp, _dp, _dp]),

@@ -45,7 +45,7 @@ __device__ __forceinline__ int64_t claim(unsigned long long *next, bool want, in
 #define HR_TRANSPORT_OCCUPANCY
 #endif
 
-template <bool kMixed>
+template <bool kMixed, bool kHard = false>
 __global__ void __launch_bounds__(256) HR_TRANSPORT_OCCUPANCY transport_kernel(hr_transport_args a,
                                                         unsigned long long *__restrict__ next) {
     Lane s;
@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(256) HR_TRANSPORT_OCCUPANCY transport_kernel(h
         }
         if (__ballot(live || !drained) == 0) break;
         if (live) {
-            const int code = step_lane<true, kMixed>(a, s);
+            const int code = step_lane<true, kMixed, kHard>(a, s);
             if (code >= 0) {
                 store_lane(a, s, (int8_t)code);
                 live = false;
@@ -132,20 +132,52 @@ __global__ void __launch_bounds__(256) HR_TRANSPORT_OCCUPANCY continue_kernel(hr
 
 __global__ void reset_counter(unsigned long long *c) { *c = 0ULL; }
 
-int g_occupancy[64];  // resident 256-lane blocks per CU of transport_kernel, per device
+int g_occupancy[2][64];  // resident 256-lane blocks per CU of transport_kernel (plain, hard), per device
 
-hr_status transport_occupancy(int block, int &occupancy) {
+hr_status transport_occupancy(int block, bool hard, int &occupancy) {
     int dev = 0;
     HR_CHECK_HIP(hipGetDevice(&dev));
     HR_REQUIRE(dev >= 0 && dev < 64, "device index %d out of range", dev);
-    if (block == 256 && g_occupancy[dev] > 0) {
-        occupancy = g_occupancy[dev];
+    if (block == 256 && g_occupancy[hard][dev] > 0) {
+        occupancy = g_occupancy[hard][dev];
         return HR_OK;
     }
     int blocks = 0;
-    HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, transport_kernel<false>, block, 0));
+    if (hard)
+        HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, transport_kernel<false, true>, block, 0));
+    else
+        HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, transport_kernel<false>, block, 0));
     occupancy = blocks > 0 ? blocks : 1;
-    if (block == 256) g_occupancy[dev] = occupancy;
+    if (block == 256) g_occupancy[hard][dev] = occupancy;
+    return HR_OK;
+}
+
+// Pointers and sizes the hard-event modes read; checked before any launch.
+hr_status check_hard(const hr_transport_args &a, const char *who) {
+    const hr_hard_events &h = a.hard;
+    HR_REQUIRE((h.rad_on == 0 || h.rad_on == 1) && (h.shell_on == 0 || h.shell_on == 1),
+               "%s: rad_on and shell_on must be 0 or 1", who);
+    if (!h.rad_on && !h.shell_on) return HR_OK;
+    HR_REQUIRE(a.energy_model == 1, "%s: the hard-event modes need the midpoint energy model", who);
+    if (h.rad_on) {
+        HR_REQUIRE(h.rad_key && h.lut_rad_soft && h.lut_rad_rate && h.rad_incident && h.rad_nominal && h.rad_top &&
+                       h.rad_chi && h.rad_z && h.rad_ncm3 && h.rad_nt && h.rad_plasma,
+                   "%s: radiative mode needs its keys, LUT columns and tables", who);
+        HR_REQUIRE(h.rad_max_t >= 2 && h.rad_n_red >= 2 && h.rad_cutoff_eV > 0.0,
+                   "%s: radiative tables need >= 2 incident rows and reduced nodes and a positive cutoff", who);
+        HR_REQUIRE(a.max_el <= 8, "%s: radiative mode takes at most 8 elements per layer (max_el=%d)", who, a.max_el);
+        HR_REQUIRE(a.cap == 0 || ((h.seg_rad_k == nullptr) == (h.seg_rad_Z == nullptr)),
+                   "%s: seg_rad_k and seg_rad_Z are written together", who);
+    }
+    if (h.shell_on) {
+        HR_REQUIRE(h.lut_sh_rate && h.lut_sh_omega2 && h.lut_sh_ch_rate && h.sh_U && h.sh_W && h.sh_branch &&
+                       h.sh_code && h.sh_nch,
+                   "%s: shell mode needs its LUT columns and channel tables", who);
+        HR_REQUIRE(h.sh_max_ch >= 1 && h.sh_cutoff_eV > 0.0, "%s: shell mode needs channels and a positive cutoff",
+                   who);
+        HR_REQUIRE(a.cap == 0 || ((h.seg_hard_W == nullptr) == (h.seg_hard_ch == nullptr)),
+                   "%s: seg_hard_W and seg_hard_ch are written together", who);
+    }
     return HR_OK;
 }
 
@@ -177,12 +209,15 @@ extern "C" hr_status hr_transport_run(const hr_transport_args *args, void *strea
                                     a.lut_alpha_f),
                "hr_transport_run: mixed precision needs the FP32 table copies");
     HR_REQUIRE(a.work != nullptr, "hr_transport_run: null claim counter (work)");
+    hr_status st = check_hard(a, "hr_transport_run");
+    if (st != HR_OK) return st;
+    const bool hard = a.hard.rad_on || a.hard.shell_on;
     int cu = 0, occupancy = 0;
-    hr_status st = device_cu(cu);
+    st = device_cu(cu);
     if (st != HR_OK) return st;
     // Whole wave64s, at most the kernel's launch bound of 256 lanes.
     const int block = a.block_size <= 0 ? 256 : (a.block_size >= 256 ? 256 : pick_block(a.block_size, 256));
-    st = transport_occupancy(block, occupancy);
+    st = transport_occupancy(block, hard, occupancy);
     if (st != HR_OK) return st;
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, a.work);
@@ -190,7 +225,11 @@ extern "C" hr_status hr_transport_run(const hr_transport_args *args, void *strea
     const int64_t need = (a.e_count + block - 1) / block;
     const int64_t resident = (int64_t)cu * occupancy;
     const int64_t grid = need < resident ? need : resident;
-    if (a.precision == 1)
+    if (hard && a.precision == 1)
+        hipLaunchKernelGGL((transport_kernel<true, true>), dim3((unsigned)grid), dim3(block), 0, s, a, a.work);
+    else if (hard)
+        hipLaunchKernelGGL((transport_kernel<false, true>), dim3((unsigned)grid), dim3(block), 0, s, a, a.work);
+    else if (a.precision == 1)
         hipLaunchKernelGGL(transport_kernel<true>, dim3((unsigned)grid), dim3(block), 0, s, a, a.work);
     else
         hipLaunchKernelGGL(transport_kernel<false>, dim3((unsigned)grid), dim3(block), 0, s, a, a.work);
@@ -226,6 +265,9 @@ extern "C" hr_status hr_transport_continue(const hr_transport_args *args, const 
     HR_REQUIRE(a.precision == 0 || (a.precision == 1 && a.lut_total_rate_f && a.lut_dEds_f && a.lut_inv_beta_f &&
                                     a.lut_cdf_f && a.lut_alpha_f),
                "hr_transport_continue: precision 0, or 1 with the FP32 table copies");
+    HR_REQUIRE(!a.hard.rad_on && !a.hard.shell_on,
+               "hr_transport_continue: the hard-event modes run on hr_transport_run only (their optical depths "
+               "and counters are not in the continuation record)");
     const unsigned grid = (unsigned)((a.e_count + 255) / 256);
     if (a.precision == 1)
         hipLaunchKernelGGL(continue_kernel<true>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, c);

@@ -11,6 +11,8 @@
 #include <math.h>
 
 #include "hr_common.hpp"
+#include "radiative.hpp"
+#include "shell.hpp"
 
 namespace hr {
 namespace {
@@ -27,7 +29,9 @@ constexpr double kPi = 3.141592653589793;
 // Row event codes of montecarlo/transport/events.py:36-43 (3 is the groove
 // surface, which this path never emits).
 enum : int8_t { EV_ELASTIC = 0, EV_SUBSTEP = 1, EV_LAYER = 2, EV_EXIT_TOP = 4, EV_EXIT_BOTTOM = 5,
-                EV_EXIT_SIDE = 6, EV_CUTOFF = 7 };
+                EV_EXIT_SIDE = 6, EV_CUTOFF = 7, EV_HARD_INELASTIC = 8, EV_HARD_RADIATIVE = 9 };
+// Salt of the hard-collision key: splitmix64(stream_key ^ salt) (hard_inelastic.py:40,77-80).
+constexpr uint64_t kHardSalt = 0x3C6EF372FE94F82BULL;
 enum : int8_t { EXIT_CUTOFF = 0, EXIT_BACK = 1, EXIT_TRANS = 2, EXIT_SIDE = 3, EXIT_STEP_LIMITED = 4,
                 EXIT_NOT_ENTERED = 5, EXIT_SUSPENDED = 6 };
 
@@ -216,6 +220,9 @@ struct Lane {
     uint64_t key, draws;
     int64_t e, id, nseg, flight, substep;
     int32_t steps;
+    // hard-event modes: open optical depths (< 0: draw at the next step), keys, counters
+    double tau_h, tau_r;
+    uint64_t hkey, rkey, hdraws, rdraws;
 };
 
 __device__ __forceinline__ void load_lane(const hr_transport_args &a, int64_t e, Lane &s) {
@@ -238,235 +245,39 @@ __device__ __forceinline__ void load_lane(const hr_transport_args &a, int64_t e,
     s.flight = 0;
     s.substep = 0;
     s.steps = 0;
+    s.tau_h = -1.0;
+    s.tau_r = -1.0;
+    s.hkey = a.hard.shell_on ? splitmix64(s.key ^ kHardSalt) : 0ULL;
+    s.rkey = a.hard.rad_on ? a.hard.rad_key[e] : 0ULL;
+    s.hdraws = 0;
+    s.rdraws = 0;
 }
 
-__device__ __forceinline__ void store_lane(const hr_transport_args &a, const Lane &s, int8_t code) {
-    const int64_t e = s.e, i = e - a.e_start;
-    a.pos[3 * e] = s.px;
-    a.pos[3 * e + 1] = s.py;
-    a.pos[3 * e + 2] = s.pz;
-    a.dirs[3 * e] = s.dx;
-    a.dirs[3 * e + 1] = s.dy;
-    a.dirs[3 * e + 2] = s.dz;
-    a.E_keV[e] = s.E;
-    a.clock[e] = s.clock;
-    if (a.straggle_on) a.stragg_dE[e] = s.stragg;
-    a.seg_count[i] = s.nseg;
-    a.exit_code[i] = code;
+__device__ __forceinline__ rad::Tables rad_tables(const hr_hard_events &h, int max_el) {
+    rad::Tables t;
+    t.incident = h.rad_incident;
+    t.nominal = h.rad_nominal;
+    t.top = h.rad_top;
+    t.chi = h.rad_chi;
+    t.z = h.rad_z;
+    t.ncm3 = h.rad_ncm3;
+    t.nt = h.rad_nt;
+    t.nel = nullptr;
+    t.plasma = h.rad_plasma;
+    t.max_el = max_el;
+    t.max_t = h.rad_max_t;
+    t.n_red = h.rad_n_red;
+    t.cutoff_eV = h.rad_cutoff_eV;
+    return t;
 }
 
-// One loop iteration of cores.py:408-1239 (per_electron, lut).  Returns the
-// exit code when the history ends, or -1 when it continues.  kRows = false
-// never writes rows (tally-only callers), whatever a.cap says.  kMixed is the
-// reference's "mixed" transport precision: FP32 table lookups, query
-// logarithms, sqrt(1 - cos^2) and azimuth sin/cos; every accumulated quantity,
-// free-path draw and the straggling stay FP64.
-template <bool kRows = true, bool kMixed = false>
-__device__ int step_lane(const hr_transport_args &a, Lane &s) {
-    const Lut<kMixed> lut{a};
-    const int nE = a.lut_n_energy;
-    const int max_el = a.max_el;
-    int L = 0;
-    if (a.n_layers != 1) {
-        int lo = 0, hi = a.n_layers - 1;  // searchsorted(side="right")
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (a.internal_bounds[mid] <= s.pz) lo = mid + 1;
-            else hi = mid;
-        }
-        L = lo;
-    }
-    const int n_el = a.L_nel[L];
-    const double z_top = a.L_top[L], z_bot = a.L_bot[L];
-    const double E = s.E;
-    const LutPos p = lut.pos(E);
-    const size_t row_off = (size_t)L * nE;
-    const double total_rate = lut.rate(row_off, p);
-    const double lam = 1e8 / total_rate;
-    if (s.tau < 0.0) {
-        s.tau = -log(stream_uniform(s.key, s.draws));
-        s.draws += 1ULL;
-    }
-    double step = s.tau * lam;
-    const double dx = s.dx, dy = s.dy, dz = s.dz, px = s.px, py = s.py, pz = s.pz;
-    bool up = false, dn = false, side = false;
-    if (a.finite_footprint) {
-        double best = INFINITY;
-        int face = -1;
-        const double hw = 0.5 * a.width_ang, hh = 0.5 * a.height_ang;
-        double t;
-        if (dx < 0.0) { t = (-hw - px) / dx; if (t > 0.0 && t < best) { best = t; face = 0; } }
-        if (dx > 0.0) { t = (hw - px) / dx; if (t > 0.0 && t < best) { best = t; face = 1; } }
-        if (dy < 0.0) { t = (-hh - py) / dy; if (t > 0.0 && t < best) { best = t; face = 2; } }
-        if (dy > 0.0) { t = (hh - py) / dy; if (t > 0.0 && t < best) { best = t; face = 3; } }
-        if (dz < 0.0) { t = (z_top - pz) / dz; if (t > 0.0 && t < best) { best = t; face = 4; } }
-        if (dz > 0.0) { t = (z_bot - pz) / dz; if (t > 0.0 && t < best) { best = t; face = 5; } }
-        if (step > best) {
-            step = best;
-            up = face == 4;
-            dn = face == 5;
-            side = face >= 0 && face <= 3;
-        }
-    } else if (dz < 0.0) {
-        const double sb = (pz - z_top) / (-dz);
-        if (step > sb) { step = sb; up = true; }
-    } else if (dz > 0.0) {
-        const double sb = (z_bot - pz) / dz;
-        if (step > sb) { step = sb; dn = true; }
-    }
-    bool exit_top = up && z_top <= 0.0;
-    bool exit_bot = dn && z_bot >= a.z_total;
-    const double dEds = lut.dEds(row_off, p);
-    const double inv_beta_j = lut.inv_beta(p);
-    bool cutoff = false, limited = false;
-    bool geometry_event = up || dn || side;
-    double E_end, t_end;
-    const bool midpoint = a.energy_model == 1;
-    if (a.straggle_on) {
-        if (a.max_dE_frac > 0.0) {
-            const double se = a.max_dE_frac * E / (-dEds);
-            if (se < step) {
-                step = se;
-                limited = true;
-                up = dn = exit_top = exit_bot = side = false;
-                geometry_event = false;
-            }
-        }
-        const uint64_t ukey = splitmix64(s.key ^ kUrbanSalt);
-        const uint64_t combined = ((uint64_t)s.flight << 32) + (uint64_t)s.substep;
-        const uint64_t fkey = splitmix64(ukey + kGolden * combined);
-        const size_t row = (size_t)L * max_el;
-        const double loss = urban_compound(a.L_Z + row, a.L_J + row, a.L_k + row, a.L_coeff + row,
-                                           a.L_E_cross + row, n_el, E, step, fkey);
-        s.stragg += loss;
-        const double delta_cut = E - s.E_cut;
-        if (loss > delta_cut || (loss == delta_cut && !geometry_event)) {
-            step = loss > 0.0 ? step * (delta_cut / loss) : 0.0;
-            cutoff = true;
-            limited = false;
-            up = dn = exit_top = exit_bot = side = false;
-            E_end = s.E_cut;
-        } else {
-            E_end = E - loss;
-        }
-        if (midpoint) {
-            const LutPos c = lut.pos(0.5 * (E + E_end));
-            t_end = s.clock + step * lut.inv_beta(c);
-        } else {
-            t_end = s.clock + step * inv_beta_j;
-        }
-    } else {
-        double cut_dist;
-        if (midpoint) {
-            const LutPos c = lut.pos(0.5 * (E + s.E_cut));
-            cut_dist = (s.E_cut - E) / lut.dEds(row_off, c);
-        } else {
-            cut_dist = (s.E_cut - E) / dEds;
-        }
-        if (cut_dist < step || (cut_dist == step && !geometry_event)) {
-            step = cut_dist;
-            cutoff = true;
-            up = dn = exit_top = exit_bot = side = false;
-        }
-        if (a.max_dE_frac > 0.0 && !cutoff) {
-            const double se = a.max_dE_frac * E / (-dEds);
-            if (se < step) {
-                step = se;
-                limited = true;
-                up = dn = exit_top = exit_bot = side = false;
-            }
-        }
-        if (midpoint) {
-            if (cutoff) {
-                E_end = s.E_cut;
-            } else {
-                const double E_pred = E + dEds * step;
-                const LutPos m = lut.pos(0.5 * (E + E_pred));
-                E_end = E + step * lut.dEds(row_off, m);
-            }
-            const LutPos c = lut.pos(0.5 * (E + E_end));
-            t_end = s.clock + step * lut.inv_beta(c);
-        } else {
-            E_end = cutoff ? s.E_cut : E + dEds * step;
-            t_end = s.clock + step * inv_beta_j;
-        }
-    }
-    int8_t event = EV_ELASTIC;
-    if (limited) event = EV_SUBSTEP;
-    else if (cutoff) event = EV_CUTOFF;
-    else if (exit_top) event = EV_EXIT_TOP;
-    else if (exit_bot) event = EV_EXIT_BOTTOM;
-    else if (side) event = EV_EXIT_SIDE;
-    else if (up || dn) event = EV_LAYER;
-
-    if (kRows && s.nseg < a.cap) {
-        const int64_t slot = (s.e - a.e_start) * a.cap + s.nseg;
-        a.seg_dir[3 * slot] = dx;
-        a.seg_dir[3 * slot + 1] = dy;
-        a.seg_dir[3 * slot + 2] = dz;
-        a.seg_mid[3 * slot] = px + 0.5 * step * dx;
-        a.seg_mid[3 * slot + 1] = py + 0.5 * step * dy;
-        a.seg_mid[3 * slot + 2] = pz + 0.5 * step * dz;
-        a.seg_len[slot] = step;
-        a.seg_E[slot] = E;
-        a.seg_t0[slot] = s.clock;
-        a.seg_id[slot] = s.id;
-        a.seg_lay[slot] = (int16_t)L;
-        if (midpoint) {
-            a.seg_E_end[slot] = E_end;
-            a.seg_t_end[slot] = t_end;
-            a.seg_flight[slot] = s.flight;
-            a.seg_substep[slot] = s.substep;
-            a.seg_event[slot] = event;
-        }
-    }
-    s.nseg += 1;
-    s.px = px + step * dx;
-    s.py = py + step * dy;
-    s.pz = pz + step * dz;
-    s.E = E_end;
-    s.clock = t_end;
-    s.tau -= step / lam;
-    if (s.tau < 0.0) s.tau = 0.0;
-    if (limited) {
-        s.substep += 1;
-        s.steps += 1;
-        return s.steps < a.max_steps ? -1 : EXIT_STEP_LIMITED;
-    }
-    if (exit_top || exit_bot || side || cutoff)
-        return exit_top ? EXIT_BACK : exit_bot ? EXIT_TRANS : side ? EXIT_SIDE : EXIT_CUTOFF;
-    s.flight += 1;
-    s.substep = 0;
-    s.tau = -1.0;
-    if (up || dn) {
-        s.pz += (s.dz > 0.0 ? 1.0 : -1.0) * 1e-6;
-        s.steps += 1;
-        return s.steps < a.max_steps ? -1 : EXIT_STEP_LIMITED;
-    }
-    // Elastic collision: element (cores.py:1094-1105), screened-Rutherford angle.
-    const size_t el_row = (size_t)L * max_el;
-    int i_el = 0;
-    if (n_el != 1) {
-        const double u = stream_uniform(s.key, s.draws);
-        s.draws += 1ULL;
-        i_el = n_el - 1;
-        for (int k = 0; k < n_el; ++k) {
-            if (lut.cdf((el_row + k) * nE, p) > u) {
-                i_el = k;
-                break;
-            }
-        }
-    }
-    const LutPos q = lut.pos(s.E);
-    const double alpha = lut.alpha((el_row + i_el) * nE, q);
-    const double cos_u = stream_uniform(s.key, s.draws);
-    const double phi_u = stream_uniform(s.key, s.draws + 1ULL);
-    s.draws += 2ULL;
-    const double cos_t = 1.0 - 2.0 * alpha * cos_u / (1.0 + alpha - cos_u);
-    // core_geometry.py:16-36
+// core_geometry.py:16-36: rotate the lane's direction by polar cosine cos_t
+// and azimuth phi about itself (kMixed: FP32 sqrt(1 - cos^2) and sin/cos).
+template <bool kMixed>
+__device__ __forceinline__ void rotate_lane(Lane &s, double cos_t, double phi) {
+    const double dx = s.dx, dy = s.dy, dz = s.dz;
     const double s2 = 1.0 - cos_t * cos_t;
     double sin_t, sin_phi, cos_phi;
-    const double phi = 2.0 * kPi * phi_u;
     if (kMixed) {
         const float s2f = (float)s2;
         sin_t = (double)sqrtf(s2f > 0.0f ? s2f : 0.0f);
@@ -495,6 +306,388 @@ __device__ int step_lane(const hr_transport_args &a, Lane &s) {
     s.dx = ox / mag;
     s.dy = oy / mag;
     s.dz = oz / mag;
+}
+
+__device__ __forceinline__ void store_lane(const hr_transport_args &a, const Lane &s, int8_t code) {
+    const int64_t e = s.e, i = e - a.e_start;
+    a.pos[3 * e] = s.px;
+    a.pos[3 * e + 1] = s.py;
+    a.pos[3 * e + 2] = s.pz;
+    a.dirs[3 * e] = s.dx;
+    a.dirs[3 * e + 1] = s.dy;
+    a.dirs[3 * e + 2] = s.dz;
+    a.E_keV[e] = s.E;
+    a.clock[e] = s.clock;
+    if (a.straggle_on) a.stragg_dE[e] = s.stragg;
+    a.seg_count[i] = s.nseg;
+    a.exit_code[i] = code;
+}
+
+// One loop iteration of cores.py:408-1239 (per_electron, lut).  Returns the
+// exit code when the history ends, or -1 when it continues.  kRows = false
+// never writes rows (tally-only callers), whatever a.cap says.  kMixed is the
+// reference's "mixed" transport precision: FP32 table lookups, query
+// logarithms, sqrt(1 - cos^2) and azimuth sin/cos; every accumulated quantity,
+// free-path draw and the straggling stay FP64.
+template <bool kRows = true, bool kMixed = false, bool kHard = false>
+__device__ int step_lane(const hr_transport_args &a, Lane &s) {
+    const Lut<kMixed> lut{a};
+    const int nE = a.lut_n_energy;
+    const int max_el = a.max_el;
+    int L = 0;
+    if (a.n_layers != 1) {
+        int lo = 0, hi = a.n_layers - 1;  // searchsorted(side="right")
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (a.internal_bounds[mid] <= s.pz) lo = mid + 1;
+            else hi = mid;
+        }
+        L = lo;
+    }
+    const int n_el = a.L_nel[L];
+    const double z_top = a.L_top[L], z_bot = a.L_bot[L];
+    const double E = s.E;
+    const LutPos p = lut.pos(E);
+    const size_t row_off = (size_t)L * nE;
+    const double total_rate = lut.rate(row_off, p);
+    const double lam = 1e8 / total_rate;
+    if (s.tau < 0.0) {
+        s.tau = -log(stream_uniform(s.key, s.draws));
+        s.draws += 1ULL;
+    }
+    double step = s.tau * lam;
+    // Hard-event modes: optical depths on their own keys against the row-start
+    // rates; a radiative proposal that wins clears a hard-collision one.
+    bool hard_j = false, rad_j = false;
+    double mu_hard = 0.0, mu_rad = 0.0, rad_dEds = 0.0;
+    if constexpr (kHard) {
+        const hr_hard_events &h = a.hard;
+        if (h.shell_on) {
+            mu_hard = lerp(h.lut_sh_rate + row_off, p);
+            if (s.tau_h < 0.0) {
+                s.tau_h = -log(stream_uniform(s.hkey, s.hdraws));
+                s.hdraws += 1ULL;
+            }
+            if (mu_hard > 0.0 && s.tau_h / mu_hard < step) {
+                step = s.tau_h / mu_hard;
+                hard_j = true;
+            }
+        }
+        if (h.rad_on) {
+            rad_dEds = lerp(h.lut_rad_soft + row_off, p);
+            mu_rad = lerp(h.lut_rad_rate + row_off, p);
+            if (s.tau_r < 0.0) {
+                s.tau_r = -log(stream_uniform(s.rkey, s.rdraws));
+                s.rdraws += 1ULL;
+            }
+            if (mu_rad > 0.0 && s.tau_r / mu_rad < step) {
+                step = s.tau_r / mu_rad;
+                rad_j = true;
+                hard_j = false;
+            }
+        }
+    }
+    const double dx = s.dx, dy = s.dy, dz = s.dz, px = s.px, py = s.py, pz = s.pz;
+    bool up = false, dn = false, side = false;
+    if (a.finite_footprint) {
+        double best = INFINITY;
+        int face = -1;
+        const double hw = 0.5 * a.width_ang, hh = 0.5 * a.height_ang;
+        double t;
+        if (dx < 0.0) { t = (-hw - px) / dx; if (t > 0.0 && t < best) { best = t; face = 0; } }
+        if (dx > 0.0) { t = (hw - px) / dx; if (t > 0.0 && t < best) { best = t; face = 1; } }
+        if (dy < 0.0) { t = (-hh - py) / dy; if (t > 0.0 && t < best) { best = t; face = 2; } }
+        if (dy > 0.0) { t = (hh - py) / dy; if (t > 0.0 && t < best) { best = t; face = 3; } }
+        if (dz < 0.0) { t = (z_top - pz) / dz; if (t > 0.0 && t < best) { best = t; face = 4; } }
+        if (dz > 0.0) { t = (z_bot - pz) / dz; if (t > 0.0 && t < best) { best = t; face = 5; } }
+        if (step > best) {
+            step = best;
+            up = face == 4;
+            dn = face == 5;
+            side = face >= 0 && face <= 3;
+        }
+    } else if (dz < 0.0) {
+        const double sb = (pz - z_top) / (-dz);
+        if (step > sb) { step = sb; up = true; }
+    } else if (dz > 0.0) {
+        const double sb = (z_bot - pz) / dz;
+        if (step > sb) { step = sb; dn = true; }
+    }
+    bool exit_top = up && z_top <= 0.0;
+    bool exit_bot = dn && z_bot >= a.z_total;
+    // soft radiative stopping joins the collision stopping as a mean loss
+    const double dEds = kHard ? lut.dEds(row_off, p) - rad_dEds : lut.dEds(row_off, p);
+    const double inv_beta_j = lut.inv_beta(p);
+    bool cutoff = false, limited = false;
+    bool geometry_event = up || dn || side;
+    double E_end, t_end;
+    const bool midpoint = a.energy_model == 1;
+    if (a.straggle_on) {
+        if (a.max_dE_frac > 0.0) {
+            const double se = a.max_dE_frac * E / (-dEds);
+            if (se < step) {
+                step = se;
+                limited = true;
+                up = dn = exit_top = exit_bot = side = false;
+                geometry_event = false;
+            }
+        }
+        const uint64_t ukey = splitmix64(s.key ^ kUrbanSalt);
+        const uint64_t combined = ((uint64_t)s.flight << 32) + (uint64_t)s.substep;
+        const uint64_t fkey = splitmix64(ukey + kGolden * combined);
+        const size_t row = (size_t)L * max_el;
+        double loss;
+        if (kHard && a.hard.shell_on) {
+            // soft collision losses only: the two-moment sampler replaces Urban's
+            const double omega2 = lerp(a.hard.lut_sh_omega2 + row_off, p);
+            uint64_t counter = 0;
+            loss = shell::soft_loss(-(dEds + rad_dEds) * step, omega2 * step, fkey, counter);
+        } else {
+            loss = urban_compound(a.L_Z + row, a.L_J + row, a.L_k + row, a.L_coeff + row, a.L_E_cross + row,
+                                  n_el, E, step, fkey);
+        }
+        if (kHard) loss += rad_dEds * step;
+        s.stragg += loss;
+        const double delta_cut = E - s.E_cut;
+        if (loss > delta_cut || (loss == delta_cut && !geometry_event)) {
+            step = loss > 0.0 ? step * (delta_cut / loss) : 0.0;
+            cutoff = true;
+            limited = false;
+            up = dn = exit_top = exit_bot = side = false;
+            E_end = s.E_cut;
+        } else {
+            E_end = E - loss;
+        }
+        if (midpoint) {
+            const LutPos c = lut.pos(0.5 * (E + E_end));
+            t_end = s.clock + step * lut.inv_beta(c);
+        } else {
+            t_end = s.clock + step * inv_beta_j;
+        }
+    } else {
+        double cut_dist;
+        if (midpoint) {
+            const LutPos c = lut.pos(0.5 * (E + s.E_cut));
+            double rate_c = lut.dEds(row_off, c);
+            if (kHard && a.hard.rad_on) rate_c -= lerp(a.hard.lut_rad_soft + row_off, c);
+            cut_dist = (s.E_cut - E) / rate_c;
+        } else {
+            cut_dist = (s.E_cut - E) / dEds;
+        }
+        if (cut_dist < step || (cut_dist == step && !geometry_event)) {
+            step = cut_dist;
+            cutoff = true;
+            up = dn = exit_top = exit_bot = side = false;
+        }
+        if (a.max_dE_frac > 0.0 && !cutoff) {
+            const double se = a.max_dE_frac * E / (-dEds);
+            if (se < step) {
+                step = se;
+                limited = true;
+                up = dn = exit_top = exit_bot = side = false;
+            }
+        }
+        if (midpoint) {
+            if (cutoff) {
+                E_end = s.E_cut;
+            } else {
+                const double E_pred = E + dEds * step;
+                const LutPos m = lut.pos(0.5 * (E + E_pred));
+                double rate_m = lut.dEds(row_off, m);
+                if (kHard && a.hard.rad_on) rate_m -= lerp(a.hard.lut_rad_soft + row_off, m);
+                E_end = E + step * rate_m;
+            }
+            const LutPos c = lut.pos(0.5 * (E + E_end));
+            t_end = s.clock + step * lut.inv_beta(c);
+        } else {
+            E_end = cutoff ? s.E_cut : E + dEds * step;
+            t_end = s.clock + step * inv_beta_j;
+        }
+    }
+    int8_t event = EV_ELASTIC;
+    if (limited) event = EV_SUBSTEP;
+    else if (cutoff) event = EV_CUTOFF;
+    else if (exit_top) event = EV_EXIT_TOP;
+    else if (exit_bot) event = EV_EXIT_BOTTOM;
+    else if (side) event = EV_EXIT_SIDE;
+    else if (up || dn) event = EV_LAYER;
+    else if (kHard && hard_j) event = EV_HARD_INELASTIC;
+    else if (kHard && rad_j) event = EV_HARD_RADIATIVE;
+
+    // Hard events are sampled before the row is written (its columns carry them).
+    double hard_W = 0.0, hard_cos = 1.0, hard_phi = 0.0, rad_k = 0.0;
+    int16_t hard_code = -1, rad_Z = 0;
+    if constexpr (kHard) {
+        const hr_hard_events &h = a.hard;
+        if (event == EV_HARD_INELASTIC) {
+            // channel, transfer, recoil, azimuth: four draws at the row-start energy
+            const double u_ch = stream_uniform(s.hkey, s.hdraws);
+            const double u_w = stream_uniform(s.hkey, s.hdraws + 1ULL);
+            const double u_q = stream_uniform(s.hkey, s.hdraws + 2ULL);
+            const double u_phi = stream_uniform(s.hkey, s.hdraws + 3ULL);
+            s.hdraws += 4ULL;
+            const double target = u_ch * mu_hard;
+            const int nch = h.sh_nch[L];
+            int ch = nch - 1;
+            double cum = 0.0;
+            for (int k = 0; k < nch; ++k) {
+                cum += lerp(h.lut_sh_ch_rate + ((size_t)L * h.sh_max_ch + k) * nE, p);
+                if (cum > target) {
+                    ch = k;
+                    break;
+                }
+            }
+            const int c = L * h.sh_max_ch + ch;
+            const double E_eV = E * 1.0e3;
+            const double W_eV = shell::sample_transfer(E_eV, h.sh_U[c], h.sh_W[c], h.sh_branch[c], h.sh_cutoff_eV, u_w);
+            hard_W = W_eV * 1.0e-3;
+            hard_code = h.sh_code[c];
+            if (E_end - hard_W <= s.E_cut) {
+                event = EV_CUTOFF;  // absorbed at the collision point: a terminal row
+                cutoff = true;
+            } else {
+                hard_cos = shell::primary_cosine(E_eV, h.sh_U[c], h.sh_W[c], h.sh_branch[c], W_eV, u_q);
+                hard_phi = 2.0 * kPi * u_phi;
+            }
+        } else if (event == EV_HARD_RADIATIVE) {
+            // element and photon from the exact moments at the row-end energy
+            const rad::Tables rt = rad_tables(h, max_el);
+            const double E_end_eV = E_end * 1.0e3;
+            const double plasma = h.rad_plasma[L];
+            const double u_el = stream_uniform(s.rkey, s.rdraws);
+            const double u_k = stream_uniform(s.rkey, s.rdraws + 1ULL);
+            s.rdraws += 2ULL;
+            double rates[8];
+            double mu_event = 0.0;
+            const int n_rad = n_el < 8 ? n_el : 8;
+            for (int k = 0; k < n_rad; ++k) {
+                const int slot = L * max_el + k;
+                rates[k] = rt.ncm3[slot] * rad::kCmPerAng *
+                           rad::moment(rad::HARD_RATE, rt, slot, E_end_eV, rt.cutoff_eV, plasma);
+                mu_event += rates[k];
+            }
+            const double target = u_el * mu_event;
+            int sel = n_rad - 1;
+            double cum = 0.0;
+            for (int k = 0; k < n_rad; ++k) {
+                cum += rates[k];
+                if (cum > target) {
+                    sel = k;
+                    break;
+                }
+            }
+            const int slot = L * max_el + sel;
+            rad_k = rad::sample_photon(rt, slot, E_end_eV, u_k, plasma);
+            rad_Z = (int16_t)rt.z[slot];
+            if (E_end - rad_k * 1.0e-3 <= s.E_cut) {
+                event = EV_CUTOFF;
+                cutoff = true;
+            }
+        }
+    }
+
+    if (kRows && s.nseg < a.cap) {
+        const int64_t slot = (s.e - a.e_start) * a.cap + s.nseg;
+        a.seg_dir[3 * slot] = dx;
+        a.seg_dir[3 * slot + 1] = dy;
+        a.seg_dir[3 * slot + 2] = dz;
+        a.seg_mid[3 * slot] = px + 0.5 * step * dx;
+        a.seg_mid[3 * slot + 1] = py + 0.5 * step * dy;
+        a.seg_mid[3 * slot + 2] = pz + 0.5 * step * dz;
+        a.seg_len[slot] = step;
+        a.seg_E[slot] = E;
+        a.seg_t0[slot] = s.clock;
+        a.seg_id[slot] = s.id;
+        a.seg_lay[slot] = (int16_t)L;
+        if (midpoint) {
+            a.seg_E_end[slot] = E_end;
+            a.seg_t_end[slot] = t_end;
+            a.seg_flight[slot] = s.flight;
+            a.seg_substep[slot] = s.substep;
+            a.seg_event[slot] = event;
+            if constexpr (kHard) {
+                if (a.hard.seg_hard_W) {
+                    a.hard.seg_hard_W[slot] = hard_W;
+                    a.hard.seg_hard_ch[slot] = hard_code;
+                }
+                if (a.hard.seg_rad_k) {
+                    a.hard.seg_rad_k[slot] = rad_k;
+                    a.hard.seg_rad_Z[slot] = rad_Z;
+                }
+            }
+        }
+    }
+    s.nseg += 1;
+    s.px = px + step * dx;
+    s.py = py + step * dy;
+    s.pz = pz + step * dz;
+    s.E = E_end;
+    s.clock = t_end;
+    s.tau -= step / lam;
+    if (s.tau < 0.0) s.tau = 0.0;
+    if constexpr (kHard) {
+        if (a.hard.shell_on) {
+            s.tau_h -= step * mu_hard;
+            if (s.tau_h < 0.0) s.tau_h = 0.0;
+        }
+        if (a.hard.rad_on) {
+            s.tau_r -= step * mu_rad;
+            if (s.tau_r < 0.0) s.tau_r = 0.0;
+        }
+    }
+    if (limited) {
+        s.substep += 1;
+        s.steps += 1;
+        return s.steps < a.max_steps ? -1 : EXIT_STEP_LIMITED;
+    }
+    if (exit_top || exit_bot || side || cutoff)
+        return exit_top ? EXIT_BACK : exit_bot ? EXIT_TRANS : side ? EXIT_SIDE : EXIT_CUTOFF;
+    s.flight += 1;
+    s.substep = 0;
+    s.tau = -1.0;
+    if constexpr (kHard) {
+        s.tau_h = -1.0;
+        s.tau_r = -1.0;
+    }
+    if (up || dn) {
+        s.pz += (s.dz > 0.0 ? 1.0 : -1.0) * 1e-6;
+        s.steps += 1;
+        return s.steps < a.max_steps ? -1 : EXIT_STEP_LIMITED;
+    }
+    if constexpr (kHard) {
+        if (event == EV_HARD_RADIATIVE) {  // debit the photon; no deflection, no elastic draw
+            s.E -= rad_k * 1.0e-3;
+            s.steps += 1;
+            return s.steps < a.max_steps ? -1 : EXIT_STEP_LIMITED;
+        }
+        if (event == EV_HARD_INELASTIC) {  // the loss and recoil deflection replace the elastic draw
+            s.E -= hard_W;
+            rotate_lane<kMixed>(s, hard_cos, hard_phi);
+            s.steps += 1;
+            return s.steps < a.max_steps ? -1 : EXIT_STEP_LIMITED;
+        }
+    }
+    // Elastic collision: element (cores.py:1094-1105), screened-Rutherford angle.
+    const size_t el_row = (size_t)L * max_el;
+    int i_el = 0;
+    if (n_el != 1) {
+        const double u = stream_uniform(s.key, s.draws);
+        s.draws += 1ULL;
+        i_el = n_el - 1;
+        for (int k = 0; k < n_el; ++k) {
+            if (lut.cdf((el_row + k) * nE, p) > u) {
+                i_el = k;
+                break;
+            }
+        }
+    }
+    const LutPos q = lut.pos(s.E);
+    const double alpha = lut.alpha((el_row + i_el) * nE, q);
+    const double cos_u = stream_uniform(s.key, s.draws);
+    const double phi_u = stream_uniform(s.key, s.draws + 1ULL);
+    s.draws += 2ULL;
+    const double cos_t = 1.0 - 2.0 * alpha * cos_u / (1.0 + alpha - cos_u);
+    rotate_lane<kMixed>(s, cos_t, 2.0 * kPi * phi_u);
     s.steps += 1;
     return s.steps < a.max_steps ? -1 : EXIT_STEP_LIMITED;
 }

@@ -69,8 +69,11 @@ class LineTables:
 
 
 def line_tables(crystal_key, hkl_list, E_grid_eV, composition=None, tab_ratio=1.0 + 2.0e-3) -> LineTables:
-    """Reflection rows and FP32 coupling tables for a crystal (``cxr/_batched.py:230-330``)."""
-    cr = CRYSTALS[crystal_key]
+    """Reflection rows and FP32 coupling tables for a crystal (``cxr/_batched.py:230-330``).
+
+    ``crystal_key`` names a catalog crystal or is a ``Crystal`` (its own
+    ``surface_hkl`` then sets the sample frame)."""
+    cr = CRYSTALS[crystal_key] if isinstance(crystal_key, str) else crystal_key
     comp = cr.composition if composition is None else list(composition)
     lo, hi = keep_window(np.asarray(E_grid_eV, dtype=float))
     E_tab = tabulation_grid(max(lo, 10.0), hi, tab_ratio)

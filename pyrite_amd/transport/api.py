@@ -60,6 +60,50 @@ def normalized_beam_dir(beam_dir):
     return beam_dir
 
 
+INELASTIC_MODELS = ("continuous", "shell-soft-hard")
+RADIATIVE_MODELS = ("uncoupled", "analytic-soft-hard")
+
+
+def _hard_modes(tables, lut, dev, inelastic_model, inelastic_cutoff_eV, radiative_model, radiative_cutoff_eV,
+                E_cut_min_keV, energy_model):
+    """Device tables of the requested hard-event modes, or None (both off).
+
+    Fail-closed like the reference (``hard_inelastic.py:50-75``,
+    ``hard_radiative.py:694-741``): unknown models, cutoffs without their
+    mode, non-midpoint runs and a radiative cutoff above the electron cutoff
+    raise."""
+    from .hard_events import DeviceHardModes, build_radiative_tables, build_shell_tables
+
+    if inelastic_model not in INELASTIC_MODELS:
+        raise ValueError(f"inelastic_model must be one of {', '.join(INELASTIC_MODELS)}")
+    if radiative_model == "bremslib-soft-hard":
+        raise NotImplementedError("radiative_model='bremslib-soft-hard' needs the fetched BremsLib tables; the hip "
+                                  "backend offers 'analytic-soft-hard' (substitute tables)")
+    if radiative_model not in RADIATIVE_MODELS:
+        raise ValueError(f"radiative_model must be one of {', '.join(RADIATIVE_MODELS)}")
+    shell_on = inelastic_model == "shell-soft-hard"
+    rad_on = radiative_model == "analytic-soft-hard"
+    if not shell_on and inelastic_cutoff_eV is not None:
+        raise ValueError("inelastic_cutoff_eV requires inelastic_model='shell-soft-hard'")
+    if not rad_on and radiative_cutoff_eV is not None:
+        raise ValueError("radiative_cutoff_eV requires radiative_model='analytic-soft-hard'")
+    if not (shell_on or rad_on):
+        return None
+    if energy_model != "midpoint":
+        raise ValueError("the hard-event modes require energy_model='midpoint'")
+    rad = None
+    if rad_on:
+        cut = E_cut_min_keV * 1e3 if radiative_cutoff_eV is None else float(radiative_cutoff_eV)
+        if cut > E_cut_min_keV * 1e3:
+            raise ValueError("radiative_cutoff_eV must not exceed the electron cutoff")
+        rad = build_radiative_tables(tables, cut)
+    shell = None
+    if shell_on:
+        cut = 1000.0 if inelastic_cutoff_eV is None else float(inelastic_cutoff_eV)
+        shell = build_shell_tables(tables, lut.E_grid, cut)
+    return DeviceHardModes(tables, lut, dev, rad=rad, shell=shell)
+
+
 def simulate_trajectories(
     E0_keV,
     Ne,
@@ -83,6 +127,10 @@ def simulate_trajectories(
     initial_state=None,
     electron_start=0,
     raise_on_step_limit=True,
+    inelastic_model="continuous",
+    inelastic_cutoff_eV=None,
+    radiative_model="uncoupled",
+    radiative_cutoff_eV=None,
 ):
     """Transport ``Ne`` electrons of ``E0_keV`` into a slab (hip core).
 
@@ -91,6 +139,18 @@ def simulate_trajectories(
     addresses electrons ``[start, start+Ne)`` of a larger run: keys are
     ``splitmix64(seed + PHI*(e+1))`` of the global id, so blocks reproduce
     the matching slice of one run (``kinematics.py:226-245``).
+
+    Hard-event modes (the reference's ``inelastic_model`` and
+    ``radiative_model`` switches, ``transport/api.py:56-60,212-250``):
+    ``inelastic_model="shell-soft-hard"`` runs explicit hard collisions above
+    ``inelastic_cutoff_eV`` (default 1 keV) with two-moment soft losses below;
+    ``radiative_model="analytic-soft-hard"`` runs explicit bremsstrahlung
+    photons above ``radiative_cutoff_eV`` (default: the electron cutoff) with
+    soft radiative stopping below.  Both run on this module's analytic
+    substitute tables (``hard_events``); ``"bremslib-soft-hard"`` needs the
+    fetched BremsLib tables and is refused.  Mode rows add
+    ``hard_inelastic_W_keV``/``hard_inelastic_channel`` and
+    ``hard_radiative_k_eV``/``hard_radiative_Z``.
     """
     import torch
 
@@ -118,6 +178,12 @@ def simulate_trajectories(
     if np.any(E_cut_h >= E_h):
         raise ValueError("each electron cutoff energy must be below its initial energy")
     lut = build_transport_lut(float(E_cut_h.min()), float(E_h.max()), tables)
+    hard = _hard_modes(tables, lut, dev, inelastic_model, inelastic_cutoff_eV, radiative_model,
+                       radiative_cutoff_eV, float(E_cut_h.min()), energy_model)
+    if hard is not None:
+        from .hard_events import lut_for
+
+        lut = lut_for(lut, hard.shell)
     dtabs = DeviceTables(tables, lut, dev)
     control = TransportControl(
         max_steps=int(max_steps),
@@ -128,7 +194,9 @@ def simulate_trajectories(
         width_ang=float(crystal_width_mm) * MM_TO_ANG if finite else 0.0,
         height_ang=float(crystal_height_mm) * MM_TO_ANG if finite else 0.0,
     )
-    core = make_hip_transport_core(dtabs, control)
+    core = make_hip_transport_core(dtabs, control, hard)
+    if hard is not None:
+        hard.set_keys(seed, electron_start, Ne)
 
     keys = torch.empty(Ne, dtype=torch.uint64, device=dev)
     _lib.call("hr_stream_keys", int(seed) & (2**64 - 1), int(electron_start), Ne, keys.data_ptr(),
@@ -141,7 +209,7 @@ def simulate_trajectories(
     alive = torch.ones(Ne, dtype=torch.uint8, device=dev)
     stragg = torch.zeros(Ne, dtype=torch.float64, device=dev)
     res = drive_batches(core, keys, (alive, clock, pos, dirs, E, E_cut), stragg, control, Ne, dev,
-                        per_electron_config)
+                        per_electron_config, extra_columns=() if hard is None else hard.row_columns)
     if raise_on_step_limit and res.counts["n_step_limited"]:
         raise TransportStepLimitError(
             f"{res.counts['n_step_limited']} histories reached max_steps={max_steps}"
@@ -180,6 +248,16 @@ def simulate_trajectories(
         out["flight_id"] = r["seg_flight"]
         out["substep_id"] = r["seg_substep"]
         out["event_kind"] = r["seg_event"]
+    if hard is not None and hard.shell is not None:
+        out["hard_inelastic_W_keV"] = r["seg_hard_W"]
+        out["hard_inelastic_channel"] = r["seg_hard_ch"]
+        out["inelastic"] = {"model": "shell-soft-hard", "cutoff_eV": hard.shell.cutoff_eV,
+                            "tables": "analytic-substitute"}
+    if hard is not None and hard.rad is not None:
+        out["hard_radiative_k_eV"] = r["seg_rad_k"]
+        out["hard_radiative_Z"] = r["seg_rad_Z"]
+        out["radiative"] = {"model": "analytic-soft-hard", "cutoff_eV": hard.rad.cutoff_eV,
+                            "plasma_energy_eV_by_layer": tuple(float(x) for x in hard.rad.plasma)}
     if not keep_segments_on_device:
         out = {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in out.items()}
     return out

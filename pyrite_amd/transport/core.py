@@ -167,7 +167,7 @@ def transport_args(tables: DeviceTables, control: TransportControl, run, state, 
     return a
 
 
-def make_hip_transport_core(tables: DeviceTables, control: TransportControl):
+def make_hip_transport_core(tables: DeviceTables, control: TransportControl, hard=None):
     """Return ``core(run, state, scratch, out, straggling, stream=None)``.
 
     ``run = (e_start, e_count, cap, keys)``; ``state = (alive, clock, pos,
@@ -176,12 +176,16 @@ def make_hip_transport_core(tables: DeviceTables, control: TransportControl):
     plus ``"work"``, the launch's 8-byte claim counter (the driver owns every
     buffer; the core allocates nothing);
     ``out = (seg_count, exit_code)``; ``straggling = (straggle_on, stragg_dE)``.
+    ``hard`` (a ``hard_events.DeviceHardModes``) switches on the hard-event
+    modes; its row columns then ride in ``scratch`` too.
     """
     lib = _lib.load()
     own = {}
 
     def core(run, state, scratch, out, straggling, stream=None, block_size=0):
         a = transport_args(tables, control, run, state, scratch, out, straggling, block_size)
+        if hard is not None and hard.on:
+            hard.fill(a.hard, scratch)
         if not a.work:  # a scratch dict without a counter: the core's own (calls of one core run in order)
             if "work" not in own:
                 import torch
@@ -193,8 +197,9 @@ def make_hip_transport_core(tables: DeviceTables, control: TransportControl):
     return core
 
 
-def alloc_rows(m: int, cap: int, midpoint: bool, device):
-    """Scratch row grid ``(m*cap)`` in the reference's column dtypes."""
+def alloc_rows(m: int, cap: int, midpoint: bool, device, extra=()):
+    """Scratch row grid ``(m*cap)`` in the reference's column dtypes, plus
+    ``extra`` ``(name, dtype name)`` columns (the hard-event modes' rows)."""
     import torch
 
     n = int(m) * int(cap)
@@ -215,6 +220,8 @@ def alloc_rows(m: int, cap: int, midpoint: bool, device):
             seg_substep=torch.empty(n, dtype=torch.int64, device=device),
             seg_event=torch.empty(n, dtype=torch.int8, device=device),
         )
+    for name, dt in extra:
+        rows[name] = torch.empty(n, dtype=getattr(torch, dt), device=device)
     return rows
 
 
@@ -304,7 +311,7 @@ class DriveResult:
 
 def drive_batches(core, keys, state, stragg_dE, control: TransportControl, Ne: int, device,
                   config: PerElectronTransportConfig = DEFAULT_CONFIG, stream=None, compact=True,
-                  cap_hint=None, alloc=None, consume=None) -> DriveResult:
+                  cap_hint=None, alloc=None, consume=None, extra_columns=()) -> DriveResult:
     """Capacity-replayed batches (``batching.py:477-628``), rows kept on device.
 
     ``compact=False`` skips the electron-major compaction and returns the slot
@@ -327,7 +334,8 @@ def drive_batches(core, keys, state, stragg_dE, control: TransportControl, Ne: i
 
     alive, clock, pos, dirs, E_keV, E_cut = state
     midpoint = control.energy_model == 1
-    row_bytes = _ROW_BYTES_MIDPOINT if midpoint else _ROW_BYTES_FROZEN
+    row_bytes = (_ROW_BYTES_MIDPOINT if midpoint else _ROW_BYTES_FROZEN) + sum(
+        {"float64": 8, "int16": 2, "int64": 8, "int8": 1}[dt] for _, dt in extra_columns)
     cap = max(1, int(config.seg_capacity if cap_hint is None else cap_hint))
     budget = int(config.scratch_budget_bytes)
     work = torch.zeros(1, dtype=torch.int64, device=device)  # the core's claim counter (this call's own)
@@ -347,7 +355,8 @@ def drive_batches(core, keys, state, stragg_dE, control: TransportControl, Ne: i
         while True:
             rows = None  # drop a replayed grid before its successor is allocated
             try:
-                rows = alloc_rows(m, cap, midpoint, device) if alloc is None else alloc(e, m, cap, midpoint, device)
+                rows = (alloc_rows(m, cap, midpoint, device, extra_columns) if alloc is None
+                        else alloc(e, m, cap, midpoint, device))
                 rows.setdefault("work", work)
                 seg_count = torch.zeros(m, dtype=torch.int64, device=device)
                 exit_code = torch.zeros(m, dtype=torch.int8, device=device)
@@ -397,7 +406,7 @@ def drive_batches(core, keys, state, stragg_dE, control: TransportControl, Ne: i
     elif batches:
         joined = {k: torch.cat([b[k] for b in batches]) for k in batches[0]}
     else:
-        joined = alloc_rows(0, 1, midpoint, device)
+        joined = alloc_rows(0, 1, midpoint, device, extra_columns)
     seg_count = torch.cat(counts_all) if counts_all else torch.zeros(0, dtype=torch.int64, device=device)
     exit_code = torch.cat(exit_all) if exit_all else torch.zeros(0, dtype=torch.int8, device=device)
     codes = torch.bincount(exit_code.to(torch.int64), minlength=6).cpu().numpy()

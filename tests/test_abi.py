@@ -66,6 +66,7 @@ STRUCTS = {
     "hr_continuation": "Continuation",
     "hr_dose_args": "DoseArgs",
     "hr_workspace": "Workspace",
+    "hr_hard_events": "HardEvents",
 }
 
 

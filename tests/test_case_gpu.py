@@ -96,3 +96,25 @@ def test_silicon_twiss_case_agrees_with_the_oracle_per_bin(device):
     assert g_lines.sum() > 0 and c_lines.sum() > 0
     _agree(g_brem, c_brem, "brem")
     _agree(g_lines, c_lines, "lines")
+
+
+def test_streamed_hopg_case_agrees_with_the_oracle_per_bin(device):
+    """The streamed route (continuation rounds: 512 lanes, 16 rows per launch,
+    each launch scored before the next) against the oracle per bin, to the
+    same binomial/4-sigma bar as the batched route."""
+    from pyrite_amd.pipeline import CaseConfig, CasePipeline
+
+    pipe = CasePipeline(CaseConfig(Ne=PER_BATCH))
+    lines, brem = [], []
+    for b in range(BATCHES):
+        pipe.spec_lines.zero_()
+        pipe.spec_brem.zero_()
+        pipe.stream(b * PER_BATCH, PER_BATCH, n_total=PER_BATCH, lanes=512, rows_per_round=16)
+        ln, br = pipe.spectra()
+        lines.append(ln)
+        brem.append(br)
+    g_lines, g_brem = np.array(lines), np.array(brem)
+    c_lines, c_brem = _cpu_batches(pipe, first=30_000_000)
+    assert g_lines.sum() > 0 and c_lines.sum() > 0
+    _agree(g_brem, c_brem, "streamed brem")
+    _agree(g_lines, c_lines, "streamed lines")

@@ -145,6 +145,7 @@ def test_brem_matches_oracle(oracle, device):
     mu = attenuation.mu_total_inv_ang(comp, E_grid).astype(np.float32)
     want = oracle.brem_reduce(seg, 1.0e6, n_hat, [6.0], [0.1136e24], E_grid, mu)
     # FP32 cells: exp(-tau) underflows near tau ~ 88 where FP64 keeps 1e-38 of a
-    # contribution, so bins are compared relative to themselves plus 1e-12 of the peak
+    # contribution, so bins are compared relative to themselves plus 1e-12 of the
+    # peak; the measured maximum on these 5 MeV segments is 1.4e-7
     err = np.abs(got - want)
-    assert np.all(err <= 1e-5 * np.abs(want) + 1e-12 * want.max()), (err / np.maximum(want, 1e-300)).max()
+    assert np.all(err <= 1e-6 * np.abs(want) + 1e-12 * want.max()), (err / np.maximum(want, 1e-300)).max()
