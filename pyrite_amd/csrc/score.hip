@@ -77,29 +77,28 @@ __global__ void __launch_bounds__(256) cxr_lines_kernel(hr_line_args p) {
     });
 }
 
-// sin(pi r) for |r| <= 1/2: odd polynomial of degree 9, r (c1 + c3 r^2 + ... +
-// c9 r^8), fitted for minimax relative error on [0, 1/2] (Lawson's iteration
-// of weighted least squares on Chebyshev-clustered nodes): fit error 5.3e-9,
-// maximum relative error 1.9e-7 once evaluated in FP32 (the degree-13 Taylor
-// series evaluates to 2.1e-7 with two more FMAs).
-constexpr float kSinPi[5] = {3.1415927410125732f, -5.167709827423096f, 2.550069808959961f,
-                             -0.5982423424720764f, 0.07756078988313675f};
-__device__ __forceinline__ float sinpi_half(float r) {
-    const float r2 = r * r;
-    float s = kSinPi[4];
-    s = fmaf(s, r2, kSinPi[3]);
-    s = fmaf(s, r2, kSinPi[2]);
-    s = fmaf(s, r2, kSinPi[1]);
-    s = fmaf(s, r2, kSinPi[0]);
-    return s * r;
-}
+// sin(pi r)/pi for |r| <= 1/2 as r P(r^2): an odd polynomial of degree 9 fitted
+// to sin(pi r) for minimax relative error on [0, 1/2] (Lawson's iteration of
+// weighted least squares on Chebyshev-clustered nodes, fit error 5.3e-9), its
+// coefficients divided by pi and rounded to FP32.
+constexpr float kSinPiOverPi[5] = {1.0f, -1.6449331045150757f, 0.8117124438285828f, -0.19042645394802094f,
+                                   0.024688366800546646f};
 
-// [sin(pi u)/(pi u)]^2 from r = u - rint(u) (|r| <= 1/2) and u in FP32: the
-// polynomial sin(pi r) and one hardware reciprocal (v_rcp_f32, 1 ulp); u == 0 is
-// x == 0, where the reference substitutes x = 1e-20 and the profile is 1.
+// [sin(pi u)/(pi u)]^2 from r = u - rint(u) (|r| <= 1/2) and u in FP32:
+// x = r P(r^2) / u with one hardware reciprocal (v_rcp_f32, 1 ulp); maximum
+// relative error 5.4e-7 over |u| <= 5e4.  |x| <= 1 in exact arithmetic (the
+// central period has u == r, elsewhere |u| >= 1/2), so fminf(x, 1) only trims
+// rounding, and at u == 0 (r == 0) it turns 0 * inf = NaN into the limit 1,
+// where the reference substitutes x = 1e-20.
 __device__ __forceinline__ float sinc2_cell(float r, float uf) {
-    const float x = sinpi_half(r) * __builtin_amdgcn_rcpf((float)kPi * uf);
-    return uf == 0.0f ? 1.0f : x * x;
+    const float r2 = r * r;
+    float s = kSinPiOverPi[4];
+    s = fmaf(s, r2, kSinPiOverPi[3]);
+    s = fmaf(s, r2, kSinPiOverPi[2]);
+    s = fmaf(s, r2, kSinPiOverPi[1]);
+    s = fmaf(s, r2, kSinPiOverPi[0]);
+    const float x = fminf(r * __builtin_amdgcn_rcpf(uf) * s, 1.0f);
+    return x * x;
 }
 
 constexpr int kRedThreads = 256;
@@ -108,14 +107,21 @@ constexpr int kSincGroup = 4;   // cells summed in FP32 before each FP64 add (al
 
 // Dense sum over a uniform grid E_k = E0 + k dE.  Block b takes line tiles
 // b, b + gridDim.x, ...; thread t owns bins t + kRedThreads*m (m < BINS_PER_THREAD)
-// of the bin window starting at blockIdx.y * kRedThreads * BINS.
+// of the bin window starting at blockIdx.y * kRedThreads * BINS.  A staged line
+// is (u0, du) in one 16-byte LDS slot (one ds_read_b128) with the weights of
+// kSincGroup lines in one float4; a group whose lines all have zero weight
+// (copies that do not carry this tally) costs one LDS read.  The phase
+// u = u0 + k du is formed and reduced in FP64 per cell: on this device an FP64
+// FMA/add issues at the rate of a non-packed FP32 one, and FP32 stepping of the
+// reduced phase would take more instructions per cell.
 template <int BINS>
 __global__ void __launch_bounds__(kRedThreads) sincsq_uniform_kernel(
     const double *__restrict__ E_r, const double *__restrict__ a, const float *__restrict__ w,
     int64_t n_lines, double E0, double dE, int64_t n_E, double *__restrict__ partial) {
-    __shared__ double s_u0[kLineTile];
-    __shared__ double s_du[kLineTile];
-    __shared__ float s_w[kLineTile];
+    static_assert(kSincGroup == 4 && kLineTile % kSincGroup == 0, "weights are read as float4 groups");
+    __shared__ double2 s_ud[kLineTile];
+    __shared__ float4 s_w4[kLineTile / kSincGroup];
+    float *s_w = reinterpret_cast<float *>(s_w4);
     const int t = threadIdx.x;
     const int64_t bin0 = (int64_t)blockIdx.y * kRedThreads * BINS;
     double acc[BINS], kd[BINS];
@@ -131,26 +137,29 @@ __global__ void __launch_bounds__(kRedThreads) sincsq_uniform_kernel(
         // u(E_k) = a (E_k - E_r)/pi = u0 + k du
         if (l < n_lines) {
             const double al = a[l] * inv_pi;
-            s_u0[t] = al * (E0 - E_r[l]);
-            s_du[t] = al * dE;
+            s_ud[t] = make_double2(al * (E0 - E_r[l]), al * dE);
             s_w[t] = w[l];
         } else {
-            s_u0[t] = 0.0;
-            s_du[t] = 0.0;
+            s_ud[t] = make_double2(0.0, 0.0);
             s_w[t] = 0.0f;
         }
         __syncthreads();
         for (int q0 = 0; q0 < kLineTile; q0 += kSincGroup) {
+            const float4 w4 = s_w4[q0 / kSincGroup];
+            // uniform across the block: every thread reads the same slots
+            if (w4.x == 0.0f && w4.y == 0.0f && w4.z == 0.0f && w4.w == 0.0f) continue;
+            const float wg[kSincGroup] = {w4.x, w4.y, w4.z, w4.w};
             float g[BINS];
 #pragma unroll
             for (int m = 0; m < BINS; ++m) g[m] = 0.0f;
-            for (int q = q0; q < q0 + kSincGroup; ++q) {
-                const float wq = s_w[q];
-                if (wq == 0.0f) continue;  // uniform across the block
-                const double u0 = s_u0[q], du = s_du[q];
+#pragma unroll
+            for (int j = 0; j < kSincGroup; ++j) {
+                const float wq = wg[j];
+                if (wq == 0.0f) continue;
+                const double2 ud = s_ud[q0 + j];
 #pragma unroll
                 for (int m = 0; m < BINS; ++m) {
-                    const double u = fma(kd[m], du, u0);
+                    const double u = fma(kd[m], ud.y, ud.x);
                     const float r = (float)(u - rint(u));
                     g[m] = fmaf(wq, sinc2_cell(r, (float)u), g[m]);
                 }
@@ -262,6 +271,7 @@ __device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_ex
 __device__ __forceinline__ float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float fast_log2(float x) { return __builtin_amdgcn_logf(x); }
 constexpr float kLn2f = 0.69314718055994531f;
+constexpr double kLn2d = 0.6931471805599453;
 
 // BH+Elwert brem over segment tiles (brem.py:396-423, brem_jit_kernel.py:76-153).
 constexpr int kBremTile = 128;
@@ -269,32 +279,33 @@ constexpr int kBremTile = 128;
 // per-element loop unrolls with its constants in registers.
 template <int BINS, int NEL>
 __global__ void __launch_bounds__(kRedThreads) brem_kernel(hr_brem_args p, double *__restrict__ partial) {
-    __shared__ float s_T[kBremTile], s_pi[kBremTile], s_p0[kBremTile], s_p1[kBremTile];
+    // per staged segment: kinetic energy, momentum, the shorter and longer
+    // escape paths from its two ends along n_hat, their mean and difference,
+    // the per-element prefactors and whether any of them is nonzero
+    __shared__ float s_T[kBremTile], s_pi[kBremTile], s_plo[kBremTile], s_phi[kBremTile], s_pmid[kBremTile],
+        s_dp[kBremTile];
     __shared__ float s_pref[kBremTile * 4];
     __shared__ int s_live[kBremTile];
     const int t = threadIdx.x;
     const int64_t bin0 = (int64_t)blockIdx.y * kRedThreads * BINS;
     constexpr int n_el = NEL;
     double acc[BINS];
-    float kev[BINS], keV[BINS], mu[BINS];
+    // per-bin constants: k [keV], k / m^2 [1/keV], ln 2 / k [1/eV], mu, mu log2(e)
+    float keV[BINS], k_m2[BINS], ln2_inv_k[BINS], mu[BINS], mu2[BINS];
 #pragma unroll
     for (int m = 0; m < BINS; ++m) {
         acc[m] = 0.0;
         const int64_t k = bin0 + t + (int64_t)m * kRedThreads;
-        kev[m] = k < p.n_E ? (float)p.E_grid_eV[k] : 0.0f;
-        keV[m] = kev[m] * 1.0e-3f;
+        const double kev = k < p.n_E ? p.E_grid_eV[k] : 0.0;
+        keV[m] = (float)(kev * 1.0e-3);
+        k_m2[m] = (float)(kev * 1.0e-3 / (kMc2 * kMc2));
+        ln2_inv_k[m] = (float)(kLn2d / fmax(kev, 1.0e-30));
         mu[m] = k < p.n_E ? p.mu[k] : 0.0f;
-    }
-    float zi[4], zi2[4], inv_kev[BINS], mu2[BINS];
-    for (int el = 0; el < 4; ++el) {
-        zi[el] = el < n_el ? (float)(2.0 * kPi * p.Z[el] * kAlpha) : 0.0f;
-        zi2[el] = (float)((double)zi[el] * kLog2e);  // exp(-z/b) = exp2(-z log2(e) / b)
-    }
-#pragma unroll
-    for (int m = 0; m < BINS; ++m) {
-        inv_kev[m] = 1.0f / fmaxf(kev[m], 1.0e-30f);
         mu2[m] = (float)((double)mu[m] * kLog2e);
     }
+    float zi2[4];
+    for (int el = 0; el < 4; ++el)  // exp(-2 pi Z alpha / beta) = exp2(-(2 pi Z alpha log2 e) / beta)
+        zi2[el] = el < n_el ? (float)(2.0 * kPi * p.Z[el] * kAlpha * kLog2e) : 0.0f;
     const int64_t n_tiles = (p.n_seg + kBremTile - 1) / kBremTile;
     for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
         if (t < kBremTile) {
@@ -308,26 +319,34 @@ __global__ void __launch_bounds__(kRedThreads) brem_kernel(hr_brem_args p, doubl
                 // escape paths from both ends along n_hat (segment_escape.py:190-211)
                 const double half = 0.5 * (double)p.L[j];
                 const double rz = p.r_mid[3 * j + 2], vz = p.v_hat[3 * j + 2];
-                s_p0[t] = (float)slab_path(rz - half * vz, p.thickness, p.n_hat[2]);
-                s_p1[t] = (float)slab_path(rz + half * vz, p.thickness, p.n_hat[2]);
+                const double p0 = slab_path(rz - half * vz, p.thickness, p.n_hat[2]);
+                const double p1 = slab_path(rz + half * vz, p.thickness, p.n_hat[2]);
+                const double plo = fmin(p0, p1), phi = fmax(p0, p1);
+                s_plo[t] = (float)plo;
+                s_phi[t] = (float)phi;
+                s_pmid[t] = (float)(0.5 * (plo + phi));
+                s_dp[t] = (float)(phi - plo);
+                int live = 0;
                 for (int el = 0; el < 4; ++el) {
                     float pref = 0.0f;
                     if (el < n_el) {
                         // n L 1e-8 (16/3) alpha r_e^2 Z^2 beta_i (1 - exp(-2 pi Z alpha/beta_i)) / p_i^2
-                        const double den_i = 1.0 - exp(-(double)zi[el] / (double)bi);
+                        const double den_i = 1.0 - exp(-2.0 * kPi * p.Z[el] * kAlpha / (double)bi);
                         const double rw = p.row_weight ? (double)p.row_weight[j] : 1.0;
                         pref = (float)(rw * p.n_cm3[el] * (double)p.L[j] * 1.0e-8 * (16.0 / 3.0) * kAlpha * kReCm2 *
                                        p.Z[el] * p.Z[el] * (double)bi * den_i / ((double)pi_ * (double)pi_));
                     }
                     s_pref[t * 4 + el] = pref;
+                    live |= pref != 0.0f;
                 }
-                s_live[t] = (s_pref[t * 4] != 0.0f) | (s_pref[t * 4 + 1] != 0.0f) | (s_pref[t * 4 + 2] != 0.0f) |
-                            (s_pref[t * 4 + 3] != 0.0f);
+                s_live[t] = live;
             } else {
                 s_T[t] = 0.0f;
                 s_pi[t] = 0.0f;
-                s_p0[t] = 0.0f;
-                s_p1[t] = 0.0f;
+                s_plo[t] = 0.0f;
+                s_phi[t] = 0.0f;
+                s_pmid[t] = 0.0f;
+                s_dp[t] = 0.0f;
                 for (int el = 0; el < 4; ++el) s_pref[t * 4 + el] = 0.0f;
                 s_live[t] = 0;
             }
@@ -351,8 +370,8 @@ __global__ void __launch_bounds__(kRedThreads) brem_kernel(hr_brem_args p, doubl
             }
             // a zero-weight row (a copy that does not carry this tally) adds nothing
             if (!s_live[q]) continue;
-            const float T = s_T[q], pi_ = s_pi[q], p0 = s_p0[q], p1 = s_p1[q];
-            const float plo = fminf(p0, p1), dp01 = fabsf(p1 - p0);
+            const float T = s_T[q], pi_ = s_pi[q];
+            const float plo = s_plo[q], phi = s_phi[q], pmid = s_pmid[q], dp = s_dp[q];
 #pragma unroll
             for (int m = 0; m < BINS; ++m) {
                 const float Tf = T - keV[m];
@@ -362,30 +381,36 @@ __global__ void __launch_bounds__(kRedThreads) brem_kernel(hr_brem_args p, doubl
                 const float TT = Tf * Tm;
                 const float rs = fast_rsq(TT);
                 const float pf = TT * rs * kInvMc2f;
-                // beta_f = p_f / (1 + T_f/m);  (p_i + p_f)/(p_i - p_f) with
-                // p_i - p_f = k (T_i + T_f + 2m) / (m^2 (p_i + p_f)) (no cancellation for k << T_i)
                 const float rcp_bf = rs * (kMc2f + Tf);
+                // ln((p_i + p_f)^2 m^2 / (k (T_i + T_f + 2m))) = ln((p_i + p_f)/(p_i - p_f)): the
+                // difference form has no cancellation for k << T_i; the argument is >= 1 up to
+                // rounding (no denormal path in the hardware log2); ln 2 / k is folded per bin
                 const float spf = pi_ + pf;
-                // ln via the hardware log2 (the argument is >= 1 up to rounding: no denormal path)
-                const float born = kLn2f * fast_log2(spf * spf * (kMc2f * kMc2f) * fast_rcp(keV[m] * (T + Tm)));
-                const float base = born * inv_kev[m] * rcp_bf;
-                float cell = 0.0f;
+                const float born_k = fast_log2(spf * spf * fast_rcp(k_m2[m] * (T + Tm))) * ln2_inv_k[m];
+                // Elwert factors sum_el pref_el / (1 - exp(-2 pi Z_el alpha / beta_f)) as one
+                // fraction num / den
+                float num = s_pref[q * 4];
+                float den = 1.0f - fast_exp2(-zi2[0] * rcp_bf);
 #pragma unroll
-                for (int el = 0; el < n_el; ++el) {
-                    const float den_f = 1.0f - fast_exp2(-zi2[el] * rcp_bf);
-                    cell += s_pref[q * 4 + el] * base * fast_rcp(den_f);
+                for (int el = 1; el < n_el; ++el) {
+                    const float d_el = 1.0f - fast_exp2(-zi2[el] * rcp_bf);
+                    num = fmaf(num, d_el, s_pref[q * 4 + el] * den);
+                    den *= d_el;
                 }
-                // segment-mean escape exp(-t_lo) (1 - exp(-d)) / d, d = |t1 - t0|
-                const float d = dp01 * mu[m];
-                float ratio;
-                if (d > 0.5f) {
-                    ratio = (1.0f - fast_exp2(-dp01 * mu2[m])) * fast_rcp(d);
+                // segment-mean escape (exp(-p_lo mu) - exp(-p_hi mu)) / d, d = (p_hi - p_lo) mu;
+                // for d <= 1 as exp(-p_mid mu) sinh(d/2)/(d/2), the series truncated after
+                // (d/2)^6 / 7! (remainder < 3e-8 relative)
+                const float d = dp * mu[m];
+                float esc;
+                if (d > 1.0f) {
+                    esc = fast_exp2(-plo * mu2[m]) - fast_exp2(-phi * mu2[m]);
+                    den *= d;
                 } else {
-                    // Taylor series of (1 - e^-d)/d, truncation < 6e-10 on [0, 1/2]
-                    ratio = 1.0f + d * (-1.0f / 2 + d * (1.0f / 6 + d * (-1.0f / 24 + d * (1.0f / 120 +
-                            d * (-1.0f / 720 + d * (1.0f / 5040 + d * (-1.0f / 40320 + d * (1.0f / 362880))))))));
+                    const float x2 = 0.25f * d * d;
+                    const float shc = fmaf(x2, fmaf(x2, fmaf(x2, 1.0f / 5040, 1.0f / 120), 1.0f / 6), 1.0f);
+                    esc = fast_exp2(-pmid * mu2[m]) * shc;
                 }
-                part[m] += cell * (fast_exp2(-plo * mu2[m]) * ratio);
+                part[m] = fmaf(num * born_k * rcp_bf * esc, fast_rcp(den), part[m]);
             }
         }
 #pragma unroll
