@@ -155,6 +155,44 @@ hr_status hr_radiative_lut(const hr_hard_events *h, int32_t n_layers, int32_t ma
 hr_status hr_radiative_sample(const hr_hard_events *h, int32_t slot, double plasma_eV, int64_t n, const double *E_eV,
                               const double *u, double *k_out, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Detector-cone splitting of elastic events (the reference's opt-in VR:
+ * src/pyrite/montecarlo/transport/cone_split.py:223-335 cone_split_event,
+ * cores.py:1189-1226 its call site, _cone_splitting.py:57-141 the policy),
+ * here on this transport's screened-Rutherford angular law.  At an elastic
+ * event with E >= min_energy_keV, a copy may represent the outcomes inside
+ * the cone C = {Omega : Omega . n >= cos_c}: its polar angle comes from the
+ * analog law restricted to the band |theta - theta_dn| <= theta_c (exact:
+ * the law's CDF (1 + alpha) mu / (alpha + mu) is inverted in closed form),
+ * its azimuth uniformly from the arc inside C, and it carries the likelihood
+ * ratio P_band dphi/pi.  A roulette against target_weight (and, with
+ * angle_scale = u_ref > 0, a second one against target max(1, (u/u_ref)^2),
+ * u = gamma angle(copy, n)) bounds the copy count; draws are counters 0-3 of
+ * the event key splitmix64(split_key + G (flight + 1)), G the SplitMix64
+ * increment, split_key = splitmix64(stream_key ^ 0xC2B2AE3D27D4EB4F).  The
+ * primary keeps its analog draw and ends with exit code 6 (cone split) when
+ * that draw lands in C.  Copy k of local electron i goes to slot i*cap + k
+ * while k < cap; count[i] counts every copy made (count > cap: the caller
+ * must rerun with a larger cap).  Not combined with the hard-event modes or
+ * the continuation/round entry points (exit code 6 is "suspended" there).
+ * ---------------------------------------------------------------------- */
+#define HR_EXIT_CONE_SPLIT 6
+typedef struct hr_cone_split {
+    int32_t on, pad_;
+    double nx, ny, nz;       /* detector direction (unit) */
+    double cos_c, sin_c;     /* cone half-angle */
+    double target_weight;    /* roulette target copy weight */
+    double min_energy_keV;   /* events below this kinetic energy stay analog */
+    double angle_scale;      /* u_ref of the angle roulette; 0 = off */
+    const uint64_t *split_key; /* [Ne] by global electron id */
+    int64_t cap;             /* copy slots per electron */
+    int64_t *count;          /* [e_count] copies made */
+    uint8_t *killed;         /* [e_count] 1: ended at a cone-entering event */
+    double *pos, *dir;       /* [e_count*cap, 3] copy state at the event */
+    double *E_keV, *clock, *weight; /* [e_count*cap] */
+    int64_t *flight;         /* [e_count*cap] flight index of the event */
+} hr_cone_split;
+
 typedef struct hr_transport_args {
     /* run */
     int64_t e_start, e_count, cap;
@@ -211,9 +249,40 @@ typedef struct hr_transport_args {
     unsigned long long *work;
     /* hard-event modes (hr_hard_events; both switches 0 = the plain step) */
     hr_hard_events hard;
+    /* detector-cone splitting (hr_cone_split; on = 0: analog) */
+    hr_cone_split cone;
 } hr_transport_args;
 
 hr_status hr_transport_run(const hr_transport_args *args, void *stream);
+
+/* Copy records of one cone-split launch, compacted in (electron, event) order
+ * (cone_split.py:412-444 _ConeSplitPass.collect).  With ladder = 1 each copy
+ * of weight w is kept with probability w/W, W = 2^ceil(log2 w), and carries W
+ * (cone_split.py:140-165 weight_ladder; the draw is counter 4 of the copy's
+ * event key).  Copy stream keys are splitmix64(event_key ^ 0x165667B19E3779F9)
+ * (cone_split.py:168-179).  Writes n_out[0] = copies kept and n_out[1] = the
+ * largest count[i]; nothing is written past out_cap copies (n_out[0] may
+ * exceed out_cap: rerun with room).  parent is the local electron index.
+ * Workspace: hr_cone_collect_workspace_bytes(e_count). */
+typedef struct hr_cone_collect_args {
+    const hr_cone_split *cone; /* the launch's copy slots, count and split keys */
+    int64_t e_start, e_count;
+    int32_t ladder, pad_;
+    int64_t out_cap;
+    int64_t *parent, *flight;
+    double *pos, *dir, *E_keV, *clock, *weight;
+    uint64_t *stream_key;
+    int64_t *n_out;            /* [2], device */
+    hr_workspace ws;
+} hr_cone_collect_args;
+int64_t hr_cone_collect_workspace_bytes(int64_t e_count);
+hr_status hr_cone_collect(const hr_cone_collect_args *args, void *stream);
+/* Split-decision keys splitmix64(stream_key[i] ^ 0xC2B2AE3D27D4EB4F) of n electrons. */
+hr_status hr_cone_split_keys(const uint64_t *stream_key, int64_t n, uint64_t *out, void *stream);
+/* One elastic event's copy, for tests: incident direction d[3], screening
+ * alpha, kinetic energy E_keV, event key; out = (made, ox, oy, oz, weight). */
+hr_status hr_cone_event(const hr_cone_split *cone, int64_t n, const double *d, const double *alpha,
+                        const double *E_keV, const uint64_t *event_key, double *out, void *stream);
 
 /* ------------------------------------------------------------------------
  * Bounded continuation (montecarlo/transport/continuation.py:1-27 and the

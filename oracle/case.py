@@ -15,7 +15,26 @@ import numpy as np
 from . import oracle
 
 
-def run_case(pipe_like, start, n, threads=0, timings=None, dose_bins=0, coherent=False):
+def _transport_all(pipe_like, keys, pos, dirs, E, E_cut, threads, clock=None, cone=None):
+    cfg = pipe_like.cfg
+    cap = 512
+    while True:
+        rows, cnt, code, st = oracle.run_transport(
+            pipe_like.layers, pipe_like.lut, keys, pos, dirs, E, E_cut, cap=cap, clock=clock,
+            max_steps=cfg.max_steps, energy_model=1, straggle_on=cfg.straggling, threads=threads, cone=cone)
+        if cnt.max(initial=0) <= cap:
+            break
+        cap = int(cnt.max() * 1.25) + 1
+    if cone is not None:  # copies beyond the slots: rerun with room (the product raises past its bound)
+        while st["cone"]["count"].max(initial=0) > cone["cap"]:
+            cone = dict(cone, cap=int(st["cone"]["count"].max()))
+            rows, cnt, code, st = oracle.run_transport(
+                pipe_like.layers, pipe_like.lut, keys, pos, dirs, E, E_cut, cap=cap, clock=clock,
+                max_steps=cfg.max_steps, energy_model=1, straggle_on=cfg.straggling, threads=threads, cone=cone)
+    return oracle.compact(rows, cnt, cap), st
+
+
+def run_case(pipe_like, start, n, threads=0, timings=None, dose_bins=0, coherent=False, cone=None):
     """Histories ``[start, start+n)`` of the case described by ``pipe_like``.
 
     ``pipe_like`` carries ``cfg``, ``layers``, ``lut``, ``beam_dir``,
@@ -27,6 +46,11 @@ def run_case(pipe_like, start, n, threads=0, timings=None, dose_bins=0, coherent
     ``coherent`` the line spectrum is the coherent one: per direction and
     reflection the fields of all these histories' segments are summed, then
     squared (``sum_d w_d sum_g wm_g |F|^2``, again not divided by Ne).
+    ``cone`` (a ``cone_split.ConeSplit``) runs detector-cone splitting: the
+    primaries split, their copies (ladder weights, powers of two) are
+    transported as one analog generation and each weight stratum of copy rows
+    is scored with the plain scorers and scaled by its weight
+    (``runner/cone_split_strata.py:25-190``).
     """
     cfg = pipe_like.cfg
     t0 = time.perf_counter()
@@ -38,16 +62,49 @@ def run_case(pipe_like, start, n, threads=0, timings=None, dose_bins=0, coherent
     pos, dirs = beam.entry(beam_args(cfg.beam, cfg.seed, start, n, cfg.E0_keV, np.deg2rad(cfg.tilt_deg)), n, start)
     E = np.full(n, cfg.E0_keV)
     E_cut = np.full(n, cfg.E_cut_keV)
-    cap = 512
-    while True:
-        rows, cnt, code, st = oracle.run_transport(
-            pipe_like.layers, pipe_like.lut, keys, pos, dirs, E, E_cut, cap=cap,
-            max_steps=cfg.max_steps, energy_model=1, straggle_on=cfg.straggling, threads=threads)
-        if cnt.max() <= cap:
-            break
-        cap = int(cnt.max() * 1.25) + 1
-    r = oracle.compact(rows, cnt, cap)
+    if cone is None:
+        r, _ = _transport_all(pipe_like, keys, pos, dirs, E, E_cut, threads)
+        strata = [(1.0, r)]
+    else:
+        if coherent or dose_bins:
+            raise ValueError("cone splitting runs the incoherent line and brem tallies only")
+        cdict = dict(n_hat=cone.n_hat, half_angle_rad=cone.half_angle_rad, target_weight=cone.target_weight,
+                     min_energy_keV=cone.min_energy_keV,
+                     angle_scale=0.0 if np.isinf(cone.angle_scale) else cone.angle_scale, cap=16)
+        r, st = _transport_all(pipe_like, keys, pos, dirs, E, E_cut, threads, cone=cdict)
+        cp = oracle.cone_collect(st["cone"], ladder=cone.weight_ladder)
+        if cp["max_count"] > cone.max_copies_per_electron:
+            raise RuntimeError("cone split copies above max_copies_per_electron")
+        strata = [(1.0, r)]
+        nc = cp["weight"].size
+        if nc:
+            rc, _ = _transport_all(pipe_like, cp["stream_key"], cp["pos"], cp["dir"], cp["E_keV"],
+                                   np.full(nc, cfg.E_cut_keV), threads, clock=cp["clock"])
+            row_w = cp["weight"][rc["seg_id"]] if rc["seg_len"].size else np.zeros(0)
+            for W in np.unique(row_w):
+                sel = row_w == W
+                strata.append((float(W), {k: v[sel] for k, v in rc.items()}))
     t1 = time.perf_counter()
+    if timings is not None:
+        timings["transport"] = timings.get("transport", 0.0) + (t1 - t0)
+    lines = np.zeros(pipe_like.E_grid.size)
+    brem = np.zeros(pipe_like.E_grid.size)
+    n_rows = 0
+    for W, rs in strata:
+        ln, br, nr = _score_rows(pipe_like, rs, threads, coherent, timings, t1)
+        lines += W * ln
+        brem += W * br
+        n_rows += nr
+    if dose_bins:
+        from . import tally
+
+        dose = tally.depth_dose(r, 0.0, float(cfg.thickness_ang) / dose_bins, dose_bins)
+        return lines, brem, n_rows, dose
+    return lines, brem, n_rows
+
+
+def _score_rows(pipe_like, r, threads, coherent, timings, t1):
+    cfg = pipe_like.cfg
     seg = {
         "r_mid": r["seg_mid"].astype(np.float32), "v_hat": r["seg_dir"].astype(np.float32),
         "L": r["seg_len"].astype(np.float32),
@@ -86,12 +143,6 @@ def run_case(pipe_like, start, n, threads=0, timings=None, dose_bins=0, coherent
                               [float(x) * 1e24 for _, x in comp], pipe_like.E_grid, mu, threads=threads)
     t3 = time.perf_counter()
     if timings is not None:
-        timings["transport"] = timings.get("transport", 0.0) + (t1 - t0)
         timings["lines"] = timings.get("lines", 0.0) + (t2 - t1)
         timings["brem"] = timings.get("brem", 0.0) + (t3 - t2)
-    if dose_bins:
-        from . import tally
-
-        dose = tally.depth_dose(r, 0.0, float(cfg.thickness_ang) / dose_bins, dose_bins)
-        return lines, brem, len(seg["L"]), dose
     return lines, brem, len(seg["L"])

@@ -99,6 +99,31 @@ double or_shell_primary_cosine(double E, double U, double W, int branch, double 
 double or_shell_secondary_cosine(double E, double U, double W, int branch, double W_eV, double u);
 double or_shell_soft_loss(double mean, double var, uint64_t key, uint64_t *counter);
 
+/* Detector-cone splitting of elastic events (or_cone.c; cone_split.py:223-335,
+ * cores.py:1189-1226) on the screened-Rutherford law.  on = 0: analog.  Copy
+ * k of local electron i in slot i*cap + k while k < cap; count[i] counts all. */
+typedef struct or_cone_split {
+    int32_t on;
+    double nx, ny, nz, cos_c, sin_c, target_weight, min_energy_keV, angle_scale;
+    const uint64_t *split_key; /* [Ne] */
+    int64_t cap;
+    int64_t *count;  /* [e_count] */
+    uint8_t *killed; /* [e_count] */
+    double *pos, *dir, *E_keV, *clock, *weight;
+    int64_t *flight;
+} or_cone_split;
+
+uint64_t or_cone_split_key(uint64_t stream_key);
+uint64_t or_cone_event_key(uint64_t split_key, int64_t flight);
+uint64_t or_cone_copy_key(uint64_t split_key, int64_t flight);
+/* out = (ox, oy, oz, weight); returns 1 when a copy is made */
+int or_cone_event(double dx, double dy, double dz, const or_cone_split *c, double alpha, double E_keV,
+                  uint64_t key, double *out);
+double or_cone_ladder_weight(double w);
+int64_t or_cone_collect(const or_cone_split *c, int64_t e_start, int64_t e_count, int ladder, int64_t out_cap,
+                        int64_t *parent, int64_t *flight, double *pos, double *dir, double *E_keV, double *clock,
+                        double *weight, uint64_t *stream_key, int64_t *max_count);
+
 typedef struct or_transport_args {
     /* run */
     int64_t e_start, e_count, cap;
@@ -144,6 +169,8 @@ typedef struct or_transport_args {
     int8_t *exit_code;
     /* hard-event modes (zeroed: off) */
     or_hard_modes hard;
+    /* detector-cone splitting (zeroed: off) */
+    or_cone_split cone;
 } or_transport_args;
 
 /* Runs electrons [e_start, e_start+e_count) with `threads` OpenMP threads

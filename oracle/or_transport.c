@@ -327,8 +327,8 @@ static int searchsorted_right(const double *b, double x, int n) {
 }
 
 /* One electron: cores.py:408-1239 with per_electron=True, lut=True; the shell
- * (inelastic) and coupled radiative modes by a->hard, every other optional
- * mode (groove, mixed, cone split) off.  The radiative mode reads its rates
+ * (inelastic) and coupled radiative modes by a->hard, detector-cone splitting
+ * by a->cone (or_cone.c), every other optional mode (groove, mixed) off.  The radiative mode reads its rates
  * from LUT columns on the transport grid (the reference's table moment mode,
  * cores.py:467-470). */
 static void transport_one(const or_transport_args *a, int64_t e) {
@@ -338,6 +338,11 @@ static void transport_one(const or_transport_args *a, int64_t e) {
     a->seg_count[i] = 0;
     a->exit_code[i] = EXIT_NOT_ENTERED;
     if (a->straggle_on) a->stragg_dE[e] = 0.0;
+    const or_cone_split *cone = &a->cone;
+    if (cone->on) {
+        cone->count[i] = 0;
+        cone->killed[i] = 0;
+    }
     if (!a->alive[e]) return;
     a->exit_code[i] = EXIT_STEP_LIMITED;
 
@@ -690,7 +695,34 @@ static void transport_one(const or_transport_args *a, int64_t e) {
         double phi_u = or_stream_uniform(key, draws);
         draws += 1ULL;
         double cos_t = 1.0 - 2.0 * alpha * cos_u / (1.0 + alpha - cos_u); /* scattering.py:234 */
+        /* cone split (cores.py:1189-1226): the in-cone copy from the incident
+         * direction and chosen element, on the key of the flight just closed */
+        int split_here = cone->on && a->E_keV[e] >= cone->min_energy_keV;
+        if (split_here) {
+            double cp[4];
+            uint64_t ek = or_cone_event_key(cone->split_key[e], flight - 1);
+            if (or_cone_event(dir[0], dir[1], dir[2], cone, alpha, a->E_keV[e], ek, cp)) {
+                int64_t k = cone->count[i];
+                if (k < cone->cap) {
+                    int64_t r = i * cone->cap + k;
+                    for (int d = 0; d < 3; ++d) {
+                        cone->pos[3 * r + d] = pos[d];
+                        cone->dir[3 * r + d] = cp[d];
+                    }
+                    cone->E_keV[r] = a->E_keV[e];
+                    cone->clock[r] = a->clock[e];
+                    cone->weight[r] = cp[3];
+                    cone->flight[r] = flight - 1;
+                }
+                cone->count[i] = k + 1;
+            }
+        }
         rotate_direction(&dir[0], &dir[1], &dir[2], cos_t, 2.0 * M_PI * phi_u);
+        if (split_here && dir[0] * cone->nx + dir[1] * cone->ny + dir[2] * cone->nz >= cone->cos_c) {
+            cone->killed[i] = 1;
+            a->exit_code[i] = 6; /* EXIT_CONE_SPLIT, cone_split.py:55 */
+            break;
+        }
         steps += 1;
     }
     a->seg_count[i] = nseg;

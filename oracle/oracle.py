@@ -45,6 +45,16 @@ class HardModes(C.Structure):
     ]
 
 
+class ConeSplit(C.Structure):
+    _fields_ = [
+        ("on", C.c_int32),
+        ("nx", C.c_double), ("ny", C.c_double), ("nz", C.c_double), ("cos_c", C.c_double), ("sin_c", C.c_double),
+        ("target_weight", C.c_double), ("min_energy_keV", C.c_double), ("angle_scale", C.c_double),
+        ("split_key", _dp), ("cap", C.c_int64), ("count", _dp), ("killed", _dp),
+        ("pos", _dp), ("dir", _dp), ("E_keV", _dp), ("clock", _dp), ("weight", _dp), ("flight", _dp),
+    ]
+
+
 class TransportArgs(C.Structure):
     _fields_ = [
         ("e_start", C.c_int64), ("e_count", C.c_int64), ("cap", C.c_int64),
@@ -66,6 +76,7 @@ class TransportArgs(C.Structure):
         ("seg_flight", _dp), ("seg_substep", _dp), ("seg_event", _dp),
         ("seg_count", _dp), ("exit_code", _dp),
         ("hard", HardModes),
+        ("cone", ConeSplit),
     ]
 
 
@@ -91,6 +102,19 @@ def lib():
         L = C.CDLL(str(LIB_PATH))
         L.or_splitmix64.restype = C.c_uint64
         L.or_splitmix64.argtypes = [C.c_uint64]
+        for name in ("or_cone_split_key",):
+            getattr(L, name).restype = C.c_uint64
+            getattr(L, name).argtypes = [C.c_uint64]
+        for name in ("or_cone_event_key", "or_cone_copy_key"):
+            getattr(L, name).restype = C.c_uint64
+            getattr(L, name).argtypes = [C.c_uint64, C.c_int64]
+        L.or_cone_event.restype = C.c_int
+        L.or_cone_event.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(ConeSplit), C.c_double,
+                                    C.c_double, C.c_uint64, _dp]
+        L.or_cone_ladder_weight.restype = C.c_double
+        L.or_cone_ladder_weight.argtypes = [C.c_double]
+        L.or_cone_collect.restype = C.c_int64
+        L.or_cone_collect.argtypes = [C.POINTER(ConeSplit), C.c_int64, C.c_int64, C.c_int, C.c_int64] + [_dp] * 9
         L.or_stream_key.restype = C.c_uint64
         L.or_stream_key.argtypes = [C.c_uint64, C.c_uint64]
         L.or_stream_uniform.restype = C.c_double
@@ -319,8 +343,12 @@ def shell_soft_loss(mean, var, key, counter=0):
 
 def run_transport(layers, lut, keys, pos, dirs, E_keV, E_cut, *, cap, e_start=0, e_count=None,
                   alive=None, clock=None, max_steps=400_000, energy_model=1, max_dE_frac=0.0,
-                  straggle_on=False, finite=None, threads=0, rad=None, shell=None):
+                  straggle_on=False, finite=None, threads=0, rad=None, shell=None, cone=None):
     """Run the oracle core on host arrays; returns (rows dict, seg_count, exit_code, state).
+
+    ``cone`` (a dict of ``n_hat, half_angle_rad, target_weight, min_energy_keV,
+    angle_scale, cap``) switches on detector-cone splitting: ``state["cone"]``
+    then holds the copy slots, ``count``, ``killed`` and the split keys.
 
     ``layers``/``lut`` are the product's host tables (the kernel inputs);
     ``pos``/``dirs``/``E_keV``/``clock`` are copied, never mutated.
@@ -413,9 +441,96 @@ def run_transport(layers, lut, keys, pos, dirs, E_keV, E_cut, *, cap, e_start=0,
             _p(hold["sh_nch"])
         a.hard.sh_max_ch, a.hard.sh_cutoff_eV = int(shell.max_ch), float(shell.cutoff_eV)
         a.hard.seg_hard_W, a.hard.seg_hard_ch = _p(rows["seg_hard_W"]), _p(rows["seg_hard_ch"])
+    if cone is not None:
+        hold["cone"] = _cone_record(cone, keys, e_count)
+        a.cone = hold["cone"]["struct"]
     lib().or_transport_lut(C.byref(a), int(threads))
     state = {"pos": pos, "dirs": dirs, "E_keV": E_keV, "clock": clock, "stragg_dE": stragg}
+    if cone is not None:
+        state["cone"] = hold["cone"]
     return rows, seg_count, exit_code, state
+
+
+def cone_split_keys(stream_keys):
+    L = lib()
+    return np.array([L.or_cone_split_key(int(k)) for k in np.asarray(stream_keys, dtype=np.uint64)],
+                    dtype=np.uint64)
+
+
+def _cone_struct(cone):
+    c = ConeSplit()
+    n = np.asarray(cone["n_hat"], dtype=np.float64)
+    n = n / np.linalg.norm(n)
+    th = float(cone["half_angle_rad"])
+    c.on = 1
+    c.nx, c.ny, c.nz = (float(v) for v in n)
+    c.cos_c, c.sin_c = np.cos(th), np.sin(th)
+    c.target_weight = float(cone["target_weight"])
+    c.min_energy_keV = float(cone.get("min_energy_keV", 0.0))
+    c.angle_scale = float(cone.get("angle_scale", 0.0))
+    return c
+
+
+def _cone_record(cone, stream_keys, e_count):
+    cap = int(cone.get("cap", 64))
+    rec = {
+        "split_key": cone_split_keys(stream_keys), "cap": cap,
+        "count": np.zeros(e_count, dtype=np.int64), "killed": np.zeros(e_count, dtype=np.uint8),
+        "pos": np.zeros((e_count * cap, 3)), "dir": np.zeros((e_count * cap, 3)),
+        "E_keV": np.zeros(e_count * cap), "clock": np.zeros(e_count * cap), "weight": np.zeros(e_count * cap),
+        "flight": np.zeros(e_count * cap, dtype=np.int64),
+    }
+    c = _cone_struct(cone)
+    c.split_key, c.cap = _p(rec["split_key"]), cap
+    for k in ("count", "killed", "pos", "dir", "E_keV", "clock", "weight", "flight"):
+        setattr(c, k, _p(rec[k]))
+    rec["struct"] = c
+    return rec
+
+
+def cone_event(d, alpha, E_keV, event_key, cone):
+    """Single elastic events: ``(made [n], out [n, 4] = (ox, oy, oz, weight))``."""
+    c = _cone_struct(cone)
+    d = np.ascontiguousarray(d, dtype=np.float64).reshape(-1, 3)
+    n = d.shape[0]
+    alpha = np.broadcast_to(np.asarray(alpha, dtype=np.float64), (n,))
+    E = np.broadcast_to(np.asarray(E_keV, dtype=np.float64), (n,))
+    keys = np.asarray(event_key, dtype=np.uint64).reshape(n)
+    made = np.zeros(n, dtype=bool)
+    out = np.zeros((n, 4))
+    buf = np.zeros(4)
+    L = lib()
+    for i in range(n):
+        made[i] = bool(L.or_cone_event(d[i, 0], d[i, 1], d[i, 2], C.byref(c), float(alpha[i]), float(E[i]),
+                                       int(keys[i]), _p(buf)))
+        out[i] = buf
+    return made, out
+
+
+def cone_event_key(split_key, flight):
+    return lib().or_cone_event_key(int(split_key), int(flight))
+
+
+def cone_ladder_weight(w):
+    return lib().or_cone_ladder_weight(float(w))
+
+
+def cone_collect(record, e_start=0, ladder=True):
+    """Copies of a cone-split run in (electron, event) order (``cone_split.py:412-444``)."""
+    e_count = record["count"].size
+    n_max = int(np.minimum(record["count"], record["cap"]).sum())
+    out = {
+        "parent": np.zeros(n_max, dtype=np.int64), "flight": np.zeros(n_max, dtype=np.int64),
+        "pos": np.zeros((n_max, 3)), "dir": np.zeros((n_max, 3)), "E_keV": np.zeros(n_max),
+        "clock": np.zeros(n_max), "weight": np.zeros(n_max), "stream_key": np.zeros(n_max, dtype=np.uint64),
+    }
+    biggest = np.zeros(1, dtype=np.int64)
+    n = lib().or_cone_collect(C.byref(record["struct"]), int(e_start), int(e_count), int(bool(ladder)), n_max,
+                              *(_p(out[k]) for k in ("parent", "flight", "pos", "dir", "E_keV", "clock", "weight",
+                                                     "stream_key")), _p(biggest))
+    out = {k: v[:n] for k, v in out.items()}
+    out["max_count"] = int(biggest[0])
+    return out
 
 
 def compact(rows, seg_count, cap):

@@ -94,6 +94,35 @@ class HardEvents(C.Structure):
     ]
 
 
+class ConeSplitArgs(C.Structure):
+    """Mirror of ``hr_cone_split`` (detector-cone splitting of elastic events)."""
+
+    _fields_ = [
+        ("on", C.c_int32), ("pad_", C.c_int32),
+        ("nx", C.c_double), ("ny", C.c_double), ("nz", C.c_double),
+        ("cos_c", C.c_double), ("sin_c", C.c_double),
+        ("target_weight", C.c_double), ("min_energy_keV", C.c_double), ("angle_scale", C.c_double),
+        ("split_key", _dp), ("cap", C.c_int64), ("count", _dp), ("killed", _dp),
+        ("pos", _dp), ("dir", _dp), ("E_keV", _dp), ("clock", _dp), ("weight", _dp), ("flight", _dp),
+    ]
+
+
+class ConeCollectArgs(C.Structure):
+    """Mirror of ``hr_cone_collect_args``."""
+
+    _fields_ = [
+        ("cone", C.POINTER(ConeSplitArgs)),
+        ("e_start", C.c_int64), ("e_count", C.c_int64),
+        ("ladder", C.c_int32), ("pad_", C.c_int32),
+        ("out_cap", C.c_int64),
+        ("parent", _dp), ("flight", _dp),
+        ("pos", _dp), ("dir", _dp), ("E_keV", _dp), ("clock", _dp), ("weight", _dp),
+        ("stream_key", _dp),
+        ("n_out", _dp),
+        ("ws", Workspace),
+    ]
+
+
 class TransportArgs(C.Structure):
     """Mirror of ``hr_transport_args`` (include/hiprite.h)."""
 
@@ -156,6 +185,7 @@ class TransportArgs(C.Structure):
         ("lut_alpha_f", _dp),
         ("work", _dp),
         ("hard", HardEvents),
+        ("cone", ConeSplitArgs),
     ]
 
 
@@ -375,6 +405,10 @@ SIGNATURES = {
     "hr_device_count": (C.c_int32, []),
     "hr_stream_keys": (C.c_int32, [C.c_uint64, C.c_int64, C.c_int64, _dp, _dp]),
     "hr_transport_run": (C.c_int32, [C.POINTER(TransportArgs), _dp]),
+    "hr_cone_collect_workspace_bytes": (C.c_int64, [C.c_int64]),
+    "hr_cone_collect": (C.c_int32, [C.POINTER(ConeCollectArgs), _dp]),
+    "hr_cone_split_keys": (C.c_int32, [_dp, C.c_int64, _dp, _dp]),
+    "hr_cone_event": (C.c_int32, [C.POINTER(ConeSplitArgs), C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp]),
     "hr_radiative_lut": (C.c_int32, [C.POINTER(HardEvents), C.c_int32, C.c_int32, _dp, _dp, C.c_int64, _dp, _dp,
                                      _dp]),
     "hr_radiative_sample": (C.c_int32, [C.POINTER(HardEvents), C.c_int32, C.c_double, C.c_int64, _dp, _dp, _dp,

@@ -26,7 +26,7 @@ BUILD = PKG / "_build"
 
 ARCH = os.environ.get("HIPRITE_ARCH", "gfx950")
 # Units checked step by step against the FP64 oracle: no fused multiply-add.
-REPRODUCIBLE = {"transport.hip", "vr.hip"}
+REPRODUCIBLE = {"transport.hip", "vr.hip", "cone.hip"}
 
 
 def _hipcc() -> str:

@@ -16,6 +16,8 @@
 // until the longest history of a static 64-electron wave ends.  Every draw and
 // every output slot is a function of the electron id alone, so the claim order
 // cannot change any result.
+#include <cmath>
+
 #include "transport_step.hpp"
 
 namespace hr {
@@ -45,7 +47,7 @@ __device__ __forceinline__ int64_t claim(unsigned long long *next, bool want, in
 #define HR_TRANSPORT_OCCUPANCY
 #endif
 
-template <bool kMixed, bool kHard = false>
+template <bool kMixed, bool kHard = false, bool kCone = false>
 __global__ void __launch_bounds__(256) HR_TRANSPORT_OCCUPANCY transport_kernel(hr_transport_args a,
                                                         unsigned long long *__restrict__ next) {
     Lane s;
@@ -60,6 +62,10 @@ __global__ void __launch_bounds__(256) HR_TRANSPORT_OCCUPANCY transport_kernel(h
                 drained = true;
             } else {
                 const int64_t e = a.e_start + i;
+                if constexpr (kCone) {
+                    a.cone.count[i] = 0;
+                    a.cone.killed[i] = 0;
+                }
                 if (!a.alive[e]) {
                     a.seg_count[i] = 0;
                     a.exit_code[i] = EXIT_NOT_ENTERED;
@@ -73,7 +79,7 @@ __global__ void __launch_bounds__(256) HR_TRANSPORT_OCCUPANCY transport_kernel(h
         }
         if (__ballot(live || !drained) == 0) break;
         if (live) {
-            const int code = step_lane<true, kMixed, kHard>(a, s);
+            const int code = step_lane<true, kMixed, kHard, kCone>(a, s);
             if (code >= 0) {
                 store_lane(a, s, (int8_t)code);
                 live = false;
@@ -132,23 +138,48 @@ __global__ void __launch_bounds__(256) HR_TRANSPORT_OCCUPANCY continue_kernel(hr
 
 __global__ void reset_counter(unsigned long long *c) { *c = 0ULL; }
 
-int g_occupancy[2][64];  // resident 256-lane blocks per CU of transport_kernel (plain, hard), per device
-
-hr_status transport_occupancy(int block, bool hard, int &occupancy) {
+// Resident 256-lane blocks per CU of transport_kernel, per variant (0 plain,
+// 1 hard-event modes, 2 cone splitting) and device.
+int g_occupancy[3][64];
+hr_status transport_occupancy(int block, int variant, int &occupancy) {
     int dev = 0;
     HR_CHECK_HIP(hipGetDevice(&dev));
     HR_REQUIRE(dev >= 0 && dev < 64, "device index %d out of range", dev);
-    if (block == 256 && g_occupancy[hard][dev] > 0) {
-        occupancy = g_occupancy[hard][dev];
+    HR_REQUIRE(variant >= 0 && variant < 3, "transport variant %d out of range", variant);
+    if (block == 256 && g_occupancy[variant][dev] > 0) {
+        occupancy = g_occupancy[variant][dev];
         return HR_OK;
     }
     int blocks = 0;
-    if (hard)
+    if (variant == 1)
         HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, transport_kernel<false, true>, block, 0));
+    else if (variant == 2)
+        HR_CHECK_HIP(
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, transport_kernel<false, false, true>, block, 0));
     else
         HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, transport_kernel<false>, block, 0));
     occupancy = blocks > 0 ? blocks : 1;
-    if (block == 256) g_occupancy[hard][dev] = occupancy;
+    if (block == 256) g_occupancy[variant][dev] = occupancy;
+    return HR_OK;
+}
+
+// Pointers and values the cone splitting reads; checked before any launch.
+hr_status check_cone(const hr_transport_args &a, const char *who) {
+    const hr_cone_split &c = a.cone;
+    HR_REQUIRE(c.on == 0 || c.on == 1, "%s: cone.on must be 0 or 1", who);
+    if (!c.on) return HR_OK;
+    HR_REQUIRE(!a.hard.rad_on && !a.hard.shell_on, "%s: cone splitting is not combined with the hard-event modes",
+               who);
+    HR_REQUIRE(c.split_key && c.count && c.killed, "%s: cone splitting needs split keys, count and killed", who);
+    HR_REQUIRE(c.cap >= 1 && c.pos && c.dir && c.E_keV && c.clock && c.weight && c.flight,
+               "%s: cone splitting needs copy slots (cap >= 1)", who);
+    const double nn = c.nx * c.nx + c.ny * c.ny + c.nz * c.nz;
+    HR_REQUIRE(fabs(nn - 1.0) < 1e-9, "%s: cone direction must be a unit vector", who);
+    HR_REQUIRE(c.sin_c > 0.0 && c.cos_c > 0.0 && fabs(c.cos_c * c.cos_c + c.sin_c * c.sin_c - 1.0) < 1e-9,
+               "%s: cone half-angle must lie in (0, pi/2)", who);
+    HR_REQUIRE(c.target_weight > 0.0 && std::isfinite(c.target_weight), "%s: target_weight must be positive", who);
+    HR_REQUIRE(c.min_energy_keV >= 0.0 && std::isfinite(c.min_energy_keV), "%s: min_energy_keV must be >= 0", who);
+    HR_REQUIRE(c.angle_scale >= 0.0 && std::isfinite(c.angle_scale), "%s: angle_scale must be >= 0 (0: off)", who);
     return HR_OK;
 }
 
@@ -211,13 +242,16 @@ extern "C" hr_status hr_transport_run(const hr_transport_args *args, void *strea
     HR_REQUIRE(a.work != nullptr, "hr_transport_run: null claim counter (work)");
     hr_status st = check_hard(a, "hr_transport_run");
     if (st != HR_OK) return st;
+    st = check_cone(a, "hr_transport_run");
+    if (st != HR_OK) return st;
     const bool hard = a.hard.rad_on || a.hard.shell_on;
+    const bool cone = a.cone.on != 0;
     int cu = 0, occupancy = 0;
     st = device_cu(cu);
     if (st != HR_OK) return st;
     // Whole wave64s, at most the kernel's launch bound of 256 lanes.
     const int block = a.block_size <= 0 ? 256 : (a.block_size >= 256 ? 256 : pick_block(a.block_size, 256));
-    st = transport_occupancy(block, hard, occupancy);
+    st = transport_occupancy(block, hard ? 1 : cone ? 2 : 0, occupancy);
     if (st != HR_OK) return st;
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, a.work);
@@ -225,7 +259,12 @@ extern "C" hr_status hr_transport_run(const hr_transport_args *args, void *strea
     const int64_t need = (a.e_count + block - 1) / block;
     const int64_t resident = (int64_t)cu * occupancy;
     const int64_t grid = need < resident ? need : resident;
-    if (hard && a.precision == 1)
+    if (cone && a.precision == 1)
+        hipLaunchKernelGGL((transport_kernel<true, false, true>), dim3((unsigned)grid), dim3(block), 0, s, a, a.work);
+    else if (cone)
+        hipLaunchKernelGGL((transport_kernel<false, false, true>), dim3((unsigned)grid), dim3(block), 0, s, a,
+                           a.work);
+    else if (hard && a.precision == 1)
         hipLaunchKernelGGL((transport_kernel<true, true>), dim3((unsigned)grid), dim3(block), 0, s, a, a.work);
     else if (hard)
         hipLaunchKernelGGL((transport_kernel<false, true>), dim3((unsigned)grid), dim3(block), 0, s, a, a.work);
@@ -265,6 +304,8 @@ extern "C" hr_status hr_transport_continue(const hr_transport_args *args, const 
     HR_REQUIRE(a.precision == 0 || (a.precision == 1 && a.lut_total_rate_f && a.lut_dEds_f && a.lut_inv_beta_f &&
                                     a.lut_cdf_f && a.lut_alpha_f),
                "hr_transport_continue: precision 0, or 1 with the FP32 table copies");
+    HR_REQUIRE(!a.cone.on, "hr_transport_continue: cone splitting runs on hr_transport_run only (exit code 6 "
+                           "is a suspended lane here)");
     HR_REQUIRE(!a.hard.rad_on && !a.hard.shell_on,
                "hr_transport_continue: the hard-event modes run on hr_transport_run only (their optical depths "
                "and counters are not in the continuation record)");

@@ -10,6 +10,7 @@
 #pragma once
 #include <math.h>
 
+#include "cone.hpp"
 #include "hr_common.hpp"
 #include "radiative.hpp"
 #include "shell.hpp"
@@ -223,6 +224,8 @@ struct Lane {
     // hard-event modes: open optical depths (< 0: draw at the next step), keys, counters
     double tau_h, tau_r;
     uint64_t hkey, rkey, hdraws, rdraws;
+    // detector-cone splitting: the electron's split-decision key
+    uint64_t skey;
 };
 
 __device__ __forceinline__ void load_lane(const hr_transport_args &a, int64_t e, Lane &s) {
@@ -251,6 +254,7 @@ __device__ __forceinline__ void load_lane(const hr_transport_args &a, int64_t e,
     s.rkey = a.hard.rad_on ? a.hard.rad_key[e] : 0ULL;
     s.hdraws = 0;
     s.rdraws = 0;
+    s.skey = a.cone.on ? a.cone.split_key[e] : 0ULL;
 }
 
 __device__ __forceinline__ rad::Tables rad_tables(const hr_hard_events &h, int max_el) {
@@ -329,7 +333,7 @@ __device__ __forceinline__ void store_lane(const hr_transport_args &a, const Lan
 // reference's "mixed" transport precision: FP32 table lookups, query
 // logarithms, sqrt(1 - cos^2) and azimuth sin/cos; every accumulated quantity,
 // free-path draw and the straggling stay FP64.
-template <bool kRows = true, bool kMixed = false, bool kHard = false>
+template <bool kRows = true, bool kMixed = false, bool kHard = false, bool kCone = false>
 __device__ int step_lane(const hr_transport_args &a, Lane &s) {
     const Lut<kMixed> lut{a};
     const int nE = a.lut_n_energy;
@@ -687,7 +691,46 @@ __device__ int step_lane(const hr_transport_args &a, Lane &s) {
     const double phi_u = stream_uniform(s.key, s.draws + 1ULL);
     s.draws += 2ULL;
     const double cos_t = 1.0 - 2.0 * alpha * cos_u / (1.0 + alpha - cos_u);
+    bool split_here = false;
+    if constexpr (kCone) {
+        // the in-cone copy of this event, from the incident direction and the
+        // chosen element, on the (electron, flight) event key (cores.py:1189-1211)
+        const hr_cone_split &c = a.cone;
+        split_here = s.E >= c.min_energy_keV;
+        if (split_here) {
+            const int64_t fl = s.flight - 1;
+            const cone::Copy cp = cone::event(s.dx, s.dy, s.dz, c.nx, c.ny, c.nz, c.cos_c, c.sin_c,
+                                              c.target_weight, c.angle_scale, alpha, s.E,
+                                              cone::event_key(s.skey, fl));
+            if (cp.made) {
+                const int64_t i = s.e - a.e_start;
+                const int64_t k = c.count[i];
+                if (k < c.cap) {
+                    const int64_t r = i * c.cap + k;
+                    c.pos[3 * r] = s.px;
+                    c.pos[3 * r + 1] = s.py;
+                    c.pos[3 * r + 2] = s.pz;
+                    c.dir[3 * r] = cp.ox;
+                    c.dir[3 * r + 1] = cp.oy;
+                    c.dir[3 * r + 2] = cp.oz;
+                    c.E_keV[r] = s.E;
+                    c.clock[r] = s.clock;
+                    c.weight[r] = cp.w;
+                    c.flight[r] = fl;
+                }
+                c.count[i] = k + 1;
+            }
+        }
+    }
     rotate_lane<kMixed>(s, cos_t, 2.0 * kPi * phi_u);
+    if constexpr (kCone) {
+        // the analog outcome lies in the cone, which the copies represent: the
+        // primary ends after this event (cores.py:1215-1226)
+        if (split_here && s.dx * a.cone.nx + s.dy * a.cone.ny + s.dz * a.cone.nz >= a.cone.cos_c) {
+            a.cone.killed[s.e - a.e_start] = 1;
+            return HR_EXIT_CONE_SPLIT;
+        }
+    }
     s.steps += 1;
     return s.steps < a.max_steps ? -1 : EXIT_STEP_LIMITED;
 }

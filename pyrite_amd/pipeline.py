@@ -67,6 +67,7 @@ class CaseConfig:
     line_roulette: dict | None = None  # LineRoulette policy fields (per-primary line roulette), or None
     depth_bins: int = 0  # > 0: also tally the depth-dose profile over this many bins of the slab
     coherent: bool = False  # sum line fields before squaring (the coherent route); brem stays incoherent
+    cone_split: dict | None = None  # ConeSplitting policy fields (detector-cone splitting of elastic events), or None
 
     def grid(self):
         return bin_centers(self.E_lo_eV, self.E_hi_eV, self.n_bins)
@@ -155,6 +156,15 @@ class CasePipeline(CaseTables):
 
             self.coherent = CoherentScorer(self.line_tables, cfg.crystal, self.E_grid, cfg.thickness_ang,
                                            self.n_hats, self.weights, dev)
+        self.cone_core = None
+        if cfg.cone_split is not None:
+            from .cone_split import ConeSplitCore, ConeSplitting
+
+            policy = ConeSplitting.from_dict(cfg.cone_split)
+            policy.validate_case(cfg)
+            self.cone_request = policy.request(self.n_hat, cfg.E0_keV)
+            self.cone_core = ConeSplitCore(self.tables, self.control, self.cone_request, dev)
+            self.cone_stats = {"copies": 0, "killed": 0, "copy_rows": 0}
         n = cfg.Ne
         self.keys = torch.empty(n, dtype=torch.uint64, device=dev)
         self.pos = torch.empty((n, 3), dtype=torch.float64, device=dev)
@@ -230,7 +240,7 @@ class CasePipeline(CaseTables):
                 torch.cuda.current_stream().wait_event(ev)
         return self._arena(self._arena_cur).rows(m, cap, midpoint)
 
-    def transport(self, batch_index: int, compact: bool = True, consume=None):
+    def transport(self, batch_index: int, compact: bool = True, consume=None, core=None):
         """Transport histories ``[b*Ne, (b+1)*Ne)``; returns the device segment rows
         (compacted; with ``compact=False`` the slot rows of each sub-batch, or,
         with ``consume``, nothing: ``consume(e0, m, cap, rows, seg_count)`` takes
@@ -260,7 +270,7 @@ class CasePipeline(CaseTables):
                                        f"capacity {cap}; pass consume= (or compact=True)")
             return self._arena_rows(e, m, cap, mid, dev)
 
-        res = drive_batches(self.core, self.keys,
+        res = drive_batches(self.core if core is None else core, self.keys,
                             (self.alive, self.clock, self.pos, self.dirs, self.E, self.E_cut),
                             self.stragg, self.control, n, self.dev, self.tcfg, stream=stream, compact=compact,
                             cap_hint=None if self._cap_hint is None else max(self._cap_hint, self.tcfg.seg_capacity),
@@ -284,9 +294,11 @@ class CasePipeline(CaseTables):
         primary id as ``"prim"``."""
         return self.slot_records(res.slots, int(res.n_segments), first_id)
 
-    def slot_records(self, slots, n_rows, first_id=None):
+    def slot_records(self, slots, n_rows, first_id=None, weights=None):
         """FP32 scoring records of slot rows ``[(e0, m, cap, rows, seg_count)]``
-        holding ``n_rows`` written rows in total."""
+        holding ``n_rows`` written rows in total; with ``weights`` (FP64, one per
+        particle of a single sub-batch) the records also carry ``"w"``, each
+        row's particle weight."""
         import ctypes as C
 
         import torch
@@ -310,7 +322,9 @@ class CasePipeline(CaseTables):
             g.n, g.rows = m, cap
             g.n_out = n_rows if len(slots) == 1 else int(cnt.sum())  # one thread per written row
             g.seg_count, g.row_offset = seg_count.data_ptr(), off.data_ptr()
-            g.weight = self._ones[:m].data_ptr()
+            if weights is not None and len(slots) != 1:
+                raise ValueError("particle weights are taken for one sub-batch at a time")
+            g.weight = (self._ones[:m] if weights is None else weights).data_ptr()
             g.seg_mid, g.seg_dir = rows["seg_mid"].data_ptr(), rows["seg_dir"].data_ptr()
             g.seg_len, g.seg_E, g.seg_E_end = rows["seg_len"].data_ptr(), rows["seg_E"].data_ptr(), \
                 rows["seg_E_end"].data_ptr()
@@ -322,7 +336,8 @@ class CasePipeline(CaseTables):
             _lib.check(lib.hr_vr_gather(C.byref(g), C.c_void_p(torch.cuda.current_stream().cuda_stream)),
                        "hr_vr_gather")
             o += int(cnt.sum()) if len(slots) > 1 else 0
-        keys = ("r_mid", "v_hat", "L", "E") + (("prim",) if first_id is not None else ())
+        keys = ("r_mid", "v_hat", "L", "E") + (("prim",) if first_id is not None else ()) + \
+            (("w",) if weights is not None else ())
         return {k: rec[k][:n_rows] for k in keys}
 
     def score_dose(self, res, Ne):
@@ -337,16 +352,23 @@ class CasePipeline(CaseTables):
         for _e0, m, cap, rows, seg_count in res.slots:
             depth_dose(self.spec_dose, 0.0, dz, rows, m, slots=cap, seg_count=seg_count, scale=1.0 / float(Ne))
 
-    def score_records(self, seg, Ne, line_weight=None):
+    def score_records(self, seg, Ne, line_weight=None, row_weight=None):
         """Line and brem tallies of FP32 scoring records; ``line_weight`` (FP32 per
-        row) weights the line tally only (the line-roulette strata)."""
+        row) weights the line tally only (the line-roulette strata),
+        ``row_weight`` (FP32 per row, the particle weight of a copy) both."""
+        if line_weight is not None and row_weight is not None:
+            raise ValueError("line_weight and row_weight are not combined")
+        lw = row_weight if line_weight is None else line_weight
         if self.cfg.line_route == "window":
-            self.spec_lines += self.lines.window_spectra(seg, Ne, x_c=self.cfg.x_c, row_weight=line_weight).sum(0)
+            self.spec_lines += self.lines.window_spectra(seg, Ne, x_c=self.cfg.x_c, row_weight=lw).sum(0)
         elif self.cfg.line_route == "dense":
-            self.lines.spectrum(seg, Ne, self.spec_lines, row_weight=line_weight)
+            self.lines.spectrum(seg, Ne, self.spec_lines, row_weight=lw)
         else:
             raise ValueError(f"unknown line route {self.cfg.line_route!r}")
-        self.brem.spectrum(seg, Ne, self.spec_brem)
+        if row_weight is None:
+            self.brem.spectrum(seg, Ne, self.spec_brem)
+        else:
+            self.brem.spectrum(seg, Ne, self.spec_brem, row_weight=row_weight)
 
     def line_screen(self, n_electrons):
         """The case's per-primary line-roulette screen over ``n_electrons`` primaries, or None."""
@@ -369,10 +391,17 @@ class CasePipeline(CaseTables):
         self.score_records(seg, Ne)
         return seg
 
-    def _score_slot(self, e0, m, cap, rows, seg_count, batch_index, n_total):
-        """Every tally of one sub-batch of slot rows (queued on the current stream)."""
+    def _score_slot(self, e0, m, cap, rows, seg_count, batch_index, n_total, weights=None):
+        """Every tally of one sub-batch of slot rows (queued on the current stream);
+        ``weights`` (FP64 per particle) weights every tally of its rows (copies)."""
         slot = [(e0, m, cap, rows, seg_count)]
         n_rows = int(seg_count.clamp(max=cap).sum())
+        if weights is not None:
+            seg = self.slot_records(slot, n_rows, weights=weights)
+            self.score_records(seg, n_total, row_weight=seg["w"])
+            self._scored()
+            self.segments += n_rows
+            return
         nb = int(self.cfg.depth_bins)
         if nb > 0:
             from .transport.tally import depth_dose
@@ -402,8 +431,37 @@ class CasePipeline(CaseTables):
         def consume(e0, m, cap, rows, seg_count):
             self._score_slot(e0, m, cap, rows, seg_count, batch_index, n_total)
 
+        if self.cone_core is not None:
+            return self._step_cone(batch_index, n_total, consume)
         res = self.transport(batch_index, compact=False, consume=consume)
         self.histories += self.cfg.Ne
+        return res
+
+    def _step_cone(self, batch_index, n_total, consume):
+        """A batch with detector-cone splitting: the primaries (rows weight 1,
+        copies collected per launch), then the copy generation, each copy's rows
+        scored with its weight (``cone_split.py:447-513``)."""
+        from .cone_split import transport_copies
+
+        self.cone_core.reset()
+        res = self.transport(batch_index, compact=False, consume=consume, core=self.cone_core)
+        self.histories += self.cfg.Ne
+        cp = self.cone_core.copies()
+        n_copies = 0 if cp is None else int(cp["weight"].numel())
+        self.cone_stats["copies"] += n_copies
+        self.cone_stats["killed"] += 0 if cp is None else int(cp["killed"].sum())
+        if n_copies:
+            w = cp["weight"]
+
+            def consume_copies(e0, m, cap, rows, seg_count):
+                self.cone_stats["copy_rows"] += int(seg_count.clamp(max=cap).sum())
+                self._score_slot(e0, m, cap, rows, seg_count, batch_index, n_total, weights=w[e0:e0 + m])
+
+            transport_copies(cp, self.tables, self.control, self.cfg.E_cut_keV, self.dev,
+                             PerElectronTransportConfig(seg_capacity=256, scratch_budget_bytes=2 << 30,
+                                                        kernel_threads=self.tcfg.kernel_threads),
+                             consume=consume_copies)
+        res.cone_copies = n_copies
         return res
 
     def run_steps(self, first_batch: int, n_steps: int, n_batches_total: int | None = None, overlap: bool = True):
@@ -418,7 +476,7 @@ class CasePipeline(CaseTables):
         import torch
 
         n_total_b = int(n_steps if n_batches_total is None else n_batches_total)
-        if not overlap or self.cfg.line_roulette is not None:
+        if not overlap or self.cfg.line_roulette is not None or self.cone_core is not None:
             for b in range(int(n_steps)):
                 self.step(int(first_batch) + b, n_total_b)
             return

@@ -67,6 +67,8 @@ STRUCTS = {
     "hr_dose_args": "DoseArgs",
     "hr_workspace": "Workspace",
     "hr_hard_events": "HardEvents",
+    "hr_cone_split": "ConeSplitArgs",
+    "hr_cone_collect_args": "ConeCollectArgs",
 }
 
 
