@@ -279,6 +279,15 @@ int64_t hr_cone_collect_workspace_bytes(int64_t e_count);
 hr_status hr_cone_collect(const hr_cone_collect_args *args, void *stream);
 /* Split-decision keys splitmix64(stream_key[i] ^ 0xC2B2AE3D27D4EB4F) of n electrons. */
 hr_status hr_cone_split_keys(const uint64_t *stream_key, int64_t n, uint64_t *out, void *stream);
+/* Importance roulette of collected copies (no counterpart in the reference's
+ * cone splitting, whose copy roulettes use the event state only): copy i of
+ * weight w[i] and predicted log-importance log_imp[i] (per unit weight)
+ * survives with p = clamp(w exp(log_imp) / T, p_min, 1), T = exp(log_T), and
+ * then carries w/p; its draw is counter 0 of splitmix64(stream_key[i] ^
+ * 0x61C8864680B583EB), disjoint from the copy's transport stream.  Writes
+ * keep[i] (0/1) and the new weight in place (0 for a dropped copy). */
+hr_status hr_cone_roulette(int64_t n, const float *log_imp, const uint64_t *stream_key, double log_T, double p_min,
+                           double *weight, uint8_t *keep, void *stream);
 /* One elastic event's copy, for tests: incident direction d[3], screening
  * alpha, kinetic energy E_keV, event key; out = (made, ox, oy, oz, weight). */
 hr_status hr_cone_event(const hr_cone_split *cone, int64_t n, const double *d, const double *alpha,

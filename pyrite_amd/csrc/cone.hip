@@ -8,6 +8,8 @@
 // the block totals, then each block scanning its electrons' kept counts and
 // writing its copies.  The ladder draw and the copy stream key depend only on
 // (split key, flight), so the output is the same for any launch shape.
+#include <cmath>
+
 #include "cone.hpp"
 
 namespace hr {
@@ -148,6 +150,21 @@ __global__ void __launch_bounds__(kCollectThreads) collect_write_kernel(hr_cone_
     }
 }
 
+constexpr uint64_t kRouletteSalt = 0x61C8864680B583EBULL;
+
+__global__ void cone_roulette_kernel(int64_t n, const float *__restrict__ log_imp,
+                                     const uint64_t *__restrict__ key, double log_T, double p_min,
+                                     double *__restrict__ weight, uint8_t *__restrict__ keep) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double w = weight[i];
+    const double s = log(w) + (double)log_imp[i] - log_T;  // log(w I / T)
+    const double p = s >= 0.0 ? 1.0 : fmax(p_min, exp(s));
+    const bool k = p >= 1.0 || stream_uniform(splitmix64(key[i] ^ kRouletteSalt), 0ULL) < p;
+    keep[i] = k ? 1 : 0;
+    weight[i] = k ? w / p : 0.0;
+}
+
 __global__ void zero_pair_kernel(int64_t *__restrict__ v) {
     v[0] = 0;
     v[1] = 0;
@@ -241,6 +258,19 @@ extern "C" hr_status hr_cone_event(const hr_cone_split *cone, int64_t n, const d
                "hr_cone_event: bad cone parameters");
     hipLaunchKernelGGL(cone_event_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                        *cone, n, d, alpha, E_keV, event_key, out);
+    HR_CHECK_HIP(hipGetLastError());
+    return HR_OK;
+}
+
+extern "C" hr_status hr_cone_roulette(int64_t n, const float *log_imp, const uint64_t *stream_key, double log_T,
+                                      double p_min, double *weight, uint8_t *keep, void *stream) {
+    using namespace hr;
+    HR_REQUIRE(n >= 0, "hr_cone_roulette: negative n");
+    if (n == 0) return HR_OK;
+    HR_REQUIRE(log_imp && stream_key && weight && keep, "hr_cone_roulette: null pointer");
+    HR_REQUIRE(p_min > 0.0 && p_min <= 1.0 && std::isfinite(log_T), "hr_cone_roulette: p_min in (0, 1], finite log_T");
+    hipLaunchKernelGGL(cone_roulette_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, n,
+                       log_imp, stream_key, log_T, p_min, weight, keep);
     HR_CHECK_HIP(hipGetLastError());
     return HR_OK;
 }

@@ -165,6 +165,9 @@ class CasePipeline(CaseTables):
             self.cone_request = policy.request(self.n_hat, cfg.E0_keV)
             self.cone_core = ConeSplitCore(self.tables, self.control, self.cone_request, dev)
             self.cone_stats = {"copies": 0, "killed": 0, "copy_rows": 0}
+            self.copy_pool = 1 << 16  # copies pooled before a copy generation runs
+            self._pool, self._pool_n, self._copy_cap_hint = [], 0, 256
+        self.cone_roulette = None  # a fitted vr.cone_roulette.LearnedConeRoulette thins the copies
         n = cfg.Ne
         self.keys = torch.empty(n, dtype=torch.uint64, device=dev)
         self.pos = torch.empty((n, 3), dtype=torch.float64, device=dev)
@@ -439,30 +442,58 @@ class CasePipeline(CaseTables):
 
     def _step_cone(self, batch_index, n_total, consume):
         """A batch with detector-cone splitting: the primaries (rows weight 1,
-        copies collected per launch), then the copy generation, each copy's rows
-        scored with its weight (``cone_split.py:447-513``)."""
-        from .cone_split import transport_copies
-
+        copies collected per launch), their copies pooled for one later copy
+        generation (``flush_copies``) once ``copy_pool`` copies are waiting."""
         self.cone_core.reset()
         res = self.transport(batch_index, compact=False, consume=consume, core=self.cone_core)
         self.histories += self.cfg.Ne
         cp = self.cone_core.copies()
+        if cp is not None and self.cone_roulette is not None:
+            cp = self.cone_roulette.apply(cp)  # learned importance roulette of the copies
         n_copies = 0 if cp is None else int(cp["weight"].numel())
         self.cone_stats["copies"] += n_copies
         self.cone_stats["killed"] += 0 if cp is None else int(cp["killed"].sum())
+        if self._pool and self._pool[0][1] != n_total:
+            self.flush_copies()
         if n_copies:
-            w = cp["weight"]
-
-            def consume_copies(e0, m, cap, rows, seg_count):
-                self.cone_stats["copy_rows"] += int(seg_count.clamp(max=cap).sum())
-                self._score_slot(e0, m, cap, rows, seg_count, batch_index, n_total, weights=w[e0:e0 + m])
-
-            transport_copies(cp, self.tables, self.control, self.cfg.E_cut_keV, self.dev,
-                             PerElectronTransportConfig(seg_capacity=256, scratch_budget_bytes=2 << 30,
-                                                        kernel_threads=self.tcfg.kernel_threads),
-                             consume=consume_copies)
+            self._pool.append((cp, n_total))
+            self._pool_n += n_copies
+        if self._pool_n >= self.copy_pool:
+            self.flush_copies()
         res.cone_copies = n_copies
         return res
+
+    def flush_copies(self):
+        """Transport every pooled copy as one analog generation and score its
+        rows with its weight (``cone_split.py:447-513``).  Copies are
+        independent of each other, so pooling several batches' copies into one
+        launch changes no tally beyond FP64 summation order, while the launch's
+        latency (the longest copy history, often one running along the slab
+        face) is paid once per pool instead of once per batch."""
+        import torch
+
+        from .cone_split import transport_copies
+
+        if not self._pool:
+            return
+        n_total = self._pool[0][1]
+        keys = ("flight", "pos", "dir", "E_keV", "clock", "weight", "stream_key")
+        cp = {k: torch.cat([c[k] for c, _ in self._pool]) for k in keys}
+        self._pool, self._pool_n = [], 0
+        w = cp["weight"]
+
+        def consume_copies(e0, m, cap, rows, seg_count):
+            self.cone_stats["copy_rows"] += int(seg_count.clamp(max=cap).sum())
+            self._score_slot(e0, m, cap, rows, seg_count, 0, n_total, weights=w[e0:e0 + m])
+
+        budget = min(self.arena_bytes, 8 << 30)
+        res = transport_copies(cp, self.tables, self.control, self.cfg.E_cut_keV, self.dev,
+                               PerElectronTransportConfig(seg_capacity=max(256, self._copy_cap_hint),
+                                                          scratch_budget_bytes=budget,
+                                                          kernel_threads=self.tcfg.kernel_threads),
+                               consume=consume_copies)
+        self._copy_cap_hint = max(self._copy_cap_hint, int(res.capacity))
+        self.cone_stats["generations"] = self.cone_stats.get("generations", 0) + 1
 
     def run_steps(self, first_batch: int, n_steps: int, n_batches_total: int | None = None, overlap: bool = True):
         """Batches ``first_batch ... first_batch + n_steps - 1`` through transport and scoring.
@@ -593,7 +624,10 @@ class CasePipeline(CaseTables):
         """``(lines [photons/eV/electron], brem [photons/eV/electron] over the cone)``.
 
         On the coherent route the lines are the squared fields of every
-        history added since ``reset_coherent()``, per electron of them."""
+        history added since ``reset_coherent()``, per electron of them.  With
+        cone splitting, pooled copies are transported and scored first."""
+        if self.cone_core is not None:
+            self.flush_copies()
         if self.coherent is not None:
             lines = self.coherent.spectrum().cpu().numpy()
         else:
