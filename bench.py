@@ -16,11 +16,11 @@ reduction: one hr_sincsq_reduce call = sincsq_uniform_kernel plus its fixed-orde
 partial-sum kernel), timed with HIP events around its launches inside the timed
 region; `cpu_baseline` times the FP64 C oracle (the CPU restatement of the
 reference path, OpenMP over histories and bins) on a bounded sample of the same
-workload.  `fom` trains the importance network on analog tallies of the same
-case and compares the PXR-peak figure of merit 1/(rel_err^2 t) of the learned
-splitting/roulette run with the plain analog run (256 VR and 768 analog
-batches of 262,144 histories, disjoint, with bootstrap intervals because the
-peak tally is heavy-tailed; after the timed region, not part of `value`).
+workload.  `fom` compares the PXR-peak figure of merit 1/(rel_err^2 t) of
+the analog run, detector-cone splitting with the reference policy's defaults,
+and cone splitting with the learned importance roulette of its copies (48
+groups of 16 batches of 65,536 histories per arm, disjoint, with bootstrap
+intervals over groups; after the timed region, not part of `value`).
 """
 
 from __future__ import annotations
@@ -57,8 +57,9 @@ def parse():
     p.add_argument("--no-overlap", action="store_true", help="run transport and scoring of successive batches "
                                                               "back to back on one stream")
     p.add_argument("--no-fom", action="store_true", help="skip the learned-importance figure of merit")
-    p.add_argument("--fom-batches", type=int, default=256, help="learned-VR batches (analog: three times as many)")
-    p.add_argument("--fom-histories", type=int, default=262144, help="histories per FOM batch")
+    p.add_argument("--fom-groups", type=int, default=48, help="FOM sample units per arm (groups of batches)")
+    p.add_argument("--fom-group-batches", type=int, default=16, help="batches per FOM group")
+    p.add_argument("--fom-histories", type=int, default=65536, help="histories per FOM batch")
     p.add_argument("--workload", choices=("hopg", "si-stream", "pixels", "coherent", "queue"), default="hopg",
                    help="hopg: the bench line (default); si-stream: the 1 mm Si 10 MeV case streamed through "
                         "continuation rounds; pixels: windowed scoring of synthetic segments over 32x32 pixels; "
@@ -74,52 +75,92 @@ def parse():
     return p.parse_args()
 
 
-def learned_fom(pipe_cfg, n_batches, Ne=262144):
-    """PXR-peak figure of merit of the learned importance vs the plain analog run."""
-    import numpy as np
+def _fom_arm(pipe, window, dE, first, n_groups, per_group, label):
+    """Per-group PXR-peak tallies of ``n_groups`` groups of ``per_group`` batches
+    (a group is the sample unit: a cone-split pipeline transports the copies of
+    a group in one generation), and the wall time."""
+    import torch
+
+    vals = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for g in range(n_groups):
+        pipe.spec_lines.zero_()
+        pipe.spec_brem.zero_()
+        for b in range(per_group):
+            pipe.step(first + g * per_group + b, per_group)
+        if pipe.cone_core is not None:
+            pipe.flush_copies()
+        vals.append(pipe.spec_lines[window].sum() * dE)
+        if g % 16 == 15:
+            print(f"fom: {label} {g + 1}/{n_groups} groups, {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+                  flush=True)
+    vals = torch.stack(vals).cpu().numpy()
+    torch.cuda.synchronize()
+    return vals, time.perf_counter() - t0
+
+
+def learned_fom(pipe_cfg, n_groups=48, per_group=16, Ne=65536):
+    """PXR-peak figure of merit 1/(rel_err^2 t) of three estimators of the same case:
+    analog; detector-cone splitting with the reference policy's defaults; and
+    cone splitting without the angle roulette, its copies thinned by the learned
+    importance roulette (the BF16 network trained on the copies' own futures).
+    Every arm runs its batches back to back on one stream, over disjoint
+    histories; gains carry 68% bootstrap intervals over groups."""
     import torch
 
     from pyrite_amd.pipeline import CaseConfig, CasePipeline
-    from pyrite_amd.vr.importance import LearnedImportance, VRConfig, figure_of_merit
+    from pyrite_amd.vr.cone_roulette import LearnedConeRoulette
+    from pyrite_amd.vr.importance import LearnedImportance, figure_of_merit
 
-    pipe = CasePipeline(CaseConfig(**{**pipe_cfg.__dict__, "Ne": Ne}))
-    vr = LearnedImportance(pipe, VRConfig())
-    t0 = time.perf_counter()
-    n_train = max(1, (8 * 65536) // Ne)  # 524,288 training histories
-    x, y = vr.training_set(n_train, first_batch=10_000)
-    vr.fit(x, y)
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t0
-    vr.measure(1, Ne, "analog", first_batch=900)
-    vr.measure(1, Ne, "vr", first_batch=901)
-    out = {"histories_per_batch": Ne, "training_s": t_train,
-           "training_histories": n_train * Ne, "window_eV": list(vr.window)}
+    base = {**pipe_cfg.__dict__, "Ne": Ne}
+    # first batch index of each arm (disjoint electron ranges; training from batch 30,000)
+    arms = (("analog", None, 0), ("cone_split", {}, 10_000), ("learned", {"angle_scale": None}, 20_000))
     rng = np.random.default_rng(0)
+    out = {"histories_per_batch": Ne, "batches_per_group": per_group, "groups": n_groups}
     boots = {}
-    # the analog estimate of the variance is the weak one (its tail is the heavier),
-    # so it gets three times the batches
-    for mode, first, nb in (("analog", 0, 3 * n_batches), ("vr", 20_000, n_batches)):
-        parts, t = [], 0.0
-        chunk = max(1, (512 * 65536) // Ne)
-        for c0 in range(0, nb, chunk):  # chunks, with a progress line each
-            v, tc = vr.measure(min(chunk, nb - c0), Ne, mode, first_batch=first + c0)
-            parts.append(np.asarray(v))
-            t += tc
-            print(f"fom: {mode} {c0 + len(parts[-1])}/{nb} batches, {t:.1f} s", file=sys.stderr, flush=True)
-        vals = np.concatenate(parts)
+    window = None
+    for mode, cs, first in arms:
+        pipe = CasePipeline(CaseConfig(**{**base, "cone_split": cs}), arena_bytes=8 << 30)
+        if window is None:
+            li = LearnedImportance(pipe)
+            window = torch.as_tensor(li.window_bins, device=pipe.dev)
+            dE = li.dE
+            out["window_eV"] = list(li.window)
+        extra = {}
+        if mode == "learned":
+            t0 = time.perf_counter()
+            lr = LearnedConeRoulette(pipe, keep_quantile=0.9)
+            x, y, w = lr.training_set(16, first_batch=30_000)
+            lr.fit(x, y, w)
+            torch.cuda.synchronize()
+            pipe.cone_roulette = lr
+            extra = {"training_s": time.perf_counter() - t0, "training_copies": int(x.shape[0]),
+                     "keep_quantile": lr.keep_quantile}
+        pipe.step(10**6 + first, 1)  # warm-up
+        pipe.spectra()
+        vals, t = _fom_arm(pipe, window, dE, first, n_groups, per_group, mode)
         mean, rel, fom = figure_of_merit(vals, t)
         boot = []
-        for _ in range(1000):  # bootstrap over batches: the peak tally is heavy-tailed
-            s = rng.choice(vals, vals.size)
-            r = s.std(ddof=1) / np.sqrt(s.size) / s.mean()
+        for _ in range(1000):
+            sm = rng.choice(vals, vals.size)
+            r = sm.std(ddof=1) / np.sqrt(sm.size) / sm.mean()
             boot.append(1.0 / (r * r * t))
         boots[mode] = np.asarray(boot)
-        out[mode] = {"batches": nb, "mean_photons_per_electron": mean, "rel_err": rel, "time_s": t, "fom": fom,
-                     "fom_boot68": [float(x) for x in np.quantile(boot, [0.16, 0.84])],
-                     "max_over_median": float(vals.max() / np.median(vals))}
-    out["gain_learned_vs_analog"] = out["vr"]["fom"] / out["analog"]["fom"]
-    ratio = boots["vr"] / boots["analog"]
-    out["gain_boot68"] = [float(x) for x in np.quantile(ratio, [0.16, 0.84])]
+        out[mode] = {"histories": n_groups * per_group * Ne, "mean_photons_per_electron": mean, "rel_err": rel,
+                     "time_s": t, "fom": fom, "fom_boot68": [float(v) for v in np.quantile(boot, [0.16, 0.84])],
+                     "max_over_median": float(vals.max() / np.median(vals)), **extra}
+        if pipe.cone_core is not None:
+            out[mode]["copies"] = int(pipe.cone_stats["copies"])
+            out[mode]["copy_rows"] = int(pipe.cone_stats["copy_rows"])
+        if pipe.cone_roulette is not None:
+            out[mode]["roulette_kept"] = pipe.cone_roulette.stats["kept"] / max(1, pipe.cone_roulette.stats["seen"])
+        pipe.release()
+        del pipe
+        torch.cuda.empty_cache()
+    for mode in ("cone_split", "learned"):
+        out[f"gain_{mode}_vs_analog"] = out[mode]["fom"] / out["analog"]["fom"]
+        out[f"gain_{mode}_boot68"] = [float(v) for v in np.quantile(boots[mode] / boots["analog"], [0.16, 0.84])]
     return out
 
 
@@ -523,7 +564,7 @@ def main():
     del pipe
     torch.cuda.empty_cache()
     if not args.no_fom:
-        out["fom"] = learned_fom(cfg, args.fom_batches, args.fom_histories)
+        out["fom"] = learned_fom(cfg, args.fom_groups, args.fom_group_batches, args.fom_histories)
         torch.cuda.empty_cache()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(host_tables, args.cpu_sample, args.cpu_sample_one)
