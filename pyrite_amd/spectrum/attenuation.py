@@ -6,8 +6,12 @@ The reference reads the EPDL2025 total cross section (a fetched table,
 Offline, this package uses the XCOM total-with-coherent attenuation directly,
 log-log interpolated between the tabulated nodes with the edge rows kept as
 one-sided values.  Ge has no XCOM table here: its cross section is the Se
-table scaled per atom by ``(32/34)^4`` (photoelectric Z^4 law) -- a stated
-substitute, not a pinned value.
+table mapped by the photoelectric law sigma ~ Z^4 / E^3 with the energy axis
+scaled by the ratio of the two K-edge energies (Se 12.660 keV, Ge 11.1031
+keV; the table's Se edge node and Bearden & Burr 1967), sigma_Ge(E) =
+(32/34)^4 s^3 sigma_Se(s E),
+s = E_K(Se)/E_K(Ge), so Ge's K edge sits at 11.10 keV -- a stated substitute,
+not a pinned value.
 
 ``mu = n sigma`` with ``sigma = (mu/rho) A / N_A`` [cm^2 -> A^2].
 """
@@ -26,15 +30,21 @@ N_A = 6.02214076e23
 # numeric columns of a row: E_MeV, coherent, incoherent, photoelectric, pair_nuclear,
 # pair_electron, total_with_coh, total_without_coh
 COL_PHOTO, COL_TOTAL = 3, 6
+# K edges: Se at the XCOM table's edge node (12.660 keV), Ge 11.1031 keV (Bearden & Burr 1967)
+SE_K_EDGE_EV, GE_K_EDGE_EV = 12660.0, 11103.1
 
 
 @lru_cache(maxsize=None)
 def _table(element: str):
     """``(E_eV, mu_rho_total, mu_rho_photo)`` rows in file order [cm^2/g]."""
     scale = 1.0
+    e_scale = 1.0
     name = element
     if element == "Ge":
-        name, scale = "Se", (32.0 / 34.0) ** 4 * TRANSPORT_ELEMENTS["Se"]["A"] / TRANSPORT_ELEMENTS["Ge"]["A"]
+        name = "Se"
+        e_scale = GE_K_EDGE_EV / SE_K_EDGE_EV  # node energies: Se edges land on Ge's
+        s = 1.0 / e_scale
+        scale = (32.0 / 34.0) ** 4 * s**3 * TRANSPORT_ELEMENTS["Se"]["A"] / TRANSPORT_ELEMENTS["Ge"]["A"]
     path = DATA / f"xcom_{name}.txt"
     if not path.exists():
         raise KeyError(f"no XCOM attenuation table for {element}")
@@ -46,7 +56,7 @@ def _table(element: str):
         vals = [float(x) for x in parts[1:]]
         rows.append(vals)
     a = np.asarray(rows)
-    E = a[:, 0] * 1.0e6
+    E = a[:, 0] * 1.0e6 * e_scale
     return E, a[:, COL_TOTAL] * scale, a[:, COL_PHOTO] * scale
 
 
