@@ -66,6 +66,7 @@ def parse():
                         "coherent: the HOPG case on the coherent line route; queue: 256 uneven configurations "
                         "in one graph-replayed work-queue kernel")
     p.add_argument("--queue-scale", type=float, default=1e-3, help="queue: history-count scale of the campaign")
+    p.add_argument("--queue-replays", type=int, default=3, help="queue: timed graph replays (the fastest counts)")
     p.add_argument("--histories", type=int, default=1 << 21, help="si-stream: histories")
     p.add_argument("--lanes", type=int, default=1 << 19, help="si-stream: lanes in flight")
     p.add_argument("--rows-per-round", type=int, default=32, help="si-stream: rows per launch")
@@ -321,15 +322,18 @@ def queue_campaign(args):
     tq = TransportQueue(cases, dev)
     tuned = tq.autotune(fraction=0.02)
     g = tq.graph()
+    t0 = time.perf_counter()
     g.replay()
     torch.cuda.synchronize()
+    print(f"queue: {tq.total} histories, warm-up replay {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     ts = []
-    for _ in range(3):
+    for r in range(max(1, int(args.queue_replays))):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         g.replay()
         torch.cuda.synchronize()
         ts.append(time.perf_counter() - t0)
+        print(f"queue: replay {r + 1}: {ts[-1]:.1f} s", file=sys.stderr, flush=True)
     t_q = min(ts)
     tally = tq.tally.cpu().numpy()
     rows = float(tally[:, 7].sum())
@@ -343,6 +347,10 @@ def queue_campaign(args):
         L.n_cfg = 1
         _lib.check(tq.lib.hr_queue_run(C.byref(L), C.c_void_p(torch.cuda.current_stream().cuda_stream)),
                    "hr_queue_run")
+        if i % 32 == 31:
+            torch.cuda.synchronize()
+            print(f"queue: serial launches {i + 1}/{len(cases)}, {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+                  flush=True)
     torch.cuda.synchronize()
     t_s = time.perf_counter() - t0
     same = bool(np.array_equal(tq.tally.cpu().numpy()[:, :9], tally[:, :9]))
