@@ -60,6 +60,7 @@ def parse():
     p.add_argument("--fom-groups", type=int, default=48, help="FOM sample units per arm (groups of batches)")
     p.add_argument("--fom-group-batches", type=int, default=16, help="batches per FOM group")
     p.add_argument("--fom-histories", type=int, default=65536, help="histories per FOM batch")
+    p.add_argument("--fom-sequential", action="store_true", help="FOM arms back to back on one stream")
     p.add_argument("--workload", choices=("hopg", "si-stream", "pixels", "coherent", "queue"), default="hopg",
                    help="hopg: the bench line (default); si-stream: the 1 mm Si 10 MeV case streamed through "
                         "continuation rounds; pixels: windowed scoring of synthetic segments over 32x32 pixels; "
@@ -76,10 +77,12 @@ def parse():
     return p.parse_args()
 
 
-def _fom_arm(pipe, window, dE, first, n_groups, per_group, label):
+def _fom_arm(pipe, window, dE, first, n_groups, per_group, label, overlap=True):
     """Per-group PXR-peak tallies of ``n_groups`` groups of ``per_group`` batches
     (a group is the sample unit: a cone-split pipeline transports the copies of
-    a group in one generation), and the wall time."""
+    a group in one generation), and the wall time.  With ``overlap`` a group's
+    batches run through ``run_steps`` on two streams (transport and copy
+    generations beside scoring), else back to back on one."""
     import torch
 
     vals = []
@@ -88,8 +91,11 @@ def _fom_arm(pipe, window, dE, first, n_groups, per_group, label):
     for g in range(n_groups):
         pipe.spec_lines.zero_()
         pipe.spec_brem.zero_()
-        for b in range(per_group):
-            pipe.step(first + g * per_group + b, per_group)
+        if overlap:
+            pipe.run_steps(first + g * per_group, per_group, per_group, overlap=True)
+        else:
+            for b in range(per_group):
+                pipe.step(first + g * per_group + b, per_group)
         if pipe.cone_core is not None:
             pipe.flush_copies()
         vals.append(pipe.spec_lines[window].sum() * dE)
@@ -101,12 +107,13 @@ def _fom_arm(pipe, window, dE, first, n_groups, per_group, label):
     return vals, time.perf_counter() - t0
 
 
-def learned_fom(pipe_cfg, n_groups=48, per_group=16, Ne=65536):
+def learned_fom(pipe_cfg, n_groups=48, per_group=16, Ne=65536, overlap=True):
     """PXR-peak figure of merit 1/(rel_err^2 t) of three estimators of the same case:
     analog; detector-cone splitting with the reference policy's defaults; and
     cone splitting without the angle roulette, its copies thinned by the learned
     importance roulette (the BF16 network trained on the copies' own futures).
-    Every arm runs its batches back to back on one stream, over disjoint
+    Every arm runs the same schedule (``overlap``: transport beside scoring on
+    two streams, as the bench's timed steps; else back to back), over disjoint
     histories; gains carry 68% bootstrap intervals over groups."""
     import torch
 
@@ -118,11 +125,12 @@ def learned_fom(pipe_cfg, n_groups=48, per_group=16, Ne=65536):
     # first batch index of each arm (disjoint electron ranges; training from batch 30,000)
     arms = (("analog", None, 0), ("cone_split", {}, 10_000), ("learned", {"angle_scale": None}, 20_000))
     rng = np.random.default_rng(0)
-    out = {"histories_per_batch": Ne, "batches_per_group": per_group, "groups": n_groups}
+    out = {"histories_per_batch": Ne, "batches_per_group": per_group, "groups": n_groups,
+           "schedule": "two streams" if overlap else "back to back"}
     boots = {}
     window = None
     for mode, cs, first in arms:
-        pipe = CasePipeline(CaseConfig(**{**base, "cone_split": cs}), arena_bytes=8 << 30)
+        pipe = CasePipeline(CaseConfig(**{**base, "cone_split": cs}))  # the timed steps' slot-row arenas
         if window is None:
             li = LearnedImportance(pipe)
             window = torch.as_tensor(li.window_bins, device=pipe.dev)
@@ -140,7 +148,7 @@ def learned_fom(pipe_cfg, n_groups=48, per_group=16, Ne=65536):
                      "keep_quantile": lr.keep_quantile}
         pipe.step(10**6 + first, 1)  # warm-up
         pipe.spectra()
-        vals, t = _fom_arm(pipe, window, dE, first, n_groups, per_group, mode)
+        vals, t = _fom_arm(pipe, window, dE, first, n_groups, per_group, mode, overlap)
         mean, rel, fom = figure_of_merit(vals, t)
         boot = []
         for _ in range(1000):
@@ -572,7 +580,8 @@ def main():
     del pipe
     torch.cuda.empty_cache()
     if not args.no_fom:
-        out["fom"] = learned_fom(cfg, args.fom_groups, args.fom_group_batches, args.fom_histories)
+        out["fom"] = learned_fom(cfg, args.fom_groups, args.fom_group_batches, args.fom_histories,
+                                 overlap=not args.fom_sequential)
         torch.cuda.empty_cache()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(host_tables, args.cpu_sample, args.cpu_sample_one)

@@ -440,7 +440,7 @@ class CasePipeline(CaseTables):
         self.histories += self.cfg.Ne
         return res
 
-    def _step_cone(self, batch_index, n_total, consume):
+    def _step_cone(self, batch_index, n_total, consume, copy_consumer=None):
         """A batch with detector-cone splitting: the primaries (rows weight 1,
         copies collected per launch), their copies pooled for one later copy
         generation (``flush_copies``) once ``copy_pool`` copies are waiting."""
@@ -454,22 +454,25 @@ class CasePipeline(CaseTables):
         self.cone_stats["copies"] += n_copies
         self.cone_stats["killed"] += 0 if cp is None else int(cp["killed"].sum())
         if self._pool and self._pool[0][1] != n_total:
-            self.flush_copies()
+            self.flush_copies(copy_consumer)
         if n_copies:
             self._pool.append((cp, n_total))
             self._pool_n += n_copies
         if self._pool_n >= self.copy_pool:
-            self.flush_copies()
+            self.flush_copies(copy_consumer)
         res.cone_copies = n_copies
         return res
 
-    def flush_copies(self):
+    def flush_copies(self, copy_consumer=None):
         """Transport every pooled copy as one analog generation and score its
         rows with its weight (``cone_split.py:447-513``).  Copies are
         independent of each other, so pooling several batches' copies into one
         launch changes no tally beyond FP64 summation order, while the launch's
         latency (the longest copy history, often one running along the slab
-        face) is paid once per pool instead of once per batch."""
+        face) is paid once per pool instead of once per batch.
+        ``copy_consumer(weights, n_total)`` returns the ``consume`` callback of
+        the copies' slot rows (the two-stream overlap of ``run_steps``); by
+        default they are scored on the current stream."""
         import torch
 
         from .cone_split import transport_copies
@@ -485,6 +488,9 @@ class CasePipeline(CaseTables):
         def consume_copies(e0, m, cap, rows, seg_count):
             self.cone_stats["copy_rows"] += int(seg_count.clamp(max=cap).sum())
             self._score_slot(e0, m, cap, rows, seg_count, 0, n_total, weights=w[e0:e0 + m])
+
+        if copy_consumer is not None:
+            consume_copies = copy_consumer(w, n_total)
 
         budget = min(self.arena_bytes, 8 << 30)
         res = transport_copies(cp, self.tables, self.control, self.cfg.E_cut_keV, self.dev,
@@ -507,7 +513,7 @@ class CasePipeline(CaseTables):
         import torch
 
         n_total_b = int(n_steps if n_batches_total is None else n_batches_total)
-        if not overlap or self.cfg.line_roulette is not None or self.cone_core is not None:
+        if not overlap or self.cfg.line_roulette is not None:
             for b in range(int(n_steps)):
                 self.step(int(first_batch) + b, n_total_b)
             return
@@ -518,6 +524,24 @@ class CasePipeline(CaseTables):
         sa, sb = self._streams
         sa.wait_stream(main)
         sb.wait_stream(main)
+
+        def copy_consumer(w, n_tot):
+            """Copies' slot rows (fresh tensors, not arena rows) scored on the
+            scoring stream; their memory is held for it by record_stream."""
+
+            def consume_copies(e0, m, cap, rows, seg_count):
+                self.cone_stats["copy_rows"] += int(seg_count.clamp(max=cap).sum())
+                done = torch.cuda.Event()
+                done.record(sa)
+                wi = w[e0:e0 + m]
+                for t in (*rows.values(), seg_count, wi):
+                    t.record_stream(sb)
+                with torch.cuda.stream(sb):
+                    sb.wait_event(done)
+                    self._score_slot(e0, m, cap, rows, seg_count, 0, n_tot, weights=wi)
+
+            return consume_copies
+
         for b in range(int(n_steps)):
             batch = int(first_batch) + b
 
@@ -530,8 +554,14 @@ class CasePipeline(CaseTables):
                     self._score_slot(e0, m, cap, rows, seg_count, batch, n_total)
 
             with torch.cuda.stream(sa):
-                self.transport(batch, compact=False, consume=consume)
-            self.histories += self.cfg.Ne
+                if self.cone_core is None:
+                    self.transport(batch, compact=False, consume=consume)
+                    self.histories += self.cfg.Ne
+                else:
+                    self._step_cone(batch, n_total, consume, copy_consumer)
+        if self.cone_core is not None:
+            with torch.cuda.stream(sa):
+                self.flush_copies(copy_consumer)
         main.wait_stream(sa)
         main.wait_stream(sb)
 

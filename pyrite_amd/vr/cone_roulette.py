@@ -74,11 +74,14 @@ class LearnedConeRoulette:
                 "E_keV": rows["seg_E"], "E_repr_keV": 0.5 * (rows["seg_E"] + rows["seg_E_end"]),
             }, p.dev)
             m2 = self.li.row_second_moments(seg, p.cfg.Ne)
-            y2.index_add_(0, rows["seg_id"], m2)
+            # compacted rows are copy-major: per-copy sums from one running sum
+            cnt = res.seg_count.to(torch.int64)
+            ends = torch.cumsum(cnt, 0)
+            cs = torch.zeros(m2.numel() + 1, dtype=torch.float64, device=p.dev)
+            cs[1:] = torch.cumsum(m2, 0)
+            y2 = cs[ends] - cs[ends - cnt]
             if self.per_cost:
-                n_rows = torch.zeros(n, dtype=torch.float64, device=p.dev)
-                n_rows.index_add_(0, rows["seg_id"], torch.ones_like(m2))
-                y2 = y2 / torch.clamp(n_rows, min=1.0)
+                y2 = y2 / torch.clamp(cnt.double(), min=1.0)
         return x, torch.sqrt(torch.clamp(y2, min=0.0))
 
     def training_set(self, n_batches, first_batch=30_000):
