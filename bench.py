@@ -463,7 +463,7 @@ def live_cells(pipe, steps, launches):
 
 # HBM bytes per sincsq_uniform_kernel launch of the default workload (mean of the
 # step's two launches), measured with rocprofv3 --pmc and a copy calibration.
-SINC_TRAFFIC_BYTES = 384.6e6
+SINC_TRAFFIC_BYTES = 265.5e6
 
 
 def main():

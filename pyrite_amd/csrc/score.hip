@@ -134,14 +134,16 @@ __global__ void __launch_bounds__(kRedThreads) sincsq_uniform_kernel(
     const int64_t n_tiles = (n_lines + kLineTile - 1) / kLineTile;
     for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
         const int64_t l = tile * kLineTile + t;
-        // u(E_k) = a (E_k - E_r)/pi = u0 + k du
-        if (l < n_lines) {
+        // u(E_k) = a (E_k - E_r)/pi = u0 + k du; the weight is read first and a
+        // zero-weight line's E_r and a are never fetched (rejected resonances
+        // come in whole waves: a ray whose in-medium root never settles)
+        const float wl = l < n_lines ? w[l] : 0.0f;
+        s_w[t] = wl;
+        if (wl != 0.0f) {
             const double al = a[l] * inv_pi;
             s_ud[t] = make_double2(al * (E0 - E_r[l]), al * dE);
-            s_w[t] = w[l];
         } else {
             s_ud[t] = make_double2(0.0, 0.0);
-            s_w[t] = 0.0f;
         }
         __syncthreads();
         for (int q0 = 0; q0 < kLineTile; q0 += kSincGroup) {
@@ -198,9 +200,10 @@ __global__ void __launch_bounds__(kRedThreads) sincsq_general_kernel(
     const int64_t n_tiles = (n_lines + kLineTile - 1) / kLineTile;
     for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
         const int64_t l = tile * kLineTile + t;
-        s_er[t] = l < n_lines ? E_r[l] : 0.0;
-        s_a[t] = l < n_lines ? a[l] * inv_pi : 0.0;
-        s_w[t] = l < n_lines ? w[l] : 0.0f;
+        const float wl = l < n_lines ? w[l] : 0.0f;  // zero-weight lines: E_r and a not fetched
+        s_w[t] = wl;
+        s_er[t] = wl != 0.0f ? E_r[l] : 0.0;
+        s_a[t] = wl != 0.0f ? a[l] * inv_pi : 0.0;
         __syncthreads();
         for (int q0 = 0; q0 < kLineTile; q0 += kSincGroup) {
             float g[BINS];
