@@ -75,7 +75,8 @@ def build(verbose: bool = True, out: Path | None = None, extra_flags=()) -> Path
         objs = list(pool.map(lambda src: _compile(src, build_dir, extra_flags), sources))
     LIB_ = lib
     tmp = LIB_.with_suffix(".so.tmp")
-    cmd = [_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp)]
+    cmd = [_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-L/opt/rocm/lib",
+           "-lrocprofiler-sdk-roctx", "-o", str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")

@@ -83,6 +83,7 @@ __global__ void __launch_bounds__(256) beam_entry_kernel(hr_beam_args p) {
 }  // namespace hr
 
 extern "C" hr_status hr_beam_entry(const hr_beam_args *p, void *stream) {
+    HR_TRACE("hr_beam_entry");
     using namespace hr;
     HR_REQUIRE(p != nullptr && p->n >= 0 && p->start >= 0, "hr_beam_entry: bad args");
     if (p->n == 0) return HR_OK;

@@ -342,6 +342,7 @@ hr_status field_plan(const hr_coherent_field_args &p, FieldPlan &P) {
 }  // namespace hr
 
 extern "C" hr_status hr_coherent_lines(const hr_coherent_args *args, void *stream) {
+    HR_TRACE("hr_coherent_lines");
     using namespace hr;
     HR_REQUIRE(args != nullptr && args->lines != nullptr, "hr_coherent_lines: null args");
     const hr_coherent_args &a = *args;
@@ -383,6 +384,7 @@ extern "C" hr_status hr_coherent_field_workspace_bytes(const hr_coherent_field_a
 }
 
 extern "C" hr_status hr_coherent_field(const hr_coherent_field_args *args, void *stream) {
+    HR_TRACE("hr_coherent_field");
     using namespace hr;
     HR_REQUIRE(args != nullptr, "hr_coherent_field: null args");
     const hr_coherent_field_args &p = *args;
@@ -416,6 +418,7 @@ extern "C" hr_status hr_coherent_field(const hr_coherent_field_args *args, void 
 
 extern "C" hr_status hr_coherent_finalize(const double *field, int32_t n_dir, int32_t n_g, const double *dir_weight,
                                           const double *wm, int64_t n_E, double scale, double *out, void *stream) {
+    HR_TRACE("hr_coherent_finalize");
     using namespace hr;
     HR_REQUIRE(n_dir >= 1 && n_g >= 1 && n_E >= 0, "hr_coherent_finalize: bad sizes (n_dir=%d n_g=%d n_E=%lld)",
                n_dir, n_g, (long long)n_E);

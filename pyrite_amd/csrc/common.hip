@@ -60,6 +60,7 @@ int32_t hr_device_count(void) {
 }
 
 hr_status hr_stream_keys(uint64_t seed, int64_t start, int64_t n, uint64_t *keys_d, void *stream) {
+    HR_TRACE("hr_stream_keys");
     HR_REQUIRE(n >= 0 && start >= 0, "hr_stream_keys: negative start or count");
     if (n == 0) return HR_OK;
     HR_REQUIRE(keys_d != nullptr, "hr_stream_keys: null output");

@@ -201,6 +201,7 @@ extern "C" int64_t hr_cone_collect_workspace_bytes(int64_t e_count) {
 }
 
 extern "C" hr_status hr_cone_collect(const hr_cone_collect_args *args, void *stream) {
+    HR_TRACE("hr_cone_collect");
     using namespace hr;
     HR_REQUIRE(args != nullptr && args->cone != nullptr, "hr_cone_collect: null args");
     const hr_cone_collect_args &p = *args;
@@ -238,6 +239,7 @@ extern "C" hr_status hr_cone_collect(const hr_cone_collect_args *args, void *str
 }
 
 extern "C" hr_status hr_cone_split_keys(const uint64_t *stream_key, int64_t n, uint64_t *out, void *stream) {
+    HR_TRACE("hr_cone_split_keys");
     using namespace hr;
     HR_REQUIRE(n >= 0, "hr_cone_split_keys: negative n");
     if (n == 0) return HR_OK;
@@ -250,6 +252,7 @@ extern "C" hr_status hr_cone_split_keys(const uint64_t *stream_key, int64_t n, u
 
 extern "C" hr_status hr_cone_event(const hr_cone_split *cone, int64_t n, const double *d, const double *alpha,
                                    const double *E_keV, const uint64_t *event_key, double *out, void *stream) {
+    HR_TRACE("hr_cone_event");
     using namespace hr;
     HR_REQUIRE(cone != nullptr && n >= 0, "hr_cone_event: null cone or negative n");
     if (n == 0) return HR_OK;
@@ -264,6 +267,7 @@ extern "C" hr_status hr_cone_event(const hr_cone_split *cone, int64_t n, const d
 
 extern "C" hr_status hr_cone_roulette(int64_t n, const float *log_imp, const uint64_t *stream_key, double log_T,
                                       double p_min, double *weight, uint8_t *keep, void *stream) {
+    HR_TRACE("hr_cone_roulette");
     using namespace hr;
     HR_REQUIRE(n >= 0, "hr_cone_roulette: negative n");
     if (n == 0) return HR_OK;

@@ -66,6 +66,7 @@ hr_status check_tables(const hr_hard_events *h, const char *who) {
 extern "C" hr_status hr_radiative_lut(const hr_hard_events *h, int32_t n_layers, int32_t max_el,
                                       const int32_t *L_nel, const double *E_grid_keV, int64_t n_E, double *soft_out,
                                       double *rate_out, void *stream) {
+    HR_TRACE("hr_radiative_lut");
     using namespace hr;
     hr_status st = check_tables(h, "hr_radiative_lut");
     if (st != HR_OK) return st;
@@ -84,6 +85,7 @@ extern "C" hr_status hr_radiative_lut(const hr_hard_events *h, int32_t n_layers,
 
 extern "C" hr_status hr_radiative_sample(const hr_hard_events *h, int32_t slot, double plasma_eV, int64_t n,
                                          const double *E_eV, const double *u, double *k_out, void *stream) {
+    HR_TRACE("hr_radiative_sample");
     using namespace hr;
     hr_status st = check_tables(h, "hr_radiative_sample");
     if (st != HR_OK) return st;

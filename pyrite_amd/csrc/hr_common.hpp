@@ -1,6 +1,7 @@
 // hr_common.hpp -- shared helpers of libhiprite: error state, launch checks,
 // counter-addressed SplitMix64 streams on the device.
 #pragma once
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -33,6 +34,17 @@ const char *last_error();
             return HR_ERR_INVALID;            \
         }                                     \
     } while (0)
+
+// A roctx range over an entry point's host work (validation and the launches
+// it enqueues), named after the entry point: `rocprofv3 --marker-trace` shows
+// it beside the kernels of the call.
+struct TraceRange {
+    explicit TraceRange(const char *name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange &) = delete;
+    TraceRange &operator=(const TraceRange &) = delete;
+};
+#define HR_TRACE(name) ::hr::TraceRange hr_trace_range_(name)
 
 // Constants of the counter-addressed streams (kinematics.py:181-193).
 constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ULL;

@@ -334,6 +334,7 @@ size_t hr_mlp_workspace_bytes(int64_t n) {
 }
 
 hr_status hr_mlp_shadow(const float *params_d, uint16_t *w1b_d, uint16_t *w2b_d, void *stream) {
+    HR_TRACE("hr_mlp_shadow");
     HR_REQUIRE(params_d && w1b_d && w2b_d, "hr_mlp_shadow: null pointer");
     hipLaunchKernelGGL(hr::mlp_shadow_kernel, dim3((hr::H * hr::H + 255) / 256), dim3(256), 0,
                        (hipStream_t)stream, params_d, (unsigned short *)w1b_d, (unsigned short *)w2b_d);
@@ -342,6 +343,7 @@ hr_status hr_mlp_shadow(const float *params_d, uint16_t *w1b_d, uint16_t *w2b_d,
 }
 
 hr_status hr_mlp_forward(const hr_mlp *net, const float *x_d, int64_t n, float *y_d, void *stream) {
+    HR_TRACE("hr_mlp_forward");
     using namespace hr;
     HR_REQUIRE(net && net->params && net->w1b && net->w2b, "hr_mlp_forward: null network");
     HR_REQUIRE(n >= 0, "hr_mlp_forward: negative n");
@@ -358,6 +360,7 @@ hr_status hr_mlp_forward(const hr_mlp *net, const float *x_d, int64_t n, float *
 
 hr_status hr_mlp_grad(const hr_mlp *net, const float *x_d, const float *t_d, int64_t n, float *workspace_d,
                       float *grad_d, float *loss_d, void *stream) {
+    HR_TRACE("hr_mlp_grad");
     using namespace hr;
     HR_REQUIRE(net && net->params && net->w1b && net->w2b, "hr_mlp_grad: null network");
     HR_REQUIRE(n > 0 && x_d && t_d && workspace_d && grad_d, "hr_mlp_grad: bad arguments");
@@ -376,6 +379,7 @@ hr_status hr_mlp_grad(const hr_mlp *net, const float *x_d, const float *t_d, int
 
 hr_status hr_mlp_adam(hr_mlp *net, const float *grad_d, float *m1_d, float *m2_d, float lr, int32_t step,
                       void *stream) {
+    HR_TRACE("hr_mlp_adam");
     using namespace hr;
     HR_REQUIRE(net && net->params && grad_d && m1_d && m2_d && step >= 1, "hr_mlp_adam: bad arguments");
     hipLaunchKernelGGL(mlp_adam_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, (hipStream_t)stream,

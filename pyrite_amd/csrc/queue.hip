@@ -161,6 +161,7 @@ struct QueueGraph {
 extern "C" {
 
 hr_status hr_queue_run(const hr_queue_launch *launch, void *stream) {
+    HR_TRACE("hr_queue_run");
     using namespace hr;
     hr_status st = queue_check(launch);
     if (st != HR_OK) return st;
@@ -206,6 +207,7 @@ hr_status hr_queue_graph_create(const hr_queue_launch *launch, void **graph) {
 }
 
 hr_status hr_queue_graph_launch(void *graph, void *stream) {
+    HR_TRACE("hr_queue_graph_launch");
     using namespace hr;
     HR_REQUIRE(graph != nullptr, "hr_queue_graph_launch: null graph");
     QueueGraph *g = (QueueGraph *)graph;

@@ -696,6 +696,7 @@ int64_t window_chunks(int cu, int64_t n_seg, int32_t n_dir) {
 }  // namespace hr
 
 extern "C" hr_status hr_cxr_lines(const hr_line_args *args, void *stream) {
+    HR_TRACE("hr_cxr_lines");
     using namespace hr;
     HR_REQUIRE(args != nullptr, "hr_cxr_lines: null args");
     const hr_line_args &p = *args;
@@ -718,6 +719,7 @@ extern "C" hr_status hr_cxr_lines(const hr_line_args *args, void *stream) {
 }
 
 extern "C" hr_status hr_sincsq_reduce(const hr_sincsq_args *args, void *stream) {
+    HR_TRACE("hr_sincsq_reduce");
     using namespace hr;
     HR_REQUIRE(args != nullptr, "hr_sincsq_reduce: null args");
     const hr_sincsq_args &p = *args;
@@ -774,6 +776,7 @@ hr_status static_scratch(void **out) {
 }  // namespace hr
 
 extern "C" hr_status hr_sincsq_reduce_static(const hr_sincsq_args *args, void *stream) {
+    HR_TRACE("hr_sincsq_reduce_static");
     using namespace hr;
     HR_REQUIRE(args != nullptr, "hr_sincsq_reduce_static: null args");
     HR_REQUIRE(args->n_lines >= 0 && args->n_E >= 0, "hr_sincsq_reduce_static: negative size");
@@ -817,6 +820,7 @@ extern "C" hr_status hr_brem_workspace_bytes(const hr_brem_args *args, size_t *b
 }
 
 extern "C" hr_status hr_brem_reduce(const hr_brem_args *args, void *stream) {
+    HR_TRACE("hr_brem_reduce");
     using namespace hr;
     HR_REQUIRE(args != nullptr, "hr_brem_reduce: null args");
     const hr_brem_args &p = *args;
@@ -893,6 +897,7 @@ extern "C" hr_status hr_window_reduce_workspace_bytes(int64_t n_lines, int64_t n
 extern "C" hr_status hr_window_reduce(const double *E_r, const double *a, const float *w, int64_t n_lines,
                                       int64_t n_E, double E0, double dE, double x_c, double *out, hr_workspace ws,
                                       void *stream) {
+    HR_TRACE("hr_window_reduce");
     using namespace hr;
     HR_REQUIRE(n_lines >= 0, "hr_window_reduce: negative n_lines");
     if (n_lines == 0 || n_E == 0) return HR_OK;
@@ -935,6 +940,7 @@ extern "C" hr_status hr_window_spectra_workspace_bytes(const hr_window_args *arg
 }
 
 extern "C" hr_status hr_window_spectra(const hr_window_args *args, void *stream) {
+    HR_TRACE("hr_window_spectra");
     using namespace hr;
     HR_REQUIRE(args != nullptr && args->lines != nullptr, "hr_window_spectra: null args");
     const hr_line_args &p = *args->lines;

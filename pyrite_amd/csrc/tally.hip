@@ -70,6 +70,7 @@ __global__ void __launch_bounds__(kDoseThreads) depth_dose_kernel(hr_dose_args p
 }  // namespace hr
 
 extern "C" hr_status hr_depth_dose(const hr_dose_args *args, void *stream) {
+    HR_TRACE("hr_depth_dose");
     using namespace hr;
     HR_REQUIRE(args != nullptr, "hr_depth_dose: null args");
     const hr_dose_args &p = *args;

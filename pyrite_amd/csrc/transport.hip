@@ -216,6 +216,7 @@ hr_status check_hard(const hr_transport_args &a, const char *who) {
 }  // namespace hr
 
 extern "C" hr_status hr_transport_run(const hr_transport_args *args, void *stream) {
+    HR_TRACE("hr_transport_run");
     using namespace hr;
     HR_REQUIRE(args != nullptr, "hr_transport_run: null args");
     const hr_transport_args &a = *args;
@@ -278,6 +279,7 @@ extern "C" hr_status hr_transport_run(const hr_transport_args *args, void *strea
 
 extern "C" hr_status hr_transport_continue(const hr_transport_args *args, const hr_continuation *cont,
                                            void *stream) {
+    HR_TRACE("hr_transport_continue");
     using namespace hr;
     HR_REQUIRE(args != nullptr && cont != nullptr, "hr_transport_continue: null args");
     const hr_transport_args &a = *args;

@@ -253,6 +253,7 @@ unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1
 extern "C" {
 
 hr_status hr_vr_round(const hr_transport_args *a, int32_t rows, const int32_t *rows_each, void *stream) {
+    HR_TRACE("hr_vr_round");
     using namespace hr;
     HR_REQUIRE(a != nullptr, "hr_vr_round: null args");
     HR_REQUIRE(rows >= 1 && rows <= a->cap, "hr_vr_round: rows %d outside [1, cap=%lld]", rows, (long long)a->cap);
@@ -285,6 +286,7 @@ hr_status hr_vr_round(const hr_transport_args *a, int32_t rows, const int32_t *r
 
 hr_status hr_vr_features(int64_t n, const double *pos, const double *dirs, const double *E, const double *seg_len,
                          double E0_keV, double thickness_ang, double tilt_deg, float *x, void *stream) {
+    HR_TRACE("hr_vr_features");
     using namespace hr;
     HR_REQUIRE(n >= 0, "hr_vr_features: negative n");
     if (n == 0) return HR_OK;
@@ -297,6 +299,7 @@ hr_status hr_vr_features(int64_t n, const double *pos, const double *dirs, const
 }
 
 hr_status hr_vr_window(const hr_vr_window_args *p, void *stream) {
+    HR_TRACE("hr_vr_window");
     using namespace hr;
     HR_REQUIRE(p != nullptr && p->n >= 0, "hr_vr_window: bad args");
     if (p->n == 0) return HR_OK;
@@ -313,6 +316,7 @@ hr_status hr_vr_window(const hr_vr_window_args *p, void *stream) {
 }
 
 hr_status hr_vr_spawn(const hr_vr_spawn_args *p, void *stream) {
+    HR_TRACE("hr_vr_spawn");
     using namespace hr;
     HR_REQUIRE(p != nullptr && p->n >= 0, "hr_vr_spawn: bad args");
     if (p->n == 0) return HR_OK;
@@ -327,6 +331,7 @@ hr_status hr_vr_spawn(const hr_vr_spawn_args *p, void *stream) {
 }
 
 hr_status hr_vr_gather(const hr_vr_gather_args *p, void *stream) {
+    HR_TRACE("hr_vr_gather");
     using namespace hr;
     HR_REQUIRE(p != nullptr && p->n >= 0 && p->rows >= 1, "hr_vr_gather: bad args");
     if (p->n == 0) return HR_OK;
@@ -346,6 +351,7 @@ hr_status hr_vr_gather(const hr_vr_gather_args *p, void *stream) {
 
 hr_status hr_line_mass(const double *E_r, const double *a, const float *w, int64_t n_seg, int32_t n_dir,
                        int32_t n_g, double lo_eV, double hi_eV, double dE_eV, double *mass, void *stream) {
+    HR_TRACE("hr_line_mass");
     using namespace hr;
     HR_REQUIRE(n_seg >= 0 && n_dir >= 1 && n_g >= 1, "hr_line_mass: bad sizes");
     if (n_seg == 0) return HR_OK;
@@ -359,6 +365,7 @@ hr_status hr_line_mass(const double *E_r, const double *a, const float *w, int64
 
 hr_status hr_line_roulette(float *w, int64_t n_seg, int32_t n_dir, int32_t n_g, const double *mass, double m_thr,
                            uint64_t key, int64_t row0, void *stream) {
+    HR_TRACE("hr_line_roulette");
     using namespace hr;
     HR_REQUIRE(n_seg >= 0 && n_dir >= 1 && n_g >= 1 && row0 >= 0, "hr_line_roulette: bad sizes");
     if (n_seg == 0) return HR_OK;
@@ -371,6 +378,7 @@ hr_status hr_line_roulette(float *w, int64_t n_seg, int32_t n_dir, int32_t n_g, 
 }
 
 hr_status hr_roulette_uniforms(uint64_t seed, int64_t first_id, int64_t n, double *out, void *stream) {
+    HR_TRACE("hr_roulette_uniforms");
     using namespace hr;
     HR_REQUIRE(n >= 0 && first_id >= 0, "hr_roulette_uniforms: bad range");
     if (n == 0) return HR_OK;
@@ -383,6 +391,7 @@ hr_status hr_roulette_uniforms(uint64_t seed, int64_t first_id, int64_t n, doubl
 
 hr_status hr_line_screen(const double *mass, int64_t n, int64_t first_id, int64_t pilot, double threshold,
                          double floor_p, uint64_t seed, double *probability, uint8_t *kept, void *stream) {
+    HR_TRACE("hr_line_screen");
     using namespace hr;
     HR_REQUIRE(n >= 0 && first_id >= 0 && pilot >= 0, "hr_line_screen: bad range");
     HR_REQUIRE(floor_p > 0.0 && floor_p <= 1.0, "hr_line_screen: floor must lie in (0, 1]");
