@@ -141,9 +141,24 @@ def test_full_histories_agree_in_aggregate(oracle, device, case):
         a, b = go[k], oo[k]
         se = math.sqrt(a.var() / n + b.var() / n)
         assert abs(a.mean() - b.mean()) <= 4.0 * se + 1e-12 * abs(b.mean()), (k, a.mean(), b.mean(), se)
-    # most histories agree step for step (host/device log/sin/cos may differ by an ulp)
-    same = np.mean(g[1] == o[1])
-    assert same > 0.5
+    # Host and device libm may differ in the last ulp of log/exp/sqrt/sin/cos, so
+    # a history could take another branch where a draw or a distance sits within
+    # an ulp of a threshold; measured on 4,000 histories of every case, none
+    # does: every history has the oracle's row count, exit code and rows.
+    np.testing.assert_array_equal(g[1], o[1])
+    np.testing.assert_array_equal(g[2], o[2])
+    worst = 0.0
+    # the frozen energy model's rows carry no end columns (batching.py:398-400)
+    cols = ("seg_mid", "seg_E", "seg_len") + (("seg_E_end", "seg_t_end") if case.energy_model == 1 else ())
+    for k in cols:
+        for e in range(n):
+            m = int(min(g[1][e], 4096))
+            a = g[0][k][e * 4096:e * 4096 + m]
+            b = o[0][k][e * 4096:e * 4096 + m]
+            if m:
+                scale = np.maximum(np.abs(b), 1.0)
+                worst = max(worst, float(np.max(np.abs(a - b) / scale)))
+    assert worst < 1e-9, worst
 
 
 @pytest.mark.parametrize("case", [SI_30KEV_STRAGGLE, HOPG_5MEV, MULTILAYER], ids=lambda c: c.name)
