@@ -3,10 +3,11 @@
 Bars, as the reference's device tests (test_transport_per_electron.py:349-460):
 * stream keys bit for bit;
 * first step of every history to rtol 1e-12 (host/device transcendental
-  functions differ in the last ulp, so later steps may diverge);
+  functions differ in the last ulp);
 * capacity replay, launch block and batch split invariance bit for bit on the
   device;
-* full histories: aggregate observables within 4 sigma of the oracle.
+* full histories: every history row for row (counts and exit codes equal,
+  rows to 1e-9), and aggregate observables within 4 sigma of the oracle.
 """
 
 import dataclasses
