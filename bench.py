@@ -361,7 +361,7 @@ def queue_campaign(args):
                   flush=True)
     torch.cuda.synchronize()
     t_s = time.perf_counter() - t0
-    same = bool(np.array_equal(tq.tally.cpu().numpy()[:, :9], tally[:, :9]))
+    same = bool(np.array_equal(tq.tally.cpu().numpy(), tally))  # every tally, energies included, bit for bit
     return {"metric": "histories/s, 256-configuration campaign in one work-queue graph", "value": tq.total / t_q,
             "unit": "histories/s", "n_gpus": 1, "seconds": t_q, "histories": int(tq.total), "rows": rows,
             "rows_per_second": rows / t_q, "serial_seconds": t_s, "speedup_vs_serial_launches": t_s / t_q,

@@ -31,14 +31,22 @@ __device__ __forceinline__ int find_config(const int64_t *__restrict__ off, int 
     return lo;
 }
 
+// Energies enter the tally quantised to 2^-12 keV (0.24 eV): FP64 sums of
+// integer multiples of the quantum are exact up to 2^53 quanta (2.2e12 keV per
+// configuration), so, like the integer-valued counts, they do not depend on
+// the order in which the atomics land (the claim order): the tallies are
+// bitwise reproducible.
+constexpr double kTallyQuantum = 1.0 / 4096.0;
+__device__ __forceinline__ double quantised(double keV) { return rint(keV * 4096.0) * kTallyQuantum; }
+
 __device__ __forceinline__ void tally_history(double *__restrict__ t, const Lane &s, int code, double E0) {
     atomicAdd(&t[code], 1.0);
     atomicAdd(&t[7], (double)s.nseg);
     atomicAdd(&t[8], (double)s.nseg * (double)s.nseg);
-    if (code == EXIT_TRANS) atomicAdd(&t[9], s.E);
-    if (code == EXIT_BACK) atomicAdd(&t[10], s.E);
+    if (code == EXIT_TRANS) atomicAdd(&t[9], quantised(s.E));
+    if (code == EXIT_BACK) atomicAdd(&t[10], quantised(s.E));
     const bool escaped = code == EXIT_BACK || code == EXIT_TRANS || code == EXIT_SIDE;
-    atomicAdd(&t[11], E0 - (escaped ? s.E : 0.0));
+    atomicAdd(&t[11], quantised(E0) - (escaped ? quantised(s.E) : 0.0));
 }
 
 template <bool kMixed>
