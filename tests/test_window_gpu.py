@@ -72,3 +72,33 @@ def test_pixel_spectra_match_oracle(oracle, device):
         w32 = np.where(og, ow * weights[d], 0.0).astype(np.float32)
         want = oracle.window_reduce(oE, oa, w32, E0, dE, n_E, 100.0)
         _close(got[d], want)
+
+
+def test_window_reduce_repeats_to_summation_order(device):
+    """The block's FP64 LDS atomics land in hardware order (as the reference's
+    device route's global FP64 atomics do, spectrum/cxr/_node_window_cuda.py:42,91),
+    so repeated launches may differ, but only in FP64 summation order: every
+    bin within 1e-13 of the spectrum's peak, where one line's deposit at its
+    own node (weight >= 0.01 here, profile ~1) is a million times larger."""
+    import torch
+
+    from pyrite_amd import _lib
+
+    rng = np.random.default_rng(12)
+    n_lines = 50000
+    n, E0, dE = 4096, 1000.0, 99000.0 / 4096
+    E_r = rng.uniform(1000.0, 100000.0, n_lines)
+    a = np.exp(rng.uniform(np.log(0.05), np.log(5.0), n_lines))
+    w = rng.uniform(0.01, 1.0, n_lines).astype(np.float32)
+    t = [torch.as_tensor(v, device=device) for v in (E_r, a, w)]
+    ws = _lib.DeviceScratch(device).get(_lib.workspace_bytes("hr_window_reduce_workspace_bytes", n_lines, n))
+    outs = []
+    for _ in range(4):
+        out = torch.zeros(n, dtype=torch.float64, device=device)
+        _lib.call("hr_window_reduce", t[0].data_ptr(), t[1].data_ptr(), t[2].data_ptr(), n_lines, n, E0, dE, 100.0,
+                  out.data_ptr(), ws, _stream())
+        outs.append(out.cpu().numpy())
+    peak = np.abs(outs[0]).max()
+    assert 1e-13 * peak < 1e-6 * 0.01
+    for o in outs[1:]:
+        assert np.abs(o - outs[0]).max() <= 1e-13 * peak
