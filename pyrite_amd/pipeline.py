@@ -68,6 +68,8 @@ class CaseConfig:
     depth_bins: int = 0  # > 0: also tally the depth-dose profile over this many bins of the slab
     coherent: bool = False  # sum line fields before squaring (the coherent route); brem stays incoherent
     cone_split: dict | None = None  # ConeSplitting policy fields (detector-cone splitting of elastic events), or None
+    elastic_model: str = "sr"  # "sr" or "mott" (SRD 64 tables from mott_tables_dir / PYRITE_MOTT_TABLES_DIR)
+    mott_tables_dir: str | None = None
 
     def grid(self):
         return bin_centers(self.E_lo_eV, self.E_hi_eV, self.n_bins)
@@ -94,7 +96,12 @@ class CaseTables:
 
     def __init__(self, cfg: CaseConfig):
         self.cfg = cfg
-        self.layers = build_layer_tables([(0.0, cfg.thickness_ang, list(cfg.composition))])
+        self.layers = build_layer_tables([(0.0, cfg.thickness_ang, list(cfg.composition))], cfg.elastic_model,
+                                         cfg.mott_tables_dir)
+        if cfg.elastic_model == "mott":
+            from .tables.mott import check_mott_coverage
+
+            check_mott_coverage(self.layers.compositions, cfg.E_cut_keV, cfg.E0_keV, cfg.mott_tables_dir)
         self.lut = build_transport_lut(cfg.E_cut_keV, cfg.E0_keV, self.layers)
         th, tilt = np.deg2rad(cfg.theta_obs_deg), np.deg2rad(cfg.tilt_deg)
         self.beam_dir, self.n_hat = tilted_geometry(th, tilt)

@@ -3,10 +3,13 @@
 The flat entry mirrors the reference's ``_simulate_trajectories``
 (``src/pyrite/montecarlo/transport/api.py:19-75`` defaults: ``E_cut_keV=5``,
 ``max_steps=400000``, ``energy_model="midpoint"``, ``max_dE_frac=0``,
-``straggling=False``) for the table-free physics the hip core implements:
-``elastic_model="sr"`` with the LUT core.  Other elastic or inelastic models
-raise ``NotImplementedError`` (fail closed, as the reference does for
-unsupported combinations).
+``straggling=False``) for the physics the hip LUT core implements:
+``elastic_model="sr"`` and ``elastic_model="mott"`` (the LUT's Browning rate
+and SRD 64-calibrated screening, ``pyrite_amd.tables.mott``; the tables come
+from ``PYRITE_MOTT_TABLES_DIR`` or ``mott_tables_dir``, and an energy range
+outside any element's table is refused before transport,
+``host_pass.py:595-611``).  ``"elsepa"`` raises ``NotImplementedError``
+(fail closed, as the reference does for unsupported combinations).
 
 The returned mapping uses the reference's segment keys (``host_pass.py:1048-1144``):
 ``r_mid``, ``v_hat``, ``L_ang``, ``E_keV``, ``E_end_keV``, ``E_repr_keV``,
@@ -131,6 +134,7 @@ def simulate_trajectories(
     inelastic_cutoff_eV=None,
     radiative_model="uncoupled",
     radiative_cutoff_eV=None,
+    mott_tables_dir=None,
 ):
     """Transport ``Ne`` electrons of ``E0_keV`` into a slab (hip core).
 
@@ -154,17 +158,16 @@ def simulate_trajectories(
     """
     import torch
 
-    if elastic_model != "sr":
+    if elastic_model not in ("sr", "mott"):
         raise NotImplementedError(
-            f"elastic_model={elastic_model!r} needs fetched ELSEPA/Mott tables; the hip core implements 'sr'"
+            f"elastic_model={elastic_model!r}: the hip LUT core implements 'sr' and 'mott' (ELSEPA needs its "
+            "tabulated angular sampler)"
         )
     if energy_model not in ("midpoint", "frozen"):
         raise ValueError("energy_model must be 'midpoint' or 'frozen'")
-    backend = require_device()
-    dev = backend.torch_device
     Ne = int(Ne)
     stack = _layers(thickness_ang, element, n_atoms_per_ang3, composition, layers)
-    tables = build_layer_tables(stack)
+    tables = build_layer_tables(stack, elastic_model, mott_tables_dir)
     finite = crystal_width_mm is not None and crystal_height_mm is not None
 
     if initial_state is None:
@@ -177,7 +180,14 @@ def simulate_trajectories(
     E_cut_h = np.full(Ne, float(E_cut_keV))
     if np.any(E_cut_h >= E_h):
         raise ValueError("each electron cutoff energy must be below its initial energy")
+    if elastic_model == "mott":
+        from ..tables.mott import check_mott_coverage
+
+        check_mott_coverage(tables.compositions, float(E_cut_h.min()), float(E_h.max()), mott_tables_dir)
     lut = build_transport_lut(float(E_cut_h.min()), float(E_h.max()), tables)
+    # host-side refusals above run before any device work
+    backend = require_device()
+    dev = backend.torch_device
     hard = _hard_modes(tables, lut, dev, inelastic_model, inelastic_cutoff_eV, radiative_model,
                        radiative_cutoff_eV, float(E_cut_h.min()), energy_model)
     if hard is not None:
@@ -231,6 +241,8 @@ def simulate_trajectories(
         "crystal_width_ang": control.width_ang if finite else None,
         "crystal_height_ang": control.height_ang if finite else None,
         "n_layers": tables.n_layers,
+        "elastic_model": elastic_model,
+        "initial_E_keV": E_h.copy(),
         "final_E_keV": E,
         "final_pos": pos,
         "final_dirs": dirs,

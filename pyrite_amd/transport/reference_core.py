@@ -22,8 +22,11 @@ CuPy arrays on its GPU path); it maps them onto one ``hr_transport_args``
 and launches on torch's current stream.  The core mutates ``pos``, ``dirs``,
 ``E_keV`` and ``clock`` in place, writes row ``s`` of local electron ``i`` to
 slot ``i*cap + s`` while ``s < cap`` and reports the true row count, as the
-reference's does.  Unsupported modes fail closed: a non-screened-Rutherford
-``elastic_model_code`` or a nonzero ``sbethe_on`` raises
+reference's does.  ``elastic_model_code`` 0 (screened Rutherford) and 1
+(Mott) both run: the reference's LUT kernel takes the code but never branches
+on it (``_jit_lut_kernel.py:74``), because the LUT columns carry the model
+(``tables.build_layer_tables(..., elastic_model=...)``).  Unsupported modes
+fail closed: the ELSEPA code 2 or a nonzero ``sbethe_on`` raises
 ``NotImplementedError``; the shell and radiative modes of this backend take a
 ``hard_events.DeviceHardModes`` through ``hard=`` (their tables live on the
 transport grid, not in the reference's ``inelastic_args`` layout, which is
@@ -63,9 +66,10 @@ def build_args(run, control, geometry, lut_args, materials, state, segments, pe_
     """``hr_transport_args`` of one positional call (no launch)."""
     e_start, e_count, cap, stream_key = run
     max_steps, _max_segments, elastic_code, energy_code, max_dE_frac = control
-    if int(elastic_code) != 0:
-        raise NotImplementedError(f"elastic_model_code={int(elastic_code)}: the hip LUT core implements "
-                                  "screened Rutherford (0) only")
+    if int(elastic_code) not in (0, 1):
+        raise NotImplementedError(f"elastic_model_code={int(elastic_code)}: the hip LUT core runs the LUT-carried "
+                                  "models, screened Rutherford (0) and Mott (1); ELSEPA (2) needs its tabulated "
+                                  "angular sampler")
     if int(energy_code) not in (0, 1):
         raise ValueError("energy_model_code must be 0 (frozen) or 1 (midpoint)")
     n_layers, internal_bounds, z_total, finite, width, height, L_nel, L_top, L_bot = geometry

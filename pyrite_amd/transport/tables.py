@@ -8,7 +8,12 @@ Restates, in NumPy, the reference's host table build:
 * the Joy-Luo / Berger-Seltzer crossover bisection --
   ``src/pyrite/montecarlo/transport/stopping.py:312-344``;
 * the uniform-in-ln(E) transport LUT -- ``src/pyrite/montecarlo/transport/lut.py:28-437``
-  (1024 intervals/decade, 256..16384 nodes, pinned endpoints).
+  (1024 intervals/decade, 256..16384 nodes, pinned endpoints);
+* the elastic columns of the two table models the LUT kernel serves
+  (``lut.py:182-203``): ``"sr"`` (screened-Rutherford rate, Joy/Bishop
+  screening) and ``"mott"`` (Browning rate ``3e-18 Z^1.7 n_cm3 / (E + 0.005
+  Z^1.7 sqrt E + 7e-4 Z^2 / sqrt E)``, screening calibrated on the SRD 64
+  transport tables, ``pyrite_amd.tables.mott``).
 
 The arrays produced here are the kernel inputs.  They are float64, C-contiguous
 and laid out exactly as the device kernels index them.
@@ -114,14 +119,32 @@ class LayerTables:
     L_sr_rate_numer: np.ndarray
     L_sr_joy_numer: np.ndarray
     compositions: list
+    elastic_model: str = "sr"
+    # Browning rate coefficients (layer_tables.py:104-110): 3e-18 Z^1.7 n_cm3, 0.005 Z^1.7, 7e-4 Z^2
+    L_mott_numer: np.ndarray | None = None
+    L_mott_denom1: np.ndarray | None = None
+    L_mott_denom2: np.ndarray | None = None
+    # per layer, per element: (log10 E_eV, log10 alpha) calibration nodes under "mott"
+    mott_tables: list | None = None
 
 
-def build_layer_tables(layers) -> LayerTables:
-    """``layers = [(z_top, z_bot, [(element, n_per_A3), ...]), ...]`` top first."""
+ELASTIC_MODELS = ("sr", "mott")
+
+
+def build_layer_tables(layers, elastic_model: str = "sr", mott_tables_dir=None) -> LayerTables:
+    """``layers = [(z_top, z_bot, [(element, n_per_A3), ...]), ...]`` top first.
+
+    ``elastic_model="mott"`` reads every element's SRD 64 table
+    (``mott_tables_dir``, else ``PYRITE_MOTT_TABLES_DIR``) here, before any
+    transport, and raises ``MottTableUnavailableError`` for a missing one."""
+    if elastic_model not in ELASTIC_MODELS:
+        raise ValueError(f"elastic_model must be one of {ELASTIC_MODELS}, got {elastic_model!r}")
     n_layers = len(layers)
     max_el = max(len(c) for _, _, c in layers)
     shape = (n_layers, max_el)
-    out = {name: np.zeros(shape) for name in ("Z", "J", "k", "coeff", "E_cross", "ncm3", "srn", "sjn")}
+    out = {name: np.zeros(shape) for name in ("Z", "J", "k", "coeff", "E_cross", "ncm3", "srn", "sjn", "mn", "md1",
+                                               "md2")}
+    mott = [[None] * len(c) for _, _, c in layers]
     nel = np.zeros(n_layers, dtype=np.int32)
     for L, (_, _, comp) in enumerate(layers):
         nel[L] = len(comp)
@@ -137,6 +160,14 @@ def build_layer_tables(layers) -> LayerTables:
             out["ncm3"][L, i] = ncm3
             out["srn"][L, i] = 5.21e-21 * Z * Z * np.float64(4.0) * np.float64(np.pi) * ncm3
             out["sjn"][L, i] = np.float64(3.4e-3) * Z ** np.float64(0.67)
+            z17 = Z ** np.float64(1.7)
+            out["mn"][L, i] = np.float64(3.0e-18) * z17 * ncm3
+            out["md1"][L, i] = np.float64(0.005) * z17
+            out["md2"][L, i] = np.float64(0.0007) * Z * Z
+            if elastic_model == "mott":
+                from ..tables.mott import mott_alpha_table
+
+                mott[L][i] = mott_alpha_table(el, Z, mott_tables_dir)
     tops = np.array([float(l[0]) for l in layers])
     bots = np.array([float(l[1]) for l in layers])
     if n_layers > 1:
@@ -160,6 +191,11 @@ def build_layer_tables(layers) -> LayerTables:
         L_sr_rate_numer=out["srn"],
         L_sr_joy_numer=out["sjn"],
         compositions=[list(c) for _, _, c in layers],
+        elastic_model=elastic_model,
+        L_mott_numer=out["mn"],
+        L_mott_denom1=out["md1"],
+        L_mott_denom2=out["md2"],
+        mott_tables=mott if elastic_model == "mott" else None,
     )
 
 
@@ -185,6 +221,12 @@ def _sr_rates(srn, sjn, E):
     return srn / (E * E) / (a * (1.0 + a)) * (rel * rel)
 
 
+def _browning_rates(numer, d1, d2, E):
+    """Browning rate [1/cm] (scattering.py:88-96, lut.py:184-189)."""
+    root = np.sqrt(E)
+    return numer / (E + d1 * root + d2 / root)
+
+
 def _evaluate_nodes(tables: LayerTables, E_grid):
     n_layers, max_el = tables.n_layers, tables.max_el
     n = E_grid.size
@@ -200,8 +242,15 @@ def _evaluate_nodes(tables: LayerTables, E_grid):
         dEds[L] = spliced_dEds(tables.L_J[L, :m], tables.L_k[L, :m], tables.L_coeff[L, :m], tables.L_E_cross[L, :m], E_grid)
         rates = np.empty((m, n))
         for i in range(m):
-            rates[i] = _sr_rates(tables.L_sr_rate_numer[L, i], tables.L_sr_joy_numer[L, i], E_grid)
-            alpha[L, i] = tables.L_sr_joy_numer[L, i] / E_grid
+            if tables.elastic_model == "mott":
+                from ..tables.mott import mott_alpha
+
+                rates[i] = _browning_rates(tables.L_mott_numer[L, i], tables.L_mott_denom1[L, i],
+                                           tables.L_mott_denom2[L, i], E_grid)
+                alpha[L, i] = mott_alpha(*tables.mott_tables[L][i], E_grid)
+            else:
+                rates[i] = _sr_rates(tables.L_sr_rate_numer[L, i], tables.L_sr_joy_numer[L, i], E_grid)
+                alpha[L, i] = tables.L_sr_joy_numer[L, i] / E_grid
         layer_total = rates.sum(axis=0)
         if np.any(~np.isfinite(layer_total)) or np.any(layer_total <= 0.0):
             raise ValueError("transport LUT produced a non-positive elastic rate")
@@ -240,7 +289,10 @@ def build_transport_lut(E_min_keV, E_max_keV, tables: LayerTables, intervals_per
         for w in (0.25, 0.5, 0.75):
             Es = np.exp((1.0 - w) * logE[:-1] + w * logE[1:])
             st, sd, sb, _, sa = _evaluate_nodes(tables, Es)
-            for node, samp in ((total, st), (dEds, sd), (inv_beta[None, :], sb[None, :])):
+            # the screening of each layer's live elements too (lut.py:251-270); the CDF is a
+            # probability and is not held to a relative bound
+            screens = [(alpha[L, :m], sa[L, :m]) for L, m in enumerate(tables.L_nel.tolist())]
+            for node, samp in ((total, st), (dEds, sd), (inv_beta[None, :], sb[None, :]), *screens):
                 pred = (1.0 - w) * node[..., :-1] + w * node[..., 1:]
                 e = np.abs(pred - samp) / np.abs(samp)
                 err = max(err, float(np.nanmax(e)))

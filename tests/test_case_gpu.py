@@ -118,3 +118,17 @@ def test_streamed_hopg_case_agrees_with_the_oracle_per_bin(device):
     assert g_lines.sum() > 0 and c_lines.sum() > 0
     _agree(g_brem, c_brem, "streamed brem")
     _agree(g_lines, c_lines, "streamed lines")
+
+
+def test_hopg_mott_case_agrees_with_the_oracle_per_bin(device):
+    """The HOPG case on the Mott LUT (the reference's synthetic SRD 64 tables:
+    Browning rates, calibrated screening), device against oracle per bin."""
+    from pyrite_amd.pipeline import CaseConfig, CasePipeline
+    from tests.transport_cases import MOTT_FIXTURE_DIR
+
+    pipe = CasePipeline(CaseConfig(Ne=PER_BATCH, elastic_model="mott", mott_tables_dir=str(MOTT_FIXTURE_DIR)))
+    assert pipe.tables.layers.elastic_model == "mott"
+    g_lines, g_brem = _gpu_batches(pipe)
+    c_lines, c_brem = _cpu_batches(pipe, first=20_000_000)
+    _agree(g_brem, c_brem, "brem")
+    _agree(g_lines, c_lines, "lines")

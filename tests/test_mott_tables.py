@@ -1,0 +1,179 @@
+"""SRD 64 Mott tables and the Mott LUT columns (CPU).
+
+Pins:
+* the reader on the reference's synthetic SRD 64-format fixture
+  (tests/data/mott_srd64_synthetic, copied to tests/golden): row count, units
+  (a0^2 -> cm^2), node energies;
+* the screened-Rutherford first moment against quadrature of the law
+  ``p(mu) ~ 1/(1 - mu + 2 alpha)^2`` and its isotropic limit;
+* the inversion against the forward moment (round trips both ways);
+* Browning's total cross section against the fixture, which the reference
+  built as ``sigma_tr = sigma_Browning * <1-cos>(alpha_Joy)``: so the
+  fixture's sigma_tr / <1-cos>(alpha_Joy) is Browning's value, and a dropped
+  term in either fails;
+* the calibration recovering the Joy/Bishop screening from that fixture at
+  every node and, through log-log interpolation (exact there), on every LUT
+  node, as the reference's fixture is built to do (tests/helpers/mott_synthetic.py);
+* coverage refusals after the reference's test_mott_coverage.py (endpoints
+  inclusive, one ulp outside refused with the element and range named, every
+  element of every layer checked) and the missing-table error.
+"""
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+from pyrite_amd.tables import mott
+from pyrite_amd.transport.tables import build_layer_tables, build_transport_lut
+from tests.transport_cases import MOTT_FIXTURE_DIR
+
+Z = {"C": 6.0, "Si": 14.0, "Ge": 32.0, "W": 74.0}
+
+
+def _joy(z, E_keV):
+    return 3.4e-3 * z**0.67 / E_keV
+
+
+def test_reader_on_the_synthetic_fixture():
+    E, s = mott.read_transport_table(MOTT_FIXTURE_DIR / "DisplayCalcTCSTableForSi.csv")
+    assert E.size == 29
+    assert E[0] == 10.0 and E[-1] == 1.0e8
+    np.testing.assert_allclose(np.diff(np.log10(E)), 0.25, rtol=1e-12)
+    # first data row "1, 10, 5.77025458916195078e+00" in a0^2
+    assert s[0] == pytest.approx(5.77025458916195078 * 2.8002852e-17, rel=1e-15)
+
+
+def test_reader_refuses_a_file_without_rows(tmp_path):
+    p = tmp_path / "DisplayCalcTCSTableForSi.csv"
+    p.write_text("SYNTHETIC\nNo, Energy,Transport cross section\n")
+    with pytest.raises(ValueError, match="no SRD 64"):
+        mott.read_transport_table(p)
+
+
+@pytest.mark.parametrize("alpha", [1e-6, 1e-3, 0.02, 0.5, 3.0, 50.0])
+def test_first_moment_against_quadrature_of_the_law(alpha):
+    w = lambda mu: 1.0 / (1.0 - mu + 2.0 * alpha) ** 2  # noqa: E731
+    pts = [1.0 - 2.0 * alpha * k for k in (1, 10, 100) if 1.0 - 2.0 * alpha * k > -1.0]
+    norm = integrate.quad(w, -1.0, 1.0, points=pts, limit=500, epsabs=0, epsrel=1e-13)[0]
+    mom = integrate.quad(lambda mu: (1.0 - mu) * w(mu), -1.0, 1.0, points=pts, limit=500, epsabs=0,
+                         epsrel=1e-13)[0]
+    assert mott.first_moment(alpha) == pytest.approx(mom / norm, rel=1e-9)
+
+
+def test_first_moment_limits():
+    # isotropic as alpha -> inf: <1 - cos> -> 1 - 1/(3 alpha)
+    a = 1e4
+    assert mott.first_moment(a) == pytest.approx(1.0 - 1.0 / (3.0 * a), rel=1e-8)
+    # small alpha: 2 a (ln(1/a) - 1) to leading order
+    a = 1e-9
+    assert mott.first_moment(a) == pytest.approx(2.0 * a * (np.log(1.0 / a) - 1.0), rel=1e-7)
+
+
+def test_inversion_round_trips():
+    t = np.logspace(-10, np.log10(0.99), 400)
+    a = mott.alpha_from_first_moment(t)
+    np.testing.assert_allclose(mott.first_moment(a), t, rtol=1e-12)
+    a0 = np.logspace(-8, 2.5, 300)  # moments below the 0.999 clip (alpha < ~333)
+    np.testing.assert_allclose(mott.alpha_from_first_moment(mott.first_moment(a0)), a0, rtol=1e-10)
+    # targets outside the law's range are clipped, as the reference's calibration does
+    assert mott.alpha_from_first_moment(np.array([0.0]))[0] == pytest.approx(
+        mott.alpha_from_first_moment(np.array([1e-12]))[0])
+
+
+@pytest.mark.parametrize("el", sorted(Z))
+def test_browning_against_the_fixture(el):
+    E_eV, sig_tr = mott.read_transport_table(MOTT_FIXTURE_DIR / f"DisplayCalcTCSTableFor{el}.csv")
+    E = E_eV * 1e-3
+    np.testing.assert_allclose(mott.browning_sigma_cm2(Z[el], E), sig_tr / mott.first_moment(_joy(Z[el], E)),
+                               rtol=1e-12)
+
+
+@pytest.mark.parametrize("el", sorted(Z))
+def test_calibration_recovers_joy_screening(el):
+    logE, logA = mott.mott_alpha_table(el, Z[el], MOTT_FIXTURE_DIR)
+    np.testing.assert_allclose(logA, np.log10(_joy(Z[el], 10.0**logE * 1e-3)), rtol=0, atol=1e-12)
+
+
+def test_mott_lut_columns():
+    layers = [(0.0, 2.0e3, [("W", 0.0632)]), (2.0e3, 3.0e4, [("Si", 0.025), ("C", 0.05)])]
+    sr = build_layer_tables(layers)
+    mt = build_layer_tables(layers, "mott", MOTT_FIXTURE_DIR)
+    lut_sr = build_transport_lut(5.0, 40.0, sr)
+    lut = build_transport_lut(5.0, 40.0, mt, audit=True)
+    E = lut.E_grid
+    # screening: the fixture's calibration is Joy/Bishop, and log-log interpolation is exact for it
+    np.testing.assert_allclose(lut.alpha, lut_sr.alpha, rtol=1e-12)
+    # rates: Browning per element times its number density
+    comps = [[("W", 0.0632)], [("Si", 0.025), ("C", 0.05)]]
+    for L, comp in enumerate(comps):
+        parts = np.array([n * 1e24 * mott.browning_sigma_cm2(Z[el], E) for el, n in comp])
+        np.testing.assert_allclose(lut.total_rate[L], parts.sum(axis=0), rtol=1e-12)
+        np.testing.assert_allclose(lut.cdf[L, :len(comp)], np.cumsum(parts, axis=0) / parts.sum(axis=0), rtol=1e-12)
+    # the two models differ in the rate (Browning vs screened Rutherford), not by a constant
+    ratio = lut.total_rate / lut_sr.total_rate
+    assert ratio.max() / ratio.min() > 1.05
+    assert lut.max_rel_interp_error < 1e-4
+    # the device inputs are the same arrays with the same layout
+    assert lut.alpha.shape == lut_sr.alpha.shape and lut.cdf.shape == lut_sr.cdf.shape
+
+
+def _write_tables(root, bounds_eV, elements=(("Si", 14.0), ("W", 74.0))):
+    """Synthetic SRD 64 exports with the real export's 50 eV - 300 keV bounds,
+    built as the reference's coverage test builds them."""
+    E = np.asarray(bounds_eV, dtype=float)
+    for el, z in elements:
+        sig = mott.browning_sigma_cm2(z, E * 1e-3) * mott.first_moment(_joy(z, E * 1e-3)) / mott.A0_SQ_CM2
+        rows = "\n".join(f"{i}, {e:.17g}, {s:.17e}" for i, (e, s) in enumerate(zip(E, sig), 1))
+        (root / f"DisplayCalcTCSTableFor{el}.csv").write_text("SYNTHETIC TEST FIXTURE\n" + rows + "\n")
+
+
+def test_coverage_endpoints_are_inclusive(tmp_path):
+    _write_tables(tmp_path, [50.0, 1e3, 1e4, 3e5])
+    mott.check_mott_coverage([[("Si", 0.05)]], 0.05, 300.0, tmp_path)
+
+
+@pytest.mark.parametrize("E_min,E_max", [(0.05, float(np.nextafter(300.0, np.inf))),
+                                         (float(np.nextafter(0.05, 0.0)), 300.0)])
+def test_coverage_refuses_one_ulp_outside(tmp_path, E_min, E_max):
+    _write_tables(tmp_path, [50.0, 1e3, 1e4, 3e5])
+    with pytest.raises(ValueError, match=r'Si.*\[0.05, 300\] keV.*elastic_model="elsepa"'):
+        mott.check_mott_coverage([[("Si", 0.05)]], E_min, E_max, tmp_path)
+
+
+def test_coverage_checks_every_layer_element(tmp_path):
+    _write_tables(tmp_path, [50.0, 1e3, 1e4, 3e5])
+    _write_tables(tmp_path, [50.0, 1e3, 1e4, 2e5], elements=(("W", 74.0),))
+    with pytest.raises(ValueError, match=r"W \[0.05, 200\] keV"):
+        mott.check_mott_coverage([[("Si", 0.05)], [("Si", 0.02), ("W", 0.06)]], 1.0, 250.0, tmp_path)
+
+
+def test_missing_table_is_an_error(tmp_path, monkeypatch):
+    monkeypatch.delenv(mott.MOTT_TABLES_ENV, raising=False)
+    with pytest.raises(mott.MottTableUnavailableError, match=mott.MOTT_TABLES_ENV):
+        build_layer_tables([(0.0, 1e4, [("Si", 0.05)])], "mott")
+    with pytest.raises(mott.MottTableUnavailableError, match="DisplayCalcTCSTableForSi.csv does not exist"):
+        build_layer_tables([(0.0, 1e4, [("Si", 0.05)])], "mott", tmp_path)
+    monkeypatch.setenv(mott.MOTT_TABLES_ENV, str(MOTT_FIXTURE_DIR))
+    assert build_layer_tables([(0.0, 1e4, [("Si", 0.05)])], "mott").elastic_model == "mott"
+    with pytest.raises(ValueError, match="elastic_model must be one of"):
+        build_layer_tables([(0.0, 1e4, [("Si", 0.05)])], "elsepa")
+
+
+def test_simulate_refuses_elsepa_before_any_device_work():
+    from pyrite_amd.transport.api import simulate_trajectories
+
+    with pytest.raises(NotImplementedError, match="'sr' and 'mott'"):
+        simulate_trajectories(30.0, 4, 1e4, "Si", 0.05, elastic_model="elsepa")
+
+
+def test_simulate_refuses_energies_outside_the_tables_before_any_device_work(tmp_path):
+    """test_mott_coverage.py: the refusal is host side (no device needed)."""
+    from pyrite_amd.transport.api import simulate_trajectories
+
+    _write_tables(tmp_path, [50.0, 1e3, 1e4, 3e5])
+    with pytest.raises(ValueError, match=r'Si.*\[0.05, 300\] keV.*elastic_model="elsepa"'):
+        simulate_trajectories(float(np.nextafter(300.0, np.inf)), 16, 1.0, "Si", 0.05, E_cut_keV=0.05,
+                              elastic_model="mott", mott_tables_dir=tmp_path, max_steps=8)
+    with pytest.raises(ValueError, match=r'Si.*\[0.05, 300\] keV'):
+        simulate_trajectories(300.0, 16, 1.0, "Si", 0.05, E_cut_keV=float(np.nextafter(0.05, 0.0)),
+                              elastic_model="mott", mott_tables_dir=tmp_path, max_steps=8)

@@ -49,8 +49,8 @@ def test_positional_call_fails_closed():
     tabs, ctl, run, state, rows, out, stragg = _setup()
     run_, control, geometry, lut_args, materials, st, segments, pe_out, straggling = reference_arguments(
         tabs, ctl, run, state, rows, out, stragg)
-    with pytest.raises(NotImplementedError, match="Rutherford"):
-        build_args(run_, (777, 0, 1, 1, 0.0), geometry, lut_args, materials, st, segments, pe_out, straggling)
+    with pytest.raises(NotImplementedError, match="ELSEPA"):
+        build_args(run_, (777, 0, 2, 1, 0.0), geometry, lut_args, materials, st, segments, pe_out, straggling)
     with pytest.raises(NotImplementedError, match="SBETHE"):
         build_args(run_, control, geometry, lut_args, materials[:5] + (1,), st, segments, pe_out, straggling)
     with pytest.raises(TypeError, match="seg_len must be float64"):
@@ -63,3 +63,22 @@ def test_positional_call_fails_closed():
         make_reference_core()(run_, control, geometry, lut_args, materials, st, segments, pe_out, straggling,
                               (1, 2, 3))
     np.testing.assert_equal(len(segments), 12)
+
+
+def test_mott_code_runs_the_lut_it_is_given():
+    """The reference's LUT kernel takes ``elastic_model_code`` without branching
+    on it (the LUT carries the model), so code 1 maps onto the same native
+    arguments as code 0."""
+    from pyrite_amd import _lib
+    from pyrite_amd.transport.reference_core import build_args, reference_arguments
+
+    tabs, ctl, run, state, rows, out, stragg = _setup()
+    run_, control, geometry, lut_args, materials, st, segments, pe_out, straggling = reference_arguments(
+        tabs, ctl, run, state, rows, out, stragg)
+    a = build_args(run_, control, geometry, lut_args, materials, st, segments, pe_out, straggling)
+    mott = (control[0], control[1], 1, control[3], control[4])
+    b = build_args(run_, mott, geometry, lut_args, materials, st, segments, pe_out, straggling)
+    for name, _ in _lib.TransportArgs._fields_:
+        if name in ("work", "hard", "cone"):
+            continue
+        assert getattr(a, name) == getattr(b, name), name

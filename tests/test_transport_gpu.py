@@ -196,6 +196,25 @@ def test_simulate_trajectories_api(device):
     np.testing.assert_allclose(out["E_repr_keV"], 0.5 * (out["E_keV"] + out["E_end_keV"]))
 
 
+def test_simulate_trajectories_mott_endpoints_are_inclusive(device, tmp_path):
+    """test_mott_coverage.py:45-53 on the device: a 300 keV beam over tables
+    ending at 300 keV, cutoff at their 50 eV start, runs; the result reports the
+    model and the initial energies."""
+    from pyrite_amd.tables import mott
+    from pyrite_amd.transport.api import simulate_trajectories
+
+    E = np.array([50.0, 1e3, 1e4, 3e5])
+    a = 3.4e-3 * 14.0**0.67 / (E * 1e-3)
+    sig = mott.browning_sigma_cm2(14.0, E * 1e-3) * mott.first_moment(a) / mott.A0_SQ_CM2
+    (tmp_path / "DisplayCalcTCSTableForSi.csv").write_text(
+        "SYNTHETIC\n" + "\n".join(f"{i}, {e:.17g}, {s:.17e}" for i, (e, s) in enumerate(zip(E, sig), 1)) + "\n")
+    out = simulate_trajectories(300.0, 16, 1.0, "Si", 0.05, E_cut_keV=0.05, seed=41, max_steps=8,
+                                elastic_model="mott", mott_tables_dir=tmp_path, raise_on_step_limit=False,
+                                keep_segments_on_device=False)
+    np.testing.assert_array_equal(out["initial_E_keV"], np.full(16, 300.0))
+    assert out["elastic_model"] == "mott" and out["Ne"] == 16
+
+
 @pytest.mark.parametrize("case", [c for c in CASES if c.energy_model == 1], ids=lambda c: c.name)
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_device_rows_keep_the_event_contract(device, case, precision):

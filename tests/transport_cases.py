@@ -1,10 +1,14 @@
 """Shared transport cases for the oracle and GPU parity tests."""
 
 from dataclasses import dataclass
+from pathlib import Path
 
 import numpy as np
 
 from pyrite_amd.transport.tables import build_layer_tables, build_transport_lut
+
+# the reference's synthetic SRD 64-format Mott tables (tests/data/mott_srd64_synthetic)
+MOTT_FIXTURE_DIR = Path(__file__).resolve().parent / "golden" / "mott_srd64_synthetic"
 
 
 @dataclass
@@ -20,9 +24,10 @@ class Case:
     max_dE_frac: float = 0.0
     beam_dir: tuple = (0.0, 0.0, 1.0)
     finite: tuple | None = None
+    elastic: str = "sr"
 
     def tables(self):
-        lt = build_layer_tables(self.layers)
+        lt = build_layer_tables(self.layers, self.elastic, MOTT_FIXTURE_DIR)
         lut = build_transport_lut(self.E_cut, self.E0, lt)
         return lt, lut
 
@@ -51,4 +56,9 @@ FRAMED = Case(
     "prism", 60.0, 300, [(0.0, 4.0e4, [("Ge", 0.0442)])], seed=3,
     beam_dir=(0.3, 0.1, 0.9), finite=(2.0e4, 1.5e4), energy_model=0,
 )
-CASES = [SI_30KEV, SI_30KEV_STRAGGLE, HOPG_5MEV, MULTILAYER, FRAMED]
+MULTILAYER_MOTT = Case(
+    "stack_mott", 40.0, 300,
+    [(0.0, 2.0e3, [("W", 0.0632)]), (2.0e3, 3.0e4, [("Si", 0.025), ("C", 0.05)])],
+    seed=6, straggle=True, max_dE_frac=0.05, elastic="mott",
+)
+CASES = [SI_30KEV, SI_30KEV_STRAGGLE, HOPG_5MEV, MULTILAYER, FRAMED, MULTILAYER_MOTT]
