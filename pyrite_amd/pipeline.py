@@ -70,6 +70,7 @@ class CaseConfig:
     cone_split: dict | None = None  # ConeSplitting policy fields (detector-cone splitting of elastic events), or None
     elastic_model: str = "sr"  # "sr" or "mott" (SRD 64 tables from mott_tables_dir / PYRITE_MOTT_TABLES_DIR)
     mott_tables_dir: str | None = None
+    atomic_electron_deflection: str | None = None  # "kawrakow": elastic rate x (1 + 1/Z), or None
 
     def grid(self):
         return bin_centers(self.E_lo_eV, self.E_hi_eV, self.n_bins)
@@ -97,7 +98,7 @@ class CaseTables:
     def __init__(self, cfg: CaseConfig):
         self.cfg = cfg
         self.layers = build_layer_tables([(0.0, cfg.thickness_ang, list(cfg.composition))], cfg.elastic_model,
-                                         cfg.mott_tables_dir)
+                                         cfg.mott_tables_dir, cfg.atomic_electron_deflection)
         if cfg.elastic_model == "mott":
             from .tables.mott import check_mott_coverage
 

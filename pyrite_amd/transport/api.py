@@ -135,6 +135,7 @@ def simulate_trajectories(
     radiative_model="uncoupled",
     radiative_cutoff_eV=None,
     mott_tables_dir=None,
+    atomic_electron_deflection=None,
 ):
     """Transport ``Ne`` electrons of ``E0_keV`` into a slab (hip core).
 
@@ -167,7 +168,9 @@ def simulate_trajectories(
         raise ValueError("energy_model must be 'midpoint' or 'frozen'")
     Ne = int(Ne)
     stack = _layers(thickness_ang, element, n_atoms_per_ang3, composition, layers)
-    tables = build_layer_tables(stack, elastic_model, mott_tables_dir)
+    shell_cut = (1000.0 if inelastic_cutoff_eV is None else float(inelastic_cutoff_eV)) \
+        if inelastic_model == "shell-soft-hard" else None
+    tables = build_layer_tables(stack, elastic_model, mott_tables_dir, atomic_electron_deflection, shell_cut)
     finite = crystal_width_mm is not None and crystal_height_mm is not None
 
     if initial_state is None:
@@ -242,6 +245,7 @@ def simulate_trajectories(
         "crystal_height_ang": control.height_ang if finite else None,
         "n_layers": tables.n_layers,
         "elastic_model": elastic_model,
+        "atomic_electron_deflection": atomic_electron_deflection,
         "initial_E_keV": E_h.copy(),
         "final_E_keV": E,
         "final_pos": pos,

@@ -129,16 +129,36 @@ class LayerTables:
 
 
 ELASTIC_MODELS = ("sr", "mott")
+ATOMIC_ELECTRON_DEFLECTION_MODELS = ("kawrakow",)
 
 
-def build_layer_tables(layers, elastic_model: str = "sr", mott_tables_dir=None) -> LayerTables:
+def build_layer_tables(layers, elastic_model: str = "sr", mott_tables_dir=None, atomic_electron_deflection=None,
+                       hard_cutoff_eV=None) -> LayerTables:
     """``layers = [(z_top, z_bot, [(element, n_per_A3), ...]), ...]`` top first.
 
     ``elastic_model="mott"`` reads every element's SRD 64 table
     (``mott_tables_dir``, else ``PYRITE_MOTT_TABLES_DIR``) here, before any
-    transport, and raises ``MottTableUnavailableError`` for a missing one."""
+    transport, and raises ``MottTableUnavailableError`` for a missing one.
+
+    ``atomic_electron_deflection="kawrakow"`` folds the deflection by the Z
+    atomic electrons into the elastic rate, ``Z^2 -> Z (Z + xi)``
+    (``atomic_electrons.py:1-20``, Kawrakow 1997).  Under continuous stopping
+    no atomic-electron collision is explicit, so ``xi = 1`` at every energy and
+    both analytic rate numerators scale by ``1 + 1/Z``; the denominators and
+    the screening are untouched (``layer_tables.py:121-130``).  With explicit
+    hard collisions (``hard_cutoff_eV``, the shell mode) ``xi`` depends on
+    energy, which these coefficients cannot carry, so that combination is
+    refused, as the reference refuses it without tabulated elastic data.
+    ``None`` leaves every coefficient bit for bit."""
     if elastic_model not in ELASTIC_MODELS:
         raise ValueError(f"elastic_model must be one of {ELASTIC_MODELS}, got {elastic_model!r}")
+    if atomic_electron_deflection is not None and atomic_electron_deflection not in ATOMIC_ELECTRON_DEFLECTION_MODELS:
+        raise ValueError("atomic_electron_deflection must be None or one of "
+                         + ", ".join(repr(m) for m in ATOMIC_ELECTRON_DEFLECTION_MODELS))
+    if atomic_electron_deflection is not None and hard_cutoff_eV is not None:
+        raise ValueError("atomic_electron_deflection with inelastic_model='shell-soft-hard' needs "
+                         "elastic_model='elsepa' (energy-dependent xi), which this backend does not run; pass "
+                         "atomic_electron_deflection=None to opt out")
     n_layers = len(layers)
     max_el = max(len(c) for _, _, c in layers)
     shape = (n_layers, max_el)
@@ -164,6 +184,10 @@ def build_layer_tables(layers, elastic_model: str = "sr", mott_tables_dir=None) 
             out["mn"][L, i] = np.float64(3.0e-18) * z17 * ncm3
             out["md1"][L, i] = np.float64(0.005) * z17
             out["md2"][L, i] = np.float64(0.0007) * Z * Z
+            if atomic_electron_deflection is not None:
+                scale = 1.0 + 1.0 / Z  # 1 + xi/Z with xi = 1 (continuous stopping)
+                out["srn"][L, i] *= scale
+                out["mn"][L, i] *= scale
             if elastic_model == "mott":
                 from ..tables.mott import mott_alpha_table
 

@@ -196,6 +196,22 @@ def test_simulate_trajectories_api(device):
     np.testing.assert_allclose(out["E_repr_keV"], 0.5 * (out["E_keV"] + out["E_end_keV"]))
 
 
+def test_atomic_electron_deflection_shrinks_first_flights(device):
+    """Rate x 15/14 at an unchanged angular law and the same optical-depth
+    draws: every device first flight shrinks by exactly 14/15
+    (test_atomic_electron_deflection.py:235-245 of the reference, per history)."""
+    from pyrite_amd.transport.api import simulate_trajectories
+
+    first = {}
+    for model in (None, "kawrakow"):
+        out = simulate_trajectories(30.0, 3000, 2.0e5, composition=[("Si", 0.04996)], seed=11,
+                                    atomic_electron_deflection=model, keep_segments_on_device=False)
+        assert out["atomic_electron_deflection"] == model
+        starts = np.flatnonzero(np.r_[True, np.diff(out["electron_id"]) != 0])
+        first[model] = out["L_ang"][starts]
+    np.testing.assert_allclose(first["kawrakow"] / first[None], 14.0 / 15.0, rtol=1e-13)
+
+
 def test_simulate_trajectories_mott_endpoints_are_inclusive(device, tmp_path):
     """test_mott_coverage.py:45-53 on the device: a 300 keV beam over tables
     ending at 300 keV, cutoff at their 50 eV start, runs; the result reports the

@@ -25,9 +25,10 @@ class Case:
     beam_dir: tuple = (0.0, 0.0, 1.0)
     finite: tuple | None = None
     elastic: str = "sr"
+    atomic: str | None = None  # atomic_electron_deflection
 
     def tables(self):
-        lt = build_layer_tables(self.layers, self.elastic, MOTT_FIXTURE_DIR)
+        lt = build_layer_tables(self.layers, self.elastic, MOTT_FIXTURE_DIR, self.atomic)
         lut = build_transport_lut(self.E_cut, self.E0, lt)
         return lt, lut
 
@@ -61,4 +62,6 @@ MULTILAYER_MOTT = Case(
     [(0.0, 2.0e3, [("W", 0.0632)]), (2.0e3, 3.0e4, [("Si", 0.025), ("C", 0.05)])],
     seed=6, straggle=True, max_dE_frac=0.05, elastic="mott",
 )
-CASES = [SI_30KEV, SI_30KEV_STRAGGLE, HOPG_5MEV, MULTILAYER, FRAMED, MULTILAYER_MOTT]
+SI_30KEV_ATOMIC = Case("si30_aed", 30.0, 400, [(0.0, 5.0e4, [("Si", 0.04996)])], seed=11, straggle=True,
+                       atomic="kawrakow")
+CASES = [SI_30KEV, SI_30KEV_STRAGGLE, HOPG_5MEV, MULTILAYER, FRAMED, MULTILAYER_MOTT, SI_30KEV_ATOMIC]
