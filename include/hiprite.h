@@ -193,6 +193,29 @@ typedef struct hr_cone_split {
     int64_t *flight;         /* [e_count*cap] flight index of the event */
 } hr_cone_split;
 
+/* Tabulated (ELSEPA) elastic angles.  Replaces the exact core's
+ * elastic_model_code == 2 branch (montecarlo/transport/_jit_kernel.py:1130-
+ * 1160; device inversion _jit_device.py:493-620): at an elastic event the
+ * chosen element's two energy rows bracketing ln E[eV] are inverted at the
+ * angle uniform (guided bisection for the panel, then the exact root of the
+ * piecewise-linear density's quadratic CDF) and the quantiles mixed linearly
+ * in ln E; cos theta = 1 - 2 mu.  The rate and element choice stay on the LUT
+ * (its columns built from the tables' log-log rates).  on = 0: the analytic
+ * screened-Rutherford angle from lut_alpha. */
+typedef struct hr_elsepa {
+    int32_t on;
+    int32_t n_mu;          /* angular nodes, shared by every row */
+    int32_t guide_cells;   /* power of two */
+    int32_t pad_;
+    const int64_t *start;  /* [n_layers, max_el] first row of each element */
+    const int64_t *length; /* [n_layers, max_el] its energy rows (>= 2) */
+    const double *logE;    /* [rows] ln E[eV] */
+    const double *mu;      /* [n_mu] */
+    const double *cdf;     /* [rows, n_mu] */
+    const double *pdf;     /* [rows, n_mu] */
+    const int32_t *guide;  /* [rows, guide_cells + 1] */
+} hr_elsepa;
+
 typedef struct hr_transport_args {
     /* run */
     int64_t e_start, e_count, cap;
@@ -251,6 +274,8 @@ typedef struct hr_transport_args {
     hr_hard_events hard;
     /* detector-cone splitting (hr_cone_split; on = 0: analog) */
     hr_cone_split cone;
+    /* tabulated elastic angles (hr_elsepa; on = 0: screened Rutherford) */
+    hr_elsepa elsepa;
 } hr_transport_args;
 
 hr_status hr_transport_run(const hr_transport_args *args, void *stream);

@@ -124,6 +124,29 @@ int64_t or_cone_collect(const or_cone_split *c, int64_t e_start, int64_t e_count
                         int64_t *parent, int64_t *flight, double *pos, double *dir, double *E_keV, double *clock,
                         double *weight, uint64_t *stream_key, int64_t *max_count);
 
+/* Tabulated (ELSEPA) elastic angles (or_elsepa.c; scattering.py:540-620,
+ * _jit_device.py:493-620).  on = 0: the screened-Rutherford angle. */
+typedef struct or_elsepa {
+    int32_t on, n_mu, guide_cells;
+    const int64_t *start, *length; /* [n_layers, max_el] */
+    const double *logE;            /* [rows] ln E[eV] */
+    const double *mu;              /* [n_mu] */
+    const double *cdf, *pdf;       /* [rows, n_mu] */
+    const int32_t *guide;          /* [rows, guide_cells + 1] */
+} or_elsepa;
+
+/* Lower node (relative to row) and fraction of x on g[row .. row+n), clamped. */
+void or_elsepa_bracket(const double *g, int64_t row, int64_t n, double x, int64_t *lo, double *f);
+/* Inversion of one angular row at xi: guided panel search / whole-row bisection. */
+double or_elsepa_invert_row(const or_elsepa *t, int64_t row, double xi);
+double or_elsepa_invert_row_full(const or_elsepa *t, int64_t row, double xi);
+/* mu of element slot `el` (L*max_el + i) at E_keV and uniform xi. */
+double or_elsepa_sample_mu(const or_elsepa *t, int64_t el, double E_keV, double xi);
+void or_elsepa_sample_mu_batch(const or_elsepa *t, int64_t el, int64_t n, const double *E_keV, const double *xi,
+                               double *out);
+void or_elsepa_invert_batch(const or_elsepa *t, int64_t n, const int64_t *row, const double *xi, int guided,
+                            double *out);
+
 typedef struct or_transport_args {
     /* run */
     int64_t e_start, e_count, cap;
@@ -171,6 +194,8 @@ typedef struct or_transport_args {
     or_hard_modes hard;
     /* detector-cone splitting (zeroed: off) */
     or_cone_split cone;
+    /* tabulated elastic angles (zeroed: screened Rutherford) */
+    or_elsepa elsepa;
 } or_transport_args;
 
 /* Runs electrons [e_start, e_start+e_count) with `threads` OpenMP threads

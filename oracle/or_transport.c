@@ -694,7 +694,11 @@ static void transport_one(const or_transport_args *a, int64_t e) {
         draws += 1ULL;
         double phi_u = or_stream_uniform(key, draws);
         draws += 1ULL;
-        double cos_t = 1.0 - 2.0 * alpha * cos_u / (1.0 + alpha - cos_u); /* scattering.py:234 */
+        double cos_t;
+        if (a->elsepa.on) /* tabulated angles at the same uniform (_jit_kernel.py:1149-1160) */
+            cos_t = 1.0 - 2.0 * or_elsepa_sample_mu(&a->elsepa, (int64_t)(el_row + i_el), a->E_keV[e], cos_u);
+        else
+            cos_t = 1.0 - 2.0 * alpha * cos_u / (1.0 + alpha - cos_u); /* scattering.py:234 */
         /* cone split (cores.py:1189-1226): the in-cone copy from the incident
          * direction and chosen element, on the key of the flight just closed */
         int split_here = cone->on && a->E_keV[e] >= cone->min_energy_keV;

@@ -55,6 +55,13 @@ class ConeSplit(C.Structure):
     ]
 
 
+class Elsepa(C.Structure):
+    _fields_ = [
+        ("on", C.c_int32), ("n_mu", C.c_int32), ("guide_cells", C.c_int32),
+        ("start", _dp), ("length", _dp), ("logE", _dp), ("mu", _dp), ("cdf", _dp), ("pdf", _dp), ("guide", _dp),
+    ]
+
+
 class TransportArgs(C.Structure):
     _fields_ = [
         ("e_start", C.c_int64), ("e_count", C.c_int64), ("cap", C.c_int64),
@@ -77,6 +84,7 @@ class TransportArgs(C.Structure):
         ("seg_count", _dp), ("exit_code", _dp),
         ("hard", HardModes),
         ("cone", ConeSplit),
+        ("elsepa", Elsepa),
     ]
 
 
@@ -111,6 +119,10 @@ def lib():
         L.or_cone_event.restype = C.c_int
         L.or_cone_event.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(ConeSplit), C.c_double,
                                     C.c_double, C.c_uint64, _dp]
+        L.or_elsepa_sample_mu_batch.restype = None
+        L.or_elsepa_sample_mu_batch.argtypes = [C.POINTER(Elsepa), C.c_int64, C.c_int64, _dp, _dp, _dp]
+        L.or_elsepa_invert_batch.restype = None
+        L.or_elsepa_invert_batch.argtypes = [C.POINTER(Elsepa), C.c_int64, _dp, _dp, C.c_int, _dp]
         L.or_cone_ladder_weight.restype = C.c_double
         L.or_cone_ladder_weight.argtypes = [C.c_double]
         L.or_cone_collect.restype = C.c_int64
@@ -444,11 +456,52 @@ def run_transport(layers, lut, keys, pos, dirs, E_keV, E_cut, *, cap, e_start=0,
     if cone is not None:
         hold["cone"] = _cone_record(cone, keys, e_count)
         a.cone = hold["cone"]["struct"]
+    if getattr(layers, "elsepa", None) is not None:  # tabulated angles ride with the layer tables
+        a.elsepa, hold["elsepa"] = elsepa_struct(layers.elsepa)
     lib().or_transport_lut(C.byref(a), int(threads))
     state = {"pos": pos, "dirs": dirs, "E_keV": E_keV, "clock": clock, "stragg_dE": stragg}
     if cone is not None:
         state["cone"] = hold["cone"]
     return rows, seg_count, exit_code, state
+
+
+def elsepa_struct(group):
+    """``(Elsepa, hold)`` of a packed ``pyrite_amd.tables.elsepa.ElsepaGroup``
+    (``hold`` keeps the arrays alive)."""
+    hold = {
+        "start": np.ascontiguousarray(group.start, dtype=np.int64),
+        "length": np.ascontiguousarray(group.length, dtype=np.int64),
+        "logE": np.ascontiguousarray(group.logE, dtype=np.float64),
+        "mu": np.ascontiguousarray(group.mu, dtype=np.float64),
+        "cdf": np.ascontiguousarray(group.cdf, dtype=np.float64),
+        "pdf": np.ascontiguousarray(group.pdf, dtype=np.float64),
+        "guide": np.ascontiguousarray(group.guide, dtype=np.int32),
+    }
+    t = Elsepa()
+    t.on, t.n_mu, t.guide_cells = 1, int(hold["mu"].size), int(hold["guide"].shape[1] - 1)
+    for k in ("start", "length", "logE", "mu", "cdf", "pdf", "guide"):
+        setattr(t, k, _p(hold[k]))
+    return t, hold
+
+
+def elsepa_sample_mu(group, el, E_keV, xi):
+    """mu of element slot ``el`` (``L * max_el + i``) at energies ``E_keV`` and uniforms ``xi``."""
+    t, hold = elsepa_struct(group)
+    xi = np.ascontiguousarray(xi, dtype=np.float64)
+    E = np.ascontiguousarray(np.broadcast_to(np.asarray(E_keV, dtype=np.float64), xi.shape))
+    out = np.empty_like(xi)
+    lib().or_elsepa_sample_mu_batch(C.byref(t), int(el), xi.size, _p(E), _p(xi), _p(out))
+    return out
+
+
+def elsepa_invert(group, rows, xi, guided=True):
+    """Inversion of angular rows ``rows`` at ``xi``: guided, or by whole-row bisection."""
+    t, hold = elsepa_struct(group)
+    xi = np.ascontiguousarray(xi, dtype=np.float64)
+    r = np.ascontiguousarray(np.broadcast_to(np.asarray(rows, dtype=np.int64), xi.shape))
+    out = np.empty_like(xi)
+    lib().or_elsepa_invert_batch(C.byref(t), xi.size, _p(r), _p(xi), int(bool(guided)), _p(out))
+    return out
 
 
 def cone_split_keys(stream_keys):

@@ -107,6 +107,15 @@ class ConeSplitArgs(C.Structure):
     ]
 
 
+class ElsepaArgs(C.Structure):
+    """Mirror of ``hr_elsepa`` (tabulated elastic angles)."""
+
+    _fields_ = [
+        ("on", C.c_int32), ("n_mu", C.c_int32), ("guide_cells", C.c_int32), ("pad_", C.c_int32),
+        ("start", _dp), ("length", _dp), ("logE", _dp), ("mu", _dp), ("cdf", _dp), ("pdf", _dp), ("guide", _dp),
+    ]
+
+
 class ConeCollectArgs(C.Structure):
     """Mirror of ``hr_cone_collect_args``."""
 
@@ -186,6 +195,7 @@ class TransportArgs(C.Structure):
         ("work", _dp),
         ("hard", HardEvents),
         ("cone", ConeSplitArgs),
+        ("elsepa", ElsepaArgs),
     ]
 
 

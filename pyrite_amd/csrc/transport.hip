@@ -170,6 +170,7 @@ hr_status check_cone(const hr_transport_args &a, const char *who) {
     if (!c.on) return HR_OK;
     HR_REQUIRE(!a.hard.rad_on && !a.hard.shell_on, "%s: cone splitting is not combined with the hard-event modes",
                who);
+    HR_REQUIRE(!a.elsepa.on, "%s: cone splitting runs on the screened-Rutherford law, not tabulated angles", who);
     HR_REQUIRE(c.split_key && c.count && c.killed, "%s: cone splitting needs split keys, count and killed", who);
     HR_REQUIRE(c.cap >= 1 && c.pos && c.dir && c.E_keV && c.clock && c.weight && c.flight,
                "%s: cone splitting needs copy slots (cap >= 1)", who);
@@ -245,6 +246,8 @@ extern "C" hr_status hr_transport_run(const hr_transport_args *args, void *strea
     if (st != HR_OK) return st;
     st = check_cone(a, "hr_transport_run");
     if (st != HR_OK) return st;
+    st = elsepa::check(a, "hr_transport_run");
+    if (st != HR_OK) return st;
     const bool hard = a.hard.rad_on || a.hard.shell_on;
     const bool cone = a.cone.on != 0;
     int cu = 0, occupancy = 0;
@@ -311,6 +314,8 @@ extern "C" hr_status hr_transport_continue(const hr_transport_args *args, const 
     HR_REQUIRE(!a.hard.rad_on && !a.hard.shell_on,
                "hr_transport_continue: the hard-event modes run on hr_transport_run only (their optical depths "
                "and counters are not in the continuation record)");
+    const hr_status est = elsepa::check(a, "hr_transport_continue");
+    if (est != HR_OK) return est;
     const unsigned grid = (unsigned)((a.e_count + 255) / 256);
     if (a.precision == 1)
         hipLaunchKernelGGL(continue_kernel<true>, dim3(grid), dim3(256), 0, (hipStream_t)stream, a, c);

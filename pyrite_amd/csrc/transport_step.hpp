@@ -11,6 +11,7 @@
 #include <math.h>
 
 #include "cone.hpp"
+#include "elsepa.hpp"
 #include "hr_common.hpp"
 #include "radiative.hpp"
 #include "shell.hpp"
@@ -685,12 +686,16 @@ __device__ int step_lane(const hr_transport_args &a, Lane &s) {
             }
         }
     }
-    const LutPos q = lut.pos(s.E);
-    const double alpha = lut.alpha((el_row + i_el) * nE, q);
     const double cos_u = stream_uniform(s.key, s.draws);
     const double phi_u = stream_uniform(s.key, s.draws + 1ULL);
     s.draws += 2ULL;
-    const double cos_t = 1.0 - 2.0 * alpha * cos_u / (1.0 + alpha - cos_u);
+    double alpha = 0.0, cos_t;
+    if (a.elsepa.on) {  // tabulated angles at the same uniform (_jit_kernel.py:1149-1160)
+        cos_t = 1.0 - 2.0 * elsepa::sample_mu(a.elsepa, el_row + i_el, s.E, cos_u);
+    } else {
+        alpha = lut.alpha((el_row + i_el) * nE, lut.pos(s.E));
+        cos_t = 1.0 - 2.0 * alpha * cos_u / (1.0 + alpha - cos_u);
+    }
     bool split_here = false;
     if constexpr (kCone) {
         // the in-cone copy of this event, from the incident direction and the

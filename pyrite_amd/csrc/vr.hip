@@ -271,6 +271,8 @@ hr_status hr_vr_round(const hr_transport_args *a, int32_t rows, const int32_t *r
     HR_REQUIRE(a->e_count < (1LL << 40), "hr_vr_round: too many particles");
     HR_REQUIRE(!a->hard.rad_on && !a->hard.shell_on, "hr_vr_round: the hard-event modes run on hr_transport_run only");
     HR_REQUIRE(!a->cone.on, "hr_vr_round: cone splitting runs on hr_transport_run only");
+    const hr_status est = elsepa::check(*a, "hr_vr_round");
+    if (est != HR_OK) return est;
     HR_REQUIRE(a->precision == 0 || (a->precision == 1 && a->lut_total_rate_f && a->lut_dEds_f &&
                                       a->lut_inv_beta_f && a->lut_cdf_f && a->lut_alpha_f),
                "hr_vr_round: precision 0, or 1 with the FP32 table copies");

@@ -71,6 +71,7 @@ class CaseConfig:
     elastic_model: str = "sr"  # "sr" or "mott" (SRD 64 tables from mott_tables_dir / PYRITE_MOTT_TABLES_DIR)
     mott_tables_dir: str | None = None
     atomic_electron_deflection: str | None = None  # "kawrakow": elastic rate x (1 + 1/Z), or None
+    elastic_tables: list | None = None  # elastic_model="elsepa": one ELSEPA native-grid table per element
 
     def grid(self):
         return bin_centers(self.E_lo_eV, self.E_hi_eV, self.n_bins)
@@ -98,11 +99,16 @@ class CaseTables:
     def __init__(self, cfg: CaseConfig):
         self.cfg = cfg
         self.layers = build_layer_tables([(0.0, cfg.thickness_ang, list(cfg.composition))], cfg.elastic_model,
-                                         cfg.mott_tables_dir, cfg.atomic_electron_deflection)
+                                         cfg.mott_tables_dir, cfg.atomic_electron_deflection,
+                                         elastic_tables=cfg.elastic_tables)
         if cfg.elastic_model == "mott":
             from .tables.mott import check_mott_coverage
 
             check_mott_coverage(self.layers.compositions, cfg.E_cut_keV, cfg.E0_keV, cfg.mott_tables_dir)
+        if cfg.elastic_model == "elsepa":
+            from .tables.elsepa import check_coverage
+
+            check_coverage(self.layers.elsepa, cfg.E_cut_keV, cfg.E0_keV)
         self.lut = build_transport_lut(cfg.E_cut_keV, cfg.E0_keV, self.layers)
         th, tilt = np.deg2rad(cfg.theta_obs_deg), np.deg2rad(cfg.tilt_deg)
         self.beam_dir, self.n_hat = tilted_geometry(th, tilt)

@@ -8,8 +8,13 @@ DCS [a0^2/sr], Sherman function, relative error``.  Panels on one angular
 grid stack into the native-grid table the reference stores, with the angular
 CDF in ``mu`` by the trapezoid rule on that grid (``dOmega = 4 pi dmu``).
 
-This module reads the format; no transport kernel of this backend consumes
-tabulated elastic cross sections yet (the step runs screened Rutherford).
+The second half packs native-grid tables for the transport step's
+``elastic_model="elsepa"`` (``src/pyrite/montecarlo/transport/scattering.py:
+333-470``): per (layer, element) the energy nodes ``ln E[eV]``, the
+macroscopic rate ``ln(n sigma)`` (log-log in energy), and per energy row the
+piecewise-linear angular density in ``mu`` with its exact CDF and a guide
+table that bounds the device's panel search.  The device inverts two
+bracketing rows at one uniform and mixes the quantiles linearly in ``ln E``.
 """
 
 from __future__ import annotations
@@ -146,3 +151,152 @@ def stack_panels(panels, energies_eV=None) -> dict:
         "transport2_cm2": np.array([p.transport2_cm2 for p in ps]),
         "absorption_cm2": np.array([p.absorption_cm2 for p in ps]),
     }
+
+
+# ---- packing for the transport step (scattering.py:333-470, 494-510) --------------------------
+#: Guide cells per angular row: a power of two, so ``xi * cells`` and the cell
+#: bounds ``j / cells`` are exact (scattering.py:356-360).
+GUIDE_CELLS = 1024
+
+
+def angular_pdf(mu, dcs):
+    """``(pdf, cdf)`` rows of piecewise-linear densities in ``mu`` from DCS rows.
+
+    ``dOmega = 4 pi dmu`` for an azimuthally symmetric DCS, so ``p(mu)`` is the
+    DCS over its trapezoid integral on the native grid; ``cdf`` is the exact
+    integral of that piecewise-linear ``pdf`` (0 at the first node, 1 at the last)."""
+    mu = np.asarray(mu, dtype=np.float64)
+    dcs = np.atleast_2d(np.asarray(dcs, dtype=np.float64))
+    if dcs.shape[1] != mu.size:
+        raise ValueError("ELSEPA DCS rows must match the angular grid")
+    panel = 0.5 * (dcs[:, 1:] + dcs[:, :-1]) * np.diff(mu)[None, :]
+    total = panel.sum(axis=1)
+    if not np.all(np.isfinite(total)) or np.any(total <= 0.0):
+        raise ValueError("ELSEPA DCS rows need a positive finite angular integral")
+    pdf = dcs / total[:, None]
+    cdf = np.concatenate([np.zeros((dcs.shape[0], 1)), np.cumsum(panel, axis=1) / total[:, None]], axis=1)
+    cdf[:, -1] = 1.0
+    return pdf, cdf
+
+
+def guide_table(cdf, cells: int = GUIDE_CELLS) -> np.ndarray:
+    """``guide[r, j]``: the last panel ``k`` in ``[0, n_mu - 2]`` with ``cdf[r, k] <= j / cells``.
+
+    For ``xi`` in cell ``j`` the panel a bisection over the whole row picks
+    (the last ``k`` with ``cdf[r, k] <= xi``) lies in ``[guide[r, j],
+    guide[r, j + 1]]``, so the bisection confined to that window picks the
+    same panel, ties and zero-mass panels included; this needs the interior
+    CDF non-decreasing, which is checked."""
+    cdf = np.atleast_2d(np.asarray(cdf, dtype=np.float64))
+    if cells < 1 or (cells & (cells - 1)) != 0:
+        raise ValueError("guide cells must be a positive power of two")
+    n_mu = cdf.shape[1]
+    inner = cdf[:, : n_mu - 1]
+    if np.any(np.diff(inner, axis=1) < 0.0):
+        raise ValueError("ELSEPA angular CDF rows must be non-decreasing")
+    bounds = np.arange(cells + 1, dtype=np.float64) / cells
+    out = np.empty((cdf.shape[0], cells + 1), dtype=np.int32)
+    for r in range(cdf.shape[0]):
+        out[r] = np.clip(np.searchsorted(inner[r], bounds, side="right") - 1, 0, n_mu - 2)
+    out[:, 0] = 0
+    out[:, cells] = n_mu - 2
+    return out
+
+
+@dataclass
+class ElsepaGroup:
+    """Packed tables of a layer stack, laid out as the device step reads them.
+
+    ``start[L, i]`` / ``length[L, i]``: the element's energy rows in
+    ``logE`` / ``log_rate`` / ``cdf`` / ``pdf`` / ``guide`` (row-major, one
+    angular grid ``mu`` shared by every table)."""
+
+    start: np.ndarray  # int64 [n_layers, max_el]
+    length: np.ndarray  # int64 [n_layers, max_el]
+    logE: np.ndarray  # ln E[eV] per row
+    log_rate: np.ndarray  # ln(n_cm3 sigma_el [1/cm]) per row
+    mu: np.ndarray  # [n_mu]
+    cdf: np.ndarray  # [rows, n_mu]
+    pdf: np.ndarray  # [rows, n_mu]
+    guide: np.ndarray  # int32 [rows, cells + 1]
+    E_range_keV: list  # per layer, per element: (lowest, highest) tabulated energy
+
+    @property
+    def guide_cells(self) -> int:
+        return int(self.guide.shape[1] - 1)
+
+
+def pack_tables(elastic_tables, L_ncm3, rate_scales=None, cells: int = GUIDE_CELLS) -> ElsepaGroup:
+    """Pack ``elastic_tables[L][i]`` (mappings with ``energy_eV``,
+    ``total_elastic_cm2``, ``mu``, ``dcs_cm2_sr``; :func:`stack_panels` makes
+    one) for number densities ``L_ncm3[L][i]`` [1/cm^3].  ``rate_scales[L][i]``
+    multiplies that element's rate node by node and leaves its angles alone."""
+    if len(elastic_tables) != len(L_ncm3):
+        raise ValueError("elastic_tables needs one entry per transport layer")
+    max_el = max(len(n) for n in L_ncm3)
+    start = np.zeros((len(L_ncm3), max_el), dtype=np.int64)
+    length = np.zeros((len(L_ncm3), max_el), dtype=np.int64)
+    logE, log_rate, pdfs, cdfs, ranges = [], [], [], [], []
+    mu_ref = None
+    row = 0
+    for L, (layer, ncm3) in enumerate(zip(elastic_tables, L_ncm3)):
+        if len(layer) != len(ncm3):
+            raise ValueError(f"elastic_tables layer {L} needs one table per element")
+        ranges.append([])
+        for i, (table, n) in enumerate(zip(layer, ncm3)):
+            E = np.asarray(table["energy_eV"], dtype=np.float64)
+            sigma = np.asarray(table["total_elastic_cm2"], dtype=np.float64)
+            mu = np.asarray(table["mu"], dtype=np.float64)
+            if mu_ref is None:
+                mu_ref = mu
+            elif not np.array_equal(mu, mu_ref):
+                raise ValueError("every ELSEPA table must share one angular grid")
+            if E.size < 2 or np.any(np.diff(E) <= 0.0) or sigma.shape != E.shape or np.any(sigma <= 0.0):
+                raise ValueError("ELSEPA tables need at least two rising energies and positive cross sections")
+            pdf, cdf = angular_pdf(mu, table["dcs_cm2_sr"])
+            if pdf.shape[0] != E.size:
+                raise ValueError("ELSEPA tables need one DCS row per energy")
+            rate = float(n) * sigma
+            if rate_scales is not None:
+                rate = rate * np.asarray(rate_scales[L][i], dtype=np.float64)
+            start[L, i], length[L, i] = row, E.size
+            row += E.size
+            logE.append(np.log(E))
+            log_rate.append(np.log(rate))
+            pdfs.append(pdf)
+            cdfs.append(cdf)
+            ranges[L].append((float(E[0]) * 1e-3, float(E[-1]) * 1e-3))
+    cdf = np.ascontiguousarray(np.vstack(cdfs))
+    return ElsepaGroup(start=start, length=length, logE=np.concatenate(logE), log_rate=np.concatenate(log_rate),
+                       mu=np.ascontiguousarray(mu_ref), cdf=cdf, pdf=np.ascontiguousarray(np.vstack(pdfs)),
+                       guide=guide_table(cdf, cells), E_range_keV=ranges)
+
+
+def bracket(logE_rows: np.ndarray, x):
+    """Lower node and fraction of ``x`` on one ascending row, clamped to its
+    ends (``_jit_shell_device.py:58-84``): below the row ``(0, 0)``, above it
+    ``(n - 2, 1)``."""
+    x = np.asarray(x, dtype=np.float64)
+    n = logE_rows.size
+    lo = np.clip(np.searchsorted(logE_rows, x, side="right") - 1, 0, n - 2)
+    f = (x - logE_rows[lo]) / (logE_rows[lo + 1] - logE_rows[lo])
+    f = np.where(x <= logE_rows[0], 0.0, np.where(x >= logE_rows[-1], 1.0, f))
+    lo = np.where(x <= logE_rows[0], 0, np.where(x >= logE_rows[-1], n - 2, lo))
+    return lo, f
+
+
+def element_rate(group: ElsepaGroup, L: int, i: int, E_keV) -> np.ndarray:
+    """Macroscopic elastic rate [1/cm] of element ``i`` of layer ``L``, log-log between nodes."""
+    s, n = int(group.start[L, i]), int(group.length[L, i])
+    lo, f = bracket(group.logE[s:s + n], np.log(np.asarray(E_keV, dtype=np.float64) * 1e3))
+    r0 = group.log_rate[s + lo]
+    return np.exp(r0 + f * (group.log_rate[s + lo + 1] - r0))
+
+
+def check_coverage(group: ElsepaGroup, E_min_keV: float, E_max_keV: float) -> None:
+    """Refuse transport outside any table's energies; never extrapolate
+    (``scattering.py:481-491``)."""
+    for layer in group.E_range_keV:
+        for lower, upper in layer:
+            if E_min_keV < lower or E_max_keV > upper:
+                raise ValueError(f"transport energy range must be within ELSEPA table [{lower:g}, {upper:g}] keV")

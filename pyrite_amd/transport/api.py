@@ -8,8 +8,10 @@ The flat entry mirrors the reference's ``_simulate_trajectories``
 and SRD 64-calibrated screening, ``pyrite_amd.tables.mott``; the tables come
 from ``PYRITE_MOTT_TABLES_DIR`` or ``mott_tables_dir``, and an energy range
 outside any element's table is refused before transport,
-``host_pass.py:595-611``).  ``"elsepa"`` raises ``NotImplementedError``
-(fail closed, as the reference does for unsupported combinations).
+``host_pass.py:595-611``), and ``elastic_model="elsepa"`` with
+``elastic_tables[L][i]`` (ELSEPA native-grid tables, e.g. from
+``pyrite_amd.tables.elsepa.stack_panels``): LUT rates from their log-log
+rates and tabulated angles in the step, energies outside a table refused.
 
 The returned mapping uses the reference's segment keys (``host_pass.py:1048-1144``):
 ``r_mid``, ``v_hat``, ``L_ang``, ``E_keV``, ``E_end_keV``, ``E_repr_keV``,
@@ -136,6 +138,7 @@ def simulate_trajectories(
     radiative_cutoff_eV=None,
     mott_tables_dir=None,
     atomic_electron_deflection=None,
+    elastic_tables=None,
 ):
     """Transport ``Ne`` electrons of ``E0_keV`` into a slab (hip core).
 
@@ -159,10 +162,9 @@ def simulate_trajectories(
     """
     import torch
 
-    if elastic_model not in ("sr", "mott"):
+    if elastic_model not in ("sr", "mott", "elsepa"):
         raise NotImplementedError(
-            f"elastic_model={elastic_model!r}: the hip LUT core implements 'sr' and 'mott' (ELSEPA needs its "
-            "tabulated angular sampler)"
+            f"elastic_model={elastic_model!r}: the hip LUT core implements 'sr', 'mott' and 'elsepa'"
         )
     if energy_model not in ("midpoint", "frozen"):
         raise ValueError("energy_model must be 'midpoint' or 'frozen'")
@@ -170,7 +172,8 @@ def simulate_trajectories(
     stack = _layers(thickness_ang, element, n_atoms_per_ang3, composition, layers)
     shell_cut = (1000.0 if inelastic_cutoff_eV is None else float(inelastic_cutoff_eV)) \
         if inelastic_model == "shell-soft-hard" else None
-    tables = build_layer_tables(stack, elastic_model, mott_tables_dir, atomic_electron_deflection, shell_cut)
+    tables = build_layer_tables(stack, elastic_model, mott_tables_dir, atomic_electron_deflection, shell_cut,
+                                elastic_tables=elastic_tables)
     finite = crystal_width_mm is not None and crystal_height_mm is not None
 
     if initial_state is None:
@@ -187,6 +190,10 @@ def simulate_trajectories(
         from ..tables.mott import check_mott_coverage
 
         check_mott_coverage(tables.compositions, float(E_cut_h.min()), float(E_h.max()), mott_tables_dir)
+    if elastic_model == "elsepa":
+        from ..tables.elsepa import check_coverage
+
+        check_coverage(tables.elsepa, float(E_cut_h.min()), float(E_h.max()))
     lut = build_transport_lut(float(E_cut_h.min()), float(E_h.max()), tables)
     # host-side refusals above run before any device work
     backend = require_device()

@@ -106,6 +106,21 @@ class DeviceTables:
         self.inv_beta_f = self.inv_beta.float()
         self.cdf_f = self.cdf.float()
         self.alpha_f = self.alpha.float()
+        # tabulated elastic angles (hr_elsepa), FP64 in both transport precisions
+        self.elsepa_args = None
+        g = getattr(layers, "elsepa", None)
+        if g is not None:
+            self._elsepa = {
+                "start": torch.as_tensor(np.ascontiguousarray(g.start, dtype=np.int64), device=device),
+                "length": torch.as_tensor(np.ascontiguousarray(g.length, dtype=np.int64), device=device),
+                "logE": f64(g.logE), "mu": f64(g.mu), "cdf": f64(g.cdf), "pdf": f64(g.pdf),
+                "guide": torch.as_tensor(np.ascontiguousarray(g.guide, dtype=np.int32), device=device),
+            }
+            t = _lib.ElsepaArgs()
+            t.on, t.n_mu, t.guide_cells = 1, int(g.mu.size), int(g.guide_cells)
+            for k, v in self._elsepa.items():
+                setattr(t, k, int(v.data_ptr()))
+            self.elsepa_args = t
 
 
 @dataclass
@@ -164,6 +179,8 @@ def transport_args(tables: DeviceTables, control: TransportControl, run, state, 
     a.precision = 1 if control.precision == "mixed" else 0
     a.lut_total_rate_f, a.lut_dEds_f = _ptr(tables.total_rate_f), _ptr(tables.dEds_f)
     a.lut_inv_beta_f, a.lut_cdf_f, a.lut_alpha_f = _ptr(tables.inv_beta_f), _ptr(tables.cdf_f), _ptr(tables.alpha_f)
+    if tables.elsepa_args is not None:
+        a.elsepa = tables.elsepa_args
     return a
 
 

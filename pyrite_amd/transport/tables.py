@@ -126,19 +126,25 @@ class LayerTables:
     L_mott_denom2: np.ndarray | None = None
     # per layer, per element: (log10 E_eV, log10 alpha) calibration nodes under "mott"
     mott_tables: list | None = None
+    # packed tabulated elastic tables under "elsepa" (pyrite_amd.tables.elsepa.ElsepaGroup)
+    elsepa: object | None = None
 
 
-ELASTIC_MODELS = ("sr", "mott")
+ELASTIC_MODELS = ("sr", "mott", "elsepa")
 ATOMIC_ELECTRON_DEFLECTION_MODELS = ("kawrakow",)
 
 
 def build_layer_tables(layers, elastic_model: str = "sr", mott_tables_dir=None, atomic_electron_deflection=None,
-                       hard_cutoff_eV=None) -> LayerTables:
+                       hard_cutoff_eV=None, elastic_tables=None) -> LayerTables:
     """``layers = [(z_top, z_bot, [(element, n_per_A3), ...]), ...]`` top first.
 
     ``elastic_model="mott"`` reads every element's SRD 64 table
     (``mott_tables_dir``, else ``PYRITE_MOTT_TABLES_DIR``) here, before any
     transport, and raises ``MottTableUnavailableError`` for a missing one.
+    ``elastic_model="elsepa"`` packs ``elastic_tables[L][i]`` (one ELSEPA
+    native-grid table per element, ``pyrite_amd.tables.elsepa``): the LUT's
+    rate and element columns then come from their log-log rates and the step
+    samples their tabulated angles (``layer_tables.py:29-40``).
 
     ``atomic_electron_deflection="kawrakow"`` folds the deflection by the Z
     atomic electrons into the elastic rate, ``Z^2 -> Z (Z + xi)``
@@ -155,6 +161,10 @@ def build_layer_tables(layers, elastic_model: str = "sr", mott_tables_dir=None, 
     if atomic_electron_deflection is not None and atomic_electron_deflection not in ATOMIC_ELECTRON_DEFLECTION_MODELS:
         raise ValueError("atomic_electron_deflection must be None or one of "
                          + ", ".join(repr(m) for m in ATOMIC_ELECTRON_DEFLECTION_MODELS))
+    if elastic_model == "elsepa" and elastic_tables is None:
+        raise ValueError("elastic_model='elsepa' needs elastic_tables (one ELSEPA table per layer element)")
+    if elastic_model != "elsepa" and elastic_tables is not None:
+        raise ValueError("elastic_tables is only valid with elastic_model='elsepa'")
     if atomic_electron_deflection is not None and hard_cutoff_eV is not None:
         raise ValueError("atomic_electron_deflection with inelastic_model='shell-soft-hard' needs "
                          "elastic_model='elsepa' (energy-dependent xi), which this backend does not run; pass "
@@ -220,7 +230,19 @@ def build_layer_tables(layers, elastic_model: str = "sr", mott_tables_dir=None, 
         L_mott_denom1=out["md1"],
         L_mott_denom2=out["md2"],
         mott_tables=mott if elastic_model == "mott" else None,
+        elsepa=_pack_elsepa(elastic_tables, out["ncm3"], out["Z"], nel, atomic_electron_deflection)
+        if elastic_model == "elsepa" else None,
     )
+
+
+def _pack_elsepa(elastic_tables, ncm3, Z, nel, atomic_electron_deflection):
+    from ..tables.elsepa import pack_tables
+
+    rows = [ncm3[L, :m] for L, m in enumerate(nel.tolist())]
+    scales = None
+    if atomic_electron_deflection is not None:  # continuous stopping: xi = 1 at every node
+        scales = [[1.0 + 1.0 / z for z in Z[L, :m]] for L, m in enumerate(nel.tolist())]
+    return pack_tables(elastic_tables, rows, scales)
 
 
 # ---- LUT (lut.py:28-437) ----------------------------------------------------
@@ -266,7 +288,11 @@ def _evaluate_nodes(tables: LayerTables, E_grid):
         dEds[L] = spliced_dEds(tables.L_J[L, :m], tables.L_k[L, :m], tables.L_coeff[L, :m], tables.L_E_cross[L, :m], E_grid)
         rates = np.empty((m, n))
         for i in range(m):
-            if tables.elastic_model == "mott":
+            if tables.elastic_model == "elsepa":
+                from ..tables.elsepa import element_rate
+
+                rates[i] = element_rate(tables.elsepa, L, i, E_grid)  # alpha unused: tabulated angles
+            elif tables.elastic_model == "mott":
                 from ..tables.mott import mott_alpha
 
                 rates[i] = _browning_rates(tables.L_mott_numer[L, i], tables.L_mott_denom1[L, i],
@@ -315,7 +341,8 @@ def build_transport_lut(E_min_keV, E_max_keV, tables: LayerTables, intervals_per
             st, sd, sb, _, sa = _evaluate_nodes(tables, Es)
             # the screening of each layer's live elements too (lut.py:251-270); the CDF is a
             # probability and is not held to a relative bound
-            screens = [(alpha[L, :m], sa[L, :m]) for L, m in enumerate(tables.L_nel.tolist())]
+            screens = [] if tables.elastic_model == "elsepa" else \
+                [(alpha[L, :m], sa[L, :m]) for L, m in enumerate(tables.L_nel.tolist())]
             for node, samp in ((total, st), (dEds, sd), (inv_beta[None, :], sb[None, :]), *screens):
                 pred = (1.0 - w) * node[..., :-1] + w * node[..., 1:]
                 e = np.abs(pred - samp) / np.abs(samp)

@@ -69,6 +69,7 @@ STRUCTS = {
     "hr_hard_events": "HardEvents",
     "hr_cone_split": "ConeSplitArgs",
     "hr_cone_collect_args": "ConeCollectArgs",
+    "hr_elsepa": "ElsepaArgs",
 }
 
 

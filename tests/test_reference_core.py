@@ -37,7 +37,7 @@ def test_positional_call_maps_onto_the_native_arguments():
     a = build_args(*pos_args)
     b = transport_args(tabs, ctl, run, state, rows, out, (ctl.straggle_on, stragg))
     for name, _ in _lib.TransportArgs._fields_:
-        if name in ("work", "hard", "cone") or name.endswith("_f"):
+        if name in ("work", "hard", "cone", "elsepa") or name.endswith("_f"):
             continue  # claim counter and FP32 copies are the caller's; modes off in both
         assert getattr(a, name) == getattr(b, name), name
     assert a.max_steps == 777 and a.max_dE_frac == 0.05 and a.e_start == 10 and a.cap == 16
@@ -79,6 +79,6 @@ def test_mott_code_runs_the_lut_it_is_given():
     mott = (control[0], control[1], 1, control[3], control[4])
     b = build_args(run_, mott, geometry, lut_args, materials, st, segments, pe_out, straggling)
     for name, _ in _lib.TransportArgs._fields_:
-        if name in ("work", "hard", "cone"):
+        if name in ("work", "hard", "cone", "elsepa"):
             continue
         assert getattr(a, name) == getattr(b, name), name

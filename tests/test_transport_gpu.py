@@ -212,6 +212,27 @@ def test_atomic_electron_deflection_shrinks_first_flights(device):
     np.testing.assert_allclose(first["kawrakow"] / first[None], 14.0 / 15.0, rtol=1e-13)
 
 
+def test_sr_shaped_elsepa_tables_reproduce_sr_transport_on_the_device(device):
+    """test_elsepa_elastic.py:332-346 of the reference on the device: the same
+    physics through the tabulated sampler and the analytic one agrees
+    statistically (backscatter fraction within 4 sigma, path per electron 3 %)."""
+    from pyrite_amd.transport.api import simulate_trajectories
+    from tests.transport_cases import elsepa_test_tables
+
+    layers = [(0.0, 2.0e5, [("Si", 0.04994)])]
+    tables = elsepa_test_tables(layers, np.geomspace(4.0, 40.0, 41))
+    n = 20_000
+    out = {}
+    for model, kw in (("sr", {}), ("elsepa", {"elastic_tables": tables})):
+        r = simulate_trajectories(30.0, n, 2.0e5, composition=[("Si", 0.04994)], seed=11, elastic_model=model,
+                                  keep_segments_on_device=False, **kw)
+        out[model] = (r["n_backscattered"] / n, r["L_ang"].sum() / n)
+    (p_sr, l_sr), (p_el, l_el) = out["sr"], out["elsepa"]
+    sigma = np.sqrt(p_sr * (1 - p_sr) / n + p_el * (1 - p_el) / n)
+    assert abs(p_sr - p_el) < 4.0 * sigma, (p_sr, p_el)
+    assert l_el == pytest.approx(l_sr, rel=0.03)
+
+
 def test_simulate_trajectories_mott_endpoints_are_inclusive(device, tmp_path):
     """test_mott_coverage.py:45-53 on the device: a 300 keV beam over tables
     ending at 300 keV, cutoff at their 50 eV start, runs; the result reports the

@@ -9,6 +9,41 @@ from pyrite_amd.transport.tables import build_layer_tables, build_transport_lut
 
 # the reference's synthetic SRD 64-format Mott tables (tests/data/mott_srd64_synthetic)
 MOTT_FIXTURE_DIR = Path(__file__).resolve().parent / "golden" / "mott_srd64_synthetic"
+# the reference's ELSEPA vendor panel (Hg, 1 keV; tests/data/xsgen/elsepa/test-run-output)
+ELSEPA_PANEL = Path(__file__).resolve().parent / "golden" / "elsepa_dcs_1p000e03.dat"
+
+
+def sr_cross_section_cm2(Z, E_keV):
+    """Total cross section of the analytic screened-Rutherford model."""
+    a = 3.4e-3 * Z**0.67 / E_keV
+    rel = (E_keV + 511.0) / (E_keV + 1024.0)
+    return 5.21e-21 * Z * Z * 4.0 * np.pi / E_keV**2 / (a * (1.0 + a)) * rel**2
+
+
+def elsepa_test_tables(layers, energies_keV=None):
+    """ELSEPA-format tables on the vendor panel's angular grid: W carries the
+    vendor DCS shape (a real, non-screened-Rutherford law) at every energy,
+    other elements the screened-Rutherford law's own DCS; both with the
+    screened-Rutherford total cross section."""
+    from pyrite_amd.tables.elsepa import read_dcs
+    from pyrite_amd.transport.elements import TRANSPORT_ELEMENTS
+
+    panel = read_dcs(ELSEPA_PANEL)
+    E = np.geomspace(4.0, 45.0, 30) if energies_keV is None else np.asarray(energies_keV, dtype=float)
+    out = []
+    for _, _, comp in layers:
+        row = []
+        for el, _ in comp:
+            Z = float(TRANSPORT_ELEMENTS[el]["Z"])
+            sigma = sr_cross_section_cm2(Z, E)
+            if el == "W":
+                dcs = np.tile(panel.dcs_cm2_sr, (E.size, 1))
+            else:
+                a = 3.4e-3 * Z**0.67 / E
+                dcs = (sigma / (4.0 * np.pi) * a * (1.0 + a))[:, None] / (panel.mu[None, :] + a[:, None]) ** 2
+            row.append({"energy_eV": E * 1e3, "total_elastic_cm2": sigma, "mu": panel.mu, "dcs_cm2_sr": dcs})
+        out.append(row)
+    return out
 
 
 @dataclass
@@ -28,7 +63,8 @@ class Case:
     atomic: str | None = None  # atomic_electron_deflection
 
     def tables(self):
-        lt = build_layer_tables(self.layers, self.elastic, MOTT_FIXTURE_DIR, self.atomic)
+        extra = {"elastic_tables": elsepa_test_tables(self.layers)} if self.elastic == "elsepa" else {}
+        lt = build_layer_tables(self.layers, self.elastic, MOTT_FIXTURE_DIR, self.atomic, **extra)
         lut = build_transport_lut(self.E_cut, self.E0, lt)
         return lt, lut
 
@@ -64,4 +100,10 @@ MULTILAYER_MOTT = Case(
 )
 SI_30KEV_ATOMIC = Case("si30_aed", 30.0, 400, [(0.0, 5.0e4, [("Si", 0.04996)])], seed=11, straggle=True,
                        atomic="kawrakow")
-CASES = [SI_30KEV, SI_30KEV_STRAGGLE, HOPG_5MEV, MULTILAYER, FRAMED, MULTILAYER_MOTT, SI_30KEV_ATOMIC]
+MULTILAYER_ELSEPA = Case(
+    "stack_elsepa", 40.0, 300,
+    [(0.0, 2.0e3, [("W", 0.0632)]), (2.0e3, 3.0e4, [("Si", 0.025), ("C", 0.05)])],
+    seed=7, straggle=True, max_dE_frac=0.05, elastic="elsepa",
+)
+CASES = [SI_30KEV, SI_30KEV_STRAGGLE, HOPG_5MEV, MULTILAYER, FRAMED, MULTILAYER_MOTT, SI_30KEV_ATOMIC,
+         MULTILAYER_ELSEPA]
