@@ -26,7 +26,8 @@ reference's does.  ``elastic_model_code`` 0 (screened Rutherford) and 1
 (Mott) both run: the reference's LUT kernel takes the code but never branches
 on it (``_jit_lut_kernel.py:74``), because the LUT columns carry the model
 (``tables.build_layer_tables(..., elastic_model=...)``).  Unsupported modes
-fail closed: the ELSEPA code 2 or a nonzero ``sbethe_on`` raises
+fail closed: the ELSEPA code 2 (its tables enter with the layer tables, not
+in the positional ``mott_group`` layout) or a nonzero ``sbethe_on`` raises
 ``NotImplementedError``; the shell and radiative modes of this backend take a
 ``hard_events.DeviceHardModes`` through ``hard=`` (their tables live on the
 transport grid, not in the reference's ``inelastic_args`` layout, which is
@@ -67,9 +68,9 @@ def build_args(run, control, geometry, lut_args, materials, state, segments, pe_
     e_start, e_count, cap, stream_key = run
     max_steps, _max_segments, elastic_code, energy_code, max_dE_frac = control
     if int(elastic_code) not in (0, 1):
-        raise NotImplementedError(f"elastic_model_code={int(elastic_code)}: the hip LUT core runs the LUT-carried "
-                                  "models, screened Rutherford (0) and Mott (1); ELSEPA (2) needs its tabulated "
-                                  "angular sampler")
+        raise NotImplementedError(f"elastic_model_code={int(elastic_code)}: the positional adapter runs the "
+                                  "LUT-carried models, screened Rutherford (0) and Mott (1); ELSEPA (2) tables enter "
+                                  "with the layer tables (build_layer_tables(..., 'elsepa', elastic_tables=...))")
     if int(energy_code) not in (0, 1):
         raise ValueError("energy_model_code must be 0 (frozen) or 1 (midpoint)")
     n_layers, internal_bounds, z_total, finite, width, height, L_nel, L_top, L_bot = geometry
