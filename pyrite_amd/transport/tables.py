@@ -153,8 +153,9 @@ def build_layer_tables(layers, elastic_model: str = "sr", mott_tables_dir=None, 
     both analytic rate numerators scale by ``1 + 1/Z``; the denominators and
     the screening are untouched (``layer_tables.py:121-130``).  With explicit
     hard collisions (``hard_cutoff_eV``, the shell mode) ``xi`` depends on
-    energy, which these coefficients cannot carry, so that combination is
-    refused, as the reference refuses it without tabulated elastic data.
+    energy (``atomic_electrons.atomic_electron_xi``): the ELSEPA tables carry
+    it node by node in their rates, and the analytic coefficients cannot, so
+    that combination is refused for "sr" and "mott", as the reference does.
     ``None`` leaves every coefficient bit for bit."""
     if elastic_model not in ELASTIC_MODELS:
         raise ValueError(f"elastic_model must be one of {ELASTIC_MODELS}, got {elastic_model!r}")
@@ -165,9 +166,9 @@ def build_layer_tables(layers, elastic_model: str = "sr", mott_tables_dir=None, 
         raise ValueError("elastic_model='elsepa' needs elastic_tables (one ELSEPA table per layer element)")
     if elastic_model != "elsepa" and elastic_tables is not None:
         raise ValueError("elastic_tables is only valid with elastic_model='elsepa'")
-    if atomic_electron_deflection is not None and hard_cutoff_eV is not None:
+    if atomic_electron_deflection is not None and hard_cutoff_eV is not None and elastic_model != "elsepa":
         raise ValueError("atomic_electron_deflection with inelastic_model='shell-soft-hard' needs "
-                         "elastic_model='elsepa' (energy-dependent xi), which this backend does not run; pass "
+                         "elastic_model='elsepa' (its energy-dependent xi rides on the table nodes); pass "
                          "atomic_electron_deflection=None to opt out")
     n_layers = len(layers)
     max_el = max(len(c) for _, _, c in layers)
@@ -230,18 +231,24 @@ def build_layer_tables(layers, elastic_model: str = "sr", mott_tables_dir=None, 
         L_mott_denom1=out["md1"],
         L_mott_denom2=out["md2"],
         mott_tables=mott if elastic_model == "mott" else None,
-        elsepa=_pack_elsepa(elastic_tables, out["ncm3"], out["Z"], nel, atomic_electron_deflection)
-        if elastic_model == "elsepa" else None,
+        elsepa=_pack_elsepa(elastic_tables, layers, out["ncm3"], out["Z"], nel, atomic_electron_deflection,
+                            hard_cutoff_eV) if elastic_model == "elsepa" else None,
     )
 
 
-def _pack_elsepa(elastic_tables, ncm3, Z, nel, atomic_electron_deflection):
+def _pack_elsepa(elastic_tables, layers, ncm3, Z, nel, atomic_electron_deflection, hard_cutoff_eV):
+    """ELSEPA group of the stack; the atomic-electron factor 1 + xi(E)/Z per
+    table node, xi from the layer's composition and the shell cutoff
+    (``layer_tables.py:131-146``)."""
     from ..tables.elsepa import pack_tables
+    from .atomic_electrons import atomic_electron_xi, elastic_rate_scale
 
     rows = [ncm3[L, :m] for L, m in enumerate(nel.tolist())]
     scales = None
-    if atomic_electron_deflection is not None:  # continuous stopping: xi = 1 at every node
-        scales = [[1.0 + 1.0 / z for z in Z[L, :m]] for L, m in enumerate(nel.tolist())]
+    if atomic_electron_deflection is not None:
+        scales = [[elastic_rate_scale(Z[L, i], atomic_electron_xi(np.asarray(t["energy_eV"], dtype=np.float64),
+                                                                    layers[L][2], hard_cutoff_eV))
+                   for i, t in enumerate(layer)] for L, layer in enumerate(elastic_tables)]
     return pack_tables(elastic_tables, rows, scales)
 
 

@@ -25,8 +25,15 @@ GE = [("Ge", 0.0442)]
 W_SI = [("W", 0.02), ("Si", 0.03)]
 
 
-def _tables(comp, E0, thick, rad_on, shell_on, rad_cut=5000.0):
-    lt = build_layer_tables([(0.0, thick, list(comp))])
+def _tables(comp, E0, thick, rad_on, shell_on, rad_cut=5000.0, elsepa_atomic=False):
+    if elsepa_atomic:  # ELSEPA tables carrying the shell mode's energy-dependent atomic-electron factor
+        from tests.transport_cases import elsepa_test_tables
+
+        layers = [(0.0, thick, list(comp))]
+        lt = build_layer_tables(layers, "elsepa", atomic_electron_deflection="kawrakow", hard_cutoff_eV=1000.0,
+                                elastic_tables=elsepa_test_tables(layers, np.geomspace(4.0, 1.2 * E0, 40)))
+    else:
+        lt = build_layer_tables([(0.0, thick, list(comp))])
     lut0 = build_transport_lut(5.0, E0, lt)
     shell = he.build_shell_tables(lt, lut0.E_grid, 1000.0) if shell_on else None
     rad = he.build_radiative_tables(lt, rad_cut) if rad_on else None
@@ -109,6 +116,27 @@ CASES = [
     ("ge_shell", GE, 3000.0, 1.0e6, False, True, False),
     ("wsi_rad", W_SI, 20000.0, 1.0e6, True, False, True),
 ]
+
+
+def test_shell_mode_on_elsepa_tables_with_atomic_deflection_matches_oracle(oracle, device):
+    """Shell mode, tabulated angles and the energy-dependent 1 + xi(E)/Z on the
+    table rates together: first steps row for row against the oracle."""
+    lt, lut0, lut, rad, shell = _tables(GE, 3000.0, 1.0e6, False, True, elsepa_atomic=True)
+    assert lt.elsepa is not None
+    n, seed, steps = 20000, 12, 4
+    g_rows, g_cnt, g_code, g_st, modes = _gpu(lt, lut0, lut, rad, shell, device, 3000.0, n, seed, False, steps,
+                                              max_steps=steps)
+    o_rows, o_cnt, o_code, o_st = _oracle(oracle, lt, lut, rad, shell, modes, 3000.0, n, seed, False, steps,
+                                          max_steps=steps)
+    np.testing.assert_array_equal(g_cnt, o_cnt)
+    np.testing.assert_array_equal(g_code, o_code)
+    used = (np.arange(steps)[None, :] < g_cnt[:, None]).reshape(-1)
+    np.testing.assert_array_equal(g_rows["seg_event"][used], o_rows["seg_event"][used])
+    assert (o_rows["seg_event"][used] == 8).sum() > 100
+    for k in ("seg_len", "seg_E", "seg_E_end", "seg_t_end"):
+        np.testing.assert_allclose(g_rows[k][used], o_rows[k][used], rtol=1e-12, atol=1e-300)
+    for k in ("E_keV", "dirs"):
+        np.testing.assert_allclose(g_st[k], o_st[k], rtol=1e-11, atol=1e-12)
 
 
 @pytest.mark.parametrize("name,comp,E0,thick,rad_on,shell_on,straggle", CASES, ids=[c[0] for c in CASES])
