@@ -317,6 +317,12 @@ hr_status hr_cone_roulette(int64_t n, const float *log_imp, const uint64_t *stre
  * alpha, kinetic energy E_keV, event key; out = (made, ox, oy, oz, weight). */
 hr_status hr_cone_event(const hr_cone_split *cone, int64_t n, const double *d, const double *alpha,
                         const double *E_keV, const uint64_t *event_key, double *out, void *stream);
+/* The same on a tabulated law, the reference's cone_split_event
+ * (montecarlo/transport/cone_split.py:223-335): element row block el[i]
+ * (layer * max_el + element) of `tables`, kinetic energy E_keV[i]. */
+hr_status hr_cone_event_tab(const hr_cone_split *cone, const hr_elsepa *tables, int64_t n, const double *d,
+                            const int64_t *el, const double *E_keV, const uint64_t *event_key, double *out,
+                            void *stream);
 
 /* ------------------------------------------------------------------------
  * Bounded continuation (montecarlo/transport/continuation.py:1-27 and the

@@ -99,6 +99,34 @@ double or_shell_primary_cosine(double E, double U, double W, int branch, double 
 double or_shell_secondary_cosine(double E, double U, double W, int branch, double W_eV, double u);
 double or_shell_soft_loss(double mean, double var, uint64_t key, uint64_t *counter);
 
+/* Tabulated (ELSEPA) elastic angles (or_elsepa.c; scattering.py:540-620,
+ * _jit_device.py:493-620).  on = 0: the screened-Rutherford angle. */
+typedef struct or_elsepa {
+    int32_t on, n_mu, guide_cells;
+    const int64_t *start, *length; /* [n_layers, max_el] */
+    const double *logE;            /* [rows] ln E[eV] */
+    const double *mu;              /* [n_mu] */
+    const double *cdf, *pdf;       /* [rows, n_mu] */
+    const int32_t *guide;          /* [rows, guide_cells + 1] */
+} or_elsepa;
+
+/* Lower node (relative to row) and fraction of x on g[row .. row+n), clamped. */
+void or_elsepa_bracket(const double *g, int64_t row, int64_t n, double x, int64_t *lo, double *f);
+/* Inversion of one angular row at xi: guided panel search / whole-row bisection. */
+double or_elsepa_invert_row(const or_elsepa *t, int64_t row, double xi);
+double or_elsepa_invert_row_full(const or_elsepa *t, int64_t row, double xi);
+/* mu of element slot `el` (L*max_el + i) at E_keV and uniform xi. */
+double or_elsepa_sample_mu(const or_elsepa *t, int64_t el, double E_keV, double xi);
+/* Bracketing first row and fraction of slot `el` at E_keV; the ln-E quantile
+ * mixture of rows row, row+1 at xi; the exact CDF of one row at mu = m. */
+void or_elsepa_energy_rows(const or_elsepa *t, int64_t el, double E_keV, int64_t *row, double *f);
+double or_elsepa_mix_mu(const or_elsepa *t, int64_t row, double f, double xi);
+double or_elsepa_cdf_at(const or_elsepa *t, int64_t row, double m);
+void or_elsepa_sample_mu_batch(const or_elsepa *t, int64_t el, int64_t n, const double *E_keV, const double *xi,
+                               double *out);
+void or_elsepa_invert_batch(const or_elsepa *t, int64_t n, const int64_t *row, const double *xi, int guided,
+                            double *out);
+
 /* Detector-cone splitting of elastic events (or_cone.c; cone_split.py:223-335,
  * cores.py:1189-1226) on the screened-Rutherford law.  on = 0: analog.  Copy
  * k of local electron i in slot i*cap + k while k < cap; count[i] counts all. */
@@ -119,33 +147,16 @@ uint64_t or_cone_copy_key(uint64_t split_key, int64_t flight);
 /* out = (ox, oy, oz, weight); returns 1 when a copy is made */
 int or_cone_event(double dx, double dy, double dz, const or_cone_split *c, double alpha, double E_keV,
                   uint64_t key, double *out);
+/* The same on the tabulated law of element slot `el` (cone_split.py:223-335). */
+int or_cone_event_tab(double dx, double dy, double dz, const or_cone_split *c, const or_elsepa *t, int64_t el,
+                      double E_keV, uint64_t key, double *out);
+void or_cone_event_tab_batch(const or_cone_split *c, const or_elsepa *t, int64_t n, const double *d,
+                             const int64_t *el, const double *E_keV, const uint64_t *key, int32_t *made,
+                             double *out);
 double or_cone_ladder_weight(double w);
 int64_t or_cone_collect(const or_cone_split *c, int64_t e_start, int64_t e_count, int ladder, int64_t out_cap,
                         int64_t *parent, int64_t *flight, double *pos, double *dir, double *E_keV, double *clock,
                         double *weight, uint64_t *stream_key, int64_t *max_count);
-
-/* Tabulated (ELSEPA) elastic angles (or_elsepa.c; scattering.py:540-620,
- * _jit_device.py:493-620).  on = 0: the screened-Rutherford angle. */
-typedef struct or_elsepa {
-    int32_t on, n_mu, guide_cells;
-    const int64_t *start, *length; /* [n_layers, max_el] */
-    const double *logE;            /* [rows] ln E[eV] */
-    const double *mu;              /* [n_mu] */
-    const double *cdf, *pdf;       /* [rows, n_mu] */
-    const int32_t *guide;          /* [rows, guide_cells + 1] */
-} or_elsepa;
-
-/* Lower node (relative to row) and fraction of x on g[row .. row+n), clamped. */
-void or_elsepa_bracket(const double *g, int64_t row, int64_t n, double x, int64_t *lo, double *f);
-/* Inversion of one angular row at xi: guided panel search / whole-row bisection. */
-double or_elsepa_invert_row(const or_elsepa *t, int64_t row, double xi);
-double or_elsepa_invert_row_full(const or_elsepa *t, int64_t row, double xi);
-/* mu of element slot `el` (L*max_el + i) at E_keV and uniform xi. */
-double or_elsepa_sample_mu(const or_elsepa *t, int64_t el, double E_keV, double xi);
-void or_elsepa_sample_mu_batch(const or_elsepa *t, int64_t el, int64_t n, const double *E_keV, const double *xi,
-                               double *out);
-void or_elsepa_invert_batch(const or_elsepa *t, int64_t n, const int64_t *row, const double *xi, int guided,
-                            double *out);
 
 typedef struct or_transport_args {
     /* run */

@@ -20,7 +20,8 @@
  * lands in the cone ends with exit code 6).
  *
  * The reference's analog law is a tabulated ELSEPA density whose band edges
- * it finds by bisection; here the law is p(mu) ~ 1/(alpha + mu)^2 on
+ * it finds by bisection (or_cone_event_tab restates that); or_cone_event runs
+ * the transport's own screened-Rutherford law, p(mu) ~ 1/(alpha + mu)^2 on
  * mu = (1 - cos theta)/2 with CDF F(mu) = (1 + alpha) mu / (alpha + mu), so
  * the band's uniform interval [F(mu_lo), F(mu_hi)] is exact.  The interval
  * is handled through the tail 1 - F = alpha (1 - mu)/(alpha + mu), which
@@ -56,8 +57,77 @@ static double arc_half(double cos_t, double sin_t, double cos_dn, double sin_dn,
     return acos(c_star);
 }
 
-int or_cone_event(double dx, double dy, double dz, const or_cone_split *c, double alpha, double E_keV,
-                  uint64_t key, double *out) {
+/* The analog law of one event: screened Rutherford (tab == NULL) with its
+ * exact band, or a tabulated row pair mixed in ln E whose band is bounded by
+ * the two rows' exact CDFs before the copy roulette and found by 64 bisection
+ * steps on the mixed quantile after it (cone_split.py:240-295). */
+typedef struct cone_law {
+    const or_elsepa *tab;
+    double alpha;                        /* screened Rutherford */
+    int64_t row;                         /* tabulated: first bracketing row */
+    double f, m_lo, m_hi, xa, p, tail_a; /* state between the steps */
+} cone_law;
+
+static double law_bound(cone_law *L, double mu_lo, double mu_hi) {
+    if (!L->tab) {
+        L->tail_a = sr_tail(L->alpha, mu_lo);
+        L->p = L->tail_a - sr_tail(L->alpha, mu_hi); /* F(mu_hi) - F(mu_lo) */
+        return L->p;
+    }
+    L->m_lo = mu_lo;
+    L->m_hi = mu_hi;
+    double fa0 = or_elsepa_cdf_at(L->tab, L->row, mu_lo), fa1 = or_elsepa_cdf_at(L->tab, L->row + 1, mu_lo);
+    double fb0 = or_elsepa_cdf_at(L->tab, L->row, mu_hi), fb1 = or_elsepa_cdf_at(L->tab, L->row + 1, mu_hi);
+    return fmax(fb0, fb1) - fmin(fa0, fa1);
+}
+
+static double law_band(cone_law *L) {
+    if (!L->tab) return L->p;
+    double lo, hi;
+    if (or_elsepa_mix_mu(L->tab, L->row, L->f, 0.0) >= L->m_lo) {
+        L->xa = 0.0;
+    } else {
+        lo = 0.0;
+        hi = 1.0;
+        for (int i = 0; i < 64; ++i) {
+            double mid = 0.5 * (lo + hi);
+            if (or_elsepa_mix_mu(L->tab, L->row, L->f, mid) >= L->m_lo)
+                hi = mid;
+            else
+                lo = mid;
+        }
+        L->xa = hi;
+    }
+    double xb;
+    if (or_elsepa_mix_mu(L->tab, L->row, L->f, 1.0) <= L->m_hi) {
+        xb = 1.0;
+    } else {
+        lo = 0.0;
+        hi = 1.0;
+        for (int i = 0; i < 64; ++i) {
+            double mid = 0.5 * (lo + hi);
+            if (or_elsepa_mix_mu(L->tab, L->row, L->f, mid) <= L->m_hi)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        xb = lo;
+    }
+    L->p = xb - L->xa;
+    return L->p;
+}
+
+static double law_mu(const cone_law *L, double u) {
+    if (!L->tab) {
+        /* xi = F(mu_lo) + u p_band, i.e. 1 - xi = tail_a - u p_band */
+        double one_minus_xi = L->tail_a - u * L->p;
+        return L->alpha * (1.0 - one_minus_xi) / (L->alpha + one_minus_xi);
+    }
+    return or_elsepa_mix_mu(L->tab, L->row, L->f, L->xa + u * L->p);
+}
+
+static int cone_event_law(double dx, double dy, double dz, const or_cone_split *c, cone_law *law, double E_keV,
+                          uint64_t key, double *out) {
     out[0] = out[1] = out[2] = out[3] = 0.0;
     double cos_dn = dx * c->nx + dy * c->ny + dz * c->nz;
     if (cos_dn > 1.0) cos_dn = 1.0;
@@ -69,17 +139,16 @@ int or_cone_event(double dx, double dy, double dz, const or_cone_split *c, doubl
     double theta_hi = fmin(M_PI, theta_dn + theta_c);
     double h_lo = sin(0.5 * theta_lo), h_hi = sin(0.5 * theta_hi);
     double mu_lo = h_lo * h_lo, mu_hi = h_hi * h_hi; /* (1 - cos theta)/2 */
-    double tail_a = sr_tail(alpha, mu_lo), tail_b = sr_tail(alpha, mu_hi);
-    double p_band = tail_a - tail_b; /* F(mu_hi) - F(mu_lo) */
+    double band_bound = law_bound(law, mu_lo, mu_hi);
     double arc_bound = sin_dn <= c->sin_c ? M_PI : asin(c->sin_c / sin_dn);
-    double bound = p_band * arc_bound / M_PI;
+    double bound = band_bound * arc_bound / M_PI;
     if (!(bound > 0.0)) return 0;
     double keep = bound / c->target_weight;
     if (keep > 1.0) keep = 1.0;
     if (or_stream_uniform(key, 0) >= keep) return 0;
-    /* xi = F(mu_lo) + u p_band, i.e. 1 - xi = tail_a - u p_band */
-    double one_minus_xi = tail_a - or_stream_uniform(key, 1) * p_band;
-    double mu_t = alpha * (1.0 - one_minus_xi) / (alpha + one_minus_xi);
+    double p_band = law_band(law);
+    if (!(p_band > 0.0)) return 0;
+    double mu_t = law_mu(law, or_stream_uniform(key, 1));
     double cos_t = 1.0 - 2.0 * mu_t;
     double sin_t = sqrt(fmax(0.0, 1.0 - cos_t * cos_t));
     double half = arc_half(cos_t, sin_t, cos_dn, sin_dn, c->cos_c);
@@ -127,6 +196,28 @@ int or_cone_event(double dx, double dy, double dz, const or_cone_split *c, doubl
     out[2] = o[2];
     out[3] = w;
     return 1;
+}
+
+int or_cone_event(double dx, double dy, double dz, const or_cone_split *c, double alpha, double E_keV,
+                  uint64_t key, double *out) {
+    cone_law law = {0};
+    law.alpha = alpha;
+    return cone_event_law(dx, dy, dz, c, &law, E_keV, key, out);
+}
+
+int or_cone_event_tab(double dx, double dy, double dz, const or_cone_split *c, const or_elsepa *t, int64_t el,
+                      double E_keV, uint64_t key, double *out) {
+    cone_law law = {0};
+    law.tab = t;
+    or_elsepa_energy_rows(t, el, E_keV, &law.row, &law.f);
+    return cone_event_law(dx, dy, dz, c, &law, E_keV, key, out);
+}
+
+void or_cone_event_tab_batch(const or_cone_split *c, const or_elsepa *t, int64_t n, const double *d,
+                             const int64_t *el, const double *E_keV, const uint64_t *key, int32_t *made,
+                             double *out) {
+    for (int64_t i = 0; i < n; ++i)
+        made[i] = or_cone_event_tab(d[3 * i], d[3 * i + 1], d[3 * i + 2], c, t, el[i], E_keV[i], key[i], out + 4 * i);
 }
 
 /* Power-of-two ladder weight: w when w is a power of two, else the next one up. */

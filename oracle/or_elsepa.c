@@ -96,13 +96,44 @@ double or_elsepa_invert_row_full(const or_elsepa *t, int64_t row, double xi) {
     return panel_root(t, row, lo, xi);
 }
 
-double or_elsepa_sample_mu(const or_elsepa *t, int64_t el, double E_keV, double xi) {
-    int64_t first = t->start[el], n = t->length[el], lo;
-    double f;
-    or_elsepa_bracket(t->logE, first, n, log(E_keV * 1.0e3), &lo, &f);
-    double mu0 = or_elsepa_invert_row(t, first + lo, xi);
-    double mu1 = or_elsepa_invert_row(t, first + lo + 1, xi);
+void or_elsepa_energy_rows(const or_elsepa *t, int64_t el, double E_keV, int64_t *row, double *f) {
+    int64_t lo;
+    or_elsepa_bracket(t->logE, t->start[el], t->length[el], log(E_keV * 1.0e3), &lo, f);
+    *row = t->start[el] + lo;
+}
+
+double or_elsepa_mix_mu(const or_elsepa *t, int64_t row, double f, double xi) {
+    double mu0 = or_elsepa_invert_row(t, row, xi);
+    double mu1 = or_elsepa_invert_row(t, row + 1, xi);
     return (1.0 - f) * mu0 + f * mu1;
+}
+
+double or_elsepa_sample_mu(const or_elsepa *t, int64_t el, double E_keV, double xi) {
+    int64_t row;
+    double f;
+    or_elsepa_energy_rows(t, el, E_keV, &row, &f);
+    return or_elsepa_mix_mu(t, row, f, xi);
+}
+
+/* Exact CDF of angular row `row` at mu = m (cone_split.py:177-197). */
+double or_elsepa_cdf_at(const or_elsepa *t, int64_t row, double m) {
+    int n = t->n_mu;
+    if (m <= t->mu[0]) return 0.0;
+    if (m >= t->mu[n - 1]) return 1.0;
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) / 2;
+        if (t->mu[mid] <= m)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    int64_t base = row * (int64_t)n;
+    double dmu = t->mu[lo + 1] - t->mu[lo];
+    double x = (m - t->mu[lo]) / dmu;
+    double p0 = t->pdf[base + lo], p1 = t->pdf[base + lo + 1];
+    double v = t->cdf[base + lo] + dmu * (p0 * x + 0.5 * (p1 - p0) * x * x);
+    return fmin(1.0, fmax(0.0, v));
 }
 
 /* Batches for the tests: mu of element slot `el` at E_keV[k], xi[k]; and row

@@ -705,7 +705,10 @@ static void transport_one(const or_transport_args *a, int64_t e) {
         if (split_here) {
             double cp[4];
             uint64_t ek = or_cone_event_key(cone->split_key[e], flight - 1);
-            if (or_cone_event(dir[0], dir[1], dir[2], cone, alpha, a->E_keV[e], ek, cp)) {
+            int made = a->elsepa.on ? or_cone_event_tab(dir[0], dir[1], dir[2], cone, &a->elsepa,
+                                                        (int64_t)(el_row + i_el), a->E_keV[e], ek, cp)
+                                    : or_cone_event(dir[0], dir[1], dir[2], cone, alpha, a->E_keV[e], ek, cp);
+            if (made) {
                 int64_t k = cone->count[i];
                 if (k < cone->cap) {
                     int64_t r = i * cone->cap + k;

@@ -123,6 +123,8 @@ def lib():
         L.or_elsepa_sample_mu_batch.argtypes = [C.POINTER(Elsepa), C.c_int64, C.c_int64, _dp, _dp, _dp]
         L.or_elsepa_invert_batch.restype = None
         L.or_elsepa_invert_batch.argtypes = [C.POINTER(Elsepa), C.c_int64, _dp, _dp, C.c_int, _dp]
+        L.or_cone_event_tab_batch.restype = None
+        L.or_cone_event_tab_batch.argtypes = [C.POINTER(ConeSplit), C.POINTER(Elsepa), C.c_int64] + [_dp] * 6
         L.or_cone_ladder_weight.restype = C.c_double
         L.or_cone_ladder_weight.argtypes = [C.c_double]
         L.or_cone_collect.restype = C.c_int64
@@ -492,6 +494,22 @@ def elsepa_sample_mu(group, el, E_keV, xi):
     out = np.empty_like(xi)
     lib().or_elsepa_sample_mu_batch(C.byref(t), int(el), xi.size, _p(E), _p(xi), _p(out))
     return out
+
+
+def cone_event_tab(d, group, el, E_keV, event_key, cone):
+    """Single-event copies on the tabulated law of element slot(s) ``el``:
+    ``(made bool [n], out [n, 4] = ox, oy, oz, weight)``."""
+    d = np.ascontiguousarray(np.atleast_2d(d), dtype=np.float64)
+    n = d.shape[0]
+    el = np.ascontiguousarray(np.broadcast_to(np.asarray(el, dtype=np.int64), (n,)))
+    E = np.ascontiguousarray(np.broadcast_to(np.asarray(E_keV, dtype=np.float64), (n,)))
+    keys = np.ascontiguousarray(np.asarray(event_key, dtype=np.uint64))
+    c = _cone_struct(cone)
+    t, hold = elsepa_struct(group)
+    made = np.zeros(n, dtype=np.int32)
+    out = np.zeros((n, 4))
+    lib().or_cone_event_tab_batch(C.byref(c), C.byref(t), n, _p(d), _p(el), _p(E), _p(keys), _p(made), _p(out))
+    return made.astype(bool), out
 
 
 def elsepa_invert(group, rows, xi, guided=True):

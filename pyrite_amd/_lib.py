@@ -420,6 +420,8 @@ SIGNATURES = {
     "hr_cone_split_keys": (C.c_int32, [_dp, C.c_int64, _dp, _dp]),
     "hr_cone_roulette": (C.c_int32, [C.c_int64, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp]),
     "hr_cone_event": (C.c_int32, [C.POINTER(ConeSplitArgs), C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "hr_cone_event_tab": (C.c_int32, [C.POINTER(ConeSplitArgs), C.POINTER(ElsepaArgs), C.c_int64, _dp, _dp, _dp, _dp,
+                                      _dp, _dp]),
     "hr_radiative_lut": (C.c_int32, [C.POINTER(HardEvents), C.c_int32, C.c_int32, _dp, _dp, C.c_int64, _dp, _dp,
                                      _dp]),
     "hr_radiative_sample": (C.c_int32, [C.POINTER(HardEvents), C.c_int32, C.c_double, C.c_int64, _dp, _dp, _dp,

@@ -189,6 +189,21 @@ __global__ void cone_event_kernel(hr_cone_split c, int64_t n, const double *__re
     out[5 * i + 4] = cp.w;
 }
 
+__global__ void cone_event_tab_kernel(hr_cone_split c, hr_elsepa t, int64_t n, const double *__restrict__ d,
+                                      const int64_t *__restrict__ el, const double *__restrict__ E,
+                                      const uint64_t *__restrict__ key, double *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    cone::TabLaw law(&t, (size_t)el[i], E[i]);
+    const cone::Copy cp = cone::event_law(d[3 * i], d[3 * i + 1], d[3 * i + 2], c.nx, c.ny, c.nz, c.cos_c, c.sin_c,
+                                          c.target_weight, c.angle_scale, law, E[i], key[i]);
+    out[5 * i] = cp.made ? 1.0 : 0.0;
+    out[5 * i + 1] = cp.ox;
+    out[5 * i + 2] = cp.oy;
+    out[5 * i + 3] = cp.oz;
+    out[5 * i + 4] = cp.w;
+}
+
 int64_t collect_blocks(int64_t e_count) { return (e_count + kCollectThreads - 1) / kCollectThreads; }
 
 }  // namespace
@@ -261,6 +276,26 @@ extern "C" hr_status hr_cone_event(const hr_cone_split *cone, int64_t n, const d
                "hr_cone_event: bad cone parameters");
     hipLaunchKernelGGL(cone_event_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                        *cone, n, d, alpha, E_keV, event_key, out);
+    HR_CHECK_HIP(hipGetLastError());
+    return HR_OK;
+}
+
+extern "C" hr_status hr_cone_event_tab(const hr_cone_split *cone, const hr_elsepa *tables, int64_t n,
+                                       const double *d, const int64_t *el, const double *E_keV,
+                                       const uint64_t *event_key, double *out, void *stream) {
+    HR_TRACE("hr_cone_event_tab");
+    using namespace hr;
+    HR_REQUIRE(cone != nullptr && tables != nullptr && n >= 0, "hr_cone_event_tab: null cone/tables or negative n");
+    if (n == 0) return HR_OK;
+    HR_REQUIRE(d && el && E_keV && event_key && out, "hr_cone_event_tab: null pointer");
+    HR_REQUIRE(cone->sin_c > 0.0 && cone->cos_c > 0.0 && cone->target_weight > 0.0 && cone->angle_scale >= 0.0,
+               "hr_cone_event_tab: bad cone parameters");
+    const hr_elsepa &t = *tables;
+    HR_REQUIRE(t.on == 1 && t.start && t.length && t.logE && t.mu && t.cdf && t.pdf && t.guide && t.n_mu >= 2 &&
+                   t.guide_cells >= 1 && (t.guide_cells & (t.guide_cells - 1)) == 0,
+               "hr_cone_event_tab: incomplete tables");
+    hipLaunchKernelGGL(cone_event_tab_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       *cone, t, n, d, el, E_keV, event_key, out);
     HR_CHECK_HIP(hipGetLastError());
     return HR_OK;
 }

@@ -75,15 +75,49 @@ __host__ __device__ __forceinline__ double invert_row(const hr_elsepa &t, int64_
     return t.mu[lo] + frac * dmu;
 }
 
+// Quantile of the ln-E mixture of rows `row` and `row + 1` (weight f on the upper) at xi.
+__host__ __device__ __forceinline__ double mix_mu(const hr_elsepa &t, int64_t row, double f, double xi) {
+    const double mu0 = invert_row(t, row, xi);
+    const double mu1 = invert_row(t, row + 1, xi);
+    return (1.0 - f) * mu0 + f * mu1;
+}
+
+// Absolute first row and fraction of the element row block `el` at E_keV.
+__host__ __device__ __forceinline__ void energy_rows(const hr_elsepa &t, size_t el, double E_keV, int64_t &row,
+                                                     double &f) {
+    const int64_t first = t.start[el];
+    int64_t lo;
+    bracket(t.logE, first, t.length[el], log(E_keV * 1.0e3), lo, f);
+    row = first + lo;
+}
+
 // mu of element row block `el` (index into start/length) at energy E_keV and uniform xi.
 __host__ __device__ __forceinline__ double sample_mu(const hr_elsepa &t, size_t el, double E_keV, double xi) {
-    const int64_t first = t.start[el], n = t.length[el];
-    int64_t lo;
+    int64_t row;
     double f;
-    bracket(t.logE, first, n, log(E_keV * 1.0e3), lo, f);
-    const double mu0 = invert_row(t, first + lo, xi);
-    const double mu1 = invert_row(t, first + lo + 1, xi);
-    return (1.0 - f) * mu0 + f * mu1;
+    energy_rows(t, el, E_keV, row, f);
+    return mix_mu(t, row, f, xi);
+}
+
+// Exact CDF of angular row `row` at mu = m: the piecewise-linear density's
+// quadratic inside the panel holding m, clamped to [0, 1]
+// (cone_split.py:177-197).
+__host__ __device__ __forceinline__ double cdf_at(const hr_elsepa &t, int64_t row, double m) {
+    const int n = t.n_mu;
+    if (m <= t.mu[0]) return 0.0;
+    if (m >= t.mu[n - 1]) return 1.0;
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (t.mu[mid] <= m) lo = mid;
+        else hi = mid;
+    }
+    const int64_t base = row * (int64_t)n;
+    const double dmu = t.mu[lo + 1] - t.mu[lo];
+    const double x = (m - t.mu[lo]) / dmu;
+    const double p0 = t.pdf[base + lo], p1 = t.pdf[base + lo + 1];
+    const double v = t.cdf[base + lo] + dmu * (p0 * x + 0.5 * (p1 - p0) * x * x);
+    return fmin(1.0, fmax(0.0, v));
 }
 
 // Pointers and sizes the tabulated angles read; checked before any launch.

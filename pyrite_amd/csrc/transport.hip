@@ -47,7 +47,7 @@ __device__ __forceinline__ int64_t claim(unsigned long long *next, bool want, in
 #define HR_TRANSPORT_OCCUPANCY
 #endif
 
-template <bool kMixed, bool kHard = false, bool kCone = false>
+template <bool kMixed, bool kHard = false, int kCone = 0>
 __global__ void __launch_bounds__(256) HR_TRANSPORT_OCCUPANCY transport_kernel(hr_transport_args a,
                                                         unsigned long long *__restrict__ next) {
     Lane s;
@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(256) HR_TRANSPORT_OCCUPANCY transport_kernel(h
                 drained = true;
             } else {
                 const int64_t e = a.e_start + i;
-                if constexpr (kCone) {
+                if constexpr (kCone != 0) {
                     a.cone.count[i] = 0;
                     a.cone.killed[i] = 0;
                 }
@@ -139,13 +139,14 @@ __global__ void __launch_bounds__(256) HR_TRANSPORT_OCCUPANCY continue_kernel(hr
 __global__ void reset_counter(unsigned long long *c) { *c = 0ULL; }
 
 // Resident 256-lane blocks per CU of transport_kernel, per variant (0 plain,
-// 1 hard-event modes, 2 cone splitting) and device.
-int g_occupancy[3][64];
+// 1 hard-event modes, 2 cone splitting, 3 cone splitting on tabulated angles)
+// and device.
+int g_occupancy[4][64];
 hr_status transport_occupancy(int block, int variant, int &occupancy) {
     int dev = 0;
     HR_CHECK_HIP(hipGetDevice(&dev));
     HR_REQUIRE(dev >= 0 && dev < 64, "device index %d out of range", dev);
-    HR_REQUIRE(variant >= 0 && variant < 3, "transport variant %d out of range", variant);
+    HR_REQUIRE(variant >= 0 && variant < 4, "transport variant %d out of range", variant);
     if (block == 256 && g_occupancy[variant][dev] > 0) {
         occupancy = g_occupancy[variant][dev];
         return HR_OK;
@@ -155,7 +156,10 @@ hr_status transport_occupancy(int block, int variant, int &occupancy) {
         HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, transport_kernel<false, true>, block, 0));
     else if (variant == 2)
         HR_CHECK_HIP(
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, transport_kernel<false, false, true>, block, 0));
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, transport_kernel<false, false, 1>, block, 0));
+    else if (variant == 3)
+        HR_CHECK_HIP(
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, transport_kernel<false, false, 2>, block, 0));
     else
         HR_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, transport_kernel<false>, block, 0));
     occupancy = blocks > 0 ? blocks : 1;
@@ -170,7 +174,6 @@ hr_status check_cone(const hr_transport_args &a, const char *who) {
     if (!c.on) return HR_OK;
     HR_REQUIRE(!a.hard.rad_on && !a.hard.shell_on, "%s: cone splitting is not combined with the hard-event modes",
                who);
-    HR_REQUIRE(!a.elsepa.on, "%s: cone splitting runs on the screened-Rutherford law, not tabulated angles", who);
     HR_REQUIRE(c.split_key && c.count && c.killed, "%s: cone splitting needs split keys, count and killed", who);
     HR_REQUIRE(c.cap >= 1 && c.pos && c.dir && c.E_keV && c.clock && c.weight && c.flight,
                "%s: cone splitting needs copy slots (cap >= 1)", who);
@@ -255,7 +258,8 @@ extern "C" hr_status hr_transport_run(const hr_transport_args *args, void *strea
     if (st != HR_OK) return st;
     // Whole wave64s, at most the kernel's launch bound of 256 lanes.
     const int block = a.block_size <= 0 ? 256 : (a.block_size >= 256 ? 256 : pick_block(a.block_size, 256));
-    st = transport_occupancy(block, hard ? 1 : cone ? 2 : 0, occupancy);
+    const bool cone_tab = cone && a.elsepa.on;
+    st = transport_occupancy(block, hard ? 1 : cone_tab ? 3 : cone ? 2 : 0, occupancy);
     if (st != HR_OK) return st;
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, a.work);
@@ -263,11 +267,14 @@ extern "C" hr_status hr_transport_run(const hr_transport_args *args, void *strea
     const int64_t need = (a.e_count + block - 1) / block;
     const int64_t resident = (int64_t)cu * occupancy;
     const int64_t grid = need < resident ? need : resident;
-    if (cone && a.precision == 1)
-        hipLaunchKernelGGL((transport_kernel<true, false, true>), dim3((unsigned)grid), dim3(block), 0, s, a, a.work);
+    if (cone_tab && a.precision == 1)
+        hipLaunchKernelGGL((transport_kernel<true, false, 2>), dim3((unsigned)grid), dim3(block), 0, s, a, a.work);
+    else if (cone_tab)
+        hipLaunchKernelGGL((transport_kernel<false, false, 2>), dim3((unsigned)grid), dim3(block), 0, s, a, a.work);
+    else if (cone && a.precision == 1)
+        hipLaunchKernelGGL((transport_kernel<true, false, 1>), dim3((unsigned)grid), dim3(block), 0, s, a, a.work);
     else if (cone)
-        hipLaunchKernelGGL((transport_kernel<false, false, true>), dim3((unsigned)grid), dim3(block), 0, s, a,
-                           a.work);
+        hipLaunchKernelGGL((transport_kernel<false, false, 1>), dim3((unsigned)grid), dim3(block), 0, s, a, a.work);
     else if (hard && a.precision == 1)
         hipLaunchKernelGGL((transport_kernel<true, true>), dim3((unsigned)grid), dim3(block), 0, s, a, a.work);
     else if (hard)

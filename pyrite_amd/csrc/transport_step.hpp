@@ -334,7 +334,10 @@ __device__ __forceinline__ void store_lane(const hr_transport_args &a, const Lan
 // reference's "mixed" transport precision: FP32 table lookups, query
 // logarithms, sqrt(1 - cos^2) and azimuth sin/cos; every accumulated quantity,
 // free-path draw and the straggling stay FP64.
-template <bool kRows = true, bool kMixed = false, bool kHard = false, bool kCone = false>
+// kCone: 0 no splitting; 1 copies on the screened-Rutherford law; 2 copies on
+// the tabulated (ELSEPA) law, a variant of its own so the bisection's
+// registers stay out of the other kernels.
+template <bool kRows = true, bool kMixed = false, bool kHard = false, int kCone = 0>
 __device__ int step_lane(const hr_transport_args &a, Lane &s) {
     const Lut<kMixed> lut{a};
     const int nE = a.lut_n_energy;
@@ -697,16 +700,22 @@ __device__ int step_lane(const hr_transport_args &a, Lane &s) {
         cos_t = 1.0 - 2.0 * alpha * cos_u / (1.0 + alpha - cos_u);
     }
     bool split_here = false;
-    if constexpr (kCone) {
+    if constexpr (kCone != 0) {
         // the in-cone copy of this event, from the incident direction and the
         // chosen element, on the (electron, flight) event key (cores.py:1189-1211)
         const hr_cone_split &c = a.cone;
         split_here = s.E >= c.min_energy_keV;
         if (split_here) {
             const int64_t fl = s.flight - 1;
-            const cone::Copy cp = cone::event(s.dx, s.dy, s.dz, c.nx, c.ny, c.nz, c.cos_c, c.sin_c,
-                                              c.target_weight, c.angle_scale, alpha, s.E,
-                                              cone::event_key(s.skey, fl));
+            cone::Copy cp;
+            if constexpr (kCone == 2) {  // the reference's law: tabulated angles of the chosen element
+                cone::TabLaw law(&a.elsepa, el_row + i_el, s.E);
+                cp = cone::event_law(s.dx, s.dy, s.dz, c.nx, c.ny, c.nz, c.cos_c, c.sin_c, c.target_weight,
+                                     c.angle_scale, law, s.E, cone::event_key(s.skey, fl));
+            } else {
+                cp = cone::event(s.dx, s.dy, s.dz, c.nx, c.ny, c.nz, c.cos_c, c.sin_c, c.target_weight,
+                                 c.angle_scale, alpha, s.E, cone::event_key(s.skey, fl));
+            }
             if (cp.made) {
                 const int64_t i = s.e - a.e_start;
                 const int64_t k = c.count[i];
@@ -728,7 +737,7 @@ __device__ int step_lane(const hr_transport_args &a, Lane &s) {
         }
     }
     rotate_lane<kMixed>(s, cos_t, 2.0 * kPi * phi_u);
-    if constexpr (kCone) {
+    if constexpr (kCone != 0) {
         // the analog outcome lies in the cone, which the copies represent: the
         // primary ends after this event (cores.py:1215-1226)
         if (split_here && s.dx * a.cone.nx + s.dy * a.cone.ny + s.dz * a.cone.nz >= a.cone.cos_c) {

@@ -25,7 +25,7 @@ import numpy as np
 import pytest
 
 from tests.test_case_gpu import _agree
-from tests.test_cone_split_oracle import HBN_50KEV, N_HAT, THETA_C
+from tests.test_cone_split_oracle import HBN_50KEV, HBN_50KEV_ELSEPA, N_HAT, THETA_C
 
 pytestmark = pytest.mark.gpu
 
@@ -197,19 +197,60 @@ def _in_cone_tally(rows, cnt, n, cap=4096):
     return np.bincount(ids, weights=L * inside, minlength=n), np.bincount(ids, weights=L, minlength=n)
 
 
-def test_split_in_cone_tally_matches_analog_on_device(device):
+def test_tabulated_single_event_copy_matches_oracle(oracle, device):
+    """hr_cone_event_tab (the reference's law: tabulated rows, band by
+    bisection on the mixed quantile) against the oracle on the same inputs."""
+    import ctypes as C
+
+    import torch
+
+    from pyrite_amd import _lib
+    from pyrite_amd.transport.core import DeviceTables
+    from pyrite_amd.transport.tables import build_transport_lut
+
+    lt, _ = HBN_50KEV_ELSEPA.tables()
+    lut = build_transport_lut(1.0, 50.0, lt)
+    tabs = DeviceTables(lt, lut, device)
+    rng = np.random.default_rng(8)
+    n = 4000
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    d[:50] = (0.0, 0.0, 1.0)
+    el = rng.integers(0, 2, n).astype(np.int64)  # B (screened-Rutherford shape) and N (vendor shape)
+    E = np.exp(rng.uniform(np.log(1.0), np.log(50.0), n))
+    keys = np.array([oracle.cone_event_key(int(k), f) for f, k in enumerate(rng.integers(0, 2**63, n))],
+                    dtype=np.uint64)
+    for target, u_ref in ((1e-3, 0.0), (0.3, 0.5)):
+        c = _cone_args((0.0, 0.0, 1.0), 0.7, target, u_ref)
+        out = torch.zeros((n, 5), dtype=torch.float64, device=device)
+        t = [torch.as_tensor(x, device=device).contiguous() for x in (d, el, E)]
+        kt = torch.as_tensor(keys.view(np.int64), device=device).contiguous()
+        _lib.check(_lib.load().hr_cone_event_tab(C.byref(c), C.byref(tabs.elsepa_args), n, t[0].data_ptr(),
+                                                 t[1].data_ptr(), t[2].data_ptr(), kt.data_ptr(), out.data_ptr(),
+                                                 None), "hr_cone_event_tab")
+        got = out.cpu().numpy()
+        made, ref = oracle.cone_event_tab(d, lt.elsepa, el, E, keys, dict(n_hat=(0, 0, 1), half_angle_rad=0.7,
+                                                                          target_weight=target, angle_scale=u_ref))
+        assert made.sum() > 100
+        np.testing.assert_array_equal(got[:, 0] == 1.0, made)
+        np.testing.assert_allclose(got[made, 1:4], ref[made, :3], rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(got[made, 4], ref[made, 3], rtol=1e-12)
+
+
+@pytest.mark.parametrize("case", [HBN_50KEV, HBN_50KEV_ELSEPA], ids=lambda c: c.name)
+def test_split_in_cone_tally_matches_analog_on_device(device, case):
     from pyrite_amd.cone_split import history_sums
 
     ne = 20000
     va, vb, ma, mb = [], [], [], []
     for seed in range(3):
-        rows, cnt, _, _, _, _, _ = _device_run(HBN_50KEV, device, 100 + seed, ne)
+        rows, cnt, _, _, _, _, _ = _device_run(case, device, 100 + seed, ne)
         ca, la = _in_cone_tally(rows, cnt, ne)
-        rows, cnt, _, cp, _, _, _ = _device_run(HBN_50KEV, device, 100 + seed, ne, cone=_request())
+        rows, cnt, _, cp, _, _, _ = _device_run(case, device, 100 + seed, ne, cone=_request())
         cb, lb = _in_cone_tally(rows, cnt, ne)
         nc = int(cp["weight"].numel())
         assert nc > 0
-        rc, cc, _, _, _, _, _ = _device_run(HBN_50KEV, device, 0, nc, keys=cp["stream_key"],
+        rc, cc, _, _, _, _, _ = _device_run(case, device, 0, nc, keys=cp["stream_key"],
                                             state=(cp["pos"], cp["dir"], cp["E_keV"], cp["clock"]))
         c2, l2 = _in_cone_tally(rc, cc, nc)
         par, w = cp["parent"].cpu().numpy(), cp["weight"].cpu().numpy()

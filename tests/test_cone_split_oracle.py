@@ -189,6 +189,10 @@ def test_policy_record():
 HBN_50KEV = Case("hbn50", 50.0, 300, [(0.0, 5.0e4, [("B", 0.0553), ("N", 0.0553)])], seed=3, E_cut=1.0,
                  finite=(5.0e5, 5.0e5))
 N_HAT = (1.0, 0.0, 0.0)
+# the same target on ELSEPA-format tables: N on the vendor DCS shape, B on the screened-Rutherford law's
+HBN_50KEV_ELSEPA = Case("hbn50_elsepa", 50.0, 300, [(0.0, 5.0e4, [("B", 0.0553), ("N", 0.0553)])], seed=3,
+                        E_cut=1.0, finite=(5.0e5, 5.0e5), elastic="elsepa", elsepa_energies=(0.8, 60.0, 40),
+                        elsepa_vendor=("N",))
 
 
 def _run(oracle, case, seed, ne, cone=None, keys=None, state=None, cap=4096):
@@ -292,3 +296,96 @@ def test_split_tallies_match_analog_within_statistical_error(oracle, options):
         # roulette keeps about 1 copy in 8 on this screened-Rutherford law and the
         # ratio is about 1.1: unbiased, not more efficient, so only the mean is held.
         assert np.sum(variances["split"], 0)[0] < np.sum(variances["analog"], 0)[0]
+
+
+# ---- the reference's own law: tabulated (ELSEPA) angles ---------------------------------------
+def _tab_group(dcs_rows, energies_eV=(1000.0, 2000.0)):
+    from pyrite_amd.tables.elsepa import pack_tables, read_dcs
+    from tests.transport_cases import ELSEPA_PANEL
+
+    panel = read_dcs(ELSEPA_PANEL)
+    E = np.asarray(energies_eV, dtype=float)
+    dcs = np.tile(panel.dcs_cm2_sr, (E.size, 1)) if dcs_rows is None else dcs_rows(panel.mu, E)
+    return pack_tables([[{"energy_eV": E, "total_elastic_cm2": np.full(E.size, 1e-16), "mu": panel.mu,
+                          "dcs_cm2_sr": dcs}]], [[1.0]]), panel
+
+
+def _tab_scores(oracle, group, direction, E_keV, target, count, theta=THETA_C, angle_scale=0.0):
+    keys = [oracle.cone_event_key(918273, i) for i in range(count)]
+    made, out = oracle.cone_event_tab(np.tile(direction, (count, 1)), group, 0, E_keV, keys,
+                                      _cone(theta=theta, target=target, angle_scale=angle_scale))
+    o, w = out[:, :3], out[:, 3]
+    assert np.all(w[made] > 0.0)
+    assert np.all(np.abs(np.linalg.norm(o[made], axis=1) - 1.0) < 1e-12)
+    assert np.all(o[made, 2] >= np.cos(theta) - 1e-12)
+    w = np.where(made, w, 0.0)
+    return np.stack([w, w * o[:, 2]], axis=1)
+
+
+@pytest.mark.parametrize("alpha", [0.02, 0.5])
+@pytest.mark.parametrize("angle", [0.1, 1.0, 2.5])
+@pytest.mark.parametrize("target", [1e-4, 0.2])
+def test_tabulated_law_reproduces_the_screened_rutherford_cap(oracle, alpha, angle, target):
+    """A table holding the screened-Rutherford density on the vendor grid: the
+    tabulated event (band bound from the rows' CDFs, band by bisection on the
+    mixed quantile) gives the law's cap probability and first moment."""
+    group, _ = _tab_group(lambda mu, E: np.tile(alpha * (1 + alpha) / (mu + alpha) ** 2, (E.size, 1)))
+    d = np.array([np.sin(angle), 0.0, np.cos(angle)])
+    scores = _tab_scores(oracle, group, d, 1.5, target, 12000)
+    p, q = _sr_cap_integrals(alpha, float(np.cos(angle)), THETA_C)
+    se = scores.std(axis=0, ddof=1) / np.sqrt(scores.shape[0])
+    assert abs(scores[:, 0].mean() - p) <= 6.0 * se[0] + 2e-4 * p, (scores[:, 0].mean(), p, se[0])
+    assert abs(scores[:, 1].mean() - q) <= 6.0 * se[1] + 2e-4 * abs(q), (scores[:, 1].mean(), q, se[1])
+
+
+@pytest.mark.parametrize("angle", [0.3, 1.2])
+def test_tabulated_event_weights_are_the_vendor_laws_cap_probability(oracle, angle):
+    """The vendor DCS between its two nodes (a quantile mixture in ln E): the
+    mean copy weight equals the cap probability computed by stratified
+    quadrature over the analog sampler's uniform, sum over xi of the arc
+    fraction half(theta(xi))/pi -- no band or bisection involved."""
+    group, _ = _tab_group(None)
+    E = 1.5
+    d = np.array([np.sin(angle), 0.0, np.cos(angle)])
+    xi = (np.arange(400_000) + 0.5) / 400_000
+    mu = oracle.elsepa_sample_mu(group, 0, E, xi)
+    cos_t = 1.0 - 2.0 * mu
+    sin_t = np.sqrt(np.maximum(0.0, 1.0 - cos_t**2))
+    cos_dn, sin_dn, cos_c = np.cos(angle), np.sin(angle), np.cos(THETA_C)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        c_star = (cos_c - cos_t * cos_dn) / (sin_t * sin_dn)
+    half = np.where(c_star <= -1.0, np.pi, np.where(c_star >= 1.0, 0.0, np.arccos(np.clip(c_star, -1.0, 1.0))))
+    p = (half / np.pi).mean()
+    assert p > 1e-4
+    scores = _tab_scores(oracle, group, d, E, 1e-3, 12000)
+    se = scores[:, 0].std(ddof=1) / np.sqrt(scores.shape[0])
+    assert abs(scores[:, 0].mean() - p) <= 6.0 * se + 1e-3 * p, (scores[:, 0].mean(), p, se)
+
+
+def test_split_tallies_match_analog_on_tabulated_laws(oracle):
+    """The split-vs-analog tally check on ELSEPA-format tables (N on the
+    vendor DCS shape): unbiased, in-cone variance lower."""
+    means = {"analog": [], "split": []}
+    variances = {"analog": [], "split": []}
+    ne = 2000
+    case = HBN_50KEV_ELSEPA
+    for seed in range(3):
+        rows, cnt, _, _ = _run(oracle, case, seed, ne)
+        cone_a, len_a = _tallies(rows, cnt, ne)
+        rows, cnt, _, st = _run(oracle, case, seed, ne, cone=_cone(N_HAT, target=0.05))
+        cone_b, len_b = _tallies(rows, cnt, ne)
+        cp = oracle.cone_collect(st["cone"])
+        nc = cp["weight"].size
+        assert nc > 0
+        rows_c, cnt_c, _, _ = _run(oracle, case, seed, nc, keys=cp["stream_key"],
+                                   state=(cp["pos"], cp["dir"], cp["E_keV"], cp["clock"]))
+        cone_c, len_c = _tallies(rows_c, cnt_c, nc)
+        cone_b = history_sums(cone_b, cone_c, cp["parent"], cp["weight"], ne)
+        len_b = history_sums(len_b, len_c, cp["parent"], cp["weight"], ne)
+        for label, values in (("analog", (cone_a, len_a)), ("split", (cone_b, len_b))):
+            means[label].append([v.mean() for v in values])
+            variances[label].append([v.var(ddof=1) / ne for v in values])
+    mean_a, mean_b = np.mean(means["analog"], axis=0), np.mean(means["split"], axis=0)
+    se = np.sqrt((np.sum(variances["analog"], 0) + np.sum(variances["split"], 0)) / 9.0)
+    assert np.all(np.abs(mean_a - mean_b) < 4.0 * se), (mean_a, mean_b, se)
+    assert np.sum(variances["split"], 0)[0] < np.sum(variances["analog"], 0)[0]

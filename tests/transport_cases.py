@@ -20,11 +20,11 @@ def sr_cross_section_cm2(Z, E_keV):
     return 5.21e-21 * Z * Z * 4.0 * np.pi / E_keV**2 / (a * (1.0 + a)) * rel**2
 
 
-def elsepa_test_tables(layers, energies_keV=None):
-    """ELSEPA-format tables on the vendor panel's angular grid: W carries the
-    vendor DCS shape (a real, non-screened-Rutherford law) at every energy,
-    other elements the screened-Rutherford law's own DCS; both with the
-    screened-Rutherford total cross section."""
+def elsepa_test_tables(layers, energies_keV=None, vendor=("W",)):
+    """ELSEPA-format tables on the vendor panel's angular grid: the elements in
+    ``vendor`` carry the vendor DCS shape (a real, non-screened-Rutherford law)
+    at every energy, the others the screened-Rutherford law's own DCS; all with
+    the screened-Rutherford total cross section."""
     from pyrite_amd.tables.elsepa import read_dcs
     from pyrite_amd.transport.elements import TRANSPORT_ELEMENTS
 
@@ -36,7 +36,7 @@ def elsepa_test_tables(layers, energies_keV=None):
         for el, _ in comp:
             Z = float(TRANSPORT_ELEMENTS[el]["Z"])
             sigma = sr_cross_section_cm2(Z, E)
-            if el == "W":
+            if el in vendor:
                 dcs = np.tile(panel.dcs_cm2_sr, (E.size, 1))
             else:
                 a = 3.4e-3 * Z**0.67 / E
@@ -61,9 +61,14 @@ class Case:
     finite: tuple | None = None
     elastic: str = "sr"
     atomic: str | None = None  # atomic_electron_deflection
+    elsepa_energies: tuple | None = None  # (lo keV, hi keV, nodes) of the ELSEPA test tables
+    elsepa_vendor: tuple = ("W",)  # elements on the vendor DCS shape
 
     def tables(self):
-        extra = {"elastic_tables": elsepa_test_tables(self.layers)} if self.elastic == "elsepa" else {}
+        extra = {}
+        if self.elastic == "elsepa":
+            E = None if self.elsepa_energies is None else np.geomspace(*self.elsepa_energies)
+            extra = {"elastic_tables": elsepa_test_tables(self.layers, E, self.elsepa_vendor)}
         lt = build_layer_tables(self.layers, self.elastic, MOTT_FIXTURE_DIR, self.atomic, **extra)
         lut = build_transport_lut(self.E_cut, self.E0, lt)
         return lt, lut
