@@ -653,6 +653,11 @@ typedef struct hr_beam_args {
     double basis[9];         /* beam-frame basis, row-major; columns 0, 1 carry (x', y') */
     double E0_keV;
     double *pos, *dirs, *E_keV, *clock; /* out, local index */
+    /* energy spread (beam_entry.py:251-297): E = E0 (1 + f z), z draw 0 of
+     * electron e's counter normals under energy_root (the spawn(6)[5] child)
+     * clamped to |z| <= 6; f = 0 leaves every E at E0 bit for bit */
+    double energy_spread_frac;
+    uint64_t energy_root;
 } hr_beam_args;
 hr_status hr_beam_entry(const hr_beam_args *args, void *stream);
 

@@ -76,3 +76,14 @@ def entry(args, n, start=0):
     else:
         dirs = np.tile(b, (n, 1))
     return pos, dirs
+
+
+def energies(args, n, start=0):
+    """Incident energies for the fields of a ``BeamArgs`` (beam_entry.py:264-297):
+    E0 (1 + f z), z draw 0 under the energy root clamped to |z| <= 6."""
+    E = np.full(n, float(args.E0_keV))
+    f = float(args.energy_spread_frac)
+    if f:
+        z = np.clip(counter_normals(args.energy_root, n, 1, start)[:, 0], -6.0, 6.0)
+        E = E * (1.0 + f * z)
+    return E

@@ -59,8 +59,9 @@ def run_case(pipe_like, start, n, threads=0, timings=None, dose_bins=0, coherent
 
     from . import beam
 
-    pos, dirs = beam.entry(beam_args(cfg.beam, cfg.seed, start, n, cfg.E0_keV, np.deg2rad(cfg.tilt_deg)), n, start)
-    E = np.full(n, cfg.E0_keV)
+    args = beam_args(cfg.beam, cfg.seed, start, n, cfg.E0_keV, np.deg2rad(cfg.tilt_deg))
+    pos, dirs = beam.entry(args, n, start)
+    E = beam.energies(args, n, start)
     E_cut = np.full(n, cfg.E_cut_keV)
     if cone is None:
         r, _ = _transport_all(pipe_like, keys, pos, dirs, E, E_cut, threads)

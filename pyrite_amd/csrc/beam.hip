@@ -29,6 +29,10 @@ __device__ __forceinline__ double counter_uniform(uint64_t root, uint64_t e, uin
 // Phi^-1(u) = -sqrt(2) erfcinv(2u) (the standard normal quantile, ndtri).
 __device__ __forceinline__ double normal_quantile(double u) { return -kSqrt2 * erfcinv(2.0 * u); }
 
+// |z| clamp of the energy-spread normal: E0 (1 + 6 |f|) is a strict upper energy
+// (beam_entry.py:244-248)
+constexpr double kSpreadClamp = 6.0;
+
 __global__ void __launch_bounds__(256) beam_entry_kernel(hr_beam_args p) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.n) return;
@@ -75,7 +79,12 @@ __global__ void __launch_bounds__(256) beam_entry_kernel(hr_beam_args p) {
     p.dirs[3 * i] = dx;
     p.dirs[3 * i + 1] = dy;
     p.dirs[3 * i + 2] = dz;
-    p.E_keV[i] = p.E0_keV;
+    double E = p.E0_keV;
+    if (p.energy_spread_frac != 0.0) {
+        const double z = fmin(kSpreadClamp, fmax(-kSpreadClamp, normal_quantile(counter_uniform(p.energy_root, e, 0))));
+        E = E * (1.0 + p.energy_spread_frac * z);
+    }
+    p.E_keV[i] = E;
     p.clock[i] = 0.0;
 }
 
@@ -92,6 +101,9 @@ extern "C" hr_status hr_beam_entry(const hr_beam_args *p, void *stream) {
     HR_REQUIRE(p->pos && p->dirs && p->E_keV && p->clock, "hr_beam_entry: null output");
     HR_REQUIRE(p->beam_dir[2] > 1e-6, "hr_beam_entry: beam must point into the slab");
     HR_REQUIRE(p->E0_keV > 0.0, "hr_beam_entry: E0 must be positive");
+    HR_REQUIRE(std::isfinite(p->energy_spread_frac) && fabs(p->energy_spread_frac) * kSpreadClamp < 1.0,
+               "hr_beam_entry: energy_spread_frac=%g could draw a non-positive energy (|f| * 6 must stay below 1; "
+               "the Gaussian spread model needs spread << 1)", p->energy_spread_frac);
     hipLaunchKernelGGL(beam_entry_kernel, dim3((unsigned)((p->n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                        *p);
     HR_CHECK_HIP(hipGetLastError());

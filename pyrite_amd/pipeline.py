@@ -20,7 +20,7 @@ from . import _lib
 from .backend import require_device
 from .spectrum.geometry import cone_detector, tilted_geometry
 from .spectrum.lines import BremScorer, LineScorer, line_tables, segments_fp32
-from .transport.beam import BeamSpec, fill_entry
+from .transport.beam import BeamSpec, fill_entry, upper_beam_energy_keV
 from .transport.core import (
     DeviceTables,
     PerElectronTransportConfig,
@@ -101,15 +101,17 @@ class CaseTables:
         self.layers = build_layer_tables([(0.0, cfg.thickness_ang, list(cfg.composition))], cfg.elastic_model,
                                          cfg.mott_tables_dir, cfg.atomic_electron_deflection,
                                          elastic_tables=cfg.elastic_tables)
+        # tables span the cutoff to the beam's strict upper energy (E0 with any spread)
+        E_top = upper_beam_energy_keV(cfg.E0_keV, cfg.beam.energy_spread_frac)
         if cfg.elastic_model == "mott":
             from .tables.mott import check_mott_coverage
 
-            check_mott_coverage(self.layers.compositions, cfg.E_cut_keV, cfg.E0_keV, cfg.mott_tables_dir)
+            check_mott_coverage(self.layers.compositions, cfg.E_cut_keV, E_top, cfg.mott_tables_dir)
         if cfg.elastic_model == "elsepa":
             from .tables.elsepa import check_coverage
 
-            check_coverage(self.layers.elsepa, cfg.E_cut_keV, cfg.E0_keV)
-        self.lut = build_transport_lut(cfg.E_cut_keV, cfg.E0_keV, self.layers)
+            check_coverage(self.layers.elsepa, cfg.E_cut_keV, E_top)
+        self.lut = build_transport_lut(cfg.E_cut_keV, E_top, self.layers)
         th, tilt = np.deg2rad(cfg.theta_obs_deg), np.deg2rad(cfg.tilt_deg)
         self.beam_dir, self.n_hat = tilted_geometry(th, tilt)
         self.n_hats, self.weights = cone_detector(th, tilt, cfg.cone_half_angle_rad, cfg.n_side)

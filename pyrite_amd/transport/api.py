@@ -34,6 +34,7 @@ from .core import (
     drive_batches,
     make_hip_transport_core,
 )
+from .beam import ENERGY_SPREAD_CLAMP_SIGMA, BeamSpec, beam_args, fill_entry, upper_beam_energy_keV
 from .tables import build_layer_tables, build_transport_lut
 
 MM_TO_ANG = 1.0e7
@@ -139,6 +140,7 @@ def simulate_trajectories(
     mott_tables_dir=None,
     atomic_electron_deflection=None,
     elastic_tables=None,
+    energy_spread_frac=None,
 ):
     """Transport ``Ne`` electrons of ``E0_keV`` into a slab (hip core).
 
@@ -147,6 +149,11 @@ def simulate_trajectories(
     addresses electrons ``[start, start+Ne)`` of a larger run: keys are
     ``splitmix64(seed + PHI*(e+1))`` of the global id, so blocks reproduce
     the matching slice of one run (``kinematics.py:226-245``).
+
+    ``energy_spread_frac``: RMS relative energy spread of the sampled beam,
+    ``E0 (1 + f z)`` with ``z`` clamped to ``|z| <= 6`` in the reference's
+    ``spawn(6)[5]`` namespace, drawn per electron on the device
+    (``api.py:161-166`` of the reference); the tables span to ``E0 (1 + 6|f|)``.
 
     Hard-event modes (the reference's ``inelastic_model`` and
     ``radiative_model`` switches, ``transport/api.py:56-60,212-250``):
@@ -176,6 +183,9 @@ def simulate_trajectories(
                                 elastic_tables=elastic_tables)
     finite = crystal_width_mm is not None and crystal_height_mm is not None
 
+    spread = float(energy_spread_frac or 0.0)
+    if spread and initial_state is not None:
+        raise ValueError("energy_spread_frac applies to the sampled beam, not to an initial_state")
     if initial_state is None:
         bd = normalized_beam_dir(beam_dir)
         pos_h = np.zeros((Ne, 3))
@@ -184,17 +194,24 @@ def simulate_trajectories(
     else:
         pos_h, dirs_h, E_h = (np.ascontiguousarray(x, dtype=np.float64) for x in initial_state)
     E_cut_h = np.full(Ne, float(E_cut_keV))
-    if np.any(E_cut_h >= E_h):
+    # with a spread the energies are drawn on the device below; the checks and the
+    # tables use its strict bounds E0 (1 -/+ 6 |f|) (beam_entry.py:251-261)
+    E_lo_h = E_h * (1.0 - ENERGY_SPREAD_CLAMP_SIGMA * abs(spread))
+    E_top = float(E_h.max()) if not spread else upper_beam_energy_keV(E0_keV, spread)
+    if np.any(E_cut_h >= E_lo_h):
         raise ValueError("each electron cutoff energy must be below its initial energy")
     if elastic_model == "mott":
         from ..tables.mott import check_mott_coverage
 
-        check_mott_coverage(tables.compositions, float(E_cut_h.min()), float(E_h.max()), mott_tables_dir)
+        check_mott_coverage(tables.compositions, float(E_cut_h.min()), E_top, mott_tables_dir)
     if elastic_model == "elsepa":
         from ..tables.elsepa import check_coverage
 
-        check_coverage(tables.elsepa, float(E_cut_h.min()), float(E_h.max()))
-    lut = build_transport_lut(float(E_cut_h.min()), float(E_h.max()), tables)
+        check_coverage(tables.elsepa, float(E_cut_h.min()), E_top)
+    spread_spec = BeamSpec(energy_spread_frac=spread)
+    if spread:
+        beam_args(spread_spec, seed, electron_start, Ne, E0_keV, 0.0)  # refuses a spread that could go non-positive
+    lut = build_transport_lut(float(E_cut_h.min()), E_top, tables)
     # host-side refusals above run before any device work
     backend = require_device()
     dev = backend.torch_device
@@ -225,9 +242,14 @@ def simulate_trajectories(
     dirs = torch.as_tensor(dirs_h, device=dev).contiguous()
     E = torch.as_tensor(E_h, device=dev).contiguous()
     clock = torch.zeros(Ne, dtype=torch.float64, device=dev)
+    if spread:  # the beam's energies, drawn per electron on the device (counter-addressed, block invariant)
+        scratch_pos, scratch_dirs = torch.empty_like(pos), torch.empty_like(dirs)
+        fill_entry(spread_spec, seed, electron_start, Ne, E0_keV, 0.0, scratch_pos, scratch_dirs, E, clock)
+        del scratch_pos, scratch_dirs
     E_cut = torch.as_tensor(E_cut_h, device=dev)
     alive = torch.ones(Ne, dtype=torch.uint8, device=dev)
     stragg = torch.zeros(Ne, dtype=torch.float64, device=dev)
+    initial_E = E.cpu().numpy() if spread else None  # before transport mutates E in place
     res = drive_batches(core, keys, (alive, clock, pos, dirs, E, E_cut), stragg, control, Ne, dev,
                         per_electron_config, extra_columns=() if hard is None else hard.row_columns)
     if raise_on_step_limit and res.counts["n_step_limited"]:
@@ -253,7 +275,7 @@ def simulate_trajectories(
         "n_layers": tables.n_layers,
         "elastic_model": elastic_model,
         "atomic_electron_deflection": atomic_electron_deflection,
-        "initial_E_keV": E_h.copy(),
+        "initial_E_keV": E_h.copy() if not spread else initial_E,
         "final_E_keV": E,
         "final_pos": pos,
         "final_dirs": dirs,

@@ -21,12 +21,20 @@ from ..spectrum.geometry import rotation_between
 
 MC2_KEV = 510.99895
 FWHM_TO_SIGMA = 1.0 / (2.0 * np.sqrt(2.0 * np.log(2.0)))
+#: |z| clamp of the energy-spread normal (``beam_entry.py:244-248``)
+ENERGY_SPREAD_CLAMP_SIGMA = 6.0
 
 
 def child_stream_root(seed: int, n_children: int, index: int) -> int:
     """Root of the ``SeedSequence(seed).spawn(n)[index]`` namespace (``kinematics.py:255-264``)."""
     child = np.random.SeedSequence(seed).spawn(n_children)[index]
     return int(child.generate_state(1, np.uint64)[0])
+
+
+def upper_beam_energy_keV(E0_keV, energy_spread_frac=None) -> float:
+    """Largest incident energy the spread can draw, ``E0 (1 + 6 |f|)``
+    (``beam_entry.py:251-261``): the top of the transport tables' range."""
+    return float(E0_keV) * (1.0 + ENERGY_SPREAD_CLAMP_SIGMA * abs(float(energy_spread_frac or 0.0)))
 
 
 def beam_entry_basis(tilt_polar_rad, tilt_azim_rad=0.0):
@@ -78,6 +86,9 @@ class BeamSpec:
 
     ``kind``: ``"pencil"``; ``"spot"`` (Gaussian spot of ``fwhm_mm`` /
     ``fwhm_y_mm``, collimated); ``"twiss"`` (planes ``twiss_x``, ``twiss_y``).
+    ``energy_spread_frac``: RMS relative energy spread, ``E = E0 (1 + f z)``
+    with ``z`` a standard normal clamped to ``|z| <= 6`` drawn in its own child
+    namespace (``beam_entry.py:264-297``); 0 is the monoenergetic beam.
     """
 
     kind: str = "pencil"
@@ -85,6 +96,7 @@ class BeamSpec:
     fwhm_y_mm: float | None = None
     twiss_x: TwissPlane | None = None
     twiss_y: TwissPlane | None = None
+    energy_spread_frac: float = 0.0
 
     @staticmethod
     def gaussian_waist(sigma_mm, divergence_rad, E0_keV):
@@ -125,6 +137,12 @@ def beam_args(spec: BeamSpec, seed, start, n, E0_keV, tilt_polar_rad, tilt_azim_
     for k in range(9):
         a.basis[k] = float(basis.reshape(-1)[k])
     a.E0_keV = float(E0_keV)
+    f = float(spec.energy_spread_frac or 0.0)
+    if not np.isfinite(f) or abs(f) * ENERGY_SPREAD_CLAMP_SIGMA >= 1.0:
+        raise ValueError(f"energy_spread_frac={f} could draw a non-finite or non-positive electron energy; the "
+                         "Gaussian spread model needs spread << 1")
+    a.energy_spread_frac = f
+    a.energy_root = child_stream_root(seed, 6, 5) if f else 0
     return a
 
 

@@ -50,12 +50,59 @@ def test_twiss_moments_and_projection():
 
 
 def test_child_roots_are_distinct():
-    assert len({child_stream_root(1, 2, 1), child_stream_root(1, 5, 4), child_stream_root(2, 5, 4)}) == 3
+    assert len({child_stream_root(1, 2, 1), child_stream_root(1, 5, 4), child_stream_root(2, 5, 4),
+                child_stream_root(1, 6, 5)}) == 4
+
+
+def test_energy_spread_draws():
+    """E = E0 (1 + f z), z the first counter normal of the spawn(6)[5] child:
+    mean E0 and RMS f E0 within sampling error, every draw inside the strict
+    bounds E0 (1 -/+ 6 f), blocks equal slices of one draw, f = 0 exactly E0."""
+    from oracle import beam
+    from pyrite_amd.transport.beam import upper_beam_energy_keV
+
+    E0, f, n = 5000.0, 0.01, 200_000
+    a = beam_args(BeamSpec(energy_spread_frac=f), 11, 0, n, E0, 0.0)
+    assert a.energy_root == child_stream_root(11, 6, 5)
+    E = beam.energies(a, n)
+    assert abs(E.mean() - E0) < 4.0 * f * E0 / np.sqrt(n)
+    assert E.std() == pytest.approx(f * E0, rel=0.01)
+    assert upper_beam_energy_keV(E0, f) == E0 * 1.06 and np.all(E < E0 * 1.06) and np.all(E > E0 * 0.94)
+    z = beam.counter_normals(a.energy_root, n, 1)[:, 0]
+    np.testing.assert_array_equal(E, E0 * (1.0 + f * np.clip(z, -6.0, 6.0)))
+    np.testing.assert_array_equal(beam.energies(beam_args(BeamSpec(energy_spread_frac=f), 11, 5, 3, E0, 0.0), 3, 5),
+                                  E[5:8])
+    mono = beam_args(BeamSpec(), 11, 0, 10, E0, 0.0)
+    assert mono.energy_spread_frac == 0.0 and np.all(beam.energies(mono, 10) == E0)
+    assert upper_beam_energy_keV(E0) == E0
+
+
+def test_energy_spread_refusals(tmp_path):
+    from pyrite_amd.transport.api import simulate_trajectories
+
+    with pytest.raises(ValueError, match="spread << 1"):
+        beam_args(BeamSpec(energy_spread_frac=0.2), 1, 0, 10, 5000.0, 0.0)
+    with pytest.raises(ValueError, match="not to an initial_state"):
+        simulate_trajectories(30.0, 2, 1e4, "Si", 0.05, energy_spread_frac=0.01,
+                              initial_state=(np.zeros((2, 3)), np.tile([0, 0, 1.0], (2, 1)), np.full(2, 30.0)))
+    # a Mott table ending at 300 keV refuses a 300 keV beam with a spread (test_mott_coverage.py:60)
+    from pyrite_amd.tables import mott
+
+    E = np.array([50.0, 1e3, 1e4, 3e5])
+    sig = mott.browning_sigma_cm2(14.0, E * 1e-3) * mott.first_moment(3.4e-3 * 14.0**0.67 / (E * 1e-3)) / mott.A0_SQ_CM2
+    (tmp_path / "DisplayCalcTCSTableForSi.csv").write_text(
+        "\n".join(f"{i}, {e:.17g}, {s:.17e}" for i, (e, s) in enumerate(zip(E, sig), 1)) + "\n")
+    with pytest.raises(ValueError, match=r'Si.*\[0.05, 300\] keV'):
+        simulate_trajectories(300.0, 16, 1.0, "Si", 0.05, E_cut_keV=0.05, elastic_model="mott",
+                              mott_tables_dir=tmp_path, energy_spread_frac=0.01)
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind,tilt", [("pencil", 0.5), ("spot", 0.0), ("spot", 1.0), ("twiss", 1.0)])
+@pytest.mark.parametrize("kind,tilt", [("pencil", 0.5), ("spot", 0.0), ("spot", 1.0), ("twiss", 1.0),
+                                       ("twiss_spread", 1.0)])
 def test_device_entry_matches_oracle(device, kind, tilt):
+    import dataclasses
+
     import torch
 
     from oracle import beam
@@ -63,7 +110,8 @@ def test_device_entry_matches_oracle(device, kind, tilt):
 
     E0 = 10_000.0
     spec = {"pencil": BeamSpec(), "spot": BeamSpec(kind="spot", fwhm_mm=2.3548),
-            "twiss": BeamSpec.gaussian_waist(1.0, 5e-4, E0)}[kind]
+            "twiss": BeamSpec.gaussian_waist(1.0, 5e-4, E0),
+            "twiss_spread": dataclasses.replace(BeamSpec.gaussian_waist(1.0, 5e-4, E0), energy_spread_frac=0.02)}[kind]
     n, start = 100_000, 1_000_000
     pos = torch.empty((n, 3), dtype=torch.float64, device=device)
     dirs = torch.empty((n, 3), dtype=torch.float64, device=device)
@@ -74,7 +122,13 @@ def test_device_entry_matches_oracle(device, kind, tilt):
     # erfcinv and scipy's ndtri agree to a few ulp; positions then inherit them
     np.testing.assert_allclose(pos.cpu().numpy(), want_p, rtol=1e-12, atol=1e-9)
     np.testing.assert_allclose(dirs.cpu().numpy(), want_d, rtol=1e-14, atol=1e-15)
-    assert torch.all(E == E0) and torch.all(clock == 0)
+    want_E = beam.energies(beam_args(spec, 3, start, n, E0, np.deg2rad(tilt)), n, start)
+    if spec.energy_spread_frac:
+        np.testing.assert_allclose(E.cpu().numpy(), want_E, rtol=1e-13)
+        assert E.std().item() == pytest.approx(0.02 * E0, rel=0.02)
+    else:
+        assert torch.all(E == E0)
+    assert torch.all(clock == 0)
 
 
 def test_allowed_reflections_follow_the_diamond_rule():

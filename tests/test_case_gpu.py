@@ -132,3 +132,17 @@ def test_hopg_mott_case_agrees_with_the_oracle_per_bin(device):
     c_lines, c_brem = _cpu_batches(pipe, first=20_000_000)
     _agree(g_brem, c_brem, "brem")
     _agree(g_lines, c_lines, "lines")
+
+
+def test_hopg_case_with_energy_spread_agrees_with_the_oracle_per_bin(device):
+    """A 2 % RMS energy spread (drawn per electron on the device and by the
+    oracle in the same namespace), device against oracle per bin."""
+    from pyrite_amd.pipeline import CaseConfig, CasePipeline
+    from pyrite_amd.transport.beam import BeamSpec
+
+    pipe = CasePipeline(CaseConfig(Ne=PER_BATCH, beam=BeamSpec(energy_spread_frac=0.02)))
+    assert pipe.lut.E_grid[-1] == pytest.approx(5000.0 * 1.12)
+    g_lines, g_brem = _gpu_batches(pipe)
+    c_lines, c_brem = _cpu_batches(pipe, first=30_000_000)
+    _agree(g_brem, c_brem, "brem")
+    _agree(g_lines, c_lines, "lines")

@@ -233,6 +233,23 @@ def test_sr_shaped_elsepa_tables_reproduce_sr_transport_on_the_device(device):
     assert l_el == pytest.approx(l_sr, rel=0.03)
 
 
+def test_simulate_trajectories_energy_spread(device):
+    """Energies drawn on the device in the reference's namespace (the oracle's
+    draw to 1e-13), block invariant through electron_start, transported."""
+    from oracle import beam
+    from pyrite_amd.transport.api import simulate_trajectories
+    from pyrite_amd.transport.beam import BeamSpec, beam_args
+
+    kw = dict(composition=[("Si", 0.04994)], seed=5, energy_spread_frac=0.01, keep_segments_on_device=False)
+    out = simulate_trajectories(30.0, 4000, 2.0e5, **kw)
+    want = beam.energies(beam_args(BeamSpec(energy_spread_frac=0.01), 5, 0, 4000, 30.0, 0.0), 4000)
+    np.testing.assert_allclose(out["initial_E_keV"], want, rtol=1e-13)
+    assert out["initial_E_keV"].std() == pytest.approx(0.3, rel=0.05)
+    assert out["n_cutoff_stopped"] + out["n_backscattered"] + out["n_transmitted"] == 4000
+    part = simulate_trajectories(30.0, 1000, 2.0e5, electron_start=3000, **kw)
+    np.testing.assert_array_equal(part["initial_E_keV"], out["initial_E_keV"][3000:])
+
+
 def test_simulate_trajectories_mott_endpoints_are_inclusive(device, tmp_path):
     """test_mott_coverage.py:45-53 on the device: a 300 keV beam over tables
     ending at 300 keV, cutoff at their 50 eV start, runs; the result reports the
