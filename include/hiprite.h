@@ -658,6 +658,10 @@ typedef struct hr_beam_args {
      * clamped to |z| <= 6; f = 0 leaves every E at E0 bit for bit */
     double energy_spread_frac;
     uint64_t energy_root;
+    /* axis of the collimated lab beam in the sample frame, along which lab
+     * spot offsets are carried to the z = 0 face (projected): the tilt's b
+     * (geometry.py:330-394), while beam_dir above sets the directions */
+    double axis[3];
 } hr_beam_args;
 hr_status hr_beam_entry(const hr_beam_args *args, void *stream);
 

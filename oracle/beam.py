@@ -62,11 +62,12 @@ def entry(args, n, start=0):
     else:
         off = np.zeros((n, 2))
     ex, ey, b = np.array(list(args.e_x)), np.array(list(args.e_y)), np.array(list(args.beam_dir))
-    if args.projected:
+    if args.projected:  # along the lab beam axis of the tilt
+        ax = np.array(list(args.axis))
         u, v = off[:, 0], off[:, 1]
-        s = -(u * ex[2] + v * ey[2]) / b[2]
-        pos[:, 0] = (u * ex[0] + v * ey[0]) + s * b[0]
-        pos[:, 1] = (u * ex[1] + v * ey[1]) + s * b[1]
+        s = -(u * ex[2] + v * ey[2]) / ax[2]
+        pos[:, 0] = (u * ex[0] + v * ey[0]) + s * ax[0]
+        pos[:, 1] = (u * ex[1] + v * ey[1]) + s * ax[1]
     else:
         pos[:, :2] = off
     if mode == 2:

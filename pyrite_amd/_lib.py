@@ -333,6 +333,7 @@ class BeamArgs(C.Structure):
         ("E0_keV", C.c_double),
         ("pos", _dp), ("dirs", _dp), ("E_keV", _dp), ("clock", _dp),
         ("energy_spread_frac", C.c_double), ("energy_root", C.c_uint64),
+        ("axis", C.c_double * 3),
     ]
 
 

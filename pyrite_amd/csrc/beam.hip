@@ -54,13 +54,13 @@ __global__ void __launch_bounds__(256) beam_entry_kernel(hr_beam_args p) {
     }
     double px = u, py = v;
     if (p.projected) {
-        // o = u e_x + v e_y; p0 = o + s* b with s* = -o_z / b_z
+        // o = u e_x + v e_y; p0 = o + s* b with s* = -o_z / b_z, b the lab beam axis
         const double ox = u * p.e_x[0] + v * p.e_y[0];
         const double oy = u * p.e_x[1] + v * p.e_y[1];
         const double oz = u * p.e_x[2] + v * p.e_y[2];
-        const double s = -oz / p.beam_dir[2];
-        px = ox + s * p.beam_dir[0];
-        py = oy + s * p.beam_dir[1];
+        const double s = -oz / p.axis[2];
+        px = ox + s * p.axis[0];
+        py = oy + s * p.axis[1];
     }
     p.pos[3 * i] = px;
     p.pos[3 * i + 1] = py;
@@ -100,6 +100,7 @@ extern "C" hr_status hr_beam_entry(const hr_beam_args *p, void *stream) {
                p->mode);
     HR_REQUIRE(p->pos && p->dirs && p->E_keV && p->clock, "hr_beam_entry: null output");
     HR_REQUIRE(p->beam_dir[2] > 1e-6, "hr_beam_entry: beam must point into the slab");
+    HR_REQUIRE(!p->projected || p->axis[2] > 1e-6, "hr_beam_entry: the projection axis must point into the slab");
     HR_REQUIRE(p->E0_keV > 0.0, "hr_beam_entry: E0 must be positive");
     HR_REQUIRE(std::isfinite(p->energy_spread_frac) && fabs(p->energy_spread_frac) * kSpreadClamp < 1.0,
                "hr_beam_entry: energy_spread_frac=%g could draw a non-positive energy (|f| * 6 must stay below 1; "

@@ -34,7 +34,7 @@ from .core import (
     drive_batches,
     make_hip_transport_core,
 )
-from .beam import ENERGY_SPREAD_CLAMP_SIGMA, BeamSpec, beam_args, fill_entry, upper_beam_energy_keV
+from .beam import ENERGY_SPREAD_CLAMP_SIGMA, beam_args, fill_entry, flat_beam_spec, upper_beam_energy_keV
 from .tables import build_layer_tables, build_transport_lut
 
 MM_TO_ANG = 1.0e7
@@ -110,6 +110,21 @@ def _hard_modes(tables, lut, dev, inelastic_model, inelastic_cutoff_eV, radiativ
     return DeviceHardModes(tables, lut, dev, rad=rad, shell=shell)
 
 
+def resolve_cutoffs_by_electron(E_cut_by_electrons, E_cut_keV, Ne):
+    """Per-electron cutoff energies [keV] as a checked ``(Ne,)`` array
+    (``arguments.py:124-138``): ``E_cut_keV`` for every electron unless
+    ``E_cut_by_electrons`` gives one each; all finite and positive."""
+    if E_cut_by_electrons is None:
+        out = np.full(int(Ne), float(E_cut_keV))
+    else:
+        out = np.asarray(E_cut_by_electrons, dtype=np.float64)
+        if out.shape != (int(Ne),):
+            raise ValueError(f"E_cut_by_electrons must have shape ({int(Ne)},), got {out.shape}")
+    if not np.all(np.isfinite(out)) or not np.all(out > 0.0):
+        raise ValueError("electron cutoff energies must be finite and strictly positive")
+    return np.ascontiguousarray(out)
+
+
 def simulate_trajectories(
     E0_keV,
     Ne,
@@ -141,6 +156,15 @@ def simulate_trajectories(
     atomic_electron_deflection=None,
     elastic_tables=None,
     energy_spread_frac=None,
+    E_cut_by_electrons=None,
+    beam_fwhm_mm=None,
+    beam_fwhm_y_mm=None,
+    transverse_distribution=None,
+    tilt_polar_rad=0.0,
+    tilt_azim_rad=0.0,
+    transport_precision="auto",
+    transport_lut_config=None,
+    _electron_start=None,
 ):
     """Transport ``Ne`` electrons of ``E0_keV`` into a slab (hip core).
 
@@ -149,6 +173,18 @@ def simulate_trajectories(
     addresses electrons ``[start, start+Ne)`` of a larger run: keys are
     ``splitmix64(seed + PHI*(e+1))`` of the global id, so blocks reproduce
     the matching slice of one run (``kinematics.py:226-245``).
+
+    Beam keywords as the reference's: ``beam_fwhm_mm`` / ``beam_fwhm_y_mm``
+    (Gaussian spot), ``transverse_distribution`` (a resolved Courant-Snyder
+    mapping, ``transverse.py:74-96``), ``tilt_polar_rad`` / ``tilt_azim_rad``
+    (the spot carried onto the tilted entrance face), directions about
+    ``beam_dir``; all drawn per electron on the device (``hr_beam_entry``).
+    ``transport_precision``: ``"auto"`` (FP64 here), ``"fp64"`` or ``"mixed"``.
+    ``transport_lut_config``: ``intervals_per_decade``, ``min_points``,
+    ``max_points`` of the LUT.
+
+    ``E_cut_by_electrons``: one cutoff [keV] per electron in place of
+    ``E_cut_keV`` (the tables span the lowest).
 
     ``energy_spread_frac``: RMS relative energy spread of the sampled beam,
     ``E0 (1 + f z)`` with ``z`` clamped to ``|z| <= 6`` in the reference's
@@ -183,17 +219,21 @@ def simulate_trajectories(
                                 elastic_tables=elastic_tables)
     finite = crystal_width_mm is not None and crystal_height_mm is not None
 
+    if _electron_start is not None:
+        electron_start = int(_electron_start)
+    if transport_precision not in ("auto", "fp64", "mixed"):
+        raise ValueError("transport_precision must be 'auto', 'fp64' or 'mixed'")
     spread = float(energy_spread_frac or 0.0)
-    if spread and initial_state is not None:
-        raise ValueError("energy_spread_frac applies to the sampled beam, not to an initial_state")
+    spec = flat_beam_spec(beam_fwhm_mm, beam_fwhm_y_mm, transverse_distribution, spread)
+    if initial_state is not None and (spec.kind != "pencil" or spread or tilt_polar_rad):
+        raise ValueError("energy_spread_frac, the spot, Courant-Snyder and tilt keywords apply to the sampled beam, "
+                         "not to an initial_state")
     if initial_state is None:
         bd = normalized_beam_dir(beam_dir)
-        pos_h = np.zeros((Ne, 3))
-        dirs_h = np.tile(bd, (Ne, 1))
         E_h = np.full(Ne, float(E0_keV))
     else:
         pos_h, dirs_h, E_h = (np.ascontiguousarray(x, dtype=np.float64) for x in initial_state)
-    E_cut_h = np.full(Ne, float(E_cut_keV))
+    E_cut_h = resolve_cutoffs_by_electron(E_cut_by_electrons, E_cut_keV, Ne)
     # with a spread the energies are drawn on the device below; the checks and the
     # tables use its strict bounds E0 (1 -/+ 6 |f|) (beam_entry.py:251-261)
     E_lo_h = E_h * (1.0 - ENERGY_SPREAD_CLAMP_SIGMA * abs(spread))
@@ -208,10 +248,11 @@ def simulate_trajectories(
         from ..tables.elsepa import check_coverage
 
         check_coverage(tables.elsepa, float(E_cut_h.min()), E_top)
-    spread_spec = BeamSpec(energy_spread_frac=spread)
-    if spread:
-        beam_args(spread_spec, seed, electron_start, Ne, E0_keV, 0.0)  # refuses a spread that could go non-positive
-    lut = build_transport_lut(float(E_cut_h.min()), E_top, tables)
+    if initial_state is None:  # refuses a spread that could go non-positive, before any device work
+        beam_args(spec, seed, electron_start, Ne, E0_keV, tilt_polar_rad, tilt_azim_rad, bd)
+    lut_kw = {} if transport_lut_config is None else {
+        k: v for k, v in dict(transport_lut_config).items() if k in ("intervals_per_decade", "min_points", "max_points")}
+    lut = build_transport_lut(float(E_cut_h.min()), E_top, tables, **lut_kw)
     # host-side refusals above run before any device work
     backend = require_device()
     dev = backend.torch_device
@@ -230,6 +271,7 @@ def simulate_trajectories(
         finite_footprint=finite,
         width_ang=float(crystal_width_mm) * MM_TO_ANG if finite else 0.0,
         height_ang=float(crystal_height_mm) * MM_TO_ANG if finite else 0.0,
+        precision="fp64" if transport_precision == "auto" else transport_precision,
     )
     core = make_hip_transport_core(dtabs, control, hard)
     if hard is not None:
@@ -238,18 +280,23 @@ def simulate_trajectories(
     keys = torch.empty(Ne, dtype=torch.uint64, device=dev)
     _lib.call("hr_stream_keys", int(seed) & (2**64 - 1), int(electron_start), Ne, keys.data_ptr(),
               None)
-    pos = torch.as_tensor(pos_h, device=dev).contiguous()
-    dirs = torch.as_tensor(dirs_h, device=dev).contiguous()
-    E = torch.as_tensor(E_h, device=dev).contiguous()
     clock = torch.zeros(Ne, dtype=torch.float64, device=dev)
-    if spread:  # the beam's energies, drawn per electron on the device (counter-addressed, block invariant)
-        scratch_pos, scratch_dirs = torch.empty_like(pos), torch.empty_like(dirs)
-        fill_entry(spread_spec, seed, electron_start, Ne, E0_keV, 0.0, scratch_pos, scratch_dirs, E, clock)
-        del scratch_pos, scratch_dirs
+    if initial_state is None:  # the sampled beam, drawn per electron on the device (counter-addressed)
+        pos = torch.empty((Ne, 3), dtype=torch.float64, device=dev)
+        dirs = torch.empty((Ne, 3), dtype=torch.float64, device=dev)
+        E = torch.empty(Ne, dtype=torch.float64, device=dev)
+        fill_entry(spec, seed, electron_start, Ne, E0_keV, tilt_polar_rad, pos, dirs, E, clock, tilt_azim_rad,
+                   beam_dir=bd)
+    else:
+        pos = torch.as_tensor(pos_h, device=dev).contiguous()
+        dirs = torch.as_tensor(dirs_h, device=dev).contiguous()
+        E = torch.as_tensor(E_h, device=dev).contiguous()
     E_cut = torch.as_tensor(E_cut_h, device=dev)
     alive = torch.ones(Ne, dtype=torch.uint8, device=dev)
     stragg = torch.zeros(Ne, dtype=torch.float64, device=dev)
-    initial_E = E.cpu().numpy() if spread else None  # before transport mutates E in place
+    # before transport mutates the state in place
+    initial_E = E.cpu().numpy() if initial_state is None else None
+    initial_r, initial_v = pos.cpu().numpy(), dirs.cpu().numpy()
     res = drive_batches(core, keys, (alive, clock, pos, dirs, E, E_cut), stragg, control, Ne, dev,
                         per_electron_config, extra_columns=() if hard is None else hard.row_columns)
     if raise_on_step_limit and res.counts["n_step_limited"]:
@@ -275,7 +322,9 @@ def simulate_trajectories(
         "n_layers": tables.n_layers,
         "elastic_model": elastic_model,
         "atomic_electron_deflection": atomic_electron_deflection,
-        "initial_E_keV": E_h.copy() if not spread else initial_E,
+        "initial_E_keV": E_h.copy() if initial_E is None else initial_E,
+        "initial_r_ang": initial_r,
+        "initial_v_hat": initial_v,
         "final_E_keV": E,
         "final_pos": pos,
         "final_dirs": dirs,

@@ -110,8 +110,13 @@ class BeamSpec:
         return BeamSpec(kind="twiss", twiss_x=plane, twiss_y=plane)
 
 
-def beam_args(spec: BeamSpec, seed, start, n, E0_keV, tilt_polar_rad, tilt_azim_rad=0.0):
-    """``hr_beam_args`` (outputs unset) for electrons ``[start, start+n)``."""
+def beam_args(spec: BeamSpec, seed, start, n, E0_keV, tilt_polar_rad, tilt_azim_rad=0.0, beam_dir=None):
+    """``hr_beam_args`` (outputs unset) for electrons ``[start, start+n)``.
+
+    The lab spot is carried to the entrance face along the tilt's beam axis;
+    the directions follow ``beam_dir`` (default: that axis), as the
+    reference's flat entry takes a slab-frame ``beam_dir`` beside the tilt
+    (``beam_entry.py:347-359``, ``geometry.py:330-394``)."""
     a = _lib.BeamArgs()
     a.start, a.n = int(start), int(n)
     e_x, e_y, b = beam_entry_basis(tilt_polar_rad, tilt_azim_rad)
@@ -131,9 +136,10 @@ def beam_args(spec: BeamSpec, seed, start, n, E0_keV, tilt_polar_rad, tilt_azim_
     else:
         raise ValueError(f"unknown beam kind {spec.kind!r}")
     a.projected = int(bool(tilt_polar_rad) and a.mode != 0)
-    for dst, src in ((a.e_x, e_x), (a.e_y, e_y), (a.beam_dir, b)):
+    d = b if beam_dir is None else np.asarray(beam_dir, dtype=np.float64)  # a unit vector from the caller
+    for dst, src in ((a.e_x, e_x), (a.e_y, e_y), (a.beam_dir, d), (a.axis, b)):
         dst[0], dst[1], dst[2] = (float(v) for v in src)
-    basis = beam_frame_basis(b)
+    basis = beam_frame_basis(d)
     for k in range(9):
         a.basis[k] = float(basis.reshape(-1)[k])
     a.E0_keV = float(E0_keV)
@@ -147,12 +153,39 @@ def beam_args(spec: BeamSpec, seed, start, n, E0_keV, tilt_polar_rad, tilt_azim_
 
 
 def fill_entry(spec: BeamSpec, seed, start, n, E0_keV, tilt_polar_rad, pos, dirs, E_keV, clock,
-               tilt_azim_rad=0.0, stream=None):
+               tilt_azim_rad=0.0, stream=None, beam_dir=None):
     """Write entry states of electrons ``[start, start+n)`` into device tensors."""
-    a = beam_args(spec, seed, start, n, E0_keV, tilt_polar_rad, tilt_azim_rad)
+    a = beam_args(spec, seed, start, n, E0_keV, tilt_polar_rad, tilt_azim_rad, beam_dir)
     a.pos, a.dirs, a.E_keV, a.clock = pos.data_ptr(), dirs.data_ptr(), E_keV.data_ptr(), clock.data_ptr()
     if stream is None:
         import torch
 
         stream = torch.cuda.current_stream().cuda_stream
     _lib.check(_lib.load().hr_beam_entry(C.byref(a), C.c_void_p(stream)), "hr_beam_entry")
+
+
+def plane_from_resolved(plane) -> TwissPlane:
+    """A ``TwissPlane`` from the reference's resolved plane mapping
+    (``transverse.py:74-87``: geometric emittance [mm rad], beta [mm/rad], alpha)."""
+    eps = float(plane["geometric_emittance_mm_rad"])
+    beta = float(plane["beta_twiss_mm_per_rad"])
+    return TwissPlane(float(np.sqrt(eps * beta)), float(np.sqrt(eps / beta)), float(plane["alpha_twiss"]))
+
+
+def flat_beam_spec(beam_fwhm_mm=None, beam_fwhm_y_mm=None, transverse_distribution=None,
+                   energy_spread_frac=None) -> BeamSpec:
+    """The ``BeamSpec`` of the reference's flat beam keywords
+    (``beam_entry.py:23-80``): a resolved Courant-Snyder mapping, else a
+    Gaussian spot of FWHM ``beam_fwhm_mm`` (``beam_fwhm_y_mm`` for y), else
+    the pencil."""
+    f = float(energy_spread_frac or 0.0)
+    if transverse_distribution is not None:
+        if beam_fwhm_mm or beam_fwhm_y_mm:
+            raise ValueError("transverse_distribution is incompatible with the spot FWHM fields")
+        td = transverse_distribution
+        return BeamSpec(kind="twiss", twiss_x=plane_from_resolved(td["x"]), twiss_y=plane_from_resolved(td["y"]),
+                        energy_spread_frac=f)
+    if beam_fwhm_mm or beam_fwhm_y_mm:
+        return BeamSpec(kind="spot", fwhm_mm=float(beam_fwhm_mm or 0.0),
+                        fwhm_y_mm=None if beam_fwhm_y_mm is None else float(beam_fwhm_y_mm), energy_spread_frac=f)
+    return BeamSpec(energy_spread_frac=f)

@@ -142,3 +142,38 @@ def test_allowed_reflections_follow_the_diamond_rule():
     refl = allowed_reflections("silicon", 16)
     assert len(refl) == 16 and len(set(refl)) == 16
     assert sum(abs(h) == 1 for h in (r[0] for r in refl)) == 8  # the eight {111}, then {220}
+
+
+def test_cutoffs_by_electron_are_checked():
+    """arguments.py:124-138: shape (Ne,), finite and positive; absent means E_cut_keV each."""
+    from pyrite_amd.transport.api import resolve_cutoffs_by_electron, simulate_trajectories
+
+    np.testing.assert_array_equal(resolve_cutoffs_by_electron(None, 5.0, 3), [5.0, 5.0, 5.0])
+    np.testing.assert_array_equal(resolve_cutoffs_by_electron([1.0, 2.0], 5.0, 2), [1.0, 2.0])
+    with pytest.raises(ValueError, match=r"shape \(3,\)"):
+        resolve_cutoffs_by_electron([1.0, 2.0], 5.0, 3)
+    with pytest.raises(ValueError, match="finite and strictly positive"):
+        resolve_cutoffs_by_electron([1.0, 0.0], 5.0, 2)
+    with pytest.raises(ValueError, match="below its initial energy"):
+        simulate_trajectories(30.0, 2, 1e4, "Si", 0.05, E_cut_by_electrons=[5.0, 30.0])
+
+
+def test_flat_beam_keywords_map_onto_the_spec():
+    from pyrite_amd.transport.beam import flat_beam_spec, plane_from_resolved
+
+    assert flat_beam_spec().kind == "pencil"
+    s = flat_beam_spec(beam_fwhm_mm=1.0)
+    assert s.kind == "spot" and s.fwhm_mm == 1.0 and s.fwhm_y_mm is None
+    p = plane_from_resolved({"geometric_emittance_mm_rad": 1e-4, "beta_twiss_mm_per_rad": 4000.0, "alpha_twiss": 0.5})
+    assert p.sigma_pos_mm == pytest.approx(np.sqrt(0.4)) and p.slope_scale_rad == pytest.approx(np.sqrt(2.5e-8))
+    plane = {"geometric_emittance_mm_rad": 1e-4, "beta_twiss_mm_per_rad": 4000.0, "alpha_twiss": 0.5}
+    with pytest.raises(ValueError, match="incompatible with the spot FWHM"):
+        flat_beam_spec(beam_fwhm_mm=1.0, transverse_distribution={"x": plane, "y": plane})
+    # a spot's x offset draw is the spot's counter normal 0 in spawn(2)[1] (test above), so the pencil
+    # stays bit for bit: zero offsets and the caller's direction
+    from oracle import beam
+
+    a = beam_args(flat_beam_spec(), 3, 0, 5, 100.0, 0.3, 0.0, np.array([0.0, 0.6, 0.8]))
+    pos, dirs = beam.entry(a, 5)
+    np.testing.assert_array_equal(pos, 0.0)
+    np.testing.assert_array_equal(dirs, np.tile([0.0, 0.6, 0.8], (5, 1)))

@@ -250,6 +250,58 @@ def test_simulate_trajectories_energy_spread(device):
     np.testing.assert_array_equal(part["initial_E_keV"], out["initial_E_keV"][3000:])
 
 
+@pytest.mark.parametrize("beam", ["pencil", "spot", "twiss"])
+def test_simulate_trajectories_beam_keywords_match_the_oracle_entry(device, beam):
+    """The reference's flat beam keywords (spot FWHM, resolved Courant-Snyder
+    mapping, tilt onto the entrance face, directions about beam_dir): the
+    device entry states against the oracle's restatement of the same draw."""
+    from oracle import beam as obeam
+    from pyrite_amd.transport.api import simulate_trajectories
+    from pyrite_amd.transport.beam import beam_args, flat_beam_spec
+
+    tilt, azim = 0.4, 0.3
+    bd = np.array([np.sin(0.2), 0.0, np.cos(0.2)])
+    td = {"beta_gamma": 20.0,
+          "x": {"geometric_emittance_mm_rad": 1e-4, "beta_twiss_mm_per_rad": 2000.0, "alpha_twiss": -0.5,
+                "sigma_position_mm": 0.45, "sigma_slope_rad": 2.5e-4},
+          "y": {"geometric_emittance_mm_rad": 5e-5, "beta_twiss_mm_per_rad": 1000.0, "alpha_twiss": 0.2,
+                "sigma_position_mm": 0.22, "sigma_slope_rad": 2.3e-4}}
+    kw = {"pencil": {}, "spot": {"beam_fwhm_mm": 1.0, "beam_fwhm_y_mm": 0.5},
+          "twiss": {"transverse_distribution": td}}[beam]
+    n, start = 3000, 700
+    out = simulate_trajectories(10_000.0, n, 1.0e5, composition=[("Si", 0.04994)], seed=21, beam_dir=bd,
+                                tilt_polar_rad=tilt, tilt_azim_rad=azim, electron_start=start,
+                                keep_segments_on_device=False, **kw)
+    spec = flat_beam_spec(**{k: v for k, v in kw.items()})
+    a = beam_args(spec, 21, start, n, 10_000.0, tilt, azim, bd / np.linalg.norm(bd))
+    want_p, want_d = obeam.entry(a, n, start)
+    np.testing.assert_allclose(out["initial_r_ang"], want_p, rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(out["initial_v_hat"], want_d, rtol=1e-14, atol=1e-15)
+    if beam == "pencil":
+        np.testing.assert_array_equal(out["initial_r_ang"], 0.0)
+        np.testing.assert_array_equal(out["initial_v_hat"], np.tile(bd / np.linalg.norm(bd), (n, 1)))
+
+
+def test_simulate_trajectories_cutoffs_by_electron(device):
+    """Per-electron cutoffs: cutoff-stopped histories end at their own cutoff,
+    and electrons sharing a cutoff transport as in a run with that cutoff
+    (tables differ only when the lowest cutoff moves, so the low group is
+    compared with a run at the low cutoff)."""
+    from pyrite_amd.transport.api import simulate_trajectories
+
+    n = 2000
+    cuts = np.where(np.arange(n) % 2 == 0, 5.0, 10.0)
+    kw = dict(composition=[("Si", 0.04994)], seed=9, keep_segments_on_device=False)
+    out = simulate_trajectories(30.0, n, 2.0e5, E_cut_by_electrons=cuts, **kw)
+    stopped = out["exit_code"] == 0
+    assert stopped.any()
+    np.testing.assert_array_equal(out["final_E_keV"][stopped], cuts[stopped])
+    low = simulate_trajectories(30.0, n, 2.0e5, E_cut_keV=5.0, **kw)
+    even = np.arange(n) % 2 == 0
+    np.testing.assert_array_equal(out["seg_count"][even], low["seg_count"][even])
+    np.testing.assert_array_equal(out["final_E_keV"][even], low["final_E_keV"][even])
+
+
 def test_simulate_trajectories_mott_endpoints_are_inclusive(device, tmp_path):
     """test_mott_coverage.py:45-53 on the device: a 300 keV beam over tables
     ending at 300 keV, cutoff at their 50 eV start, runs; the result reports the
