@@ -339,7 +339,8 @@ class ConeSplitCore:
 
     def copies(self):
         """Every collected copy of the calls since ``reset()``, in (electron, event)
-        order, ``parent`` as a global electron index of the run."""
+        order; ``parent`` indexes the primaries of the driven run (each
+        sub-batch's local parent plus the sub-batch start)."""
         import torch
 
         recs = [self.records[k] for k in sorted(self.records)]

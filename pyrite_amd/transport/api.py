@@ -165,6 +165,7 @@ def simulate_trajectories(
     transport_precision="auto",
     transport_lut_config=None,
     _electron_start=None,
+    _cone_split=None,
 ):
     """Transport ``Ne`` electrons of ``E0_keV`` into a slab (hip core).
 
@@ -273,7 +274,16 @@ def simulate_trajectories(
         height_ang=float(crystal_height_mm) * MM_TO_ANG if finite else 0.0,
         precision="fp64" if transport_precision == "auto" else transport_precision,
     )
-    core = make_hip_transport_core(dtabs, control, hard)
+    if _cone_split is not None:
+        from ..cone_split import ConeSplit, ConeSplitCore
+
+        if not isinstance(_cone_split, ConeSplit):
+            raise TypeError("_cone_split must be a pyrite_amd.cone_split.ConeSplit request")
+        if hard is not None:
+            raise ValueError("detector-cone splitting is not combined with the hard-event modes")
+        core = ConeSplitCore(dtabs, control, _cone_split, dev)
+    else:
+        core = make_hip_transport_core(dtabs, control, hard)
     if hard is not None:
         hard.set_keys(seed, electron_start, Ne)
 
@@ -299,6 +309,58 @@ def simulate_trajectories(
     initial_r, initial_v = pos.cpu().numpy(), dirs.cpu().numpy()
     res = drive_batches(core, keys, (alive, clock, pos, dirs, E, E_cut), stragg, control, Ne, dev,
                         per_electron_config, extra_columns=() if hard is None else hard.row_columns)
+    out = _assemble(res, Ne, tables, control, finite, elastic_model, atomic_electron_deflection,
+                    E_h.copy() if initial_E is None else initial_E, initial_r, initial_v, E, pos, dirs, stragg,
+                    straggling, energy_model, hard, raise_on_step_limit, max_steps)
+    if _cone_split is not None:
+        rec = _copy_generation(core, electron_start, E_cut, dtabs, control, dev, per_electron_config, tables, finite,
+                               elastic_model, atomic_electron_deflection, straggling, energy_model,
+                               raise_on_step_limit, max_steps)
+        if rec["copies"] is not None and not keep_segments_on_device:
+            rec["copies"] = _to_host(rec["copies"])
+        out["cone_split"] = rec
+    return out if keep_segments_on_device else _to_host(out)
+
+
+def _copy_generation(core, electron_start, E_cut, dtabs, control, dev, config, tables, finite, elastic_model,
+                     atomic_electron_deflection, straggling, energy_model, raise_on_step_limit, max_steps):
+    """The primaries' copy records and the copies transported once, analog, as
+    one generation with their own stream keys and their parents' cutoffs
+    (``cone_split.py:457-513``); ``parent`` indexes this call's primaries."""
+    import torch
+
+    cp = core.copies()
+    if cp is None:
+        return {"parent": np.zeros(0, dtype=np.int64), "copies": None, "electron_start": int(electron_start)}
+    # copies() offsets each sub-batch's parents by the sub-batch start within this
+    # call, so they already index this call's primaries (0 .. Ne-1)
+    parent = cp["parent"]
+    parent_h = parent.cpu().numpy()
+    if parent_h.size and (parent_h.min() < 0 or parent_h.max() >= E_cut.numel()):
+        raise RuntimeError("cone splitting: copy parents outside this call's primaries")
+    rec = {"parent": parent_h, "flight": cp["flight"].cpu().numpy(),
+           "weight": cp["weight"].cpu().numpy(), "stream_keys": cp["stream_key"].cpu().numpy(),
+           "r_ang": cp["pos"].cpu().numpy(), "v_hat": cp["dir"].cpu().numpy(), "E_keV": cp["E_keV"].cpu().numpy(),
+           "t_ang": cp["clock"].cpu().numpy(), "killed": cp["killed"].cpu().numpy().astype(bool),
+           "electron_start": int(electron_start), "copies": None}
+    n = int(parent.numel())
+    if n == 0:
+        return rec
+    state = (torch.ones(n, dtype=torch.uint8, device=dev), cp["clock"].clone(), cp["pos"].clone(),
+             cp["dir"].clone(), cp["E_keV"].clone(), E_cut[parent].contiguous())
+    stragg = torch.zeros(n, dtype=torch.float64, device=dev)
+    E0s, r0, v0 = rec["E_keV"].copy(), rec["r_ang"].copy(), rec["v_hat"].copy()
+    res = drive_batches(make_hip_transport_core(dtabs, control), cp["stream_key"], state, stragg, control, n, dev,
+                        config)
+    rec["copies"] = _assemble(res, n, tables, control, finite, elastic_model, atomic_electron_deflection, E0s, r0, v0,
+                              state[4], state[2], state[3], stragg, straggling, energy_model, None,
+                              raise_on_step_limit, max_steps)
+    return rec
+
+
+def _assemble(res, Ne, tables, control, finite, elastic_model, atomic_electron_deflection, initial_E, initial_r,
+              initial_v, E, pos, dirs, stragg, straggling, energy_model, hard, raise_on_step_limit, max_steps):
+    """The result mapping of one transport (the reference's segment keys)."""
     if raise_on_step_limit and res.counts["n_step_limited"]:
         raise TransportStepLimitError(
             f"{res.counts['n_step_limited']} histories reached max_steps={max_steps}"
@@ -322,7 +384,7 @@ def simulate_trajectories(
         "n_layers": tables.n_layers,
         "elastic_model": elastic_model,
         "atomic_electron_deflection": atomic_electron_deflection,
-        "initial_E_keV": E_h.copy() if initial_E is None else initial_E,
+        "initial_E_keV": initial_E,
         "initial_r_ang": initial_r,
         "initial_v_hat": initial_v,
         "final_E_keV": E,
@@ -352,6 +414,8 @@ def simulate_trajectories(
         out["hard_radiative_Z"] = r["seg_rad_Z"]
         out["radiative"] = {"model": "analytic-soft-hard", "cutoff_eV": hard.rad.cutoff_eV,
                             "plasma_energy_eV_by_layer": tuple(float(x) for x in hard.rad.plasma)}
-    if not keep_segments_on_device:
-        out = {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in out.items()}
     return out
+
+
+def _to_host(out):
+    return {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in out.items()}

@@ -382,3 +382,52 @@ def test_overlapped_cone_steps_equal_back_to_back(device):
     assert ca["copies"] > 0 and ca == {k: o.cone_stats[k] for k in ca}
     np.testing.assert_array_equal(la, lo)
     np.testing.assert_array_equal(ba, bo)
+
+
+def test_flat_entry_cone_split_record_blocks_and_tally(device):
+    """simulate_trajectories(_cone_split=ConeSplit) as the reference's
+    transport_cone_split: the primaries' result carries the copy records and
+    the copies' own transport; parents index the call's primaries (block-local
+    under electron_start, and a block equals that slice of the single call);
+    primary plus weighted copies keep the in-cone path tally of analog."""
+    from pyrite_amd.cone_split import ConeSplit, history_sums
+    from pyrite_amd.transport.api import simulate_trajectories
+
+    comp = [("B", 0.0553), ("N", 0.0553)]
+    kw = dict(composition=comp, E_cut_keV=1.0, crystal_width_mm=50.0, crystal_height_mm=50.0,
+              keep_segments_on_device=False)
+    req = ConeSplit(N_HAT, THETA_C, 0.05, max_copies_per_electron=1024, weight_ladder=True)
+    full = simulate_trajectories(50.0, 3000, 5.0e4, seed=4, _cone_split=req, **kw)
+    rec = full["cone_split"]
+    assert rec["parent"].size > 50 and rec["copies"]["Ne"] == rec["parent"].size
+    assert np.all(full["exit_code"][rec["killed"]] == 6)
+    np.testing.assert_array_equal(rec["copies"]["initial_r_ang"], rec["r_ang"])
+    assert np.all(rec["v_hat"] @ np.asarray(N_HAT) >= math.cos(THETA_C) - 1e-12)
+    block = simulate_trajectories(50.0, 1000, 5.0e4, seed=4, _cone_split=req, electron_start=2000, **kw)
+    b = block["cone_split"]
+    sel = rec["parent"] >= 2000
+    np.testing.assert_array_equal(b["parent"], rec["parent"][sel] - 2000)
+    for name in ("weight", "stream_keys", "r_ang", "v_hat", "E_keV"):
+        np.testing.assert_array_equal(b[name], rec[name][sel], err_msg=name)
+
+    ma, mb, va, vb = [], [], [], []
+    n = 10_000
+    for seed in range(3):
+        a = simulate_trajectories(50.0, n, 5.0e4, seed=50 + seed, **kw)
+        s = simulate_trajectories(50.0, n, 5.0e4, seed=50 + seed, _cone_split=req, **kw)
+        ta = np.bincount(a["electron_id"], weights=a["L_ang"] * (a["v_hat"] @ np.asarray(N_HAT) >= math.cos(THETA_C)),
+                         minlength=n)
+        tp = np.bincount(s["electron_id"], weights=s["L_ang"] * (s["v_hat"] @ np.asarray(N_HAT) >= math.cos(THETA_C)),
+                         minlength=n)
+        c = s["cone_split"]
+        co = c["copies"]
+        tc = np.bincount(co["electron_id"], weights=co["L_ang"] * (co["v_hat"] @ np.asarray(N_HAT) >= math.cos(THETA_C)),
+                         minlength=c["parent"].size)
+        tb = history_sums(tp, tc, c["parent"], c["weight"], n)
+        ma.append(ta.mean())
+        mb.append(tb.mean())
+        va.append(ta.var(ddof=1) / n)
+        vb.append(tb.var(ddof=1) / n)
+    se = math.sqrt((sum(va) + sum(vb)) / 9.0)
+    assert abs(np.mean(ma) - np.mean(mb)) < 4.0 * se, (np.mean(ma), np.mean(mb), se)
+    assert sum(vb) < sum(va)
