@@ -57,7 +57,10 @@ def parse():
     p.add_argument("--no-overlap", action="store_true", help="run transport and scoring of successive batches "
                                                               "back to back on one stream")
     p.add_argument("--no-fom", action="store_true", help="skip the learned-importance figure of merit")
-    p.add_argument("--fom-groups", type=int, default=48, help="FOM sample units per arm (groups of batches)")
+    p.add_argument("--fom-groups", type=int, default=96,
+                   help="FOM sample units per arm (groups of batches); 96 x 16 x 65536 = 1.0e8 histories per arm")
+    p.add_argument("--fom-train-batches", type=int, default=153,
+                   help="batches whose copies train the learned roulette (153 x 65536 = 1.0e7 histories)")
     p.add_argument("--fom-group-batches", type=int, default=16, help="batches per FOM group")
     p.add_argument("--fom-histories", type=int, default=65536, help="histories per FOM batch")
     p.add_argument("--fom-sequential", action="store_true", help="FOM arms back to back on one stream")
@@ -107,7 +110,7 @@ def _fom_arm(pipe, window, dE, first, n_groups, per_group, label, overlap=True):
     return vals, time.perf_counter() - t0
 
 
-def learned_fom(pipe_cfg, n_groups=48, per_group=16, Ne=65536, overlap=True):
+def learned_fom(pipe_cfg, n_groups=96, per_group=16, Ne=65536, overlap=True, train_batches=153):
     """PXR-peak figure of merit 1/(rel_err^2 t) of three estimators of the same case:
     analog; detector-cone splitting with the reference policy's defaults; and
     cone splitting without the angle roulette, its copies thinned by the learned
@@ -140,12 +143,12 @@ def learned_fom(pipe_cfg, n_groups=48, per_group=16, Ne=65536, overlap=True):
         if mode == "learned":
             t0 = time.perf_counter()
             lr = LearnedConeRoulette(pipe, keep_quantile=0.9)
-            x, y, w = lr.training_set(16, first_batch=30_000)
+            x, y, w = lr.training_set(train_batches, first_batch=30_000)
             lr.fit(x, y, w)
             torch.cuda.synchronize()
             pipe.cone_roulette = lr
             extra = {"training_s": time.perf_counter() - t0, "training_copies": int(x.shape[0]),
-                     "keep_quantile": lr.keep_quantile}
+                     "training_histories": int(train_batches) * int(Ne), "keep_quantile": lr.keep_quantile}
         pipe.step(10**6 + first, 1)  # warm-up
         pipe.spectra()
         vals, t = _fom_arm(pipe, window, dE, first, n_groups, per_group, mode, overlap)
@@ -170,6 +173,11 @@ def learned_fom(pipe_cfg, n_groups=48, per_group=16, Ne=65536, overlap=True):
     for mode in ("cone_split", "learned"):
         out[f"gain_{mode}_vs_analog"] = out[mode]["fom"] / out["analog"]["fom"]
         out[f"gain_{mode}_boot68"] = [float(v) for v in np.quantile(boots[mode] / boots["analog"], [0.16, 0.84])]
+    # the learned roulette against the method without it: both arms' tails are bounded
+    # by the copies' weights, so this ratio does not hang on the analog arm's rare events
+    out["gain_learned_vs_cone_split"] = out["learned"]["fom"] / out["cone_split"]["fom"]
+    out["gain_learned_vs_cone_split_boot68"] = [float(v) for v in
+                                               np.quantile(boots["learned"] / boots["cone_split"], [0.16, 0.84])]
     return out
 
 
@@ -581,7 +589,7 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_fom:
         out["fom"] = learned_fom(cfg, args.fom_groups, args.fom_group_batches, args.fom_histories,
-                                 overlap=not args.fom_sequential)
+                                 overlap=not args.fom_sequential, train_batches=args.fom_train_batches)
         torch.cuda.empty_cache()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(host_tables, args.cpu_sample, args.cpu_sample_one)
