@@ -252,7 +252,8 @@ def simulate_trajectories(
     if initial_state is None:  # refuses a spread that could go non-positive, before any device work
         beam_args(spec, seed, electron_start, Ne, E0_keV, tilt_polar_rad, tilt_azim_rad, bd)
     lut_kw = {} if transport_lut_config is None else {
-        k: v for k, v in dict(transport_lut_config).items() if k in ("intervals_per_decade", "min_points", "max_points")}
+        k: v for k, v in dict(transport_lut_config).items()
+        if k in ("intervals_per_decade", "min_points", "max_points", "rel_tol")}
     lut = build_transport_lut(float(E_cut_h.min()), E_top, tables, **lut_kw)
     # host-side refusals above run before any device work
     backend = require_device()

@@ -318,9 +318,21 @@ def _evaluate_nodes(tables: LayerTables, E_grid):
     return total, dEds, inv_beta, cdf, alpha
 
 
+class TransportLUTToleranceWarning(UserWarning):
+    """The built transport LUT does not meet its requested interpolation tolerance."""
+
+
 def build_transport_lut(E_min_keV, E_max_keV, tables: LayerTables, intervals_per_decade=1024,
-                        min_points=256, max_points=16384, audit=False) -> TransportLUT:
-    """Uniform-ln(E) LUT over ``[E_min, E_max]`` (lut.py:275-437)."""
+                        min_points=256, max_points=16384, audit=None, rel_tol=1e-4) -> TransportLUT:
+    """Uniform-ln(E) LUT over ``[E_min, E_max]`` (lut.py:275-437).
+
+    With ``rel_tol > 0`` (the default 1e-4) the linear interpolant's relative
+    error is measured at the quarter, half and three-quarter points of every
+    interval in the total rate, stopping, 1/beta and the live elements'
+    screening (the CDF is a probability and is not held to it) and a
+    ``TransportLUTToleranceWarning`` says when it is unmet (``lut.py:380-419``);
+    ``max_rel_interp_error`` reports what was achieved.  ``audit=False`` skips
+    the measurement."""
     E_min_keV = float(E_min_keV)
     E_max_keV = float(E_max_keV)
     if not (np.isfinite(E_min_keV) and np.isfinite(E_max_keV)) or E_min_keV <= 0.0:
@@ -342,6 +354,9 @@ def build_transport_lut(E_min_keV, E_max_keV, tables: LayerTables, intervals_per
     E_grid[-1] = E_max_keV
     total, dEds, inv_beta, cdf, alpha = _evaluate_nodes(tables, E_grid)
     err = 0.0
+    worst_keV = E_min_keV
+    if audit is None:
+        audit = float(rel_tol) > 0.0
     if audit:
         for w in (0.25, 0.5, 0.75):
             Es = np.exp((1.0 - w) * logE[:-1] + w * logE[1:])
@@ -352,8 +367,25 @@ def build_transport_lut(E_min_keV, E_max_keV, tables: LayerTables, intervals_per
                 [(alpha[L, :m], sa[L, :m]) for L, m in enumerate(tables.L_nel.tolist())]
             for node, samp in ((total, st), (dEds, sd), (inv_beta[None, :], sb[None, :]), *screens):
                 pred = (1.0 - w) * node[..., :-1] + w * node[..., 1:]
-                e = np.abs(pred - samp) / np.abs(samp)
-                err = max(err, float(np.nanmax(e)))
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    e = np.abs(pred - samp) / np.abs(samp)
+                e = np.where(np.isfinite(e), e, 0.0)
+                while e.ndim > 1:
+                    e = e.max(axis=0)
+                if e.size and float(e.max()) > err:
+                    err = float(e.max())
+                    worst_keV = float(Es[int(np.argmax(e))])
+        if float(rel_tol) > 0.0 and err > float(rel_tol):
+            import warnings
+
+            capped = n_energy < requested
+            warnings.warn(
+                f"transport LUT tolerance unmet: measured max relative interpolation error {err:.3e} > rel_tol "
+                f"{float(rel_tol):.3e} at {worst_keV:.6g} keV with {n_energy} points over {E_min_keV:.6g}-"
+                f"{E_max_keV:.6g} keV" + (f"; max_points={max_points} capped the requested {requested} points"
+                                          if capped else "; raise intervals_per_decade, or accept the O(h) error "
+                                          "a model join leaves in the interval that straddles it"),
+                TransportLUTToleranceWarning, stacklevel=2)
     return TransportLUT(
         log_E_min=log_E_min,
         inv_dlogE=float(inv_dlogE),

@@ -181,3 +181,21 @@ def test_simulate_refuses_energies_outside_the_tables_before_any_device_work(tmp
     with pytest.raises(ValueError, match=r'Si.*\[0.05, 300\] keV'):
         simulate_trajectories(300.0, 16, 1.0, "Si", 0.05, E_cut_keV=float(np.nextafter(0.05, 0.0)),
                               elastic_model="mott", mott_tables_dir=tmp_path, max_steps=8)
+
+
+def test_lut_audit_warns_when_its_tolerance_is_unmet():
+    """lut.py:380-419: the audit runs by default and warns when the grid misses rel_tol."""
+    import warnings
+
+    from pyrite_amd.transport.tables import TransportLUTToleranceWarning
+
+    lt = build_layer_tables([(0.0, 1e5, [("Si", 0.05)])])
+    with warnings.catch_warnings():
+        warnings.simplefilter("error", TransportLUTToleranceWarning)
+        lut = build_transport_lut(5.0, 5000.0, lt)
+    assert 0.0 < lut.max_rel_interp_error < 1e-4
+    # 100 nodes over three decades: ~6e-4 measured
+    with pytest.warns(TransportLUTToleranceWarning, match="max_points=100 capped"):
+        coarse = build_transport_lut(5.0, 5000.0, lt, min_points=2, max_points=100)
+    assert coarse.max_rel_interp_error > 1e-4
+    assert build_transport_lut(5.0, 5000.0, lt, rel_tol=0.0).max_rel_interp_error == 0.0
