@@ -166,6 +166,7 @@ def simulate_trajectories(
     transport_lut_config=None,
     _electron_start=None,
     _cone_split=None,
+    collect_diagnostics=False,
 ):
     """Transport ``Ne`` electrons of ``E0_keV`` into a slab (hip core).
 
@@ -183,6 +184,10 @@ def simulate_trajectories(
     ``transport_precision``: ``"auto"`` (FP64 here), ``"fp64"`` or ``"mixed"``.
     ``transport_lut_config``: ``intervals_per_decade``, ``min_points``,
     ``max_points`` of the LUT.
+
+    ``collect_diagnostics``: ``result["transport_diagnostics"]``, percentile
+    summaries of each flight's fractional energy loss, relative elastic-hazard
+    change, clock error estimate and cutoff overshoot (``diagnostics.py``).
 
     ``E_cut_by_electrons``: one cutoff [keV] per electron in place of
     ``E_cut_keV`` (the tables span the lowest).
@@ -313,6 +318,13 @@ def simulate_trajectories(
     out = _assemble(res, Ne, tables, control, finite, elastic_model, atomic_electron_deflection,
                     E_h.copy() if initial_E is None else initial_E, initial_r, initial_v, E, pos, dirs, stragg,
                     straggling, energy_model, hard, raise_on_step_limit, max_steps)
+    if collect_diagnostics:  # batching.py:98-217: percentile summaries of the per-flight errors
+        from .diagnostics import flight_diagnostic_summary
+
+        host = lambda v: v.cpu().numpy() if hasattr(v, "cpu") else np.asarray(v)  # noqa: E731
+        out["transport_diagnostics"] = flight_diagnostic_summary(host(out["E_keV"]), host(out["L_ang"]),
+                                                                 host(out["electron_id"]), host(out["layer"]),
+                                                                 E_cut_h, tables)
     if _cone_split is not None:
         rec = _copy_generation(core, electron_start, E_cut, dtabs, control, dev, per_electron_config, tables, finite,
                                elastic_model, atomic_electron_deflection, straggling, energy_model,

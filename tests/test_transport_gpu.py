@@ -282,6 +282,22 @@ def test_simulate_trajectories_beam_keywords_match_the_oracle_entry(device, beam
         np.testing.assert_array_equal(out["initial_v_hat"], np.tile(bd / np.linalg.norm(bd), (n, 1)))
 
 
+def test_flight_diagnostics_use_the_elsepa_hazard(device):
+    """test_elsepa_elastic.py:358-378: every row is a flight and the hazard
+    summaries are finite on ELSEPA tables."""
+    from pyrite_amd.transport.api import simulate_trajectories
+    from tests.transport_cases import elsepa_test_tables
+
+    layers = [(0.0, 2.0e5, [("Si", 0.04994)])]
+    out = simulate_trajectories(30.0, 2000, 2.0e5, composition=[("Si", 0.04994)], seed=11, elastic_model="elsepa",
+                                elastic_tables=elsepa_test_tables(layers, np.geomspace(4.0, 40.0, 41)),
+                                collect_diagnostics=True, keep_segments_on_device=False)
+    summary = out["transport_diagnostics"]
+    assert summary["n_flights"] == out["L_ang"].size
+    assert np.isfinite(summary["relative_hazard_change"]["max"])
+    assert 0.0 <= summary["fractional_energy_loss"]["p50"] <= summary["fractional_energy_loss"]["max"] < 1.0
+
+
 def test_simulate_trajectories_cutoffs_by_electron(device):
     """Per-electron cutoffs: cutoff-stopped histories end at their own cutoff,
     and electrons sharing a cutoff transport as in a run with that cutoff
