@@ -167,6 +167,18 @@ def simulate_trajectories(
     _electron_start=None,
     _cone_split=None,
     collect_diagnostics=False,
+    transport_core="auto",
+    _energy_range_keV=None,
+    stopping_tables=None,
+    inelastic_materials=None,
+    bremslib_tables=None,
+    gdf_source=None,
+    secondary_threshold_eV=None,
+    pair_production_model=None,
+    positron_transport=False,
+    mixed_elastic=None,
+    groove=None,
+    radiative_moments="table",
 ):
     """Transport ``Ne`` electrons of ``E0_keV`` into a slab (hip core).
 
@@ -227,6 +239,23 @@ def simulate_trajectories(
 
     if _electron_start is not None:
         electron_start = int(_electron_start)
+    # the reference's core selector: every device core of this backend is the hip LUT core
+    if transport_core not in ("auto", "per-electron", "cuda", "hip"):
+        raise NotImplementedError(f"transport_core={transport_core!r}: this backend runs the per-electron hip core "
+                                  "('auto', 'per-electron', 'cuda' and 'hip' select it)")
+    # inputs that need fetched data or subsystems this backend does not build fail closed
+    unsupported = {"stopping_tables": stopping_tables, "inelastic_materials": inelastic_materials,
+                   "bremslib_tables": bremslib_tables, "gdf_source": gdf_source,
+                   "secondary_threshold_eV": secondary_threshold_eV, "pair_production_model": pair_production_model,
+                   "mixed_elastic": mixed_elastic, "groove": groove}
+    for key, value in unsupported.items():
+        if value is not None:
+            raise NotImplementedError(f"{key} is not supported by the hip backend (fetched tables or a subsystem "
+                                      "outside this build); pass None")
+    if positron_transport:
+        raise NotImplementedError("positron_transport is not supported by the hip backend")
+    if radiative_moments not in ("table", "exact"):
+        raise ValueError("radiative_moments must be 'table' or 'exact'")
     if transport_precision not in ("auto", "fp64", "mixed"):
         raise ValueError("transport_precision must be 'auto', 'fp64' or 'mixed'")
     spread = float(energy_spread_frac or 0.0)
@@ -259,7 +288,13 @@ def simulate_trajectories(
     lut_kw = {} if transport_lut_config is None else {
         k: v for k, v in dict(transport_lut_config).items()
         if k in ("intervals_per_decade", "min_points", "max_points", "rel_tol")}
-    lut = build_transport_lut(float(E_cut_h.min()), E_top, tables, **lut_kw)
+    lo, hi = float(E_cut_h.min()), E_top
+    if _energy_range_keV is not None:  # a block's population range, so its tables equal the single call's
+        r_lo, r_hi = (float(v) for v in _energy_range_keV)
+        if r_lo > lo or r_hi < hi:
+            raise ValueError("_energy_range_keV must cover every transported electron")
+        lo, hi = r_lo, r_hi
+    lut = build_transport_lut(lo, hi, tables, **lut_kw)
     # host-side refusals above run before any device work
     backend = require_device()
     dev = backend.torch_device

@@ -177,3 +177,16 @@ def test_flat_beam_keywords_map_onto_the_spec():
     pos, dirs = beam.entry(a, 5)
     np.testing.assert_array_equal(pos, 0.0)
     np.testing.assert_array_equal(dirs, np.tile([0.0, 0.6, 0.8], (5, 1)))
+
+
+def test_flat_entry_fails_closed_on_unsupported_inputs():
+    from pyrite_amd.transport.api import simulate_trajectories
+
+    kw = dict(E0_keV=30.0, Ne=2, thickness_ang=1e4, element="Si", n_atoms_per_ang3=0.05)
+    with pytest.raises(NotImplementedError, match="transport_core='lockstep'"):
+        simulate_trajectories(transport_core="lockstep", **kw)
+    for key in ("stopping_tables", "bremslib_tables", "gdf_source", "secondary_threshold_eV", "mixed_elastic"):
+        with pytest.raises(NotImplementedError, match=key):
+            simulate_trajectories(**{key: object()}, **kw)
+    with pytest.raises(ValueError, match="must cover every transported electron"):
+        simulate_trajectories(_energy_range_keV=(6.0, 30.0), **kw)

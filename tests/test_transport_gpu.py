@@ -316,6 +316,12 @@ def test_simulate_trajectories_cutoffs_by_electron(device):
     even = np.arange(n) % 2 == 0
     np.testing.assert_array_equal(out["seg_count"][even], low["seg_count"][even])
     np.testing.assert_array_equal(out["final_E_keV"][even], low["final_E_keV"][even])
+    # a block whose own cutoffs are all 10 keV, on the population's table range (host_pass
+    # _energy_range_keV), equals that slice of the single call bit for bit
+    blk = simulate_trajectories(30.0, 1, 2.0e5, E_cut_by_electrons=[10.0], electron_start=1,
+                                _energy_range_keV=(5.0, 30.0), **kw)
+    np.testing.assert_array_equal(blk["seg_count"], out["seg_count"][1:2])
+    np.testing.assert_array_equal(blk["final_E_keV"], out["final_E_keV"][1:2])
 
 
 def test_simulate_trajectories_mott_endpoints_are_inclusive(device, tmp_path):
