@@ -632,6 +632,32 @@ hr_status hr_coherent_field_workspace_bytes(const hr_coherent_field_args *args, 
 hr_status hr_coherent_finalize(const double *field, int32_t n_dir, int32_t n_g, const double *dir_weight,
                                const double *wm, int64_t n_E, double scale, double *out, void *stream);
 
+/* ---- inter-electron decoherence blend
+ * Replaces the grouped-intensity reduction _grouped_intensity_kernel
+ * (src/pyrite/montecarlo/spectrum/coherent_grouped_jit_kernel.py:62-230) and
+ * the row estimator mixed_row_power (coherent_population.py:83-96).
+ * grouped[r][k] += sum_e (|F_s,e|^2 + |F_p,e|^2), F_.,e the field of record
+ * columns [group_start[e], group_start[e+1]) of row r (the same cell as
+ * hr_coherent_field); group_start is ascending from 0 to field.n_lines
+ * (reads are clamped into [0, n_lines]).  One writer per cell per call:
+ * fixed-order partial sums in field.ws (hr_coherent_grouped_workspace_bytes);
+ * field.field is not read. */
+typedef struct hr_coherent_grouped_args {
+    hr_coherent_field_args field;
+    const int64_t *group_start; /* [n_groups + 1] device */
+    int64_t n_groups;
+    double *grouped;            /* [n_rows][n_E] FP64, accumulated */
+} hr_coherent_grouped_args;
+hr_status hr_coherent_grouped(const hr_coherent_grouped_args *args, void *stream);
+hr_status hr_coherent_grouped_workspace_bytes(const hr_coherent_grouped_args *args, size_t *bytes);
+/* out[k] += scale sum_d dir_weight[d] sum_g wm[g] M_dg(k), with P = |F_s|^2 + |F_p|^2
+ * of the all-line field, G the grouped term and F = factor[k]:
+ * pair_scale < 0 (sampled bunch): M = (1 - F) G + F P;
+ * pair_scale >= 0 (physical population, (N-1)/(M-1)): M = G + pair_scale F (P - G). */
+hr_status hr_coherent_blend_finalize(const double *field, const double *grouped, int32_t n_dir, int32_t n_g,
+                                     const double *dir_weight, const double *wm, int64_t n_E, const double *factor,
+                                     double pair_scale, double scale, double *out, void *stream);
+
 /* ---- incident states on the entrance face
  * Replaces initial_beam_positions + primary_directions for an ungrooved
  * analytic beam (src/pyrite/montecarlo/transport/beam_entry.py:23-126,

@@ -727,6 +727,35 @@ def coherent_field(rec, E_grid, dom, F=None, bound=None, threads=0):
     return F
 
 
+def coherent_grouped(rec, group_start, E_grid, dom, threads=0):
+    """Grouped term of the decoherence blend for one row
+    (coherent_grouped_jit_kernel.py:62-230): ``G[k] = sum_e sum_c F_e[c, k]^2``
+    with ``F_e`` the FP64 field of record lines ``[group_start[e],
+    group_start[e+1])``.  Also returns ``size[k] = sum_e b_e (sum_c |F_e,c| + b_e)``
+    (``b_e`` the L1 size of group ``e``'s sum): a per-cell field error
+    ``delta b_e`` moves ``G`` by at most ``4 delta size``."""
+    rec = _f64(np.asarray(rec).reshape(-1, 13))
+    gs = np.asarray(group_start, dtype=np.int64)
+    G = np.zeros(len(E_grid))
+    size = np.zeros(len(E_grid))
+    for e in range(len(gs) - 1):
+        if gs[e + 1] <= gs[e]:
+            continue
+        b = np.zeros(len(E_grid))
+        F = coherent_field(rec[gs[e]:gs[e + 1]], E_grid, dom, bound=b, threads=threads)
+        G += (F * F).sum(0)
+        size += b * (np.abs(F).sum(0) + b)
+    return G, size
+
+
+def mixed_row_power(grouped, flat, factor, pair=None):
+    """The reference's row estimator (coherent_population.py:83-96): sampled
+    bunch ``(1 - F) G + F P``; physical population ``G + pair F (P - G)``."""
+    if pair is None:
+        return (1.0 - factor) * grouped + factor * flat
+    return grouped + pair * factor * (flat - grouped)
+
+
 def sincsq_reduce(E_r, a, w, E_grid, out=None, threads=0):
     E_r, a, w, E_grid = _f64(E_r), _f64(a), _f64(w), _f64(E_grid)
     out = np.zeros(len(E_grid)) if out is None else out

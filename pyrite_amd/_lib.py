@@ -318,6 +318,14 @@ class CoherentFieldArgs(C.Structure):
     ]
 
 
+class CoherentGroupedArgs(C.Structure):
+    """Mirror of ``hr_coherent_grouped_args``."""
+
+    _fields_ = [
+        ("field", CoherentFieldArgs), ("group_start", _dp), ("n_groups", C.c_int64), ("grouped", _dp),
+    ]
+
+
 class BeamArgs(C.Structure):
     """Mirror of ``hr_beam_args``."""
 
@@ -459,6 +467,10 @@ SIGNATURES = {
     "hr_coherent_field": (C.c_int32, [C.POINTER(CoherentFieldArgs), _dp]),
     "hr_coherent_field_workspace_bytes": (C.c_int32, [C.POINTER(CoherentFieldArgs), C.POINTER(C.c_size_t)]),
     "hr_coherent_finalize": (C.c_int32, [_dp, C.c_int32, C.c_int32, _dp, _dp, C.c_int64, C.c_double, _dp, _dp]),
+    "hr_coherent_grouped": (C.c_int32, [C.POINTER(CoherentGroupedArgs), _dp]),
+    "hr_coherent_grouped_workspace_bytes": (C.c_int32, [C.POINTER(CoherentGroupedArgs), C.POINTER(C.c_size_t)]),
+    "hr_coherent_blend_finalize": (C.c_int32, [_dp, _dp, C.c_int32, C.c_int32, _dp, _dp, C.c_int64, _dp, C.c_double,
+                                               C.c_double, _dp, _dp]),
     "hr_vr_round": (C.c_int32, [C.POINTER(TransportArgs), C.c_int32, _dp, _dp]),
     "hr_vr_features": (C.c_int32, [C.c_int64, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp, _dp]),
     "hr_vr_window": (C.c_int32, [C.POINTER(VrWindowArgs), _dp]),

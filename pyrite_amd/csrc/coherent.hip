@@ -157,6 +157,34 @@ struct FieldTile {
     int flag[kFieldTile];  // 0 rejected, 1 live, 2 live and |q| small enough for the series branch
 };
 
+// Stage record column j (row planes R, plane stride `plane`) into tile slot t.
+__device__ __forceinline__ void stage_line(FieldTile &T, const double *__restrict__ R, int64_t plane, int64_t j,
+                                           int tid) {
+        const double cs_re = R[4 * plane + j], cs_im = R[5 * plane + j];
+        const double cp_re = R[6 * plane + j], cp_im = R[7 * plane + j];
+        const bool live = cs_re != 0.0 || cs_im != 0.0 || cp_re != 0.0 || cp_im != 0.0;
+        const double a = live ? R[plane + j] : 0.0;
+        const double Ec = live ? R[j] : 0.0;
+        const double apb = live ? R[10 * plane + j] : 0.0, bma = live ? R[11 * plane + j] : 0.0;
+        const double q = live ? R[12 * plane + j] : 0.0;
+        T.a_t[tid] = a * kInv2Pi;
+        T.aEc_t[tid] = a * Ec * kInv2Pi;
+        T.ps_t[tid] = live ? R[2 * plane + j] * kInv2Pi : 0.0;
+        T.gp_t[tid] = live ? R[3 * plane + j] * kInv2Pi : 0.0;
+        T.Le_t[tid] = live ? R[8 * plane + j] * kInv2Pi : 0.0;
+        T.hdL_t[tid] = live ? R[9 * plane + j] * kInv2Pi : 0.0;
+        T.c[0][tid] = (float)cs_re;
+        T.c[1][tid] = (float)cs_im;
+        T.c[2][tid] = (float)cp_re;
+        T.c[3][tid] = (float)cp_im;
+        T.h_apb[tid] = (float)(0.5 * apb);
+        T.h_bma[tid] = (float)(0.5 * bma);
+        T.h_qapb[tid] = (float)(0.5 * q * apb);
+        T.h_qbma[tid] = (float)(0.5 * q * bma);
+        T.q[tid] = (float)q;
+        T.flag[tid] = live ? ((float)q * (float)q < kSeriesW2 ? 2 : 1) : 0;
+}
+
 // One (line l, bin m) cell added to the FP32 group sums.
 template <bool kSeries, int BINS>
 __device__ __forceinline__ void field_cell(const FieldTile &T, int l, const double (&E)[BINS],
@@ -188,6 +216,29 @@ __device__ __forceinline__ void field_cell(const FieldTile &T, int l, const doub
         g[m][1] = fmaf(csr, xi, fmaf(csi, xr, g[m][1]));
         g[m][2] = fmaf(cpr, xr, fmaf(-cpi, xi, g[m][2]));
         g[m][3] = fmaf(cpr, xi, fmaf(cpi, xr, g[m][3]));
+    }
+}
+
+// acc += the fields of tile lines [0, nt), in FP32 groups of kFieldGroup lines.
+template <int BINS>
+__device__ __forceinline__ void tile_fields(const FieldTile &T, int nt, const double (&E)[BINS],
+                                            const double (&D)[BINS], double (&acc)[BINS][4]) {
+    for (int l0 = 0; l0 < nt; l0 += kFieldGroup) {
+        float g[BINS][4];
+#pragma unroll
+        for (int m = 0; m < BINS; ++m)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) g[m][c] = 0.0f;
+        const int l1 = l0 + kFieldGroup < nt ? l0 + kFieldGroup : nt;
+        for (int l = l0; l < l1; ++l) {
+            const int f = T.flag[l];  // uniform over the block
+            if (f == 1) field_cell<false, BINS>(T, l, E, D, g);
+            else if (f == 2) field_cell<true, BINS>(T, l, E, D, g);
+        }
+#pragma unroll
+        for (int m = 0; m < BINS; ++m)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[m][c] += (double)g[m][c];
     }
 }
 
@@ -223,50 +274,9 @@ __global__ void __launch_bounds__(kFieldThreads) coherent_field_kernel(hr_cohere
         const int64_t j0 = tile * kFieldTile;
         const int nt = (int)(p.n_lines - j0 < kFieldTile ? p.n_lines - j0 : kFieldTile);
         __syncthreads();
-        if (tid < nt) {
-            const int64_t j = j0 + tid;
-            const double cs_re = R[4 * plane + j], cs_im = R[5 * plane + j];
-            const double cp_re = R[6 * plane + j], cp_im = R[7 * plane + j];
-            const bool live = cs_re != 0.0 || cs_im != 0.0 || cp_re != 0.0 || cp_im != 0.0;
-            const double a = live ? R[plane + j] : 0.0;
-            const double Ec = live ? R[j] : 0.0;
-            const double apb = live ? R[10 * plane + j] : 0.0, bma = live ? R[11 * plane + j] : 0.0;
-            const double q = live ? R[12 * plane + j] : 0.0;
-            T.a_t[tid] = a * kInv2Pi;
-            T.aEc_t[tid] = a * Ec * kInv2Pi;
-            T.ps_t[tid] = live ? R[2 * plane + j] * kInv2Pi : 0.0;
-            T.gp_t[tid] = live ? R[3 * plane + j] * kInv2Pi : 0.0;
-            T.Le_t[tid] = live ? R[8 * plane + j] * kInv2Pi : 0.0;
-            T.hdL_t[tid] = live ? R[9 * plane + j] * kInv2Pi : 0.0;
-            T.c[0][tid] = (float)cs_re;
-            T.c[1][tid] = (float)cs_im;
-            T.c[2][tid] = (float)cp_re;
-            T.c[3][tid] = (float)cp_im;
-            T.h_apb[tid] = (float)(0.5 * apb);
-            T.h_bma[tid] = (float)(0.5 * bma);
-            T.h_qapb[tid] = (float)(0.5 * q * apb);
-            T.h_qbma[tid] = (float)(0.5 * q * bma);
-            T.q[tid] = (float)q;
-            T.flag[tid] = live ? ((float)q * (float)q < kSeriesW2 ? 2 : 1) : 0;
-        }
+        if (tid < nt) stage_line(T, R, plane, j0 + tid, tid);
         __syncthreads();
-        for (int l0 = 0; l0 < nt; l0 += kFieldGroup) {
-            float g[BINS][4];
-#pragma unroll
-            for (int m = 0; m < BINS; ++m)
-#pragma unroll
-                for (int c = 0; c < 4; ++c) g[m][c] = 0.0f;
-            const int l1 = l0 + kFieldGroup < nt ? l0 + kFieldGroup : nt;
-            for (int l = l0; l < l1; ++l) {
-                const int f = T.flag[l];  // uniform over the block
-                if (f == 1) field_cell<false, BINS>(T, l, E, D, g);
-                else if (f == 2) field_cell<true, BINS>(T, l, E, D, g);
-            }
-#pragma unroll
-            for (int m = 0; m < BINS; ++m)
-#pragma unroll
-                for (int c = 0; c < 4; ++c) acc[m][c] += (double)g[m][c];
-        }
+        tile_fields<BINS>(T, nt, E, D, acc);
     }
 #pragma unroll
     for (int m = 0; m < BINS; ++m) {
@@ -277,6 +287,64 @@ __global__ void __launch_bounds__(kFieldThreads) coherent_field_kernel(hr_cohere
             if (S == 1) out[((int64_t)row * 4 + c) * n_E + k] += acc[m][c];
             else out[(((int64_t)split * p.n_rows + row) * 4 + c) * n_E + k] = acc[m][c];
         }
+    }
+}
+
+// Block (chunk s, row r, bin window w): groups [n_groups s / S, n_groups (s+1) / S)
+// of row r, each group's columns [gs[e], gs[e+1]) summed to one FP64 field
+// whose |F_s|^2 + |F_p|^2 is added to the thread's bins; writes its partial
+// (or adds it when S == 1).
+template <int BINS>
+__global__ void __launch_bounds__(kFieldThreads) coherent_grouped_kernel(hr_coherent_field_args p,
+                                                                         const int64_t *__restrict__ gs,
+                                                                         int64_t n_groups, int W, int S,
+                                                                         double *__restrict__ out) {
+    __shared__ FieldTile T;
+    const int tid = threadIdx.x;
+    const int row = (int)(blockIdx.y / W), win = (int)(blockIdx.y % W);
+    const int split = (int)blockIdx.x;
+    const int64_t n_E = p.n_E;
+    double E[BINS], D[BINS];
+    bool valid[BINS];
+#pragma unroll
+    for (int m = 0; m < BINS; ++m) {
+        const int64_t k = ((int64_t)win * BINS + m) * kFieldThreads + tid;
+        valid[m] = k < n_E;
+        E[m] = valid[m] ? p.E_grid[k] : 0.0;
+        D[m] = valid[m] ? p.dom[k] : 0.0;
+    }
+    double G[BINS];
+#pragma unroll
+    for (int m = 0; m < BINS; ++m) G[m] = 0.0;
+    const int64_t plane = (int64_t)p.n_rows * p.ld;
+    const double *R = p.rec + (int64_t)row * p.ld;
+    const int64_t g_lo = n_groups * split / S, g_hi = n_groups * (split + 1) / S;
+    for (int64_t grp = g_lo; grp < g_hi; ++grp) {
+        int64_t b = gs[grp], e = gs[grp + 1];
+        b = b < 0 ? 0 : (b > p.n_lines ? p.n_lines : b);
+        e = e < b ? b : (e > p.n_lines ? p.n_lines : e);
+        double acc[BINS][4];
+#pragma unroll
+        for (int m = 0; m < BINS; ++m)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[m][c] = 0.0;
+        for (int64_t j0 = b; j0 < e; j0 += kFieldTile) {
+            const int nt = (int)(e - j0 < kFieldTile ? e - j0 : kFieldTile);
+            __syncthreads();
+            if (tid < nt) stage_line(T, R, plane, j0 + tid, tid);
+            __syncthreads();
+            tile_fields<BINS>(T, nt, E, D, acc);
+        }
+#pragma unroll
+        for (int m = 0; m < BINS; ++m)
+            G[m] += acc[m][0] * acc[m][0] + acc[m][1] * acc[m][1] + acc[m][2] * acc[m][2] + acc[m][3] * acc[m][3];
+    }
+#pragma unroll
+    for (int m = 0; m < BINS; ++m) {
+        if (!valid[m]) continue;
+        const int64_t k = ((int64_t)win * BINS + m) * kFieldThreads + tid;
+        if (S == 1) out[(int64_t)row * n_E + k] += G[m];
+        else out[((int64_t)split * p.n_rows + row) * n_E + k] = G[m];
     }
 }
 
@@ -309,6 +377,34 @@ __global__ void __launch_bounds__(256) coherent_finalize_kernel(const double *__
     out[k] += scale * total;
 }
 
+// Decoherence blend per bin (coherent_population.py:83-96): with P the
+// squared all-line field and G the grouped term of each row,
+// pair < 0: (1 - F) G + F P; otherwise G + pair F (P - G).
+__global__ void __launch_bounds__(256) coherent_blend_kernel(const double *__restrict__ field,
+                                                             const double *__restrict__ grouped, int n_dir, int n_g,
+                                                             const double *__restrict__ dir_weight,
+                                                             const double *__restrict__ wm,
+                                                             const double *__restrict__ factor, double pair,
+                                                             int64_t n_E, double scale, double *__restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_E) return;
+    const double F = factor[k];
+    double total = 0.0;
+    for (int d = 0; d < n_dir; ++d) {
+        double dir = 0.0;
+        for (int g = 0; g < n_g; ++g) {
+            const int64_t r = (int64_t)d * n_g + g;
+            const double *A = field + r * 4 * n_E + k;
+            const double P = A[0] * A[0] + A[n_E] * A[n_E] + A[2 * n_E] * A[2 * n_E] + A[3 * n_E] * A[3 * n_E];
+            const double Gr = grouped[r * n_E + k];
+            const double mixed = pair < 0.0 ? (1.0 - F) * Gr + F * P : Gr + pair * F * (P - Gr);
+            dir += wm[g] * mixed;
+        }
+        total += dir_weight[d] * dir;
+    }
+    out[k] += scale * total;
+}
+
 constexpr int kFieldBins = 4;
 constexpr size_t kMaxPartialBytes = size_t(1) << 30;
 
@@ -335,6 +431,25 @@ hr_status field_plan(const hr_coherent_field_args &p, FieldPlan &P) {
     if (S < 1) S = 1;
     P.S = S;
     P.bytes = (S > 1 && p.n_lines > 0) ? (size_t)S * (size_t)cells * sizeof(double) : 0;
+    return HR_OK;
+}
+
+// Launch shape of hr_coherent_grouped: S chunks of groups (as the field plan,
+// never more than there are groups), partials of n_rows*n_E cells.
+hr_status grouped_plan(const hr_coherent_field_args &p, int64_t n_groups, FieldPlan &P) {
+    int cu = 0;
+    hr_status st = device_cu(cu);
+    if (st != HR_OK) return st;
+    const int64_t per_block = (int64_t)kFieldThreads * kFieldBins;
+    P.W = (p.n_E + per_block - 1) / per_block;
+    const int64_t cells = (int64_t)p.n_rows * p.n_E;
+    int64_t S = (int64_t)cu * 8 / ((int64_t)p.n_rows * P.W);
+    if (S > n_groups) S = n_groups;
+    const int64_t s_cap = (int64_t)(kMaxPartialBytes / ((size_t)cells * sizeof(double)));
+    if (S > s_cap) S = s_cap;
+    if (S < 1) S = 1;
+    P.S = S;
+    P.bytes = (S > 1 && n_groups > 0) ? (size_t)S * (size_t)cells * sizeof(double) : 0;
     return HR_OK;
 }
 
@@ -426,6 +541,71 @@ extern "C" hr_status hr_coherent_finalize(const double *field, int32_t n_dir, in
     HR_REQUIRE(field && dir_weight && wm && out, "hr_coherent_finalize: null pointer");
     hipLaunchKernelGGL(coherent_finalize_kernel, dim3((unsigned)blocks_of(n_E, 256)), dim3(256), 0,
                        (hipStream_t)stream, field, n_dir, n_g, dir_weight, wm, n_E, scale, out);
+    HR_CHECK_HIP(hipGetLastError());
+    return HR_OK;
+}
+
+extern "C" hr_status hr_coherent_grouped_workspace_bytes(const hr_coherent_grouped_args *args, size_t *bytes) {
+    using namespace hr;
+    HR_REQUIRE(args != nullptr && bytes != nullptr, "hr_coherent_grouped_workspace_bytes: null argument");
+    const hr_coherent_field_args &p = args->field;
+    HR_REQUIRE(p.n_lines >= 0 && p.n_rows >= 1 && p.n_E >= 1 && args->n_groups >= 0,
+               "hr_coherent_grouped_workspace_bytes: bad sizes");
+    FieldPlan P;
+    hr_status st = grouped_plan(p, args->n_groups, P);
+    if (st != HR_OK) return st;
+    *bytes = P.bytes;
+    return HR_OK;
+}
+
+extern "C" hr_status hr_coherent_grouped(const hr_coherent_grouped_args *args, void *stream) {
+    HR_TRACE("hr_coherent_grouped");
+    using namespace hr;
+    HR_REQUIRE(args != nullptr, "hr_coherent_grouped: null args");
+    const hr_coherent_field_args &p = args->field;
+    const int64_t n_groups = args->n_groups;
+    HR_REQUIRE(p.n_lines >= 0 && p.ld >= p.n_lines && p.n_rows >= 1 && p.n_E >= 1 && n_groups >= 0,
+               "hr_coherent_grouped: bad sizes (n_lines=%lld ld=%lld n_rows=%d n_E=%lld n_groups=%lld)",
+               (long long)p.n_lines, (long long)p.ld, p.n_rows, (long long)p.n_E, (long long)n_groups);
+    if (p.n_lines == 0 || n_groups == 0) return HR_OK;
+    HR_REQUIRE(p.rec && p.E_grid && p.dom && args->group_start && args->grouped,
+               "hr_coherent_grouped: null pointer");
+    FieldPlan P;
+    hr_status st = grouped_plan(p, n_groups, P);
+    if (st != HR_OK) return st;
+    const int64_t W = P.W, S = P.S;
+    const int64_t cells = (int64_t)p.n_rows * p.n_E;
+    HR_REQUIRE((int64_t)p.n_rows * W <= 65535, "hr_coherent_grouped: %lld row windows exceed the grid",
+               (long long)(p.n_rows * W));
+    HR_REQUIRE_WS(p.ws, P.bytes, "hr_coherent_grouped");
+    hipStream_t s = (hipStream_t)stream;
+    double *out = S > 1 ? (double *)p.ws.ptr : args->grouped;
+    hipLaunchKernelGGL(coherent_grouped_kernel<kFieldBins>, dim3((unsigned)S, (unsigned)(p.n_rows * W)),
+                       dim3(kFieldThreads), 0, s, p, args->group_start, n_groups, (int)W, (int)S, out);
+    HR_CHECK_HIP(hipGetLastError());
+    if (S > 1) {
+        hipLaunchKernelGGL(field_partials_kernel, dim3((unsigned)blocks_of(cells, 256)), dim3(256), 0, s,
+                           (const double *)p.ws.ptr, (int)S, cells, args->grouped);
+        HR_CHECK_HIP(hipGetLastError());
+    }
+    return HR_OK;
+}
+
+extern "C" hr_status hr_coherent_blend_finalize(const double *field, const double *grouped, int32_t n_dir,
+                                                int32_t n_g, const double *dir_weight, const double *wm,
+                                                int64_t n_E, const double *factor, double pair_scale, double scale,
+                                                double *out, void *stream) {
+    HR_TRACE("hr_coherent_blend_finalize");
+    using namespace hr;
+    HR_REQUIRE(n_dir >= 1 && n_g >= 1 && n_E >= 0,
+               "hr_coherent_blend_finalize: bad sizes (n_dir=%d n_g=%d n_E=%lld)", n_dir, n_g, (long long)n_E);
+    HR_REQUIRE(pair_scale < 0.0 || (pair_scale >= 0.0 && pair_scale <= 1.0e300),
+               "hr_coherent_blend_finalize: pair_scale must be negative (sampled bunch) or finite");
+    if (n_E == 0) return HR_OK;
+    HR_REQUIRE(field && grouped && dir_weight && wm && factor && out, "hr_coherent_blend_finalize: null pointer");
+    hipLaunchKernelGGL(coherent_blend_kernel, dim3((unsigned)blocks_of(n_E, 256)), dim3(256), 0,
+                       (hipStream_t)stream, field, grouped, n_dir, n_g, dir_weight, wm, factor, pair_scale, n_E, scale,
+                       out);
     HR_CHECK_HIP(hipGetLastError());
     return HR_OK;
 }

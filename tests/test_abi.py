@@ -61,6 +61,7 @@ STRUCTS = {
     "hr_window_args": "WindowArgs",
     "hr_coherent_args": "CoherentArgs",
     "hr_coherent_field_args": "CoherentFieldArgs",
+    "hr_coherent_grouped_args": "CoherentGroupedArgs",
     "hr_queue_config": "QueueConfig",
     "hr_queue_launch": "QueueLaunch",
     "hr_continuation": "Continuation",

@@ -254,3 +254,106 @@ def test_hopg_coherent_case_agrees_with_the_oracle_per_bin(device):
     g_lines, c_lines = np.array(g_lines), np.array(c_lines)
     assert g_lines.sum() > 0 and c_lines.sum() > 0
     _agree(g_lines, c_lines, "coherent lines")
+
+
+def _grouped_scorer(pipe):
+    from pyrite_amd.spectrum.coherent import CoherentScorer
+
+    return CoherentScorer(pipe.line_tables, pipe.cfg.crystal, pipe.E_grid, pipe.cfg.thickness_ang, pipe.n_hats,
+                          pipe.weights, pipe.coherent.device, grouped=True)
+
+
+def test_device_grouped_term_equals_the_oracle(hopg_batch):
+    """Each electron's field squared (hr_coherent_grouped) against the FP64
+    oracle, within the field's parity bound carried through the square."""
+    import torch
+
+    from oracle import oracle as O
+
+    pipe, res = hopg_batch
+    sc = _grouped_scorer(pipe)
+    (_e0, m, cap, rows, cnt), = res.slots
+    rec_d = sc.records(rows, cnt, m, cap).contiguous()
+    n_lines = rec_d.shape[2]
+    c = cnt[:m].clamp(max=cap)
+    gs = torch.cat((torch.zeros(1, dtype=torch.int64, device=c.device), torch.cumsum(c, 0)))
+    print(f"grouped: {m} electrons, {n_lines} lines, {sc.n_rows} rows", flush=True)
+    sc._launch_grouped(rec_d, n_lines, gs)
+    G = sc.grouped.cpu().numpy()
+    rec = rec_d.cpu().numpy()
+    dom = sc.dom.cpu().numpy()
+    gs_h = gs.cpu().numpy()
+    live = 0
+    for row in range(sc.n_rows):
+        ref, size = O.coherent_grouped(rec[:, row, :].T, gs_h, pipe.E_grid, dom)
+        live += ref.max() > 0
+        err = np.abs(G[row] - ref)
+        assert np.all(err <= 4 * 2e-6 * size + 1e-300), (row, np.max(err / np.maximum(size, 1e-300)))
+    assert live >= sc.n_rows // 2
+
+
+def test_blend_limits_and_the_physical_estimator_on_the_device(hopg_batch):
+    """factor 1 (sampled) is the flat spectrum, factor 0 the grouped one, and a
+    physical population follows G + (N-1)/(M-1) F (P - G) of the same planes."""
+    from pyrite_amd.spectrum.coherent import gaussian_longitudinal_factor, pair_scale
+
+    pipe, res = hopg_batch
+    sc = _grouped_scorer(pipe)
+    sc.add_transport(res, pipe.cfg.Ne)
+    n_E = sc.n_E
+    flat = sc.spectrum().cpu().numpy()
+    np.testing.assert_allclose(sc.spectrum(factor=np.ones(n_E)).cpu().numpy(), flat, rtol=1e-12, atol=1e-300)
+    field = sc.field.cpu().numpy()
+    G = sc.grouped.cpu().numpy()
+    w, wm = pipe.weights, pipe.line_tables.wm
+    P = (field ** 2).sum(1)
+    grouped_only = sum(w[r // len(wm)] * wm[r % len(wm)] * G[r] for r in range(sc.n_rows)) / pipe.cfg.Ne
+    np.testing.assert_allclose(sc.spectrum(factor=np.zeros(n_E)).cpu().numpy(), grouped_only, rtol=1e-12,
+                               atol=1e-300)
+    assert np.all(G >= 0.0) and grouped_only.max() > 0
+    F = gaussian_longitudinal_factor(pipe.E_grid, 2e-4)
+    assert 0.0 < F.min() < F.max() < 1.0
+    pair = pair_scale(6.2e6, pipe.cfg.Ne)
+    got = sc.spectrum(factor=F, pair_scale=pair).cpu().numpy()
+    want = sum(w[r // len(wm)] * wm[r % len(wm)] * (G[r] + pair * F * (P[r] - G[r]))
+               for r in range(sc.n_rows)) / pipe.cfg.Ne
+    np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-12 * np.abs(want).max())
+    sampled = sc.spectrum(factor=F).cpu().numpy()
+    want = sum(w[r // len(wm)] * wm[r % len(wm)] * ((1 - F) * G[r] + F * P[r])
+               for r in range(sc.n_rows)) / pipe.cfg.Ne
+    np.testing.assert_allclose(sampled, want, rtol=1e-12, atol=1e-300)
+
+
+def test_grouped_passes_split_only_between_electrons(hopg_batch, monkeypatch):
+    """Record passes end between electrons, so the grouped term of many passes
+    is the one-pass term up to summation order; grouping per-segment rows by
+    electron id gives the slot rows' grouping."""
+    from pyrite_amd.spectrum import coherent
+
+    pipe, res = hopg_batch
+    sc = _grouped_scorer(pipe)
+    sc.add_transport(res, pipe.cfg.Ne)
+    one = sc.grouped.cpu().numpy()
+    scale = one.max()
+    assert scale > 0
+    monkeypatch.setattr(coherent, "RECORD_BYTES", 13 * sc.n_rows * 8 * 3001)
+    sc.reset()
+    sc.add_transport(res, pipe.cfg.Ne)
+    # each electron's FP32 line groups are the same in every pass: only the
+    # chunk partials' summation order moves
+    np.testing.assert_allclose(sc.grouped.cpu().numpy(), one, rtol=0, atol=1e-10 * scale)
+    # the same lines as one row per segment, electrons given by id
+    (_e0, m, cap, rows, cnt), = res.slots
+    h = _host_rows(res)
+    c = np.minimum(cnt[:m].cpu().numpy(), cap)
+    elec = np.repeat(np.arange(m), c)
+    seg = {"r_mid": h["seg_mid"], "v_hat": h["seg_dir"], "L_ang": h["seg_len"], "E_keV": h["seg_E"],
+           "t_ang": h["seg_t0"]}
+    srows, scnt, n = coherent.segment_rows(seg, sc.device)
+    srows["seg_E_end"] = srows["seg_E"].new_tensor(h["seg_E_end"])
+    sc.reset()
+    sc.add_rows(srows, scnt, n, 1, electron=elec)
+    np.testing.assert_allclose(sc.grouped.cpu().numpy(), one, rtol=0, atol=1e-10 * scale)
+    sc.reset()
+    sc.add_rows(srows, scnt, n, 1)  # each segment its own electron: the incoherent limit
+    assert np.max(np.abs(sc.grouped.cpu().numpy() - one)) > 1e-3 * scale
