@@ -64,6 +64,8 @@ def parse():
     p.add_argument("--fom-group-batches", type=int, default=16, help="batches per FOM group")
     p.add_argument("--fom-histories", type=int, default=65536, help="histories per FOM batch")
     p.add_argument("--fom-sequential", action="store_true", help="FOM arms back to back on one stream")
+    p.add_argument("--coherent-grouped", action="store_true",
+                   help="coherent workload: also accumulate the decoherence blend's per-electron grouped term")
     p.add_argument("--workload", choices=("hopg", "si-stream", "pixels", "coherent", "queue"), default="hopg",
                    help="hopg: the bench line (default); si-stream: the 1 mm Si 10 MeV case streamed through "
                         "continuation rounds; pixels: windowed scoring of synthetic segments over 32x32 pixels; "
@@ -254,7 +256,7 @@ def coherent_case(args):
     from pyrite_amd import roofline
     from pyrite_amd.pipeline import CaseConfig, CasePipeline
 
-    cfg = CaseConfig(Ne=args.ne, n_side=args.n_side, coherent=True)
+    cfg = CaseConfig(Ne=args.ne, n_side=args.n_side, coherent=True, coherent_grouped=args.coherent_grouped)
     pipe = CasePipeline(cfg)
     pipe.run_steps(1000, args.warmup, overlap=not args.no_overlap)  # warm-up through the timed path
     torch.cuda.synchronize()
@@ -290,7 +292,8 @@ def coherent_case(args):
         "data": "synthetic (seeded pencil beam; analytic tables)",
         "config": {"workload": "HOPG 100 um, 5 MeV pencil, (002) coherent PXR/CBS fields over a 10 mrad cone "
                                "(3x3 rays) + BH brem, 1024 bins 1-50 keV",
-                   "histories_per_step": cfg.Ne, "segments_per_history": pipe.segments / max(pipe.histories, 1)},
+                   "histories_per_step": cfg.Ne, "segments_per_history": pipe.segments / max(pipe.histories, 1),
+                   "grouped_term": bool(args.coherent_grouped)},
         "roofline": {
             "kernel": "hr_coherent_field: coherent_field_kernel + field_partials_kernel",
             "bound": "valu", "pipe": "vector ALU, FP32 peak (FP64 phase reduction counted as FP32 work)",

@@ -67,6 +67,7 @@ class CaseConfig:
     line_roulette: dict | None = None  # LineRoulette policy fields (per-primary line roulette), or None
     depth_bins: int = 0  # > 0: also tally the depth-dose profile over this many bins of the slab
     coherent: bool = False  # sum line fields before squaring (the coherent route); brem stays incoherent
+    coherent_grouped: bool = False  # also keep the decoherence blend's per-electron grouped term (batched route)
     cone_split: dict | None = None  # ConeSplitting policy fields (detector-cone splitting of elastic events), or None
     elastic_model: str = "sr"  # "sr" or "mott" (SRD 64 tables from mott_tables_dir / PYRITE_MOTT_TABLES_DIR)
     mott_tables_dir: str | None = None
@@ -137,6 +138,8 @@ class CasePipeline(CaseTables):
                  arena_bytes: int | None = None):
         import torch
 
+        if cfg.coherent_grouped and not cfg.coherent:
+            raise ValueError("coherent_grouped needs the coherent route (coherent=True)")
         super().__init__(cfg)
         self.dev = require_device().torch_device
         dev = self.dev
@@ -171,7 +174,7 @@ class CasePipeline(CaseTables):
             from .spectrum.coherent import CoherentScorer
 
             self.coherent = CoherentScorer(self.line_tables, cfg.crystal, self.E_grid, cfg.thickness_ang,
-                                           self.n_hats, self.weights, dev)
+                                           self.n_hats, self.weights, dev, grouped=cfg.coherent_grouped)
         self.cone_core = None
         if cfg.cone_split is not None:
             from .cone_split import ConeSplitCore, ConeSplitting
@@ -598,6 +601,10 @@ class CasePipeline(CaseTables):
         import torch
 
         from .transport.continuation import ContinuationDriver
+
+        if self.coherent is not None and self.coherent.grouped is not None:
+            raise ValueError("the grouped coherent term squares whole histories, which continuation rounds "
+                             "split; run it on the batched route")
 
         cfg = self.cfg
         if cfg.line_roulette is not None:

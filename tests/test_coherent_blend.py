@@ -114,3 +114,10 @@ def test_population_and_pair_scale():
         coh.pair_scale(10.0, 1)
     with pytest.raises(ValueError):
         coh.pair_scale(0.5, 10)
+
+
+def test_pipeline_refuses_grouped_without_coherent_route():
+    from pyrite_amd.pipeline import CaseConfig, CasePipeline
+
+    with pytest.raises(ValueError, match="coherent route"):
+        CasePipeline(CaseConfig(Ne=16, coherent_grouped=True))

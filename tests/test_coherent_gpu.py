@@ -357,3 +357,11 @@ def test_grouped_passes_split_only_between_electrons(hopg_batch, monkeypatch):
     sc.reset()
     sc.add_rows(srows, scnt, n, 1)  # each segment its own electron: the incoherent limit
     assert np.max(np.abs(sc.grouped.cpu().numpy() - one)) > 1e-3 * scale
+
+
+def test_stream_route_refuses_the_grouped_term(device):
+    from pyrite_amd.pipeline import CaseConfig, CasePipeline
+
+    pipe = CasePipeline(CaseConfig(Ne=64, coherent=True, coherent_grouped=True))
+    with pytest.raises(ValueError, match="continuation rounds"):
+        pipe.stream(0, 64)
